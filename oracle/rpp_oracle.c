@@ -1,0 +1,151 @@
+/*
+ * rpp_oracle.c -- CPU oracle for the truncated tensor algebra hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file restates, in plain C, the semantics of
+ * the reference strategy cpu::single_thread of RoughPathPrimitives v1.0.2 for
+ * ft_mul, ft_fma, ft_inplace_fma, ft_inplace_mul, ft_exp, ft_fmexp, ft_log,
+ * tensor_pairing, sparse_matrix_vector_product (CSR and CSC), ft_adj_lmul and
+ * ft_adj_rmul, plus the caller-side Chen recursion (signature) and its
+ * gradient composed from the reference adjoints.  Only tests/, the smoke test
+ * and the CPU leg of bench.py load it, and only as the checker.
+ *
+ * Pinning (each function is checked by tests/test_oracle_*.py):
+ *   - against the real reference compiled from its own headers (oracle/_ref,
+ *     built by oracle/Makefile in the container that holds the reference) on
+ *     seeded inputs and on every degree-window case of
+ *     tests/rpp/ft_degree_range_cases.hpp;
+ *   - against committed golden fixtures written from oracle/_ref
+ *     (tests/golden/);
+ *   - against mathematics: exact dyadic brute force of the concatenation
+ *     product, unit identity, associativity, exp(x)_w = prod x_{w_i}/k! for a
+ *     degree-one x, log(exp(x)) = x, fmexp(exp x, y) = exp(x) exp(y),
+ *     the adjoint pairing identity <L_a^* g, t> = <g, a t>, and central finite
+ *     differences of the signature for the gradient.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_MAX_DEPTH 32
+
+typedef struct {
+    int width;
+    int depth;
+    long long pow[ORC_MAX_DEPTH + 2];   /* width^k                     */
+    long long start[ORC_MAX_DEPTH + 2]; /* start_of_degree(k), k<=N+1  */
+} orc_basis;
+
+/* Degree offsets of TensorBasis (basis/tensor_basis.hpp:122-130):
+ * start(k) = 1 + d + ... + d^(k-1). */
+static int orc_basis_init(orc_basis* B, int width, int depth)
+{
+    if (width < 1 || depth < 0 || depth > ORC_MAX_DEPTH) return 1;
+    B->width = width;
+    B->depth = depth;
+    B->pow[0] = 1;
+    B->start[0] = 0;
+    for (int k = 1; k <= depth + 1; ++k) {
+        B->pow[k] = B->pow[k - 1] * width;
+        B->start[k] = B->start[k - 1] + B->pow[k - 1];
+    }
+    return 0;
+}
+
+long long orc_tensor_dimension(int width, int depth)
+{
+    orc_basis B;
+    if (orc_basis_init(&B, width, depth)) return -1;
+    return B.start[depth + 1];
+}
+
+/* Writer kinds of the reference (ft_multiply.hpp:176-435):
+ *   W_SCALED:  out = beta * tile                       (ScaledTileWriter)
+ *   W_ADD:     out = alpha * add + beta * tile, where the
+ *              addend is read only inside its window   (ScaledTileAddWriter)
+ */
+enum { W_SCALED = 0, W_ADD = 1 };
+
+/* View::truncate(a, b) (views/dense_tensor_view.hpp:29-35). */
+static void trunc_window(int mn, int mx, int a, int b, int* omn, int* omx)
+{
+    *omn = mn > a ? mn : a;
+    *omx = mx < b ? mx : b;
+}
+
+#define T double
+#define SFX f64
+#include "rpp_oracle_impl.h"
+#undef T
+#undef SFX
+
+#define T float
+#define SFX f32
+#include "rpp_oracle_impl.h"
+#undef T
+#undef SFX
+
+/* Gradient of loss_p = <ell, S_p> with respect to the increments of each path,
+ * composed from the reference adjoints (operations/basic/ft_adj_lmul.hpp,
+ * ft_adj_rmul.hpp) as the reference leaves it to the caller:
+ *   G_L = ell;  for t = L..1:  E_t = exp(x_t)
+ *     Ebar    = ft_adj_lmul(op = S_{t-1}, arg = G_t)
+ *     xbar_t  = J_exp(x_t)^T Ebar,   exp(x)_w = prod_i x_{w_i} / |w|!
+ *     G_{t-1} = ft_adj_rmul(op = E_t, arg = G_t)
+ * FP64 only.  ell is one functional (dim) shared by every path. */
+int orc_signature_vjp_f64(int width, int depth, long long n_paths,
+                          const long long* offsets, const double* increments,
+                          const double* ell, double* grad)
+{
+    orc_basis B;
+    if (orc_basis_init(&B, width, depth)) return 3;
+    const long long dim = B.start[depth + 1];
+    double* scratch = (double*)malloc(sizeof(double) * (size_t)(B.pow[depth] + 1));
+    double* e = (double*)calloc((size_t)dim, sizeof(double));
+    double* E = (double*)malloc(sizeof(double) * (size_t)dim);
+    double* G = (double*)malloc(sizeof(double) * (size_t)dim);
+    double* G2 = (double*)malloc(sizeof(double) * (size_t)dim);
+    double* Eb = (double*)malloc(sizeof(double) * (size_t)dim);
+    double* fact = (double*)malloc(sizeof(double) * (size_t)(depth + 1));
+    int* letters = (int*)malloc(sizeof(int) * (size_t)(depth + 1));
+    fact[0] = 1.0;
+    for (int k = 1; k <= depth; ++k) fact[k] = fact[k - 1] * k;
+    for (long long p = 0; p < n_paths; ++p) {
+        const long long L = offsets[p + 1] - offsets[p];
+        double* S = (double*)malloc(sizeof(double) * (size_t)(dim * (L + 1)));
+        for (long long i = 0; i < dim; ++i) S[i] = 0.0;
+        S[0] = 1.0;
+        for (long long t = 0; t < L; ++t) {
+            const double* x = increments + (offsets[p] + t) * width;
+            for (int l = 0; l < width; ++l) e[1 + l] = x[l];
+            orc_ft_fmexp_f64(&B, scratch, S + (t + 1) * dim, 0, depth,
+                             S + t * dim, 0, depth, e, 1, 1);
+        }
+        for (long long i = 0; i < dim; ++i) G[i] = ell[i];
+        for (long long t = L - 1; t >= 0; --t) {
+            const double* x = increments + (offsets[p] + t) * width;
+            double* xb = grad + (offsets[p] + t) * width;
+            for (int l = 0; l < width; ++l) e[1 + l] = x[l];
+            orc_ft_exp_f64(&B, scratch, E, 0, depth, e, 1, 1);
+            orc_ft_adj_lmul_f64(&B, Eb, 0, depth, S + t * dim, 0, depth, G, 0, depth);
+            for (int l = 0; l < width; ++l) xb[l] = 0.0;
+            for (int k = 1; k <= depth; ++k) {
+                for (long long w = 0; w < B.pow[k]; ++w) {
+                    const double g = Eb[B.start[k] + w] / fact[k];
+                    if (g == 0.0) continue;
+                    long long r = w;
+                    for (int i = k - 1; i >= 0; --i) { letters[i] = (int)(r % width); r /= width; }
+                    for (int i = 0; i < k; ++i) {
+                        double prod = 1.0;
+                        for (int j = 0; j < k; ++j)
+                            if (j != i) prod *= x[letters[j]];
+                        xb[letters[i]] += g * prod;
+                    }
+                }
+            }
+            orc_ft_adj_rmul_f64(&B, G2, 0, depth, E, 0, depth, G, 0, depth);
+            double* tmp = G; G = G2; G2 = tmp;
+        }
+        free(S);
+    }
+    free(scratch); free(e); free(E); free(G); free(G2); free(Eb); free(fact); free(letters);
+    return 0;
+}

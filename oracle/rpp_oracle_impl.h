@@ -1,0 +1,518 @@
+/*
+ * Type-generic body of the oracle, included twice by rpp_oracle.c with
+ *   T   = the scalar and accumulator type (float or double),
+ *   SFX = the name suffix (f32 or f64).
+ *
+ * TEST INFRASTRUCTURE ONLY: this is a plain-C restatement of the reference
+ * strategy cpu::single_thread (RoughPathPrimitives v1.0.2).  It is the parity
+ * checker for the HIP library and is never linked into it.
+ *
+ * Every object is addressed as (base pointer, min_degree, max_degree), where
+ * the base pointer is the address of coefficient 0 (the empty word), exactly
+ * like DenseGradedVectorView (views/dense_graded_vector_view.hpp:17-128).
+ * Batches are (pointer, stride in elements) like layouts::StrideLayout
+ * (views/batch.hpp:49-68); a stride of 0 broadcasts one object.
+ */
+
+#define CAT2(a, b) a##_##b
+#define CAT(a, b) CAT2(a, b)
+#define FN(name) CAT(name, SFX)
+
+/* ---- product core ---------------------------------------------------------
+ * Level k of lhs (x) rhs restricted to the operand windows, accumulated in T:
+ *   tile[w] = sum_{j} lhs_{k-j}[w / d^j] * rhs_j[w % d^j]
+ * over j in [max(rhs.min, k - lhs.max), min(rhs.max, k - lhs.min)], j
+ * ascending, which is the rhs-degree order of contiguous_level_terms
+ * (cpu/single_thread/operations/basic/detail/ft_multiply.hpp:989-1027).
+ */
+static void FN(level_product)(const orc_basis* B, T* tile, int k,
+                              const T* lhs, int lmin, int lmax,
+                              const T* rhs, int rmin, int rmax)
+{
+    const long long n = B->pow[k];
+    for (long long w = 0; w < n; ++w) tile[w] = (T)0;
+    int jlo = rmin > k - lmax ? rmin : k - lmax;
+    int jhi = rmax < k - lmin ? rmax : k - lmin;
+    if (jlo < 0) jlo = 0;
+    if (jhi > k) jhi = k;
+    for (int j = jlo; j <= jhi; ++j) {
+        const long long pj = B->pow[j];
+        const T* L = lhs + B->start[k - j];
+        const T* R = rhs + B->start[j];
+        for (long long w = 0; w < n; ++w) {
+            tile[w] += L[w / pj] * R[w % pj];
+        }
+    }
+}
+
+/* Writer kinds W_SCALED / W_ADD are defined in rpp_oracle.c. */
+
+/* multiply(): output degrees are visited in descending order and each level
+ * is fully formed in scratch before it is written, so out may alias lhs, rhs
+ * or the addend (ft_multiply.hpp:1596-1640). */
+static void FN(multiply)(const orc_basis* B, T* scratch,
+                         T* out, int omin, int omax,
+                         const T* lhs, int lmin, int lmax,
+                         const T* rhs, int rmin, int rmax,
+                         int writer, const T* add, int amin, int amax,
+                         T alpha, T beta)
+{
+    for (int k = omax; k >= omin; --k) {
+        FN(level_product)(B, scratch, k, lhs, lmin, lmax, rhs, rmin, rmax);
+        const long long n = B->pow[k];
+        T* o = out + B->start[k];
+        if (writer == W_ADD && k >= amin && k <= amax) {
+            const T* a = add + B->start[k];
+            for (long long w = 0; w < n; ++w) o[w] = alpha * a[w] + beta * scratch[w];
+        } else {
+            for (long long w = 0; w < n; ++w) o[w] = beta * scratch[w];
+        }
+    }
+}
+
+/* --- sub-operations of the composites ------------------------------------ */
+
+/* TensorSetIdentity (cpu/.../basic/tensor_set_identity.hpp): zero the window,
+ * then put `s` in degree 0 if the window holds it. */
+static void FN(set_identity)(const orc_basis* B, T* t, int mn, int mx, T s)
+{
+    if (mn > mx) return;
+    for (long long i = B->start[mn]; i < B->start[mx + 1]; ++i) t[i] = (T)0;
+    if (mn == 0) t[0] = s;
+}
+
+/* TensorAddIdentity (cpu/.../basic/tensor_add_identity.hpp). */
+static void FN(add_identity)(T* t, int mn, int mx, T s)
+{
+    if (mn == 0 && mx >= 0) t[0] = t[0] + s;
+}
+
+/* VectorSetConstant (cpu/.../linalg/vector_set_constant.hpp). */
+static void FN(set_constant)(const orc_basis* B, T* t, int mn, int mx, T v)
+{
+    if (mn > mx) return;
+    for (long long i = B->start[mn]; i < B->start[mx + 1]; ++i) t[i] = v;
+}
+
+/* VectorAssign (cpu/.../linalg/vector_assign.hpp:32-61): copy on the overlap
+ * of the windows, zero the rest of out's window. */
+static void FN(assign)(const orc_basis* B, T* out, int omin, int omax,
+                       const T* arg, int amin, int amax)
+{
+    if (omin > omax) return;
+    int lo = omin > amin ? omin : amin;
+    int hi = omax < amax ? omax : amax;
+    const long long ob = B->start[omin], oe = B->start[omax + 1];
+    if (hi < lo) {
+        for (long long i = ob; i < oe; ++i) out[i] = (T)0;
+        return;
+    }
+    const long long b = B->start[lo], e = B->start[hi + 1];
+    for (long long i = ob; i < b; ++i) out[i] = (T)0;
+    for (long long i = b; i < e; ++i) out[i] = arg[i];
+    for (long long i = e; i < oe; ++i) out[i] = (T)0;
+}
+
+/* --- public single-object operations ------------------------------------- */
+
+/* FTMul (operations/basic/ft_mul.hpp; cpu/.../basic/ft_mul.hpp:42-56). */
+void FN(orc_ft_mul)(const orc_basis* B, T* scratch, T* out, int omin, int omax,
+                    const T* lhs, int lmin, int lmax, const T* rhs, int rmin,
+                    int rmax, T beta)
+{
+    FN(multiply)(B, scratch, out, omin, omax, lhs, lmin, lmax, rhs, rmin, rmax,
+                 W_SCALED, 0, 1, 0, (T)0, beta);
+}
+
+/* FTFma (cpu/.../basic/ft_fma.hpp): out = alpha*a + beta*b(x)c. */
+void FN(orc_ft_fma)(const orc_basis* B, T* scratch, T* out, int omin, int omax,
+                    const T* a, int amin, int amax, const T* b, int bmin,
+                    int bmax, const T* c, int cmin, int cmax, T alpha, T beta)
+{
+    FN(multiply)(B, scratch, out, omin, omax, b, bmin, bmax, c, cmin, cmax,
+                 W_ADD, a, amin, amax, alpha, beta);
+}
+
+/* FTInplaceFma (cpu/.../basic/ft_inplace_fma.hpp:54-67):
+ *   231: a <- alpha*a + beta*b(x)c
+ *   123: a <- alpha*c + beta*a(x)b
+ *   213: a <- alpha*c + beta*b(x)a */
+void FN(orc_ft_inplace_fma)(const orc_basis* B, T* scratch, int variant,
+                            T* a, int amin, int amax, const T* b, int bmin,
+                            int bmax, const T* c, int cmin, int cmax, T alpha,
+                            T beta)
+{
+    if (variant == 231) {
+        FN(multiply)(B, scratch, a, amin, amax, b, bmin, bmax, c, cmin, cmax,
+                     W_ADD, a, amin, amax, alpha, beta);
+    } else if (variant == 123) {
+        FN(multiply)(B, scratch, a, amin, amax, a, amin, amax, b, bmin, bmax,
+                     W_ADD, c, cmin, cmax, alpha, beta);
+    } else {
+        FN(multiply)(B, scratch, a, amin, amax, b, bmin, bmax, a, amin, amax,
+                     W_ADD, c, cmin, cmax, alpha, beta);
+    }
+}
+
+/* FTInplaceMul (cpu/.../basic/ft_inplace_mul.hpp): lhs <- beta*lhs(x)rhs. */
+void FN(orc_ft_inplace_mul)(const orc_basis* B, T* scratch, T* lhs, int lmin,
+                            int lmax, const T* rhs, int rmin, int rmax, T beta)
+{
+    FN(multiply)(B, scratch, lhs, lmin, lmax, lhs, lmin, lmax, rhs, rmin, rmax,
+                 W_SCALED, 0, 1, 0, (T)0, beta);
+}
+
+/* FTExp (operations/intermediate/ft_exp.hpp:70-81), literal Horner order. */
+void FN(orc_ft_exp)(const orc_basis* B, T* scratch, T* out, int omin, int omax,
+                    const T* arg, int amin, int amax)
+{
+    const int N = B->depth;
+    int tmin, tmax;
+    FN(set_identity)(B, out, omin, omax, (T)1);
+    trunc_window(amin, amax, 1, N, &tmin, &tmax);
+    for (int d = N; d > 0; --d) {
+        const T divisor = (T)1 / (T)d;
+        FN(multiply)(B, scratch, out, omin, omax, out, omin, omax, arg, tmin,
+                     tmax, W_SCALED, 0, 1, 0, (T)0, divisor);
+        FN(add_identity)(out, omin, omax, (T)1);
+    }
+}
+
+/* FTFMExp (operations/intermediate/ft_fmexp.hpp:66-88), literal Horner order:
+ * out = mult; for d=N..1: out = mult + (1/d) out (x) exponent_[1, N-d+1]. */
+void FN(orc_ft_fmexp)(const orc_basis* B, T* scratch, T* out, int omin,
+                      int omax, const T* mult, int mmin, int mmax,
+                      const T* ex, int emin, int emax)
+{
+    const int N = B->depth;
+    FN(assign)(B, out, omin, omax, mult, mmin, mmax);
+    for (int d = N; d > 0; --d) {
+        int tmin, tmax;
+        trunc_window(emin, emax, 1, N - d + 1, &tmin, &tmax);
+        const T divisor = (T)1 / (T)d;
+        FN(multiply)(B, scratch, out, omin, omax, out, omin, omax, ex, tmin,
+                     tmax, W_ADD, mult, mmin, mmax, (T)1, divisor);
+    }
+}
+
+/* FTLog (operations/intermediate/ft_log.hpp:70-84), literal Horner order. */
+void FN(orc_ft_log)(const orc_basis* B, T* scratch, T* out, int omin, int omax,
+                    const T* arg, int amin, int amax)
+{
+    const int N = B->depth;
+    FN(set_constant)(B, out, omin, omax, (T)0);
+    for (int d = N; d > 0; --d) {
+        const T val = (d % 2 == 0 ? (T)-1 : (T)1) / (T)d;
+        FN(add_identity)(out, omin, omax, val);
+        int o_lo, o_hi, a_lo, a_hi;
+        trunc_window(omin, omax, 0, N - d + 1, &o_lo, &o_hi);
+        trunc_window(amin, amax, 1, N - d + 1, &a_lo, &a_hi);
+        FN(multiply)(B, scratch, out, o_lo, o_hi, out, o_lo, o_hi, arg, a_lo,
+                     a_hi, W_SCALED, 0, 1, 0, (T)0, (T)1);
+    }
+}
+
+/* TensorPairing (cpu/.../basic/tensor_pairing.hpp:43-65): sum over the
+ * overlap of the two windows; 0 when it is empty. */
+T FN(orc_tensor_pairing)(const orc_basis* B, const T* f, int fmin, int fmax,
+                         const T* x, int xmin, int xmax)
+{
+    int lo = fmin > xmin ? fmin : xmin;
+    int hi = fmax < xmax ? fmax : xmax;
+    T acc = (T)0;
+    if (hi < lo) return acc;
+    for (long long i = B->start[lo]; i < B->start[hi + 1]; ++i) acc += f[i] * x[i];
+    return acc;
+}
+
+/* FTAdjLMul (cpu/.../basic/ft_adj_lmul.hpp:40-115): out[w] = sum_u op[u] arg[uw]
+ * with argument-degree-major traversal and per-step rounding to T. */
+void FN(orc_ft_adj_lmul)(const orc_basis* B, T* out, int omin, int omax,
+                         const T* op, int pmin, int pmax, const T* arg,
+                         int gmin, int gmax)
+{
+    FN(set_constant)(B, out, omin, omax, (T)0);
+    int gmx = gmax < omax + pmax ? gmax : omax + pmax;
+    int gmn = gmin > omin + pmin ? gmin : omin + pmin;
+    if (gmn > gmx) return;
+    for (int ad = gmx; ad >= gmn; --ad) {
+        int olo = ad - pmax > omin ? ad - pmax : omin;
+        int ohi = ad - pmin < omax ? ad - pmin : omax;
+        const T* A = arg + B->start[ad];
+        for (int od = ohi; od >= olo; --od) {
+            const int pd = ad - od;
+            const T* P = op + B->start[pd];
+            T* O = out + B->start[od];
+            const long long osz = B->pow[od], psz = B->pow[pd];
+            for (long long u = 0; u < psz; ++u) {
+                const T pv = P[u];
+                for (long long i = 0; i < osz; ++i) O[i] = O[i] + pv * A[i + u * osz];
+            }
+        }
+    }
+}
+
+/* FTAdjRMul (cpu/.../basic/ft_adj_rmul.hpp:102-134), which is
+ * antipode o adj_lmul(antipode op, antipode arg); the signs cancel and it
+ * equals out[w] = sum_u arg[wu] op[u] over the same degree windows. */
+void FN(orc_ft_adj_rmul)(const orc_basis* B, T* out, int omin, int omax,
+                         const T* op, int pmin, int pmax, const T* arg,
+                         int gmin, int gmax)
+{
+    FN(set_constant)(B, out, omin, omax, (T)0);
+    int gmx = gmax < omax + pmax ? gmax : omax + pmax;
+    int gmn = gmin > omin + pmin ? gmin : omin + pmin;
+    if (gmn > gmx) return;
+    for (int ad = gmx; ad >= gmn; --ad) {
+        int olo = ad - pmax > omin ? ad - pmax : omin;
+        int ohi = ad - pmin < omax ? ad - pmin : omax;
+        const T* A = arg + B->start[ad];
+        for (int od = ohi; od >= olo; --od) {
+            const int pd = ad - od;
+            const T* P = op + B->start[pd];
+            T* O = out + B->start[od];
+            const long long osz = B->pow[od], psz = B->pow[pd];
+            for (long long i = 0; i < osz; ++i) {
+                T acc = O[i];
+                for (long long u = 0; u < psz; ++u) acc = acc + A[i * psz + u] * P[u];
+                O[i] = acc;
+            }
+        }
+    }
+}
+
+/* SparseMatrixVectorProduct, CSR rows (cpu/.../linalg/sparse_matrix_vector.hpp
+ * :45-73).  out_begin/arg_begin are view.begin(): row 0 is out_begin. */
+void FN(orc_spmv_csr)(T* out_begin, const T* arg_begin, const T* values,
+                      const long long* inner, const long long* offsets,
+                      long long rows, T alpha)
+{
+    for (long long r = 0; r < rows; ++r) {
+        T acc = (T)0;
+        for (long long e = offsets[r]; e < offsets[r + 1]; ++e)
+            acc += values[e] * arg_begin[inner[e]];
+        out_begin[r] = alpha * acc;
+    }
+}
+
+/* CSC scatter (sparse_matrix_vector.hpp:133-163): zero the out window first. */
+void FN(orc_spmv_csc)(T* out_begin, long long out_size, const T* arg_begin,
+                      const T* values, const long long* inner,
+                      const long long* offsets, long long cols, T alpha)
+{
+    for (long long i = 0; i < out_size; ++i) out_begin[i] = (T)0;
+    for (long long c = 0; c < cols; ++c) {
+        const T av = alpha * arg_begin[c];
+        for (long long e = offsets[c]; e < offsets[c + 1]; ++e)
+            out_begin[inner[e]] = out_begin[inner[e]] + values[e] * av;
+    }
+}
+
+/* --- batched wrappers (the strategy's serial batch loop,
+ * cpu/single_thread/strategy.hpp:115-124) ---------------------------------- */
+
+#define OBJ(p, s, i) ((p) + (long long)(s) * (long long)(i))
+
+int FN(orc_batch_ft_mul)(int width, int depth, long long n, T* out,
+                         long long os, int omin, int omax, const T* lhs,
+                         long long ls, int lmin, int lmax, const T* rhs,
+                         long long rs, int rmin, int rmax, T beta)
+{
+    orc_basis B;
+    if (orc_basis_init(&B, width, depth)) return 3;
+    T* scratch = (T*)malloc(sizeof(T) * (size_t)(B.pow[depth] + 1));
+    for (long long i = 0; i < n; ++i)
+        FN(orc_ft_mul)(&B, scratch, OBJ(out, os, i), omin, omax, OBJ(lhs, ls, i),
+                       lmin, lmax, OBJ(rhs, rs, i), rmin, rmax, beta);
+    free(scratch);
+    return 0;
+}
+
+int FN(orc_batch_ft_fma)(int width, int depth, long long n, T* out,
+                         long long os, int omin, int omax, const T* a,
+                         long long as, int amin, int amax, const T* b,
+                         long long bs, int bmin, int bmax, const T* c,
+                         long long cs, int cmin, int cmax, T alpha, T beta)
+{
+    orc_basis B;
+    if (orc_basis_init(&B, width, depth)) return 3;
+    T* scratch = (T*)malloc(sizeof(T) * (size_t)(B.pow[depth] + 1));
+    for (long long i = 0; i < n; ++i)
+        FN(orc_ft_fma)(&B, scratch, OBJ(out, os, i), omin, omax, OBJ(a, as, i),
+                       amin, amax, OBJ(b, bs, i), bmin, bmax, OBJ(c, cs, i),
+                       cmin, cmax, alpha, beta);
+    free(scratch);
+    return 0;
+}
+
+int FN(orc_batch_ft_inplace_fma)(int variant, int width, int depth,
+                                 long long n, T* a, long long as, int amin,
+                                 int amax, const T* b, long long bs, int bmin,
+                                 int bmax, const T* c, long long cs, int cmin,
+                                 int cmax, T alpha, T beta)
+{
+    orc_basis B;
+    if (variant != 231 && variant != 123 && variant != 213) return 3;
+    if (orc_basis_init(&B, width, depth)) return 3;
+    T* scratch = (T*)malloc(sizeof(T) * (size_t)(B.pow[depth] + 1));
+    for (long long i = 0; i < n; ++i)
+        FN(orc_ft_inplace_fma)(&B, scratch, variant, OBJ(a, as, i), amin, amax,
+                               OBJ(b, bs, i), bmin, bmax, OBJ(c, cs, i), cmin,
+                               cmax, alpha, beta);
+    free(scratch);
+    return 0;
+}
+
+int FN(orc_batch_ft_inplace_mul)(int width, int depth, long long n, T* lhs,
+                                 long long ls, int lmin, int lmax,
+                                 const T* rhs, long long rs, int rmin,
+                                 int rmax, T beta)
+{
+    orc_basis B;
+    if (orc_basis_init(&B, width, depth)) return 3;
+    T* scratch = (T*)malloc(sizeof(T) * (size_t)(B.pow[depth] + 1));
+    for (long long i = 0; i < n; ++i)
+        FN(orc_ft_inplace_mul)(&B, scratch, OBJ(lhs, ls, i), lmin, lmax,
+                               OBJ(rhs, rs, i), rmin, rmax, beta);
+    free(scratch);
+    return 0;
+}
+
+int FN(orc_batch_ft_exp)(int width, int depth, long long n, T* out,
+                         long long os, int omin, int omax, const T* arg,
+                         long long as, int amin, int amax)
+{
+    orc_basis B;
+    if (orc_basis_init(&B, width, depth)) return 3;
+    T* scratch = (T*)malloc(sizeof(T) * (size_t)(B.pow[depth] + 1));
+    for (long long i = 0; i < n; ++i)
+        FN(orc_ft_exp)(&B, scratch, OBJ(out, os, i), omin, omax,
+                       OBJ(arg, as, i), amin, amax);
+    free(scratch);
+    return 0;
+}
+
+int FN(orc_batch_ft_fmexp)(int width, int depth, long long n, T* out,
+                           long long os, int omin, int omax, const T* mult,
+                           long long ms, int mmin, int mmax, const T* ex,
+                           long long es, int emin, int emax)
+{
+    orc_basis B;
+    if (orc_basis_init(&B, width, depth)) return 3;
+    T* scratch = (T*)malloc(sizeof(T) * (size_t)(B.pow[depth] + 1));
+    for (long long i = 0; i < n; ++i)
+        FN(orc_ft_fmexp)(&B, scratch, OBJ(out, os, i), omin, omax,
+                         OBJ(mult, ms, i), mmin, mmax, OBJ(ex, es, i), emin,
+                         emax);
+    free(scratch);
+    return 0;
+}
+
+int FN(orc_batch_ft_log)(int width, int depth, long long n, T* out,
+                         long long os, int omin, int omax, const T* arg,
+                         long long as, int amin, int amax)
+{
+    orc_basis B;
+    if (orc_basis_init(&B, width, depth)) return 3;
+    T* scratch = (T*)malloc(sizeof(T) * (size_t)(B.pow[depth] + 1));
+    for (long long i = 0; i < n; ++i)
+        FN(orc_ft_log)(&B, scratch, OBJ(out, os, i), omin, omax,
+                       OBJ(arg, as, i), amin, amax);
+    free(scratch);
+    return 0;
+}
+
+int FN(orc_batch_tensor_pairing)(int width, int depth, long long n, T* out,
+                                 long long ostride, const T* f, long long fs,
+                                 int fmin, int fmax, const T* x, long long xs,
+                                 int xmin, int xmax)
+{
+    orc_basis B;
+    if (orc_basis_init(&B, width, depth)) return 3;
+    for (long long i = 0; i < n; ++i)
+        out[ostride * i] = FN(orc_tensor_pairing)(&B, OBJ(f, fs, i), fmin, fmax,
+                                                  OBJ(x, xs, i), xmin, xmax);
+    return 0;
+}
+
+int FN(orc_batch_ft_adj_lmul)(int width, int depth, long long n, T* out,
+                              long long os, int omin, int omax, const T* op,
+                              long long ps, int pmin, int pmax, const T* arg,
+                              long long gs, int gmin, int gmax)
+{
+    orc_basis B;
+    if (orc_basis_init(&B, width, depth)) return 3;
+    for (long long i = 0; i < n; ++i)
+        FN(orc_ft_adj_lmul)(&B, OBJ(out, os, i), omin, omax, OBJ(op, ps, i),
+                            pmin, pmax, OBJ(arg, gs, i), gmin, gmax);
+    return 0;
+}
+
+int FN(orc_batch_ft_adj_rmul)(int width, int depth, long long n, T* out,
+                              long long os, int omin, int omax, const T* op,
+                              long long ps, int pmin, int pmax, const T* arg,
+                              long long gs, int gmin, int gmax)
+{
+    orc_basis B;
+    if (orc_basis_init(&B, width, depth)) return 3;
+    for (long long i = 0; i < n; ++i)
+        FN(orc_ft_adj_rmul)(&B, OBJ(out, os, i), omin, omax, OBJ(op, ps, i),
+                            pmin, pmax, OBJ(arg, gs, i), gmin, gmax);
+    return 0;
+}
+
+/* Batched SpMV over graded views: out_begin = out + os*i + out_offset, etc. */
+int FN(orc_batch_spmv)(int csc, long long n, T* out, long long os,
+                       long long out_offset, long long out_size, const T* arg,
+                       long long as, long long arg_offset, const T* values,
+                       const long long* inner, const long long* offsets,
+                       long long outer, T alpha)
+{
+    for (long long i = 0; i < n; ++i) {
+        T* o = OBJ(out, os, i) + out_offset;
+        const T* a = OBJ(arg, as, i) + arg_offset;
+        if (csc)
+            FN(orc_spmv_csc)(o, out_size, a, values, inner, offsets, outer, alpha);
+        else
+            FN(orc_spmv_csr)(o, a, values, inner, offsets, outer, alpha);
+    }
+    return 0;
+}
+
+/* Signature of piecewise-linear paths by the caller-side Chen recursion the
+ * reference leaves to its users: S_0 = 1, S_t = ft_fmexp(S_{t-1}, dX_t) with
+ * the increment held in a full tensor whose only non-zero degree is 1 and the
+ * exponent view (1,1).  Increments are packed [sum(len), width] row-major,
+ * path p owning rows offsets[p]..offsets[p+1]. */
+int FN(orc_signature)(int width, int depth, long long n_paths,
+                      const long long* offsets, const T* increments, T* out,
+                      long long os)
+{
+    orc_basis B;
+    if (orc_basis_init(&B, width, depth)) return 3;
+    const long long dim = B.start[depth + 1];
+    T* scratch = (T*)malloc(sizeof(T) * (size_t)(B.pow[depth] + 1));
+    T* a = (T*)malloc(sizeof(T) * (size_t)dim);
+    T* b = (T*)malloc(sizeof(T) * (size_t)dim);
+    T* e = (T*)calloc((size_t)dim, sizeof(T));
+    for (long long p = 0; p < n_paths; ++p) {
+        for (long long i = 0; i < dim; ++i) a[i] = (T)0;
+        a[0] = (T)1;
+        T* cur = a;
+        T* nxt = b;
+        for (long long t = offsets[p]; t < offsets[p + 1]; ++t) {
+            for (int l = 0; l < width; ++l) e[1 + l] = increments[t * width + l];
+            FN(orc_ft_fmexp)(&B, scratch, nxt, 0, depth, cur, 0, depth, e, 1, 1);
+            T* tmp = cur; cur = nxt; nxt = tmp;
+        }
+        T* o = out + os * p;
+        for (long long i = 0; i < dim; ++i) o[i] = cur[i];
+    }
+    free(scratch); free(a); free(b); free(e);
+    return 0;
+}
+
+#undef OBJ
+#undef FN
+#undef CAT
+#undef CAT2
