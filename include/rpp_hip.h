@@ -1,0 +1,303 @@
+/*
+ * rpp_hip.h -- C-ABI of librpp_hip.so, the MI355X (gfx950) strategy for the
+ * truncated tensor algebra hot path of RoughPathPrimitives.
+ *
+ * Every entry point is asynchronous on `stream` (a hipStream_t passed as a
+ * plain pointer; NULL is the default stream), exactly like the reference GPU
+ * strategy, whose launch() enqueues on LaunchConfig.stream and returns
+ * (gpu/device.hpp:21-24, gpu/block/strategy.hpp:312-365).  All tensor data are
+ * device pointers owned by the caller; the library owns only its per-device
+ * caches (Lie-basis matrices per (width, depth), workspaces), which are
+ * created on first use and kept until unload.
+ *
+ * Return values are the integer codes of rpp::ErrorCode
+ * (support/error.hpp:28-39).  On a non-zero return, rpp_hip_last_error()
+ * gives a message for the calling thread.  Descriptors are validated on the
+ * host before any HIP call.
+ */
+#ifndef RPP_HIP_H
+#define RPP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same numbering as rpp::ErrorCode (support/error.hpp:28-39). */
+enum rpp_hip_status {
+    RPP_HIP_SUCCESS = 0,
+    RPP_HIP_CANCELLED = 1,
+    RPP_HIP_UNKNOWN = 2,
+    RPP_HIP_INVALID_ARGUMENT = 3,
+    RPP_HIP_TIMEOUT = 4,
+    RPP_HIP_OUT_OF_RESOURCES = 5,
+    RPP_HIP_CONTRACT_VIOLATION = 6,
+    RPP_HIP_OUT_OF_BOUNDS = 7,
+    RPP_HIP_NOT_IMPLEMENTED = 8,
+    RPP_HIP_INTERNAL = 9
+};
+
+/* TensorBasis(width, depth) (basis/tensor_basis.hpp:40-118).  Degree offsets
+ * start_of_degree(k) = (width^k - 1)/(width - 1) are implied. */
+typedef struct rpp_hip_basis {
+    int32_t width;
+    int32_t depth;
+} rpp_hip_basis;
+
+/* One Batch of DenseTensorView (views/batch.hpp:101-145,
+ * views/dense_tensor_view.hpp:38-91): object b, coefficient i lives at
+ *   data[b * batch_stride + i * coeff_stride].
+ * coeff_stride == 1 is layouts::StrideLayout (object-major); batch_stride == 1
+ * with coeff_stride == num_batches is the batch-interleaved layout; a
+ * batch_stride of 0 broadcasts one object to the whole batch.  Only degrees
+ * [min_degree, max_degree] are read or written (the view's window). */
+typedef struct rpp_hip_tensor_batch {
+    void* data;
+    int64_t batch_stride;
+    int64_t coeff_stride;
+    int32_t min_degree;
+    int32_t max_degree;
+} rpp_hip_tensor_batch;
+
+/* A batch of ScalarView (views/scalar_view.hpp:17-60): object b at
+ * data[b * stride]; make_scalar_batch uses stride 1. */
+typedef struct rpp_hip_scalar_batch {
+    void* data;
+    int64_t stride;
+} rpp_hip_scalar_batch;
+
+/* ---------------------------------------------------------------- products */
+
+/* rpp::ops::ft_mul (operations/basic/ft_mul.hpp:53-79):
+ *   out = beta * lhs (x) rhs on out's window; out may alias lhs or rhs. */
+typedef struct rpp_hip_ft_mul_desc {
+    rpp_hip_basis basis;
+    int64_t num_batches;
+    rpp_hip_tensor_batch out, lhs, rhs;
+    double beta;
+} rpp_hip_ft_mul_desc;
+int rpp_hip_ft_mul_f32(const rpp_hip_ft_mul_desc* desc, void* stream);
+int rpp_hip_ft_mul_f64(const rpp_hip_ft_mul_desc* desc, void* stream);
+
+/* rpp::ops::ft_fma (operations/basic/ft_fma.hpp:54-83):
+ *   out = alpha * a + beta * b (x) c; a is read only inside its window. */
+typedef struct rpp_hip_ft_fma_desc {
+    rpp_hip_basis basis;
+    int64_t num_batches;
+    rpp_hip_tensor_batch out, a, b, c;
+    double alpha, beta;
+} rpp_hip_ft_fma_desc;
+int rpp_hip_ft_fma_f32(const rpp_hip_ft_fma_desc* desc, void* stream);
+int rpp_hip_ft_fma_f64(const rpp_hip_ft_fma_desc* desc, void* stream);
+
+/* rpp::ops::ft_inplace_fma<FMAType> (operations/basic/ft_inplace_fma.hpp
+ * :15-19,64-100).  variant 231: a <- alpha*a + beta*b(x)c,
+ * 123: a <- alpha*c + beta*a(x)b, 213: a <- alpha*c + beta*b(x)a. */
+typedef struct rpp_hip_ft_inplace_fma_desc {
+    rpp_hip_basis basis;
+    int64_t num_batches;
+    int32_t variant;
+    rpp_hip_tensor_batch a, b, c;
+    double alpha, beta;
+} rpp_hip_ft_inplace_fma_desc;
+int rpp_hip_ft_inplace_fma_f32(const rpp_hip_ft_inplace_fma_desc* desc, void* stream);
+int rpp_hip_ft_inplace_fma_f64(const rpp_hip_ft_inplace_fma_desc* desc, void* stream);
+
+/* rpp::ops::ft_inplace_mul (operations/basic/ft_inplace_mul.hpp:38-69):
+ *   lhs <- beta * lhs (x) rhs. */
+typedef struct rpp_hip_ft_inplace_mul_desc {
+    rpp_hip_basis basis;
+    int64_t num_batches;
+    rpp_hip_tensor_batch lhs, rhs;
+    double beta;
+} rpp_hip_ft_inplace_mul_desc;
+int rpp_hip_ft_inplace_mul_f32(const rpp_hip_ft_inplace_mul_desc* desc, void* stream);
+int rpp_hip_ft_inplace_mul_f64(const rpp_hip_ft_inplace_mul_desc* desc, void* stream);
+
+/* ------------------------------------------------------------- composites */
+
+/* rpp::ops::ft_exp (operations/intermediate/ft_exp.hpp:89-112):
+ *   out = exp(arg_{>=1}); and rpp::ops::ft_log (ft_log.hpp:91-114):
+ *   out = log(1 + arg_{>=1}).  One kernel per call. */
+typedef struct rpp_hip_ft_unary_desc {
+    rpp_hip_basis basis;
+    int64_t num_batches;
+    rpp_hip_tensor_batch out, arg;
+} rpp_hip_ft_unary_desc;
+int rpp_hip_ft_exp_f32(const rpp_hip_ft_unary_desc* desc, void* stream);
+int rpp_hip_ft_exp_f64(const rpp_hip_ft_unary_desc* desc, void* stream);
+int rpp_hip_ft_log_f32(const rpp_hip_ft_unary_desc* desc, void* stream);
+int rpp_hip_ft_log_f64(const rpp_hip_ft_unary_desc* desc, void* stream);
+
+/* rpp::ops::ft_fmexp (operations/intermediate/ft_fmexp.hpp:96-121):
+ *   out = multiplier (x) exp(exponent_{>=1}); out must not alias multiplier.
+ *   One kernel per call. */
+typedef struct rpp_hip_ft_fmexp_desc {
+    rpp_hip_basis basis;
+    int64_t num_batches;
+    rpp_hip_tensor_batch out, multiplier, exponent;
+} rpp_hip_ft_fmexp_desc;
+int rpp_hip_ft_fmexp_f32(const rpp_hip_ft_fmexp_desc* desc, void* stream);
+int rpp_hip_ft_fmexp_f64(const rpp_hip_ft_fmexp_desc* desc, void* stream);
+
+/* rpp::ops::tensor_pairing (operations/basic/tensor_pairing.hpp:47-73):
+ *   out[b] = sum over the overlap of the windows of functional[i]*arg[i]. */
+typedef struct rpp_hip_tensor_pairing_desc {
+    rpp_hip_basis basis;
+    int64_t num_batches;
+    rpp_hip_scalar_batch out;
+    rpp_hip_tensor_batch functional, arg;
+} rpp_hip_tensor_pairing_desc;
+int rpp_hip_tensor_pairing_f32(const rpp_hip_tensor_pairing_desc* desc, void* stream);
+int rpp_hip_tensor_pairing_f64(const rpp_hip_tensor_pairing_desc* desc, void* stream);
+
+/* rpp::ops::ft_adj_lmul (operations/basic/ft_adj_lmul.hpp:49-73):
+ *   out[w] = sum_u op[u] arg[uw];  rpp::ops::ft_adj_rmul (ft_adj_rmul.hpp
+ *   :47-71): out[w] = sum_u arg[wu] op[u]. */
+typedef struct rpp_hip_ft_adjoint_desc {
+    rpp_hip_basis basis;
+    int64_t num_batches;
+    rpp_hip_tensor_batch out, op, arg;
+} rpp_hip_ft_adjoint_desc;
+int rpp_hip_ft_adj_lmul_f32(const rpp_hip_ft_adjoint_desc* desc, void* stream);
+int rpp_hip_ft_adj_lmul_f64(const rpp_hip_ft_adjoint_desc* desc, void* stream);
+int rpp_hip_ft_adj_rmul_f32(const rpp_hip_ft_adjoint_desc* desc, void* stream);
+int rpp_hip_ft_adj_rmul_f64(const rpp_hip_ft_adjoint_desc* desc, void* stream);
+
+/* --------------------------------------------------------- sparse and Lie */
+
+enum rpp_hip_matrix_format { RPP_HIP_CSR = 0, RPP_HIP_CSC = 1 };
+
+/* CompressedMatrix (sparse/detail/compressed_matrix.hpp:38-126) with device
+ * arrays: values[nnz], indices[nnz] (inner index), offsets[outer + 1]. */
+typedef struct rpp_hip_sparse_matrix {
+    int32_t format;
+    const void* values;      /* float or double, as the entry point */
+    const int64_t* indices;
+    const int64_t* offsets;
+    int64_t nnz;
+    int64_t rows;
+    int64_t cols;
+} rpp_hip_sparse_matrix;
+
+/* A batch of vectors for SpMV: object b, entry i of the view at
+ * data[b*batch_stride + (begin + i)*coeff_stride], where begin is
+ * view.begin_index() and size the view's length (R9: row 0 <-> out.begin(),
+ * column 0 <-> arg.begin()). */
+typedef struct rpp_hip_vector_batch {
+    void* data;
+    int64_t batch_stride;
+    int64_t coeff_stride;
+    int64_t begin;
+    int64_t size;
+} rpp_hip_vector_batch;
+
+/* rpp::ops::sparse_matrix_vector_product
+ * (operations/linalg/sparse_matrix_vector.hpp:57-96): out = alpha * M arg. */
+typedef struct rpp_hip_spmv_desc {
+    int64_t num_batches;
+    rpp_hip_vector_batch out, arg;
+    rpp_hip_sparse_matrix matrix;
+    double alpha;
+} rpp_hip_spmv_desc;
+int rpp_hip_sparse_matrix_vector_product_f32(const rpp_hip_spmv_desc* desc, void* stream);
+int rpp_hip_sparse_matrix_vector_product_f64(const rpp_hip_spmv_desc* desc, void* stream);
+
+/* rpp::ops::tensor_to_lie (operations/basic/tensor_to_lie.hpp:65-103) and
+ * rpp::ops::lie_to_tensor (operations/basic/lie_to_tensor.hpp:70-108) with the
+ * library's own Hall-basis T2L / L2T matrices, built on the host and
+ * uploaded once per (width, depth, dtype) as CSR.  Lie objects hold
+ * true_size() slots (slot 0 unused) and Lie windows start at degree >= 1. */
+typedef struct rpp_hip_lie_convert_desc {
+    rpp_hip_basis basis;
+    int64_t num_batches;
+    rpp_hip_tensor_batch tensor;
+    rpp_hip_tensor_batch lie;
+} rpp_hip_lie_convert_desc;
+int rpp_hip_tensor_to_lie_f32(const rpp_hip_lie_convert_desc* desc, void* stream);
+int rpp_hip_tensor_to_lie_f64(const rpp_hip_lie_convert_desc* desc, void* stream);
+int rpp_hip_lie_to_tensor_f32(const rpp_hip_lie_convert_desc* desc, void* stream);
+int rpp_hip_lie_to_tensor_f64(const rpp_hip_lie_convert_desc* desc, void* stream);
+
+/* Host-side Hall basis metadata (basis/hall_basis.hpp:16-88,
+ * sparse/l2t_matrix.hpp:58-114, sparse/t2l_matrix.hpp:57-142).  Sizes first:
+ * sizes[0] = true_size (incl. slot 0), [1] = L2T nnz, [2] = T2L nnz,
+ * [3] = tensor dimension.  Then the arrays: pairs[2*true_size],
+ * degree_begin[depth+2], and both matrices in CSC exactly as the reference
+ * builders return them (L2T: columns = Lie keys 1.., rows = tensor words;
+ * T2L: columns = tensor words, rows = Lie key - 1).  Any output pointer may be
+ * NULL. */
+int rpp_hip_lie_basis_sizes(int32_t width, int32_t depth, int64_t* sizes);
+int rpp_hip_lie_basis_copy(int32_t width, int32_t depth, int64_t* pairs,
+                           int64_t* degree_begin, double* l2t_values,
+                           int64_t* l2t_indices, int64_t* l2t_offsets,
+                           double* t2l_values, int64_t* t2l_indices,
+                           int64_t* t2l_offsets);
+
+/* ------------------------------------------------- batched Chen recursion */
+
+/* Signatures of many piecewise-linear paths of uneven length in ONE
+ * persistent launch (the caller-side chain of ft_fmexp, SURVEYed as the
+ * reference's usage: S_0 = 1, S_t = S_{t-1} (x) exp(dX_t)).
+ *   increments: device [total, width] row-major; path p owns rows
+ *               offsets[p] .. offsets[p+1]-1 (offsets: device int64[n+1]).
+ *   out:        tensor batch with the full window [0, depth].
+ *   checkpoint_every / checkpoints: optional (0 / NULL).  When set, the
+ *               state after every checkpoint_every-th increment of path p is
+ *               stored at checkpoints + (ckpt_offsets[p] + j) * dim, used by
+ *               the backward pass to re-anchor its reconstruction.
+ * The tile shape (threads per path, letters fixed per work-group) comes from
+ * the autotune table for (width, depth, dtype) unless tile_threads > 0. */
+typedef struct rpp_hip_signature_desc {
+    rpp_hip_basis basis;
+    int64_t num_paths;
+    const void* increments;
+    const int64_t* offsets;
+    rpp_hip_tensor_batch out;
+    int32_t checkpoint_every;
+    void* checkpoints;
+    const int64_t* ckpt_offsets;
+    int32_t tile_threads;
+    int32_t tile_split;
+} rpp_hip_signature_desc;
+int rpp_hip_signature_f32(const rpp_hip_signature_desc* desc, void* stream);
+int rpp_hip_signature_f64(const rpp_hip_signature_desc* desc, void* stream);
+
+/* Backward of rpp_hip_signature_*: given the final signatures S_L (out of the
+ * forward pass) and the cotangent G_L = d loss / d S_L (a tensor batch, or one
+ * functional broadcast with batch_stride 0), write d loss / d increments
+ * (device [total, width]).  Intermediate states are reconstructed backward
+ * with S_{t-1} = S_t (x) exp(-dX_t); when the forward pass stored checkpoints
+ * the reconstruction is re-anchored at each of them (sqrt(L) checkpointing).
+ * scratch: device workspace of rpp_hip_signature_backward_scratch_bytes(). */
+typedef struct rpp_hip_signature_backward_desc {
+    rpp_hip_basis basis;
+    int64_t num_paths;
+    const void* increments;
+    const int64_t* offsets;
+    rpp_hip_tensor_batch signatures;
+    rpp_hip_tensor_batch cotangent;
+    void* grad_increments;
+    int32_t checkpoint_every;
+    const void* checkpoints;
+    const int64_t* ckpt_offsets;
+    void* scratch;
+} rpp_hip_signature_backward_desc;
+int64_t rpp_hip_signature_backward_scratch_bytes(const rpp_hip_signature_backward_desc* desc,
+                                                 int32_t dtype_bytes);
+int rpp_hip_signature_backward_f32(const rpp_hip_signature_backward_desc* desc, void* stream);
+int rpp_hip_signature_backward_f64(const rpp_hip_signature_backward_desc* desc, void* stream);
+
+/* ------------------------------------------------------------------ misc */
+
+/* Message for the last non-zero return on the calling thread. */
+const char* rpp_hip_last_error(void);
+/* Library version string. */
+const char* rpp_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RPP_HIP_H */

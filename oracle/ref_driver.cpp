@@ -60,8 +60,9 @@ auto batch(P* data, long long stride, TBasis const& basis, int mn, int mx) {
                                   static_cast<Degree>(mx));
 }
 
+// rpp::Error converts to true on success (support/error.hpp:140-142).
 int code_of(rpp::Error<std::string> const& err) {
-    return err ? static_cast<int>(err.code()) : 0;
+    return err ? 0 : static_cast<int>(err.code());
 }
 
 // Split [0,n) over `threads` std::threads, each calling fn(first, count).
@@ -231,7 +232,7 @@ int signature(int w, int N, long long n, long long L, const T* incr, T* out,
                 batch<T>(static_cast<const T*>(cur), dim, bh.basis, 0, N),
                 batch<T>(static_cast<const T*>(e.data()), w + 1, bh.basis, 1, 1),
                 bh.basis, static_cast<Index>(c));
-            if (err) return code_of(err);
+            if (!err) return code_of(err);
             std::swap(cur, nxt);
         }
         std::memcpy(out + f * dim, cur, sizeof(T) * static_cast<size_t>(c * dim));

@@ -1,0 +1,417 @@
+// extern "C" entry points of librpp_hip.so (include/rpp_hip.h): descriptor
+// validation on the host, then dispatch to the kernels.  Validation follows
+// the reference's contract: windows inside [0, depth], out not broadcast,
+// fmexp's out distinct from its multiplier (operations/intermediate/
+// ft_fmexp.hpp:73-86), Lie windows from degree 1.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "common.hpp"
+#include "ft_ops.hpp"
+#include "lie.hpp"
+#include "signature.hpp"
+
+namespace rpp_hip {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+const char* last_error_cstr() { return g_last_error.c_str(); }
+
+int fail(int code, const std::string& msg) {
+    set_error(msg);
+    return code;
+}
+
+int map_hip_error(hipError_t err) {
+    switch (err) {
+        case hipSuccess: return RPP_HIP_SUCCESS;
+        case hipErrorOutOfMemory: return RPP_HIP_OUT_OF_RESOURCES;
+        case hipErrorInvalidValue:
+        case hipErrorInvalidDevicePointer:
+        case hipErrorInvalidConfiguration:
+        case hipErrorInvalidResourceHandle:
+        case hipErrorInvalidDevice: return RPP_HIP_INVALID_ARGUMENT;
+        case hipErrorNotSupported:
+        case hipErrorNoBinaryForGpu:
+        case hipErrorInvalidDeviceFunction: return RPP_HIP_NOT_IMPLEMENTED;
+        case hipErrorLaunchTimeOut: return RPP_HIP_TIMEOUT;
+        case hipErrorLaunchOutOfResources: return RPP_HIP_OUT_OF_RESOURCES;
+        default: return RPP_HIP_INTERNAL;
+    }
+}
+
+int check_hip(hipError_t err, const char* what) {
+    if (err == hipSuccess) return 0;
+    return fail(map_hip_error(err), std::string(what) + ": " + hipGetErrorString(err));
+}
+
+unsigned grid_for(int64_t items, unsigned cap) {
+    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(items, cap)));
+}
+
+DevBasis make_basis(int width, int depth) {
+    DevBasis B{};
+    B.width = width;
+    B.depth = depth;
+    uint32_t L = 0;
+    while ((uint64_t{1} << L) < static_cast<uint64_t>(width)) ++L;
+    const uint64_t num = uint64_t{1} << (32 + L);
+    B.magic = num / static_cast<uint64_t>(width) + ((num % static_cast<uint64_t>(width)) ? 1 : 0);
+    B.shift = 32 + L;
+    uint64_t p = 1;
+    int64_t s = 0;
+    for (int k = 0; k <= depth + 1 && k < kMaxDepth + 2; ++k) {
+        B.pow[k] = static_cast<uint32_t>(std::min<uint64_t>(p, 0xffffffffu));
+        B.start[k] = s;
+        s += static_cast<int64_t>(p);
+        p *= static_cast<uint64_t>(width);
+    }
+    return B;
+}
+
+namespace {
+
+int check_basis(const rpp_hip_basis& b) {
+    if (b.width < 1 || b.width > 255) return fail(RPP_HIP_INVALID_ARGUMENT, "width must be in [1, 255]");
+    if (b.depth < 0 || b.depth > kMaxDepth)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "depth must be in [0, 30]");
+    uint64_t p = 1;
+    for (int k = 0; k < b.depth; ++k) p *= static_cast<uint64_t>(b.width);
+    if (p >= (uint64_t{1} << 31))
+        return fail(RPP_HIP_OUT_OF_BOUNDS, "width^depth must be below 2^31 words per level");
+    return 0;
+}
+
+int check_view(const rpp_hip_tensor_batch& t, const rpp_hip_basis& b, int64_t n, const char* name,
+               bool output) {
+    if (t.min_degree < 0 || t.max_degree > b.depth)
+        return fail(RPP_HIP_INVALID_ARGUMENT, std::string(name) + ": window outside [0, depth]");
+    if (t.coeff_stride < 1)
+        return fail(RPP_HIP_INVALID_ARGUMENT, std::string(name) + ": coeff_stride must be >= 1");
+    if (t.batch_stride < 0)
+        return fail(RPP_HIP_INVALID_ARGUMENT, std::string(name) + ": negative batch_stride");
+    if (output && n > 1 && t.batch_stride == 0 && t.min_degree <= t.max_degree)
+        return fail(RPP_HIP_INVALID_ARGUMENT, std::string(name) + ": an output cannot be broadcast");
+    if (n > 0 && t.min_degree <= t.max_degree && t.data == nullptr)
+        return fail(RPP_HIP_INVALID_ARGUMENT, std::string(name) + ": null data");
+    return 0;
+}
+
+#define RPP_TRY(expr)               \
+    do {                            \
+        if (int rc_ = (expr)) return rc_; \
+    } while (0)
+
+hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+template <typename T>
+int ft_mul_impl(const rpp_hip_ft_mul_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    RPP_TRY(check_basis(d->basis));
+    RPP_TRY(check_view(d->out, d->basis, d->num_batches, "out", true));
+    RPP_TRY(check_view(d->lhs, d->basis, d->num_batches, "lhs", false));
+    RPP_TRY(check_view(d->rhs, d->basis, d->num_batches, "rhs", false));
+    const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    return launch_product<T>(B, d->num_batches, d->out, d->lhs, d->rhs, nullptr, 0.0, d->beta, as_stream(s));
+}
+
+template <typename T>
+int ft_fma_impl(const rpp_hip_ft_fma_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    RPP_TRY(check_basis(d->basis));
+    RPP_TRY(check_view(d->out, d->basis, d->num_batches, "out", true));
+    RPP_TRY(check_view(d->a, d->basis, d->num_batches, "a", false));
+    RPP_TRY(check_view(d->b, d->basis, d->num_batches, "b", false));
+    RPP_TRY(check_view(d->c, d->basis, d->num_batches, "c", false));
+    const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    return launch_product<T>(B, d->num_batches, d->out, d->b, d->c, &d->a, d->alpha, d->beta, as_stream(s));
+}
+
+template <typename T>
+int ft_inplace_fma_impl(const rpp_hip_ft_inplace_fma_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    RPP_TRY(check_basis(d->basis));
+    RPP_TRY(check_view(d->a, d->basis, d->num_batches, "a", true));
+    RPP_TRY(check_view(d->b, d->basis, d->num_batches, "b", false));
+    RPP_TRY(check_view(d->c, d->basis, d->num_batches, "c", false));
+    const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    switch (d->variant) {
+        case 231:  // a <- alpha*a + beta*b(x)c
+            return launch_product<T>(B, d->num_batches, d->a, d->b, d->c, &d->a, d->alpha, d->beta,
+                                     as_stream(s));
+        case 123:  // a <- alpha*c + beta*a(x)b
+            return launch_product<T>(B, d->num_batches, d->a, d->a, d->b, &d->c, d->alpha, d->beta,
+                                     as_stream(s));
+        case 213:  // a <- alpha*c + beta*b(x)a
+            return launch_product<T>(B, d->num_batches, d->a, d->b, d->a, &d->c, d->alpha, d->beta,
+                                     as_stream(s));
+        default: return fail(RPP_HIP_INVALID_ARGUMENT, "variant must be 231, 123 or 213");
+    }
+}
+
+template <typename T>
+int ft_inplace_mul_impl(const rpp_hip_ft_inplace_mul_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    RPP_TRY(check_basis(d->basis));
+    RPP_TRY(check_view(d->lhs, d->basis, d->num_batches, "lhs", true));
+    RPP_TRY(check_view(d->rhs, d->basis, d->num_batches, "rhs", false));
+    const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    return launch_product<T>(B, d->num_batches, d->lhs, d->lhs, d->rhs, nullptr, 0.0, d->beta, as_stream(s));
+}
+
+template <typename T>
+int unary_impl(int op, const rpp_hip_ft_unary_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    RPP_TRY(check_basis(d->basis));
+    RPP_TRY(check_view(d->out, d->basis, d->num_batches, "out", true));
+    RPP_TRY(check_view(d->arg, d->basis, d->num_batches, "arg", false));
+    if (d->out.data == d->arg.data && d->out.min_degree <= d->out.max_degree &&
+        d->arg.min_degree <= d->arg.max_degree)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "out must not alias arg");
+    const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    return launch_horner<T>(op, B, d->num_batches, d->out, d->arg, nullptr, as_stream(s));
+}
+
+template <typename T>
+int fmexp_impl(const rpp_hip_ft_fmexp_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    RPP_TRY(check_basis(d->basis));
+    RPP_TRY(check_view(d->out, d->basis, d->num_batches, "out", true));
+    RPP_TRY(check_view(d->multiplier, d->basis, d->num_batches, "multiplier", false));
+    RPP_TRY(check_view(d->exponent, d->basis, d->num_batches, "exponent", false));
+    if (d->out.data == d->multiplier.data && d->out.min_degree <= d->out.max_degree)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "ft_fmexp: out must not alias the multiplier");
+    if (d->out.data == d->exponent.data && d->out.min_degree <= d->out.max_degree)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "ft_fmexp: out must not alias the exponent");
+    const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    return launch_horner<T>(kOpFmexp, B, d->num_batches, d->out, d->exponent, &d->multiplier, as_stream(s));
+}
+
+template <typename T>
+int pairing_impl(const rpp_hip_tensor_pairing_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    RPP_TRY(check_basis(d->basis));
+    RPP_TRY(check_view(d->functional, d->basis, d->num_batches, "functional", false));
+    RPP_TRY(check_view(d->arg, d->basis, d->num_batches, "arg", false));
+    if (d->num_batches > 0 && d->out.data == nullptr) return fail(RPP_HIP_INVALID_ARGUMENT, "out: null data");
+    const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    return launch_pairing<T>(B, d->num_batches, d->out, d->functional, d->arg, as_stream(s));
+}
+
+template <typename T>
+int adjoint_impl(bool left, const rpp_hip_ft_adjoint_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    RPP_TRY(check_basis(d->basis));
+    RPP_TRY(check_view(d->out, d->basis, d->num_batches, "out", true));
+    RPP_TRY(check_view(d->op, d->basis, d->num_batches, "op", false));
+    RPP_TRY(check_view(d->arg, d->basis, d->num_batches, "arg", false));
+    if ((d->out.data == d->op.data || d->out.data == d->arg.data) && d->out.min_degree <= d->out.max_degree)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "adjoint multiplication: out must not alias an operand");
+    const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    return launch_adjoint<T>(left, B, d->num_batches, d->out, d->op, d->arg, as_stream(s));
+}
+
+int check_vec(const rpp_hip_vector_batch& v, const char* name) {
+    if (v.begin < 0 || v.size < 0 || v.coeff_stride < 1 || v.batch_stride < 0)
+        return fail(RPP_HIP_INVALID_ARGUMENT, std::string(name) + ": invalid vector batch");
+    return 0;
+}
+
+template <typename T>
+int spmv_impl(const rpp_hip_spmv_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    RPP_TRY(check_vec(d->out, "out"));
+    RPP_TRY(check_vec(d->arg, "arg"));
+    const auto& M = d->matrix;
+    if (M.format != RPP_HIP_CSR && M.format != RPP_HIP_CSC)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "matrix format must be CSR or CSC");
+    if (M.rows < 0 || M.cols < 0 || M.nnz < 0) return fail(RPP_HIP_INVALID_ARGUMENT, "negative matrix shape");
+    if (d->num_batches > 0 && (M.offsets == nullptr || (M.nnz > 0 && (!M.values || !M.indices))))
+        return fail(RPP_HIP_INVALID_ARGUMENT, "null matrix arrays");
+    return launch_spmv<T>(d->num_batches, d->out, d->arg, M, d->alpha, as_stream(s));
+}
+
+template <typename T>
+int lie_convert_impl(bool to_lie, const rpp_hip_lie_convert_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    RPP_TRY(check_basis(d->basis));
+    if (d->basis.width < 2) return fail(RPP_HIP_NOT_IMPLEMENTED, "Lie conversions need width >= 2");
+    RPP_TRY(check_view(d->tensor, d->basis, d->num_batches, "tensor", !to_lie));
+    if (d->lie.min_degree < 1 || d->lie.max_degree > d->basis.depth || d->lie.coeff_stride < 1 ||
+        d->lie.batch_stride < 0)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "lie: window must lie in [1, depth]");
+    const DeviceCsr* l2t = nullptr;
+    const DeviceCsr* t2l = nullptr;
+    RPP_TRY(device_lie_matrices<T>(d->basis.width, d->basis.depth, &l2t, &t2l));
+    auto md = lie_metadata(d->basis.width, d->basis.depth);
+    const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    rpp_hip_vector_batch tv{d->tensor.data, d->tensor.batch_stride, d->tensor.coeff_stride,
+                            B.start[d->tensor.min_degree], md->tensor_dim - B.start[d->tensor.min_degree]};
+    const int64_t lbeg = md->degree_begin[static_cast<size_t>(d->lie.min_degree)];
+    rpp_hip_vector_batch lv{d->lie.data, d->lie.batch_stride, d->lie.coeff_stride, lbeg, md->true_size - lbeg};
+    const DeviceCsr* m = to_lie ? t2l : l2t;
+    rpp_hip_sparse_matrix M{RPP_HIP_CSR, m->values, m->indices, m->offsets, m->nnz, m->rows, m->cols};
+    // R9: matrix row 0 <-> out.begin(), column 0 <-> arg.begin().
+    return to_lie ? launch_spmv<T>(d->num_batches, lv, tv, M, 1.0, as_stream(s))
+                  : launch_spmv<T>(d->num_batches, tv, lv, M, 1.0, as_stream(s));
+}
+
+}  // namespace
+}  // namespace rpp_hip
+
+using namespace rpp_hip;
+
+extern "C" {
+
+#define RPP_C_ENTRY(NAME, DESC, CALL)                                             \
+    int NAME##_f32(const DESC* d, void* s) { return CALL<float>(d, s); }          \
+    int NAME##_f64(const DESC* d, void* s) { return CALL<double>(d, s); }
+
+RPP_C_ENTRY(rpp_hip_ft_mul, rpp_hip_ft_mul_desc, ft_mul_impl)
+RPP_C_ENTRY(rpp_hip_ft_fma, rpp_hip_ft_fma_desc, ft_fma_impl)
+RPP_C_ENTRY(rpp_hip_ft_inplace_fma, rpp_hip_ft_inplace_fma_desc, ft_inplace_fma_impl)
+RPP_C_ENTRY(rpp_hip_ft_inplace_mul, rpp_hip_ft_inplace_mul_desc, ft_inplace_mul_impl)
+RPP_C_ENTRY(rpp_hip_ft_fmexp, rpp_hip_ft_fmexp_desc, fmexp_impl)
+RPP_C_ENTRY(rpp_hip_tensor_pairing, rpp_hip_tensor_pairing_desc, pairing_impl)
+RPP_C_ENTRY(rpp_hip_sparse_matrix_vector_product, rpp_hip_spmv_desc, spmv_impl)
+
+int rpp_hip_ft_exp_f32(const rpp_hip_ft_unary_desc* d, void* s) { return unary_impl<float>(kOpExp, d, s); }
+int rpp_hip_ft_exp_f64(const rpp_hip_ft_unary_desc* d, void* s) { return unary_impl<double>(kOpExp, d, s); }
+int rpp_hip_ft_log_f32(const rpp_hip_ft_unary_desc* d, void* s) { return unary_impl<float>(kOpLog, d, s); }
+int rpp_hip_ft_log_f64(const rpp_hip_ft_unary_desc* d, void* s) { return unary_impl<double>(kOpLog, d, s); }
+
+int rpp_hip_ft_adj_lmul_f32(const rpp_hip_ft_adjoint_desc* d, void* s) { return adjoint_impl<float>(true, d, s); }
+int rpp_hip_ft_adj_lmul_f64(const rpp_hip_ft_adjoint_desc* d, void* s) { return adjoint_impl<double>(true, d, s); }
+int rpp_hip_ft_adj_rmul_f32(const rpp_hip_ft_adjoint_desc* d, void* s) { return adjoint_impl<float>(false, d, s); }
+int rpp_hip_ft_adj_rmul_f64(const rpp_hip_ft_adjoint_desc* d, void* s) { return adjoint_impl<double>(false, d, s); }
+
+int rpp_hip_tensor_to_lie_f32(const rpp_hip_lie_convert_desc* d, void* s) {
+    return lie_convert_impl<float>(true, d, s);
+}
+int rpp_hip_tensor_to_lie_f64(const rpp_hip_lie_convert_desc* d, void* s) {
+    return lie_convert_impl<double>(true, d, s);
+}
+int rpp_hip_lie_to_tensor_f32(const rpp_hip_lie_convert_desc* d, void* s) {
+    return lie_convert_impl<float>(false, d, s);
+}
+int rpp_hip_lie_to_tensor_f64(const rpp_hip_lie_convert_desc* d, void* s) {
+    return lie_convert_impl<double>(false, d, s);
+}
+
+int rpp_hip_lie_basis_sizes(int32_t width, int32_t depth, int64_t* sizes) {
+    const rpp_hip_basis b{width, depth};
+    if (int rc = check_basis(b)) return rc;
+    if (!sizes) return fail(RPP_HIP_INVALID_ARGUMENT, "null sizes");
+    auto md = lie_metadata(width, depth);
+    sizes[0] = md->true_size;
+    sizes[1] = static_cast<int64_t>(md->l2t_csc_values.size());
+    sizes[2] = static_cast<int64_t>(md->t2l_csc_values.size());
+    sizes[3] = md->tensor_dim;
+    return 0;
+}
+
+int rpp_hip_lie_basis_copy(int32_t width, int32_t depth, int64_t* pairs, int64_t* degree_begin,
+                           double* l2t_values, int64_t* l2t_indices, int64_t* l2t_offsets,
+                           double* t2l_values, int64_t* t2l_indices, int64_t* t2l_offsets) {
+    const rpp_hip_basis b{width, depth};
+    if (int rc = check_basis(b)) return rc;
+    auto md = lie_metadata(width, depth);
+    auto cp = [](auto* dst, const auto& v) {
+        if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    };
+    cp(pairs, md->pairs);
+    cp(degree_begin, md->degree_begin);
+    cp(l2t_values, md->l2t_csc_values);
+    cp(l2t_indices, md->l2t_csc_indices);
+    cp(l2t_offsets, md->l2t_csc_offsets);
+    cp(t2l_values, md->t2l_csc_values);
+    cp(t2l_indices, md->t2l_csc_indices);
+    cp(t2l_offsets, md->t2l_csc_offsets);
+    return 0;
+}
+
+const char* rpp_hip_last_error(void) { return last_error_cstr(); }
+
+const char* rpp_hip_version(void) { return "rpp_hip 0.1.0 (gfx950)"; }
+
+}  // extern "C"
+
+// ------------------------------------------------------- batched signatures
+
+namespace rpp_hip {
+namespace {
+
+template <typename T>
+int signature_impl(const rpp_hip_signature_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    RPP_TRY(check_basis(d->basis));
+    if (d->num_paths < 0) return fail(RPP_HIP_INVALID_ARGUMENT, "negative num_paths");
+    if (d->num_paths == 0) return 0;
+    RPP_TRY(check_view(d->out, d->basis, d->num_paths, "out", true));
+    if (d->out.min_degree != 0 || d->out.max_degree != d->basis.depth)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "signature: out must hold the full window [0, depth]");
+    if (!d->increments || !d->offsets) return fail(RPP_HIP_INVALID_ARGUMENT, "null increments or offsets");
+    if (d->checkpoint_every < 0) return fail(RPP_HIP_INVALID_ARGUMENT, "negative checkpoint_every");
+    if (d->checkpoint_every > 0 && (!d->checkpoints || !d->ckpt_offsets))
+        return fail(RPP_HIP_INVALID_ARGUMENT, "checkpoints requested without buffers");
+    const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    hipStream_t stream = as_stream(s);
+    std::vector<int64_t> lengths;
+    int64_t* order = nullptr;
+    RPP_TRY(path_order(d->offsets, d->num_paths, lengths, &order, stream));
+    return signature_forward_generic<T>(B, *d, order, stream);
+}
+
+template <typename T>
+int signature_backward_impl(const rpp_hip_signature_backward_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    RPP_TRY(check_basis(d->basis));
+    if (d->num_paths < 0) return fail(RPP_HIP_INVALID_ARGUMENT, "negative num_paths");
+    if (d->num_paths == 0) return 0;
+    RPP_TRY(check_view(d->signatures, d->basis, d->num_paths, "signatures", false));
+    RPP_TRY(check_view(d->cotangent, d->basis, d->num_paths, "cotangent", false));
+    if (d->signatures.min_degree != 0 || d->signatures.max_degree != d->basis.depth ||
+        d->cotangent.min_degree != 0 || d->cotangent.max_degree != d->basis.depth)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "signature backward: full windows [0, depth] required");
+    if (!d->increments || !d->offsets || !d->grad_increments)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "null increments, offsets or gradient");
+    if (d->checkpoint_every < 0) return fail(RPP_HIP_INVALID_ARGUMENT, "negative checkpoint_every");
+    if (d->checkpoint_every > 0 && (!d->checkpoints || !d->ckpt_offsets))
+        return fail(RPP_HIP_INVALID_ARGUMENT, "checkpoints requested without buffers");
+    const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    hipStream_t stream = as_stream(s);
+    std::vector<int64_t> lengths;
+    int64_t* order = nullptr;
+    RPP_TRY(path_order(d->offsets, d->num_paths, lengths, &order, stream));
+    return signature_backward_generic<T>(B, *d, order, stream);
+}
+
+}  // namespace
+}  // namespace rpp_hip
+
+extern "C" {
+
+int rpp_hip_signature_f32(const rpp_hip_signature_desc* d, void* s) { return signature_impl<float>(d, s); }
+int rpp_hip_signature_f64(const rpp_hip_signature_desc* d, void* s) { return signature_impl<double>(d, s); }
+int rpp_hip_signature_backward_f32(const rpp_hip_signature_backward_desc* d, void* s) {
+    return signature_backward_impl<float>(d, s);
+}
+int rpp_hip_signature_backward_f64(const rpp_hip_signature_backward_desc* d, void* s) {
+    return signature_backward_impl<double>(d, s);
+}
+
+int64_t rpp_hip_signature_backward_scratch_bytes(const rpp_hip_signature_backward_desc* d, int32_t dtype_bytes) {
+    if (!d || check_basis(d->basis)) return -1;
+    const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    const int64_t grid = std::min<int64_t>(std::max<int64_t>(d->num_paths, 1), device_cu_count() * 4);
+    return backward_slot_elems(B) * grid * dtype_bytes + 256;
+}
+
+}  // extern "C"

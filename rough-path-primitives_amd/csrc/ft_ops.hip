@@ -1,0 +1,498 @@
+// Per-operation kernels of the hip::block strategy: products (ft_mul, ft_fma,
+// ft_inplace_fma, ft_inplace_mul), the composites (ft_exp, ft_fmexp, ft_log),
+// tensor_pairing, the adjoint multiplications and sparse matrix-vector
+// products.
+//
+// Execution model: one work-group per object of the batch (grid-stride over
+// the batch), threads over the words of one level, output levels in
+// DESCENDING degree order with a work-group barrier between levels.  An output
+// word of degree k reads operand words of degree < k, plus at most the same
+// word of degree k (the j = 0 or j = k split), which the same thread reads
+// before it writes.  That is why out may alias an operand exactly as in the
+// reference CPU strategy (ft_multiply.hpp:1596-1605).
+//
+// Word arithmetic never divides by a runtime level size: the split
+// w -> (prefix, suffix) at j letters is advanced one letter at a time with a
+// division by the width, compile-time for the common widths.
+#include "common.hpp"
+#include "ft_ops.hpp"
+
+#include <algorithm>
+
+namespace rpp_hip {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <bool CS1>
+__device__ __forceinline__ int64_t at(int64_t idx, int64_t cs) {
+    if constexpr (CS1) {
+        return idx;
+    } else {
+        return idx * cs;
+    }
+}
+
+// One pass of  out = [alpha * add] + beta * lhs (x) rhs  over out's window,
+// degrees descending, each level finished by a barrier.  `add` is consulted
+// only on levels inside its window (ScaledTileAddWriter semantics,
+// ft_multiply.hpp:260-395); without it the level is beta * product
+// (ScaledTileWriter, :176-258).
+template <typename T, int W, bool CS1>
+__device__ void product_pass(const DevBasis& B, T* o, int64_t ocs, int omn, int omx,
+                             const T* l, int64_t lcs, int lmn, int lmx,
+                             const T* r, int64_t rcs, int rmn, int rmx,
+                             const T* a, int64_t acs, int amn, int amx, bool with_add,
+                             T alpha, T beta) {
+    using L = Letters<W>;
+    const uint32_t width = L::width(B);
+    for (int k = omx; k >= omn; --k) {
+        int jlo = max(rmn, k - lmx);
+        int jhi = min(rmx, k - lmn);
+        jlo = max(jlo, 0);
+        jhi = min(jhi, k);
+        const bool use_add = with_add && amn <= k && k <= amx;
+        const uint32_t nk = B.pow[k];
+        const int64_t ok = B.start[k];
+        for (uint32_t w = threadIdx.x; w < nk; w += blockDim.x) {
+            T acc = T(0);
+            if (jlo <= jhi) {
+                uint32_t p = w;
+                uint32_t s = 0;
+                for (int j = 0; j < jlo; ++j) {
+                    const uint32_t q = L::div(B, p);
+                    s += (p - q * width) * B.pow[j];
+                    p = q;
+                }
+                for (int j = jlo;; ++j) {
+                    acc += l[at<CS1>(B.start[k - j] + p, lcs)] * r[at<CS1>(B.start[j] + s, rcs)];
+                    if (j == jhi) break;
+                    const uint32_t q = L::div(B, p);
+                    s += (p - q * width) * B.pow[j];
+                    p = q;
+                }
+            }
+            T v = beta * acc;
+            if (use_add) v = alpha * a[at<CS1>(ok + w, acs)] + v;
+            o[at<CS1>(ok + w, ocs)] = v;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, bool CS1>
+__device__ void fill_window(const DevBasis& B, T* o, int64_t cs, int mn, int mx, T v) {
+    if (mn > mx) return;
+    const int64_t b = B.start[mn], e = B.start[mx + 1];
+    for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) o[at<CS1>(i, cs)] = v;
+}
+
+// ---------------------------------------------------------------- kernels
+
+template <typename T, int W, bool CS1>
+__global__ void __launch_bounds__(kBlock)
+k_product(DevBasis B, View<T> out, View<const T> lhs, View<const T> rhs, View<const T> add,
+          int with_add, T alpha, T beta, int64_t n) {
+    for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
+        product_pass<T, W, CS1>(B, out.obj(b), out.cs, out.mn, out.mx, lhs.obj(b), lhs.cs, lhs.mn,
+                                lhs.mx, rhs.obj(b), rhs.cs, rhs.mn, rhs.mx,
+                                with_add ? add.obj(b) : nullptr, add.cs, add.mn, add.mx,
+                                with_add != 0, alpha, beta);
+    }
+}
+
+// The composites in the reference's literal Horner order, all N steps in one
+// launch (operations/intermediate/ft_exp.hpp:70-81, ft_fmexp.hpp:66-88,
+// ft_log.hpp:70-84).  Used whenever the window layout leaves no shortcut.
+template <typename T, int W, bool CS1>
+__global__ void __launch_bounds__(kBlock)
+k_horner(int op, DevBasis B, View<T> out, View<const T> arg, View<const T> mult, int64_t n) {
+    const int N = B.depth;
+    for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
+        T* o = out.obj(b);
+        const T* x = arg.obj(b);
+        if (op == kOpExp) {
+            fill_window<T, CS1>(B, o, out.cs, out.mn, out.mx, T(0));
+            __syncthreads();
+            if (threadIdx.x == 0 && out.mn == 0 && out.mx >= 0) o[0] = T(1);
+            __syncthreads();
+            const int tmn = max(arg.mn, 1), tmx = min(arg.mx, N);
+            for (int d = N; d > 0; --d) {
+                const T divisor = T(1) / T(d);
+                product_pass<T, W, CS1>(B, o, out.cs, out.mn, out.mx, o, out.cs, out.mn, out.mx, x,
+                                        arg.cs, tmn, tmx, nullptr, 0, 1, 0, false, T(0), divisor);
+                if (threadIdx.x == 0 && out.mn == 0 && out.mx >= 0) o[0] = o[0] + T(1);
+                __syncthreads();
+            }
+        } else if (op == kOpFmexp) {
+            const T* m = mult.obj(b);
+            // VectorAssign (cpu/.../linalg/vector_assign.hpp:32-61)
+            if (out.mn <= out.mx) {
+                const int lo = max(out.mn, mult.mn), hi = min(out.mx, mult.mx);
+                const int64_t bb = B.start[out.mn], ee = B.start[out.mx + 1];
+                const int64_t cb = lo <= hi ? B.start[lo] : ee;
+                const int64_t ce = lo <= hi ? B.start[hi + 1] : ee;
+                for (int64_t i = bb + threadIdx.x; i < ee; i += blockDim.x)
+                    o[at<CS1>(i, out.cs)] = (i >= cb && i < ce) ? m[at<CS1>(i, mult.cs)] : T(0);
+            }
+            __syncthreads();
+            for (int d = N; d > 0; --d) {
+                const int tmn = max(arg.mn, 1), tmx = min(arg.mx, N - d + 1);
+                const T divisor = T(1) / T(d);
+                product_pass<T, W, CS1>(B, o, out.cs, out.mn, out.mx, o, out.cs, out.mn, out.mx, x,
+                                        arg.cs, tmn, tmx, m, mult.cs, mult.mn, mult.mx, true, T(1),
+                                        divisor);
+            }
+        } else {  // kOpLog
+            fill_window<T, CS1>(B, o, out.cs, out.mn, out.mx, T(0));
+            __syncthreads();
+            for (int d = N; d > 0; --d) {
+                const T val = (d % 2 == 0 ? T(-1) : T(1)) / T(d);
+                if (threadIdx.x == 0 && out.mn == 0 && out.mx >= 0) o[0] = o[0] + val;
+                __syncthreads();
+                const int omx = min(out.mx, N - d + 1);
+                const int tmn = max(arg.mn, 1), tmx = min(arg.mx, N - d + 1);
+                product_pass<T, W, CS1>(B, o, out.cs, out.mn, omx, o, out.cs, out.mn, omx, x, arg.cs,
+                                        tmn, tmx, nullptr, 0, 1, 0, false, T(0), T(1));
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ft_fmexp when the exponent is a degree-one increment (window (1,1)) and
+// out starts at degree 0:  out_k[w] = sum_j mult_{k-j}[w / d^j] e_j(suffix),
+// e_j(s) = prod_i x[s_i] / j!.  One pass over out instead of N; the result is
+// the same truncated product mult (x) exp(x) the Horner recursion computes.
+template <typename T, int W, bool CS1>
+__global__ void __launch_bounds__(kBlock)
+k_fmexp_deg1(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int64_t n) {
+    using L = Letters<W>;
+    __shared__ T xs[256];
+    __shared__ T inv[kMaxDepth + 2];
+    const uint32_t width = L::width(B);
+    for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
+        T* o = out.obj(b);
+        const T* m = mult.obj(b);
+        const T* x = ex.obj(b);
+        for (uint32_t i = threadIdx.x; i < width; i += blockDim.x) xs[i] = x[at<CS1>(1 + i, ex.cs)];
+        if (threadIdx.x <= static_cast<unsigned>(B.depth))
+            inv[threadIdx.x] = threadIdx.x == 0 ? T(1) : T(1) / T(threadIdx.x);
+        __syncthreads();
+        for (int k = out.mx; k >= 0; --k) {
+            const uint32_t nk = B.pow[k];
+            const int64_t ok = B.start[k];
+            for (uint32_t w = threadIdx.x; w < nk; w += blockDim.x) {
+                T acc = mult.has(k) ? m[at<CS1>(ok + w, mult.cs)] : T(0);
+                uint32_t p = w;
+                T e = T(1);
+                for (int j = 1; j <= k; ++j) {
+                    const uint32_t q = L::div(B, p);
+                    e = e * xs[p - q * width] * inv[j];
+                    p = q;
+                    if (mult.has(k - j)) acc += m[at<CS1>(B.start[k - j] + p, mult.cs)] * e;
+                }
+                o[at<CS1>(ok + w, out.cs)] = acc;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ft_exp of a degree-one argument, out from degree 0: exp(x)_w = prod x / k!.
+template <typename T, int W, bool CS1>
+__global__ void __launch_bounds__(kBlock)
+k_exp_deg1(DevBasis B, View<T> out, View<const T> arg, int64_t n) {
+    using L = Letters<W>;
+    __shared__ T xs[256];
+    const uint32_t width = L::width(B);
+    for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
+        T* o = out.obj(b);
+        const T* x = arg.obj(b);
+        for (uint32_t i = threadIdx.x; i < width; i += blockDim.x) xs[i] = x[at<CS1>(1 + i, arg.cs)];
+        __syncthreads();
+        for (int k = 0; k <= out.mx; ++k) {
+            const uint32_t nk = B.pow[k];
+            const int64_t ok = B.start[k];
+            for (uint32_t w = threadIdx.x; w < nk; w += blockDim.x) {
+                uint32_t p = w;
+                T e = T(1);
+                for (int j = 1; j <= k; ++j) {
+                    const uint32_t q = L::div(B, p);
+                    e = e * xs[p - q * width] / T(j);
+                    p = q;
+                }
+                o[at<CS1>(ok + w, out.cs)] = e;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Work-group sum of one value per thread (wave64 shuffles, then LDS).
+template <typename T>
+__device__ T block_sum(T v) {
+    __shared__ T part[kBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) part[wave] = v;
+    __syncthreads();
+    T r = T(0);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < static_cast<int>(blockDim.x >> 6); ++i) r += part[i];
+    __syncthreads();
+    return r;
+}
+
+template <typename T, bool CS1>
+__global__ void __launch_bounds__(kBlock)
+k_pairing(DevBasis B, T* out, int64_t os, View<const T> f, View<const T> x, int64_t n) {
+    const int lo = max(f.mn, x.mn), hi = min(f.mx, x.mx);
+    for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
+        T acc = T(0);
+        if (lo <= hi) {
+            const T* F = f.obj(b);
+            const T* X = x.obj(b);
+            for (int64_t i = B.start[lo] + threadIdx.x; i < B.start[hi + 1]; i += blockDim.x)
+                acc += F[at<CS1>(i, f.cs)] * X[at<CS1>(i, x.cs)];
+        }
+        acc = block_sum(acc);
+        if (threadIdx.x == 0) out[b * os] = acc;
+    }
+}
+
+// out[w] = sum_u op[u] arg[u w]   (left)   or   sum_u arg[w u] op[u]   (right),
+// over op degrees pd with |w| + pd inside arg's window, pd descending as the
+// reference's argument-degree-major loop (cpu/.../ft_adj_lmul.hpp:69-114).
+template <typename T, bool LEFT, bool CS1>
+__global__ void __launch_bounds__(kBlock)
+k_adjoint(DevBasis B, View<T> out, View<const T> op, View<const T> arg, int64_t n) {
+    for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
+        T* o = out.obj(b);
+        const T* P = op.obj(b);
+        const T* A = arg.obj(b);
+        for (int od = out.mx; od >= out.mn; --od) {
+            const int pmx = min(op.mx, arg.mx - od);
+            const int pmn = max(op.mn, arg.mn - od);
+            const uint32_t nk = B.pow[od];
+            for (uint32_t w = threadIdx.x; w < nk; w += blockDim.x) {
+                T acc = T(0);
+                for (int pd = pmx; pd >= pmn; --pd) {
+                    const int64_t ab = B.start[od + pd];
+                    const int64_t pb = B.start[pd];
+                    const uint32_t np = B.pow[pd];
+                    if constexpr (LEFT) {
+                        for (uint32_t u = 0; u < np; ++u)
+                            acc += P[at<CS1>(pb + u, op.cs)] *
+                                   A[at<CS1>(ab + static_cast<int64_t>(u) * nk + w, arg.cs)];
+                    } else {
+                        const int64_t base = ab + static_cast<int64_t>(w) * np;
+                        for (uint32_t u = 0; u < np; ++u)
+                            acc += A[at<CS1>(base + u, arg.cs)] * P[at<CS1>(pb + u, op.cs)];
+                    }
+                }
+                o[at<CS1>(B.start[od] + w, out.cs)] = acc;
+            }
+        }
+    }
+}
+
+// CSR rows (cpu/.../linalg/sparse_matrix_vector.hpp:45-73): one thread per
+// row, grid.y over objects.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_spmv_csr(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t abs_,
+           int64_t acs, int64_t abeg, const T* values, const int64_t* indices,
+           const int64_t* offsets, int64_t rows, T alpha, int64_t n) {
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    const int64_t e0 = offsets[row], e1 = offsets[row + 1];
+    for (int64_t b = blockIdx.y; b < n; b += gridDim.y) {
+        const T* a = arg + b * abs_;
+        T acc = T(0);
+        for (int64_t e = e0; e < e1; ++e) acc += values[e] * a[(abeg + indices[e]) * acs];
+        out[b * obs + (obeg + row) * ocs] = alpha * acc;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_vec_zero(T* out, int64_t obs, int64_t ocs, int64_t obeg, int64_t size, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size) return;
+    for (int64_t b = blockIdx.y; b < n; b += gridDim.y) out[b * obs + (obeg + i) * ocs] = T(0);
+}
+
+// CSC columns (sparse_matrix_vector.hpp:133-163) as an atomic scatter after
+// the zero fill; the summation order of a row is therefore not fixed.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_spmv_csc(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t abs_,
+           int64_t acs, int64_t abeg, const T* values, const int64_t* indices,
+           const int64_t* offsets, int64_t cols, T alpha, int64_t n) {
+    const int64_t col = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (col >= cols) return;
+    const int64_t e0 = offsets[col], e1 = offsets[col + 1];
+    for (int64_t b = blockIdx.y; b < n; b += gridDim.y) {
+        const T av = alpha * arg[b * abs_ + (abeg + col) * acs];
+        T* o = out + b * obs;
+        for (int64_t e = e0; e < e1; ++e) atomicAdd(&o[(obeg + indices[e]) * ocs], values[e] * av);
+    }
+}
+
+bool all_contig(std::initializer_list<const rpp_hip_tensor_batch*> ts) {
+    for (auto* t : ts)
+        if (t && t->coeff_stride != 1) return false;
+    return true;
+}
+
+// Width/contiguity dispatch: compile-time widths for contiguous batches.
+#define RPP_DISPATCH_W(WIDTH, CONTIG, KERNEL_CALL)        \
+    do {                                                 \
+        if (CONTIG) {                                    \
+            switch (WIDTH) {                             \
+                case 2: { constexpr int W_ = 2; constexpr bool C_ = true; KERNEL_CALL; } break; \
+                case 3: { constexpr int W_ = 3; constexpr bool C_ = true; KERNEL_CALL; } break; \
+                case 4: { constexpr int W_ = 4; constexpr bool C_ = true; KERNEL_CALL; } break; \
+                case 5: { constexpr int W_ = 5; constexpr bool C_ = true; KERNEL_CALL; } break; \
+                case 6: { constexpr int W_ = 6; constexpr bool C_ = true; KERNEL_CALL; } break; \
+                case 8: { constexpr int W_ = 8; constexpr bool C_ = true; KERNEL_CALL; } break; \
+                default: { constexpr int W_ = 0; constexpr bool C_ = true; KERNEL_CALL; } break; \
+            }                                            \
+        } else {                                         \
+            constexpr int W_ = 0; constexpr bool C_ = false; KERNEL_CALL; \
+        }                                                \
+    } while (0)
+
+}  // namespace
+
+// ------------------------------------------------------------ host launchers
+
+template <typename T>
+int launch_product(const DevBasis& B, int64_t n, const rpp_hip_tensor_batch& out,
+                   const rpp_hip_tensor_batch& lhs, const rpp_hip_tensor_batch& rhs,
+                   const rpp_hip_tensor_batch* add, double alpha, double beta, hipStream_t stream) {
+    if (n <= 0) return 0;
+    const View<T> o = make_view<T>(out);
+    const View<const T> l = make_cview<T>(lhs), r = make_cview<T>(rhs);
+    const View<const T> a = add ? make_cview<T>(*add) : View<const T>{nullptr, 0, 1, 1, 0};
+    const int with_add = add ? 1 : 0;
+    const bool contig = all_contig({&out, &lhs, &rhs, add});
+    const unsigned grid = grid_for(n);
+    RPP_DISPATCH_W(B.width, contig,
+                   (k_product<T, W_, C_><<<grid, kBlock, 0, stream>>>(B, o, l, r, a, with_add, T(alpha),
+                                                                      T(beta), n)));
+    return check_hip(hipGetLastError(), "k_product launch");
+}
+
+template <typename T>
+int launch_horner(int op, const DevBasis& B, int64_t n, const rpp_hip_tensor_batch& out,
+                  const rpp_hip_tensor_batch& arg, const rpp_hip_tensor_batch* mult,
+                  hipStream_t stream) {
+    if (n <= 0) return 0;
+    const View<T> o = make_view<T>(out);
+    const View<const T> x = make_cview<T>(arg);
+    const View<const T> m = mult ? make_cview<T>(*mult) : View<const T>{nullptr, 0, 1, 1, 0};
+    const bool contig = all_contig({&out, &arg, mult});
+    const unsigned grid = grid_for(n);
+    // Shortcuts that compute the same truncated series in one pass.
+    const bool deg1 = B.depth >= 1 && arg.max_degree == 1 && arg.min_degree <= 1 &&
+                      out.min_degree == 0 && out.max_degree >= 0;
+    if (op == kOpFmexp && deg1 && B.width <= 256) {
+        RPP_DISPATCH_W(B.width, contig,
+                       (k_fmexp_deg1<T, W_, C_><<<grid, kBlock, 0, stream>>>(B, o, m, x, n)));
+        return check_hip(hipGetLastError(), "k_fmexp_deg1 launch");
+    }
+    if (op == kOpExp && deg1 && B.width <= 256) {
+        RPP_DISPATCH_W(B.width, contig, (k_exp_deg1<T, W_, C_><<<grid, kBlock, 0, stream>>>(B, o, x, n)));
+        return check_hip(hipGetLastError(), "k_exp_deg1 launch");
+    }
+    RPP_DISPATCH_W(B.width, contig,
+                   (k_horner<T, W_, C_><<<grid, kBlock, 0, stream>>>(op, B, o, x, m, n)));
+    return check_hip(hipGetLastError(), "k_horner launch");
+}
+
+template <typename T>
+int launch_pairing(const DevBasis& B, int64_t n, const rpp_hip_scalar_batch& out,
+                   const rpp_hip_tensor_batch& f, const rpp_hip_tensor_batch& x, hipStream_t stream) {
+    if (n <= 0) return 0;
+    const unsigned grid = grid_for(n);
+    if (f.coeff_stride == 1 && x.coeff_stride == 1)
+        k_pairing<T, true><<<grid, kBlock, 0, stream>>>(B, static_cast<T*>(out.data), out.stride,
+                                                        make_cview<T>(f), make_cview<T>(x), n);
+    else
+        k_pairing<T, false><<<grid, kBlock, 0, stream>>>(B, static_cast<T*>(out.data), out.stride,
+                                                         make_cview<T>(f), make_cview<T>(x), n);
+    return check_hip(hipGetLastError(), "k_pairing launch");
+}
+
+template <typename T>
+int launch_adjoint(bool left, const DevBasis& B, int64_t n, const rpp_hip_tensor_batch& out,
+                   const rpp_hip_tensor_batch& op, const rpp_hip_tensor_batch& arg,
+                   hipStream_t stream) {
+    if (n <= 0) return 0;
+    const unsigned grid = grid_for(n);
+    const bool contig = all_contig({&out, &op, &arg});
+    const View<T> o = make_view<T>(out);
+    const View<const T> p = make_cview<T>(op), g = make_cview<T>(arg);
+    if (left) {
+        if (contig) k_adjoint<T, true, true><<<grid, kBlock, 0, stream>>>(B, o, p, g, n);
+        else k_adjoint<T, true, false><<<grid, kBlock, 0, stream>>>(B, o, p, g, n);
+    } else {
+        if (contig) k_adjoint<T, false, true><<<grid, kBlock, 0, stream>>>(B, o, p, g, n);
+        else k_adjoint<T, false, false><<<grid, kBlock, 0, stream>>>(B, o, p, g, n);
+    }
+    return check_hip(hipGetLastError(), "k_adjoint launch");
+}
+
+template <typename T>
+int launch_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg,
+                const rpp_hip_sparse_matrix& M, double alpha, hipStream_t stream) {
+    if (n <= 0) return 0;
+    const unsigned gy = static_cast<unsigned>(std::min<int64_t>(n, 65535));
+    T* o = static_cast<T*>(out.data);
+    const T* a = static_cast<const T*>(arg.data);
+    const T* v = static_cast<const T*>(M.values);
+    if (M.format == RPP_HIP_CSR) {
+        if (M.rows == 0) return 0;
+        dim3 grid(static_cast<unsigned>((M.rows + kBlock - 1) / kBlock), gy);
+        k_spmv_csr<T><<<grid, kBlock, 0, stream>>>(o, out.batch_stride, out.coeff_stride, out.begin, a,
+                                                   arg.batch_stride, arg.coeff_stride, arg.begin, v,
+                                                   M.indices, M.offsets, M.rows, T(alpha), n);
+        return check_hip(hipGetLastError(), "k_spmv_csr launch");
+    }
+    if (out.size > 0) {
+        dim3 zg(static_cast<unsigned>((out.size + kBlock - 1) / kBlock), gy);
+        k_vec_zero<T><<<zg, kBlock, 0, stream>>>(o, out.batch_stride, out.coeff_stride, out.begin,
+                                                 out.size, n);
+    }
+    if (M.cols > 0) {
+        dim3 grid(static_cast<unsigned>((M.cols + kBlock - 1) / kBlock), gy);
+        k_spmv_csc<T><<<grid, kBlock, 0, stream>>>(o, out.batch_stride, out.coeff_stride, out.begin, a,
+                                                   arg.batch_stride, arg.coeff_stride, arg.begin, v,
+                                                   M.indices, M.offsets, M.cols, T(alpha), n);
+    }
+    return check_hip(hipGetLastError(), "k_spmv_csc launch");
+}
+
+#define RPP_INSTANTIATE(T)                                                                      \
+    template int launch_product<T>(const DevBasis&, int64_t, const rpp_hip_tensor_batch&,         \
+                                   const rpp_hip_tensor_batch&, const rpp_hip_tensor_batch&,      \
+                                   const rpp_hip_tensor_batch*, double, double, hipStream_t);     \
+    template int launch_horner<T>(int, const DevBasis&, int64_t, const rpp_hip_tensor_batch&,     \
+                                  const rpp_hip_tensor_batch&, const rpp_hip_tensor_batch*,       \
+                                  hipStream_t);                                                   \
+    template int launch_pairing<T>(const DevBasis&, int64_t, const rpp_hip_scalar_batch&,         \
+                                   const rpp_hip_tensor_batch&, const rpp_hip_tensor_batch&,      \
+                                   hipStream_t);                                                  \
+    template int launch_adjoint<T>(bool, const DevBasis&, int64_t, const rpp_hip_tensor_batch&,   \
+                                   const rpp_hip_tensor_batch&, const rpp_hip_tensor_batch&,      \
+                                   hipStream_t);                                                  \
+    template int launch_spmv<T>(int64_t, const rpp_hip_vector_batch&, const rpp_hip_vector_batch&, \
+                                const rpp_hip_sparse_matrix&, double, hipStream_t);
+
+RPP_INSTANTIATE(float)
+RPP_INSTANTIATE(double)
+
+}  // namespace rpp_hip

@@ -1,0 +1,33 @@
+// Host-side launchers of the per-operation kernels (ft_ops.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace rpp_hip {
+
+enum { kOpExp = 0, kOpFmexp = 1, kOpLog = 2 };
+
+template <typename T>
+int launch_product(const DevBasis& B, int64_t n, const rpp_hip_tensor_batch& out,
+                   const rpp_hip_tensor_batch& lhs, const rpp_hip_tensor_batch& rhs,
+                   const rpp_hip_tensor_batch* add, double alpha, double beta, hipStream_t stream);
+
+template <typename T>
+int launch_horner(int op, const DevBasis& B, int64_t n, const rpp_hip_tensor_batch& out,
+                  const rpp_hip_tensor_batch& arg, const rpp_hip_tensor_batch* mult,
+                  hipStream_t stream);
+
+template <typename T>
+int launch_pairing(const DevBasis& B, int64_t n, const rpp_hip_scalar_batch& out,
+                   const rpp_hip_tensor_batch& f, const rpp_hip_tensor_batch& x, hipStream_t stream);
+
+template <typename T>
+int launch_adjoint(bool left, const DevBasis& B, int64_t n, const rpp_hip_tensor_batch& out,
+                   const rpp_hip_tensor_batch& op, const rpp_hip_tensor_batch& arg,
+                   hipStream_t stream);
+
+template <typename T>
+int launch_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg,
+                const rpp_hip_sparse_matrix& M, double alpha, hipStream_t stream);
+
+}  // namespace rpp_hip

@@ -1,0 +1,316 @@
+// Host-side Hall-basis metadata: the Hall set, the bracket of two Hall
+// elements expressed in the Hall basis, and the Lie->tensor (L2T) and
+// tensor->Lie (T2L) matrices, built once per (width, depth) and uploaded to
+// each device once per dtype as CSR.
+//
+// The Hall set is enumerated in the same order as the reference
+// (basis/hall_basis.hpp:64-88) so that Lie coordinates agree index for index:
+// key 0 is the unused (0,0) slot, keys 1..width are the letters (0, l), and
+// each degree lists the pairs (left < right) with left_parent(right) <= left,
+// by left degree, then left key, then right key.  Everything else is fixed by
+// the mathematics: the expansion of a bracket in a basis is unique, L2T
+// column h is the commutator expansion of h, and T2L column w is the right
+// bracketing [w1,[w2,[...]]] divided by |w| (Dynkin-Specht-Wever).
+#include "lie.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <unordered_map>
+
+#include "common.hpp"
+
+namespace rpp_hip {
+
+namespace {
+
+inline uint64_t pair_key(int64_t a, int64_t b) {
+    return (static_cast<uint64_t>(a) << 32) | static_cast<uint64_t>(b);
+}
+
+using Terms = std::vector<std::pair<int64_t, int64_t>>;  // (key, integer coefficient)
+
+class HallSet {
+public:
+    HallSet(int width, int depth) : width_(width), depth_(depth) {
+        pairs_ = {0, 0};
+        degree_begin_ = {0, 1};
+        deg_ = {0};
+        if (depth >= 1) {
+            for (int64_t l = 1; l <= width; ++l) add(0, l, 1);
+            degree_begin_.push_back(1 + width);
+        }
+        for (int d = 2; d <= depth; ++d) {
+            for (int ld = 1; 2 * ld <= d; ++ld) {
+                const int rd = d - ld;
+                for (int64_t left = degree_begin_[ld]; left < degree_begin_[ld + 1]; ++left) {
+                    for (int64_t right = std::max(left + 1, degree_begin_[rd]);
+                         right < degree_begin_[rd + 1]; ++right) {
+                        if (pairs_[2 * right] <= left) add(left, right, d);
+                    }
+                }
+            }
+            degree_begin_.push_back(static_cast<int64_t>(deg_.size()));
+        }
+        if (depth == 0) degree_begin_.resize(2);
+    }
+
+    int64_t size() const { return static_cast<int64_t>(deg_.size()); }
+    int degree(int64_t key) const { return deg_[static_cast<size_t>(key)]; }
+    int64_t left(int64_t key) const { return pairs_[2 * key]; }
+    int64_t right(int64_t key) const { return pairs_[2 * key + 1]; }
+    const std::vector<int64_t>& pairs() const { return pairs_; }
+    const std::vector<int64_t>& degree_begin() const { return degree_begin_; }
+    int depth() const { return depth_; }
+    int width() const { return width_; }
+
+    int64_t find(int64_t a, int64_t b) const {
+        auto it = index_.find(pair_key(a, b));
+        return it == index_.end() ? 0 : it->second;
+    }
+
+    // [a, b] in the Hall basis (integer coefficients, zero terms dropped).
+    const Terms& bracket(int64_t a, int64_t b) {
+        static const Terms empty;
+        if (a <= 0 || b <= 0 || a == b || a >= size() || b >= size()) return empty;
+        if (degree(a) + degree(b) > depth_) return empty;
+        const uint64_t k = pair_key(a, b);
+        auto it = memo_.find(k);
+        if (it != memo_.end()) return it->second;
+        Terms out;
+        if (int64_t h = find(a, b)) {
+            out.emplace_back(h, 1);
+        } else if (int64_t h2 = find(b, a)) {
+            out.emplace_back(h2, -1);
+        } else if (b < a) {
+            const Terms rev = bracket(b, a);
+            for (auto const& [h, c] : rev) out.emplace_back(h, -c);
+        } else {
+            // a < b and b = [b1, b2] is not a letter:
+            // [a, [b1, b2]] = [[a, b1], b2] - [[a, b2], b1]   (Jacobi)
+            const int64_t b1 = left(b), b2 = right(b);
+            std::map<int64_t, int64_t> acc;
+            auto half = [&](int64_t outer_right, int64_t inner_right, int64_t sign) {
+                const Terms outer = bracket(a, outer_right);
+                for (auto const& [h, c] : outer) {
+                    const Terms inner = bracket(h, inner_right);
+                    for (auto const& [g, e] : inner) acc[g] += sign * c * e;
+                }
+            };
+            half(b1, b2, 1);
+            half(b2, b1, -1);
+            for (auto const& [h, c] : acc)
+                if (c != 0) out.emplace_back(h, c);
+        }
+        return memo_.emplace(k, std::move(out)).first->second;
+    }
+
+private:
+    void add(int64_t l, int64_t r, int d) {
+        index_.emplace(pair_key(l, r), static_cast<int64_t>(deg_.size()));
+        pairs_.push_back(l);
+        pairs_.push_back(r);
+        deg_.push_back(d);
+    }
+
+    int width_, depth_;
+    std::vector<int64_t> pairs_;
+    std::vector<int64_t> degree_begin_;
+    std::vector<int> deg_;
+    std::unordered_map<uint64_t, int64_t> index_;
+    std::unordered_map<uint64_t, Terms> memo_;
+};
+
+struct Csc {
+    std::vector<double> values;
+    std::vector<int64_t> indices;
+    std::vector<int64_t> offsets{0};
+    void close_column() { offsets.push_back(static_cast<int64_t>(values.size())); }
+};
+
+Csr to_csr(const Csc& m, int64_t rows) {
+    Csr r;
+    r.rows = rows;
+    r.offsets.assign(static_cast<size_t>(rows) + 1, 0);
+    for (int64_t i : m.indices) ++r.offsets[static_cast<size_t>(i) + 1];
+    for (int64_t i = 0; i < rows; ++i) r.offsets[i + 1] += r.offsets[i];
+    r.values.resize(m.values.size());
+    r.indices.resize(m.values.size());
+    std::vector<int64_t> fill(r.offsets.begin(), r.offsets.end() - 1);
+    const int64_t cols = static_cast<int64_t>(m.offsets.size()) - 1;
+    for (int64_t c = 0; c < cols; ++c) {
+        for (int64_t e = m.offsets[c]; e < m.offsets[c + 1]; ++e) {
+            const int64_t dst = fill[static_cast<size_t>(m.indices[e])]++;
+            r.values[dst] = m.values[e];
+            r.indices[dst] = c;
+        }
+    }
+    r.cols = cols;
+    return r;
+}
+
+}  // namespace
+
+std::shared_ptr<const LieMetadata> build_lie_metadata(int width, int depth) {
+    auto md = std::make_shared<LieMetadata>();
+    HallSet hall(width, depth);
+    md->width = width;
+    md->depth = depth;
+    md->pairs = hall.pairs();
+    md->degree_begin = hall.degree_begin();
+    const int64_t true_size = hall.size();
+    md->true_size = true_size;
+
+    std::vector<int64_t> tstart(static_cast<size_t>(depth) + 2, 0), tpow(static_cast<size_t>(depth) + 2, 1);
+    for (int k = 1; k <= depth + 1; ++k) {
+        tpow[k] = tpow[k - 1] * width;
+        tstart[k] = tstart[k - 1] + tpow[k - 1];
+    }
+    const int64_t dim = tstart[depth + 1];
+    md->tensor_dim = dim;
+
+    // L2T, CSC: column key-1 is the tensor expansion of Hall element `key`
+    // (sparse/l2t_matrix.hpp:58-114): letters map to their word, and
+    // [a, b] -> a(x)b - b(x)a, a concatenation of the two columns.
+    Csc l2t;
+    std::vector<std::map<int64_t, double>> expansion(static_cast<size_t>(true_size));
+    for (int64_t key = 1; key < true_size; ++key) {
+        auto& col = expansion[static_cast<size_t>(key)];
+        if (hall.left(key) == 0) {
+            col[hall.right(key)] = 1.0;  // tensor index of the letter word
+        } else {
+            const int64_t a = hall.left(key), b = hall.right(key);
+            const int da = hall.degree(a), db = hall.degree(b);
+            const int d = da + db;
+            for (auto const& [ia, ca] : expansion[static_cast<size_t>(a)]) {
+                for (auto const& [ib, cb] : expansion[static_cast<size_t>(b)]) {
+                    const int64_t la = ia - tstart[da], lb = ib - tstart[db];
+                    col[tstart[d] + la * tpow[db] + lb] += ca * cb;
+                    col[tstart[d] + lb * tpow[da] + la] -= cb * ca;
+                }
+            }
+            for (auto it = col.begin(); it != col.end();) it = it->second == 0.0 ? col.erase(it) : std::next(it);
+        }
+        for (auto const& [row, v] : col) {
+            l2t.values.push_back(v);
+            l2t.indices.push_back(row);
+        }
+        l2t.close_column();
+    }
+    // T2L, CSC: column w is the right bracketing of the word, over |w|
+    // (sparse/t2l_matrix.hpp:57-142); rows are key - 1.
+    Csc t2l;
+    std::vector<std::vector<std::pair<int64_t, double>>> rb(static_cast<size_t>(dim));
+    for (int k = 1; k <= depth; ++k) {
+        for (int64_t local = 0; local < tpow[k]; ++local) {
+            const int64_t g = tstart[k] + local;
+            auto& col = rb[static_cast<size_t>(g)];
+            if (k == 1) {
+                col.emplace_back(local + 1, 1.0);
+                continue;
+            }
+            const int64_t first = local / tpow[k - 1];
+            const int64_t rest = tstart[k - 1] + local % tpow[k - 1];
+            std::map<int64_t, double> acc;
+            for (auto const& [key, c] : rb[static_cast<size_t>(rest)]) {
+                for (auto const& [h, e] : hall.bracket(first + 1, key)) acc[h] += c * static_cast<double>(e);
+            }
+            for (auto const& [h, v] : acc)
+                if (v != 0.0) col.emplace_back(h, v);
+        }
+    }
+    t2l.close_column();  // the empty word
+    for (int64_t g = 1; g < dim; ++g) {
+        int k = 1;
+        while (tstart[k + 1] <= g) ++k;
+        const double divisor = 1.0 / static_cast<double>(k);
+        for (auto const& [h, v] : rb[static_cast<size_t>(g)]) {
+            t2l.values.push_back(k >= 2 ? v * divisor : v);
+            t2l.indices.push_back(h - 1);
+        }
+        t2l.close_column();
+    }
+    md->l2t_csc_values = l2t.values;
+    md->l2t_csc_indices = l2t.indices;
+    md->l2t_csc_offsets = l2t.offsets;
+    md->t2l_csc_values = t2l.values;
+    md->t2l_csc_indices = t2l.indices;
+    md->t2l_csc_offsets = t2l.offsets;
+    md->l2t = to_csr(l2t, dim);
+    md->t2l = to_csr(t2l, std::max<int64_t>(true_size - 1, 0));
+    return md;
+}
+
+std::shared_ptr<const LieMetadata> lie_metadata(int width, int depth) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, std::shared_ptr<const LieMetadata>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto key = std::make_pair(width, depth);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    auto md = build_lie_metadata(width, depth);
+    cache.emplace(key, md);
+    return md;
+}
+
+namespace {
+
+struct DeviceMatrices {
+    DeviceCsr l2t, t2l;
+};
+
+template <typename T>
+int upload(const Csr& h, DeviceCsr& d) {
+    d.rows = h.rows;
+    d.cols = h.cols;
+    d.nnz = static_cast<int64_t>(h.values.size());
+    std::vector<T> vals(h.values.begin(), h.values.end());
+    const size_t nv = std::max<size_t>(vals.size(), 1);
+    if (hipMalloc(&d.values, nv * sizeof(T)) != hipSuccess) return 5;
+    if (hipMalloc(reinterpret_cast<void**>(&d.indices), nv * sizeof(int64_t)) != hipSuccess) return 5;
+    if (hipMalloc(reinterpret_cast<void**>(&d.offsets), h.offsets.size() * sizeof(int64_t)) != hipSuccess)
+        return 5;
+    if (!vals.empty()) {
+        if (hipMemcpy(d.values, vals.data(), vals.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            return 9;
+        if (hipMemcpy(d.indices, h.indices.data(), h.indices.size() * sizeof(int64_t),
+                      hipMemcpyHostToDevice) != hipSuccess)
+            return 9;
+    }
+    if (hipMemcpy(d.offsets, h.offsets.data(), h.offsets.size() * sizeof(int64_t), hipMemcpyHostToDevice) !=
+        hipSuccess)
+        return 9;
+    return 0;
+}
+
+}  // namespace
+
+template <typename T>
+int device_lie_matrices(int width, int depth, const DeviceCsr** l2t, const DeviceCsr** t2l) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int, int>, std::unique_ptr<DeviceMatrices>> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(9, "hipGetDevice failed");
+    auto md = lie_metadata(width, depth);
+    std::lock_guard<std::mutex> lock(mu);
+    auto key = std::make_tuple(width, depth, static_cast<int>(sizeof(T)), dev);
+    auto it = cache.find(key);
+    if (it == cache.end()) {
+        auto dm = std::make_unique<DeviceMatrices>();
+        if (int rc = upload<T>(md->l2t, dm->l2t)) return fail(rc, "uploading the L2T matrix failed");
+        if (int rc = upload<T>(md->t2l, dm->t2l)) return fail(rc, "uploading the T2L matrix failed");
+        it = cache.emplace(key, std::move(dm)).first;
+    }
+    *l2t = &it->second->l2t;
+    *t2l = &it->second->t2l;
+    return 0;
+}
+
+template int device_lie_matrices<float>(int, int, const DeviceCsr**, const DeviceCsr**);
+template int device_lie_matrices<double>(int, int, const DeviceCsr**, const DeviceCsr**);
+
+}  // namespace rpp_hip

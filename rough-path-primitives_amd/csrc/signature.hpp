@@ -1,0 +1,26 @@
+// Batched Chen recursion (signature.hip): host launchers.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace rpp_hip {
+
+// Workspace slots of the library (see workspace()).
+enum { kSlotOrder = 0, kSlotState = 1, kSlotCounter = 2, kSlotTmp = 3 };
+
+int path_order(const int64_t* d_offsets, int64_t n, std::vector<int64_t>& lengths_host, int64_t** d_order,
+               hipStream_t stream);
+
+template <typename T>
+int signature_forward_generic(const DevBasis& B, const rpp_hip_signature_desc& d, const int64_t* d_order,
+                              hipStream_t stream);
+
+template <typename T>
+int signature_backward_generic(const DevBasis& B, const rpp_hip_signature_backward_desc& d,
+                               const int64_t* d_order, hipStream_t stream);
+
+int64_t backward_slot_elems(const DevBasis& B);
+
+}  // namespace rpp_hip

@@ -1,0 +1,360 @@
+"""Operation API over torch device tensors, mirroring the reference's free
+functions (rpp::ops::ft_mul etc., operations/**.hpp) with the hip::block
+strategy implicit: ``strategy`` and ``LaunchConfig`` collapse into an
+optional ``stream`` (the reference's GPU LaunchConfig carries exactly a
+stream, gpu/device.hpp:21-24).
+
+Batches mirror ``make_tensor_batch`` (views/dense_tensor_view.hpp:38-91):
+
+* a 2-D tensor ``[n, stride]`` is object-major (``layouts::StrideLayout``);
+* a 1-D tensor is one object broadcast to the batch (stride 0);
+* ``interleaved(t)`` marks a ``[dim, n]`` tensor as batch-interleaved
+  (coefficient i of object b at ``t[i, b]``).
+
+Errors raise :class:`RppHipError` carrying the rpp::ErrorCode value
+(support/error.hpp:28-39); nothing falls back to the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import torch
+
+from . import _capi as A
+
+ERROR_NAMES = {0: "Success", 1: "Cancelled", 2: "Unknown", 3: "InvalidArgument", 4: "Timeout",
+               5: "OutOfResources", 6: "ContractViolation", 7: "OutOfBounds", 8: "NotImplemented",
+               9: "Internal"}
+
+
+class RppHipError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{ERROR_NAMES.get(code, code)}: {message}")
+        self.code = code
+
+
+@dataclass(frozen=True)
+class TensorBasis:
+    """TensorBasis(width, depth) (basis/tensor_basis.hpp:40-118)."""
+    width: int
+    depth: int
+
+    def start_of_degree(self, k: int) -> int:
+        s, p = 0, 1
+        for _ in range(k):
+            s += p
+            p *= self.width
+        return s
+
+    def end_of_degree(self, k: int) -> int:
+        return self.start_of_degree(k + 1)
+
+    def size_of_degree(self, k: int) -> int:
+        return self.width ** k
+
+    def size(self) -> int:
+        return self.start_of_degree(self.depth + 1)
+
+    def _c(self):
+        return A.Basis(self.width, self.depth)
+
+
+def tensor_dimension(width: int, depth: int) -> int:
+    return TensorBasis(width, depth).size()
+
+
+@dataclass
+class Batch:
+    """A batch of graded views: data + window [min_degree, max_degree]."""
+    data: torch.Tensor
+    min_degree: int
+    max_degree: int
+    interleaved: bool = False
+
+    def truncate(self, mn: int, mx: int) -> "Batch":
+        return Batch(self.data, max(mn, self.min_degree), min(mx, self.max_degree), self.interleaved)
+
+    def _c(self) -> A.TensorBatch:
+        t = self.data
+        if not t.is_cuda:
+            raise RppHipError(3, "batch data must be a device tensor")
+        if self.interleaved:
+            if t.dim() != 2 or t.stride(1) != 1:
+                raise RppHipError(3, "interleaved batches are [dim, n] with unit batch stride")
+            return A.TensorBatch(t.data_ptr(), 1, t.stride(0), self.min_degree, self.max_degree)
+        if t.dim() == 1:
+            if t.stride(0) != 1:
+                raise RppHipError(3, "a broadcast object must be contiguous")
+            return A.TensorBatch(t.data_ptr(), 0, 1, self.min_degree, self.max_degree)
+        if t.dim() != 2 or t.stride(1) != 1:
+            raise RppHipError(3, "object-major batches are [n, stride] with unit coefficient stride")
+        return A.TensorBatch(t.data_ptr(), t.stride(0), 1, self.min_degree, self.max_degree)
+
+    @property
+    def num_objects(self) -> int:
+        t = self.data
+        if t.dim() == 1:
+            return 1
+        return t.shape[1] if self.interleaved else t.shape[0]
+
+
+def make_tensor_batch(data: torch.Tensor, basis: TensorBasis, min_degree: int = 0,
+                      max_degree: Optional[int] = None) -> Batch:
+    return Batch(data, min_degree, basis.depth if max_degree is None else max_degree)
+
+
+def interleaved(data: torch.Tensor, basis: TensorBasis, min_degree: int = 0,
+                max_degree: Optional[int] = None) -> Batch:
+    return Batch(data, min_degree, basis.depth if max_degree is None else max_degree, True)
+
+
+def _as_batch(x, basis) -> Batch:
+    if isinstance(x, Batch):
+        return x
+    return make_tensor_batch(x, basis)
+
+
+def _sfx(*batches) -> str:
+    dt = None
+    for b in batches:
+        d = b.data.dtype if isinstance(b, Batch) else b.dtype
+        if dt is None:
+            dt = d
+        elif d != dt:
+            raise RppHipError(3, f"mixed dtypes {dt} and {d}")
+    if dt == torch.float64:
+        return "_f64"
+    if dt == torch.float32:
+        return "_f32"
+    raise RppHipError(8, f"unsupported dtype {dt}")
+
+
+def _stream(stream) -> int:
+    if stream is None:
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, torch.cuda.Stream):
+        return stream.cuda_stream
+    return int(stream)
+
+
+def _call(name: str, desc, stream) -> None:
+    lib = A.lib()
+    rc = getattr(lib, name)(C.byref(desc), C.c_void_p(_stream(stream)))
+    if rc != 0:
+        msg = lib.rpp_hip_last_error()
+        raise RppHipError(rc, msg.decode() if msg else name)
+
+
+def _n(num_batches, *batches) -> int:
+    if num_batches is not None:
+        return int(num_batches)
+    return max(b.num_objects for b in batches)
+
+
+# ------------------------------------------------------------------ products
+
+def ft_mul(out, lhs, rhs, basis: TensorBasis, num_batches=None, beta: float = 1.0, stream=None):
+    """rpp::ops::ft_mul (operations/basic/ft_mul.hpp:53-79)."""
+    out, lhs, rhs = (_as_batch(x, basis) for x in (out, lhs, rhs))
+    d = A.FtMulDesc(basis._c(), _n(num_batches, out), out._c(), lhs._c(), rhs._c(), beta)
+    _call("rpp_hip_ft_mul" + _sfx(out, lhs, rhs), d, stream)
+
+
+def ft_fma(out, a, b, c, basis: TensorBasis, num_batches=None, alpha: float = 1.0, beta: float = 1.0,
+           stream=None):
+    """rpp::ops::ft_fma (operations/basic/ft_fma.hpp:54-83)."""
+    out, a, b, c = (_as_batch(x, basis) for x in (out, a, b, c))
+    d = A.FtFmaDesc(basis._c(), _n(num_batches, out), out._c(), a._c(), b._c(), c._c(), alpha, beta)
+    _call("rpp_hip_ft_fma" + _sfx(out, a, b, c), d, stream)
+
+
+def ft_inplace_fma(variant: int, a, b, c, basis: TensorBasis, num_batches=None, alpha: float = 1.0,
+                   beta: float = 1.0, stream=None):
+    """rpp::ops::ft_inplace_fma<FMAType> (operations/basic/ft_inplace_fma.hpp:64-100);
+    variant 231 (AEqualsBCPlusA), 123 (AEqualsABPlusC) or 213 (AEqualsBAPlusC)."""
+    a, b, c = (_as_batch(x, basis) for x in (a, b, c))
+    d = A.FtInplaceFmaDesc(basis._c(), _n(num_batches, a), variant, a._c(), b._c(), c._c(), alpha, beta)
+    _call("rpp_hip_ft_inplace_fma" + _sfx(a, b, c), d, stream)
+
+
+def ft_inplace_mul(lhs, rhs, basis: TensorBasis, num_batches=None, beta: float = 1.0, stream=None):
+    """rpp::ops::ft_inplace_mul (operations/basic/ft_inplace_mul.hpp:38-69)."""
+    lhs, rhs = (_as_batch(x, basis) for x in (lhs, rhs))
+    d = A.FtInplaceMulDesc(basis._c(), _n(num_batches, lhs), lhs._c(), rhs._c(), beta)
+    _call("rpp_hip_ft_inplace_mul" + _sfx(lhs, rhs), d, stream)
+
+
+# ---------------------------------------------------------------- composites
+
+def ft_exp(out, arg, basis: TensorBasis, num_batches=None, stream=None):
+    """rpp::ops::ft_exp (operations/intermediate/ft_exp.hpp:89-112)."""
+    out, arg = (_as_batch(x, basis) for x in (out, arg))
+    d = A.FtUnaryDesc(basis._c(), _n(num_batches, out), out._c(), arg._c())
+    _call("rpp_hip_ft_exp" + _sfx(out, arg), d, stream)
+
+
+def ft_log(out, arg, basis: TensorBasis, num_batches=None, stream=None):
+    """rpp::ops::ft_log (operations/intermediate/ft_log.hpp:91-114)."""
+    out, arg = (_as_batch(x, basis) for x in (out, arg))
+    d = A.FtUnaryDesc(basis._c(), _n(num_batches, out), out._c(), arg._c())
+    _call("rpp_hip_ft_log" + _sfx(out, arg), d, stream)
+
+
+def ft_fmexp(out, multiplier, exponent, basis: TensorBasis, num_batches=None, stream=None):
+    """rpp::ops::ft_fmexp (operations/intermediate/ft_fmexp.hpp:96-121)."""
+    out, multiplier, exponent = (_as_batch(x, basis) for x in (out, multiplier, exponent))
+    d = A.FtFmexpDesc(basis._c(), _n(num_batches, out), out._c(), multiplier._c(), exponent._c())
+    _call("rpp_hip_ft_fmexp" + _sfx(out, multiplier, exponent), d, stream)
+
+
+def tensor_pairing(out: torch.Tensor, functional, arg, basis: TensorBasis, num_batches=None, stream=None):
+    """rpp::ops::tensor_pairing (operations/basic/tensor_pairing.hpp:47-73);
+    ``out`` is a 1-D device tensor (make_scalar_batch)."""
+    functional, arg = (_as_batch(x, basis) for x in (functional, arg))
+    if not out.is_cuda or out.dim() != 1:
+        raise RppHipError(3, "out must be a 1-D device tensor")
+    n = _n(num_batches, functional, arg) if num_batches is None else num_batches
+    d = A.PairingDesc(basis._c(), n, A.ScalarBatch(out.data_ptr(), out.stride(0)), functional._c(), arg._c())
+    _call("rpp_hip_tensor_pairing" + _sfx(out, functional, arg), d, stream)
+
+
+def ft_adj_lmul(out, op, arg, basis: TensorBasis, num_batches=None, stream=None):
+    """rpp::ops::ft_adj_lmul (operations/basic/ft_adj_lmul.hpp:49-73)."""
+    out, op, arg = (_as_batch(x, basis) for x in (out, op, arg))
+    d = A.AdjointDesc(basis._c(), _n(num_batches, out), out._c(), op._c(), arg._c())
+    _call("rpp_hip_ft_adj_lmul" + _sfx(out, op, arg), d, stream)
+
+
+def ft_adj_rmul(out, op, arg, basis: TensorBasis, num_batches=None, stream=None):
+    """rpp::ops::ft_adj_rmul (operations/basic/ft_adj_rmul.hpp:47-71)."""
+    out, op, arg = (_as_batch(x, basis) for x in (out, op, arg))
+    d = A.AdjointDesc(basis._c(), _n(num_batches, out), out._c(), op._c(), arg._c())
+    _call("rpp_hip_ft_adj_rmul" + _sfx(out, op, arg), d, stream)
+
+
+# ------------------------------------------------------------- sparse, Lie
+
+@dataclass
+class SparseMatrix:
+    """CompressedMatrix with device arrays (sparse/detail/compressed_matrix.hpp:38-126)."""
+    format: str  # "csr" or "csc"
+    values: torch.Tensor
+    indices: torch.Tensor  # int64
+    offsets: torch.Tensor  # int64
+    rows: int
+    cols: int
+
+    def _c(self):
+        return A.SparseMatrix(0 if self.format == "csr" else 1, self.values.data_ptr(),
+                              self.indices.data_ptr(), self.offsets.data_ptr(), self.values.numel(),
+                              self.rows, self.cols)
+
+
+def _vec(t: torch.Tensor, begin: int, size: int) -> A.VectorBatch:
+    if t.dim() == 1:
+        return A.VectorBatch(t.data_ptr(), 0, t.stride(0), begin, size)
+    return A.VectorBatch(t.data_ptr(), t.stride(0), t.stride(1), begin, size)
+
+
+def sparse_matrix_vector_product(out: torch.Tensor, arg: torch.Tensor, matrix: SparseMatrix,
+                                 out_begin: int = 0, arg_begin: int = 0, out_size: Optional[int] = None,
+                                 alpha: float = 1.0, num_batches=None, stream=None):
+    """rpp::ops::sparse_matrix_vector_product (operations/linalg/sparse_matrix_vector.hpp:57-96)
+    with view-relative indexing: row 0 <-> out_begin, column 0 <-> arg_begin."""
+    n = out.shape[0] if num_batches is None else num_batches
+    if out_size is None:
+        out_size = matrix.rows
+    d = A.SpmvDesc(n, _vec(out, out_begin, out_size), _vec(arg, arg_begin, matrix.cols), matrix._c(), alpha)
+    _call("rpp_hip_sparse_matrix_vector_product" + _sfx(out, arg, matrix.values), d, stream)
+
+
+def tensor_to_lie(lie_out, tensor_in, basis: TensorBasis, num_batches=None, stream=None):
+    """rpp::ops::tensor_to_lie (operations/basic/tensor_to_lie.hpp:65-103) with the
+    library's Hall-basis T2L matrix.  ``lie_out`` holds true_size slots."""
+    lie_out = lie_out if isinstance(lie_out, Batch) else Batch(lie_out, 1, basis.depth)
+    tensor_in = _as_batch(tensor_in, basis)
+    d = A.LieConvertDesc(basis._c(), _n(num_batches, tensor_in), tensor_in._c(), lie_out._c())
+    _call("rpp_hip_tensor_to_lie" + _sfx(lie_out, tensor_in), d, stream)
+
+
+def lie_to_tensor(tensor_out, lie_in, basis: TensorBasis, num_batches=None, stream=None):
+    """rpp::ops::lie_to_tensor (operations/basic/lie_to_tensor.hpp:70-108)."""
+    lie_in = lie_in if isinstance(lie_in, Batch) else Batch(lie_in, 1, basis.depth)
+    tensor_out = _as_batch(tensor_out, basis)
+    d = A.LieConvertDesc(basis._c(), _n(num_batches, tensor_out), tensor_out._c(), lie_in._c())
+    _call("rpp_hip_lie_to_tensor" + _sfx(tensor_out, lie_in), d, stream)
+
+
+def lie_basis(width: int, depth: int) -> dict:
+    """Hall basis pairs, degree_begin and the CSC L2T/T2L matrices (host numpy)."""
+    import numpy as np
+    lib = A.lib()
+    sizes = np.zeros(4, dtype=np.int64)
+    rc = lib.rpp_hip_lie_basis_sizes(width, depth, sizes.ctypes.data)
+    if rc != 0:
+        raise RppHipError(rc, lib.rpp_hip_lie_basis_sizes.__name__)
+    ts, l2t_nnz, t2l_nnz, dim = (int(v) for v in sizes)
+    pairs = np.zeros(2 * ts, dtype=np.int64)
+    db = np.zeros(depth + 2, dtype=np.int64)
+    l2t = (np.zeros(l2t_nnz), np.zeros(l2t_nnz, dtype=np.int64), np.zeros(ts, dtype=np.int64))
+    t2l = (np.zeros(t2l_nnz), np.zeros(t2l_nnz, dtype=np.int64), np.zeros(dim + 1, dtype=np.int64))
+    rc = lib.rpp_hip_lie_basis_copy(width, depth, pairs.ctypes.data, db.ctypes.data,
+                                    *(a.ctypes.data for a in l2t), *(a.ctypes.data for a in t2l))
+    if rc != 0:
+        raise RppHipError(rc, "rpp_hip_lie_basis_copy")
+    return dict(true_size=ts, tensor_dim=dim, pairs=pairs.reshape(-1, 2), degree_begin=db,
+                l2t=dict(data=l2t[0], indices=l2t[1], offsets=l2t[2]),
+                t2l=dict(data=t2l[0], indices=t2l[1], offsets=t2l[2]))
+
+
+# -------------------------------------------------- batched Chen recursion
+
+def _offsets_of(lengths, device) -> torch.Tensor:
+    lengths = torch.as_tensor(lengths, dtype=torch.int64)
+    off = torch.zeros(lengths.numel() + 1, dtype=torch.int64)
+    off[1:] = torch.cumsum(lengths, 0)
+    return off.to(device)
+
+
+def signature(increments: torch.Tensor, offsets: torch.Tensor, basis: TensorBasis,
+              out: Optional[torch.Tensor] = None, checkpoint_every: int = 0,
+              checkpoints: Optional[torch.Tensor] = None, ckpt_offsets: Optional[torch.Tensor] = None,
+              tile_threads: int = 0, tile_split: int = 0, stream=None) -> torch.Tensor:
+    """Signatures of many paths of uneven length in one launch.
+
+    increments: [total, width] device tensor; offsets: int64 device tensor [n+1].
+    """
+    n = offsets.numel() - 1
+    dim = basis.size()
+    if out is None:
+        out = torch.empty((n, dim), dtype=increments.dtype, device=increments.device)
+    inc = increments.contiguous()
+    d = A.SignatureDesc(basis._c(), n, inc.data_ptr(), offsets.data_ptr(),
+                        Batch(out, 0, basis.depth)._c(), checkpoint_every,
+                        checkpoints.data_ptr() if checkpoints is not None else None,
+                        ckpt_offsets.data_ptr() if ckpt_offsets is not None else None,
+                        tile_threads, tile_split)
+    _call("rpp_hip_signature" + _sfx(inc, out), d, stream)
+    return out
+
+
+def signature_backward(increments: torch.Tensor, offsets: torch.Tensor, basis: TensorBasis,
+                       signatures: torch.Tensor, cotangent: torch.Tensor,
+                       grad: Optional[torch.Tensor] = None, checkpoint_every: int = 0,
+                       checkpoints: Optional[torch.Tensor] = None,
+                       ckpt_offsets: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
+    """d loss / d increments given d loss / d signatures (cotangent may be 1-D,
+    one functional for every path)."""
+    n = offsets.numel() - 1
+    inc = increments.contiguous()
+    if grad is None:
+        grad = torch.empty_like(inc)
+    d = A.SignatureBackwardDesc(basis._c(), n, inc.data_ptr(), offsets.data_ptr(),
+                                Batch(signatures, 0, basis.depth)._c(),
+                                Batch(cotangent, 0, basis.depth)._c(), grad.data_ptr(), checkpoint_every,
+                                checkpoints.data_ptr() if checkpoints is not None else None,
+                                ckpt_offsets.data_ptr() if ckpt_offsets is not None else None, None)
+    _call("rpp_hip_signature_backward" + _sfx(inc, signatures, cotangent), d, stream)
+    return grad

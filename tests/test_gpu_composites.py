@@ -1,0 +1,182 @@
+"""GPU parity of ft_exp, ft_fmexp, ft_log, tensor_pairing and the adjoint
+multiplications against the oracle."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import rpp_hip
+from helpers import assert_close, damped_batch, degree_begin, random_batch, window_mask
+
+pytestmark = pytest.mark.gpu
+
+DTYPES = [np.float64, np.float32]
+SHAPES = [(1, 5), (2, 6), (3, 4), (4, 4), (5, 3), (6, 3), (7, 2), (16, 2)]
+
+
+def to_dev(x, dev):
+    return torch.tensor(np.ascontiguousarray(x), device=dev)
+
+
+def deg1(rng, n, w, N, dtype, scale=0.5):
+    dim = degree_begin(w, N)[-1]
+    x = rng.uniform(-1, 1, size=(n, dim)).astype(dtype)  # junk outside degree 1
+    x[:, 1:1 + w] = rng.normal(0.0, scale, size=(n, w))
+    return x
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("ew", [(1, 1), (0, 1), None], ids=["deg1", "deg01", "full"])
+def test_ft_fmexp(dev, shape, ew, dtype):
+    w, N = shape
+    rng = np.random.default_rng(w * 10 + N)
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    n = 5
+    mult = damped_batch(rng, n, w, N, dtype)
+    ex = deg1(rng, n, w, N, dtype) if ew is not None else damped_batch(rng, n, w, N, dtype)
+    ew = ew or (0, N)
+    expected = np.full((n, dim), np.nan, dtype=dtype)
+    O.orc.ft_fmexp(w, N, expected, mult, ex, ew=ew)
+    out = to_dev(np.full((n, dim), np.nan, dtype=dtype), dev)
+    rpp_hip.ft_fmexp(out, to_dev(mult, dev), rpp_hip.Batch(to_dev(ex, dev), *ew), basis)
+    assert_close(out.cpu().numpy(), expected, dtype, f"fmexp {shape} {ew}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("ranges", [((0, 4), (1, 4)), ((0, 2), (1, 1)), ((1, 3), (1, 2)), ((2, 4), (0, 4)),
+                                    ((0, 0), (1, 4)), ((0, 4), (3, 4))])
+@pytest.mark.parametrize("out_max", [0, 2, 4])
+def test_ft_fmexp_windows(dev, ranges, out_max, dtype):
+    """Windows of multiplier and exponent, outputs of every prefix length
+    (ft_intermediate_initialization_tests.cpp:24-109)."""
+    mw, ew = ranges
+    w, N = 4, 4
+    rng = np.random.default_rng(111)
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    n = 3
+    mult = damped_batch(rng, n, w, N, dtype)
+    ex = damped_batch(rng, n, w, N, dtype)
+    ow = (0, out_max)
+    init = np.full((n, dim), np.nan, dtype=dtype)
+    expected = init.copy()
+    O.orc.ft_fmexp(w, N, expected, mult, ex, ow=ow, mw=mw, ew=ew)
+    out = to_dev(init, dev)
+    rpp_hip.ft_fmexp(rpp_hip.Batch(out, *ow), rpp_hip.Batch(to_dev(mult, dev), *mw),
+                     rpp_hip.Batch(to_dev(ex, dev), *ew), basis)
+    got = out.cpu().numpy()
+    m = window_mask(w, N, ow)
+    assert np.all(np.isnan(got[:, ~m]))
+    assert_close(got[:, m], expected[:, m], dtype, f"fmexp windows {ranges} {out_max}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("op", ["exp", "log"])
+@pytest.mark.parametrize("aw", [(1, 1), (0, 99), (2, 3)], ids=["deg1", "full", "deg23"])
+def test_ft_exp_log(dev, shape, op, aw, dtype):
+    w, N = shape
+    aw = (aw[0], min(aw[1], N))
+    if aw[0] > N:
+        pytest.skip("window beyond depth")
+    rng = np.random.default_rng(17 * w + N)
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    n = 4
+    arg = damped_batch(rng, n, w, N, dtype)
+    expected = np.full((n, dim), np.nan, dtype=dtype)
+    getattr(O.orc, "ft_" + op)(w, N, expected, arg, aw=aw)
+    out = to_dev(np.full((n, dim), np.nan, dtype=dtype), dev)
+    getattr(rpp_hip, "ft_" + op)(out, rpp_hip.Batch(to_dev(arg, dev), *aw), basis)
+    assert_close(out.cpu().numpy(), expected, dtype, f"{op} {shape} {aw}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("op", ["exp", "log"])
+@pytest.mark.parametrize("ow", [(0, 2), (1, 3), (2, 4), (0, 0)])
+def test_ft_exp_log_output_windows(dev, op, ow, dtype):
+    w, N = 3, 4
+    rng = np.random.default_rng(4)
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    n = 3
+    arg = damped_batch(rng, n, w, N, dtype)
+    init = np.full((n, dim), np.nan, dtype=dtype)
+    expected = init.copy()
+    getattr(O.orc, "ft_" + op)(w, N, expected, arg, ow=ow)
+    out = to_dev(init, dev)
+    getattr(rpp_hip, "ft_" + op)(rpp_hip.Batch(out, *ow), to_dev(arg, dev), basis)
+    got = out.cpu().numpy()
+    m = window_mask(w, N, ow)
+    assert np.all(np.isnan(got[:, ~m]))
+    assert_close(got[:, m], expected[:, m], dtype, f"{op} out {ow}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("windows", [((0, 4), (0, 4)), ((1, 3), (2, 4)), ((3, 4), (0, 2)), ((2, 2), (2, 2))])
+def test_tensor_pairing(dev, windows, dtype):
+    fw, xw = windows
+    w, N = 3, 4
+    rng = np.random.default_rng(21)
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    n = 9
+    f = random_batch(rng, n, dim, dtype)
+    x = random_batch(rng, n, dim, dtype)
+    expected = np.full(n, np.nan, dtype=dtype)
+    O.orc.tensor_pairing(w, N, expected, f, x, fw=fw, xw=xw)
+    out = to_dev(np.full(n, np.nan, dtype=dtype), dev)
+    rpp_hip.tensor_pairing(out, rpp_hip.Batch(to_dev(f, dev), *fw), rpp_hip.Batch(to_dev(x, dev), *xw), basis)
+    got = out.cpu().numpy()
+    if fw[0] > xw[1] or xw[0] > fw[1]:
+        assert np.all(got == 0)  # R8: empty overlap writes 0
+    assert_close(got, expected, dtype, f"pairing {windows}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("side", ["lmul", "rmul"])
+@pytest.mark.parametrize("windows", [((0, 4), (0, 4), (0, 4)), ((1, 3), (0, 2), (2, 4)),
+                                     ((0, 2), (1, 1), (1, 4)), ((2, 4), (0, 0), (0, 4))])
+def test_adjoint_multiplications(dev, side, windows, dtype):
+    ow, pw, gw = windows
+    w, N = 3, 4
+    rng = np.random.default_rng(31)
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    n = 4
+    op = random_batch(rng, n, dim, dtype)
+    arg = random_batch(rng, n, dim, dtype)
+    init = np.full((n, dim), np.nan, dtype=dtype)
+    expected = init.copy()
+    getattr(O.orc, "ft_adj_" + side)(w, N, expected, op, arg, ow=ow, pw=pw, gw=gw)
+    out = to_dev(init, dev)
+    getattr(rpp_hip, "ft_adj_" + side)(rpp_hip.Batch(out, *ow), rpp_hip.Batch(to_dev(op, dev), *pw),
+                                       rpp_hip.Batch(to_dev(arg, dev), *gw), basis)
+    got = out.cpu().numpy()
+    m = window_mask(w, N, ow)
+    assert np.all(np.isnan(got[:, ~m]))
+    assert_close(got[:, m], expected[:, m], dtype, f"adj_{side} {windows}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_chained_fmexp_c2_shape(dev, dtype):
+    """BASELINE C2 shape (width 5, depth 6), 64 paths x 16 increments of the
+    plain chain, against the oracle chain."""
+    w, N, n, L = 5, 6, 64, 16
+    rng = np.random.default_rng(0xC2)
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    inc = rng.normal(0.0, np.sqrt(1.0 / 128), size=(n, L, w))
+    expected = O.signature(w, N, inc.reshape(n * L, w), np.arange(n + 1) * L, dtype=dtype)
+    padded = np.zeros((n, 1 + L * w), dtype=dtype)
+    padded[:, 1:] = inc.reshape(n, L * w)
+    pad_d = to_dev(padded, dev)
+    cur = torch.zeros((n, dim), dtype=pad_d.dtype, device=dev)
+    cur[:, 0] = 1
+    nxt = torch.empty_like(cur)
+    for t in range(L):
+        rpp_hip.ft_fmexp(nxt, cur, rpp_hip.Batch(pad_d[:, t * w: t * w + 1 + w], 1, 1), basis)
+        cur, nxt = nxt, cur
+    assert_close(cur.cpu().numpy(), expected, dtype, "C2 chain")

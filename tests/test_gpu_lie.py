@@ -1,0 +1,96 @@
+"""GPU parity of sparse_matrix_vector_product, tensor_to_lie and lie_to_tensor."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import rpp_hip
+from helpers import assert_close, damped_batch
+
+pytestmark = pytest.mark.gpu
+
+
+def to_dev(x, dev):
+    return torch.tensor(np.ascontiguousarray(x), device=dev)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("fmt", ["csr", "csc"])
+def test_spmv_view_relative(dev, fmt, dtype):
+    """R9: row 0 <-> out.begin(), column 0 <-> arg.begin()
+    (sparse_matrix_vector_product_tests.cpp:192-226)."""
+    rng = np.random.default_rng(3)
+    rows, cols, n = 7, 9, 5
+    dense = rng.uniform(-1, 1, size=(rows, cols)) * (rng.uniform(size=(rows, cols)) < 0.4)
+    if fmt == "csr":
+        offs = np.concatenate([[0], np.cumsum((dense != 0).sum(axis=1))])
+        idx = np.concatenate([np.nonzero(dense[r])[0] for r in range(rows)])
+        vals = np.concatenate([dense[r][dense[r] != 0] for r in range(rows)])
+    else:
+        offs = np.concatenate([[0], np.cumsum((dense != 0).sum(axis=0))])
+        idx = np.concatenate([np.nonzero(dense[:, c])[0] for c in range(cols)])
+        vals = np.concatenate([dense[:, c][dense[:, c] != 0] for c in range(cols)])
+    out_begin, arg_begin, width_out, width_arg = 3, 2, 12, 14
+    arg = rng.uniform(-1, 1, size=(n, width_arg)).astype(dtype)
+    init = np.full((n, width_out), 7.0, dtype=dtype)
+    expected = init.copy()
+    O.spmv(vals, idx, offs, rows if fmt == "csr" else cols, expected, arg, csc=(fmt == "csc"),
+           out_offset=out_begin, out_size=rows, arg_offset=arg_begin, alpha=0.5)
+    M = rpp_hip.SparseMatrix(fmt, to_dev(vals.astype(dtype), dev), to_dev(idx.astype(np.int64), dev),
+                             to_dev(offs.astype(np.int64), dev), rows, cols)
+    out = to_dev(init, dev)
+    rpp_hip.sparse_matrix_vector_product(out, to_dev(arg, dev), M, out_begin=out_begin, arg_begin=arg_begin,
+                                         out_size=rows, alpha=0.5)
+    got = out.cpu().numpy()
+    assert np.all(got[:, :out_begin] == 7) and np.all(got[:, out_begin + rows:] == 7)
+    assert_close(got, expected, dtype, f"spmv {fmt}")
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", [(2, 4), (3, 4), (3, 8), (4, 5), (5, 3)])
+def test_tensor_to_lie_and_back(dev, shape, dtype):
+    w, N = shape
+    rng = np.random.default_rng(w + N)
+    basis = rpp_hip.TensorBasis(w, N)
+    lie = rpp_hip.lie_basis(w, N)
+    ts, dim = lie["true_size"], lie["tensor_dim"]
+    n = 6
+    x = damped_batch(rng, n, w, N, dtype)
+    # oracle: CSC products with the (reference-identical) matrices
+    t2l = lie["t2l"]
+    exp_lie = np.zeros((n, ts), dtype=dtype)
+    O.spmv(t2l["data"], t2l["indices"], t2l["offsets"], dim, exp_lie, x, csc=True, out_offset=1,
+           out_size=ts - 1)
+    lie_out = to_dev(np.zeros((n, ts), dtype=dtype), dev)
+    rpp_hip.tensor_to_lie(lie_out, to_dev(x, dev), basis)
+    got_lie = lie_out.cpu().numpy()
+    assert_close(got_lie, exp_lie, dtype, f"t2l {shape}")
+    l2t = lie["l2t"]
+    exp_t = np.zeros((n, dim), dtype=dtype)
+    O.spmv(l2t["data"], l2t["indices"], l2t["offsets"], ts - 1, exp_t, got_lie, csc=True, out_offset=0,
+           out_size=dim, arg_offset=1)
+    t_out = to_dev(np.full((n, dim), np.nan, dtype=dtype), dev)
+    rpp_hip.lie_to_tensor(t_out, lie_out, basis)
+    assert_close(t_out.cpu().numpy(), exp_t, dtype, f"l2t {shape}")
+
+
+def test_log_signature_pipeline_matches_reference(dev):
+    """C3 pipeline at (3,8): signature -> ft_log -> tensor_to_lie, against the
+    reference ops (oracle/_ref) where built."""
+    if not O.has_ref():
+        pytest.skip("reference driver not built on this machine")
+    w, N, n, L = 3, 8, 4, 64
+    rng = np.random.default_rng(0xC3)
+    inc = rng.normal(0, np.sqrt(1 / 64), size=(n, L, w))
+    basis = rpp_hip.TensorBasis(w, N)
+    sig_ref = O.ref_signature(w, N, inc)
+    log_ref = np.zeros_like(sig_ref)
+    O.refimpl.ft_log(w, N, log_ref, sig_ref)
+    lie_ref = O.ref_lie_convert(True, True, w, N, log_ref)
+    sig = rpp_hip.signature(torch.tensor(inc.reshape(n * L, w), device=dev),
+                            torch.arange(n + 1, device=dev, dtype=torch.int64) * L, basis)
+    lg = torch.empty_like(sig)
+    rpp_hip.ft_log(lg, sig, basis)
+    lie = torch.zeros((n, lie_ref.shape[1]), dtype=torch.float64, device=dev)
+    rpp_hip.tensor_to_lie(lie, lg, basis)
+    assert_close(lie.cpu().numpy()[:, 1:], lie_ref[:, 1:], np.float64, "C3 pipeline")

@@ -1,0 +1,106 @@
+"""GPU parity of the batched Chen recursion (forward) and its backward pass."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import rpp_hip
+from helpers import assert_close, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def paths(rng, n, w, lo, hi, dtype=np.float64):
+    lengths = rng.integers(lo, hi + 1, size=n)
+    offsets = np.zeros(n + 1, dtype=np.int64)
+    offsets[1:] = np.cumsum(lengths)
+    inc = np.concatenate([rng.normal(0.0, np.sqrt(1.0 / max(L, 1)), size=(L, w)) for L in lengths]) \
+        if offsets[-1] > 0 else np.zeros((0, w))
+    return inc.astype(dtype), offsets
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", [(2, 5), (3, 4), (4, 4), (5, 3), (5, 6), (6, 3), (8, 3), (7, 3)])
+def test_signature_uneven(dev, shape, dtype):
+    w, N = shape
+    rng = np.random.default_rng(w * 100 + N)
+    inc, off = paths(rng, 13, w, 0, 9, dtype)
+    basis = rpp_hip.TensorBasis(w, N)
+    expected = O.signature(w, N, inc.astype(np.float64), off)
+    got = rpp_hip.signature(torch.tensor(inc, device=dev), torch.tensor(off, device=dev), basis)
+    assert_close(got.cpu().numpy(), expected.astype(dtype) if dtype == np.float64 else expected,
+                 dtype, f"signature {shape}")
+
+
+def test_signature_matches_reference_chain_fp64(dev):
+    """Same inputs as the reference's own chain of ft_fmexp (oracle/_ref when built)."""
+    if not O.has_ref():
+        pytest.skip("reference driver not built on this machine")
+    w, N, n, L = 4, 4, 8, 12
+    rng = np.random.default_rng(1)
+    inc = rng.normal(0, 0.3, size=(n, L, w))
+    expected = O.ref_signature(w, N, inc)
+    got = rpp_hip.signature(torch.tensor(inc.reshape(n * L, w), device=dev),
+                            torch.arange(n + 1, device=dev, dtype=torch.int64) * L, rpp_hip.TensorBasis(w, N))
+    assert_close(got.cpu().numpy(), expected, np.float64, "vs reference chain")
+
+
+@pytest.mark.parametrize("shape", [(2, 5), (3, 4), (4, 3), (5, 3), (6, 2)])
+@pytest.mark.parametrize("ckpt", [0, 3])
+def test_signature_backward_fp64(dev, shape, ckpt):
+    w, N = shape
+    rng = np.random.default_rng(5 + w * 7 + N)
+    n = 6
+    inc, off = paths(rng, n, w, 1, 8)
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    ell = rng.normal(size=dim)
+    expected = O.signature_vjp(w, N, inc, off, ell)
+    inc_d = torch.tensor(inc, device=dev)
+    off_d = torch.tensor(off, device=dev)
+    kwargs = {}
+    if ckpt:
+        lengths = np.diff(off)
+        nck = lengths // ckpt
+        ck_off = np.zeros(n + 1, dtype=np.int64)
+        ck_off[1:] = np.cumsum(nck)
+        ckpts = torch.zeros((max(int(ck_off[-1]), 1), dim), dtype=torch.float64, device=dev)
+        kwargs = dict(checkpoint_every=ckpt, checkpoints=ckpts, ckpt_offsets=torch.tensor(ck_off, device=dev))
+    sig = rpp_hip.signature(inc_d, off_d, basis, **kwargs)
+    grad = rpp_hip.signature_backward(inc_d, off_d, basis, sig, torch.tensor(ell, device=dev), **kwargs)
+    err = rel_err(grad.cpu().numpy(), expected)
+    assert err < 1e-10, f"backward {shape} ckpt={ckpt}: {err:.3e}"
+
+
+def test_signature_backward_finite_differences(dev):
+    """Central differences of <ell, S(path)> in FP64 on the oracle forward."""
+    w, N, L = 3, 4, 5
+    rng = np.random.default_rng(99)
+    inc = rng.normal(0, 0.4, size=(L, w))
+    off = np.array([0, L], dtype=np.int64)
+    basis = rpp_hip.TensorBasis(w, N)
+    ell = rng.normal(size=basis.size())
+    h = 1e-6
+    fd = np.zeros_like(inc)
+    for t in range(L):
+        for a in range(w):
+            p = inc.copy(); p[t, a] += h
+            m = inc.copy(); m[t, a] -= h
+            fd[t, a] = (O.signature(w, N, p, off)[0] @ ell - O.signature(w, N, m, off)[0] @ ell) / (2 * h)
+    inc_d = torch.tensor(inc, device=dev)
+    off_d = torch.tensor(off, device=dev)
+    sig = rpp_hip.signature(inc_d, off_d, basis)
+    grad = rpp_hip.signature_backward(inc_d, off_d, basis, sig, torch.tensor(ell, device=dev))
+    assert rel_err(grad.cpu().numpy(), fd) < 1e-7
+
+
+def test_signature_empty_paths(dev):
+    w, N = 3, 3
+    basis = rpp_hip.TensorBasis(w, N)
+    off = torch.tensor([0, 0, 2, 2], dtype=torch.int64, device=dev)
+    inc = torch.tensor(np.random.default_rng(0).normal(size=(2, w)), device=dev)
+    sig = rpp_hip.signature(inc, off, basis).cpu().numpy()
+    unit = np.zeros(basis.size()); unit[0] = 1
+    assert np.array_equal(sig[0], unit) and np.array_equal(sig[2], unit)
+    expected = O.signature(w, N, inc.cpu().numpy(), off.cpu().numpy())
+    assert_close(sig, expected, np.float64, "empty paths")
