@@ -270,7 +270,9 @@ int rpp_hip_signature_f64(const rpp_hip_signature_desc* desc, void* stream);
  * (device [total, width]).  Intermediate states are reconstructed backward
  * with S_{t-1} = S_t (x) exp(-dX_t); when the forward pass stored checkpoints
  * the reconstruction is re-anchored at each of them (sqrt(L) checkpointing).
- * scratch: device workspace of rpp_hip_signature_backward_scratch_bytes(). */
+ * scratch: device workspace of rpp_hip_signature_backward_scratch_bytes() for
+ * the storage-generic kernel (NULL: library workspace).  tile_threads /
+ * tile_split select a register tile as for the forward pass (split <= 1). */
 typedef struct rpp_hip_signature_backward_desc {
     rpp_hip_basis basis;
     int64_t num_paths;
@@ -283,11 +285,21 @@ typedef struct rpp_hip_signature_backward_desc {
     const void* checkpoints;
     const int64_t* ckpt_offsets;
     void* scratch;
+    int32_t tile_threads;
+    int32_t tile_split;
 } rpp_hip_signature_backward_desc;
 int64_t rpp_hip_signature_backward_scratch_bytes(const rpp_hip_signature_backward_desc* desc,
                                                  int32_t dtype_bytes);
 int rpp_hip_signature_backward_f32(const rpp_hip_signature_backward_desc* desc, void* stream);
 int rpp_hip_signature_backward_f64(const rpp_hip_signature_backward_desc* desc, void* stream);
+
+/* The compiled register tiles of rpp_hip_signature_*: writes up to
+ * `capacity` quadruples (width, depth, threads, split) for the dtype (4 or 8
+ * bytes) and returns how many exist.  A descriptor selects one with
+ * tile_threads/tile_split; tile_threads = 0 takes the library default for
+ * (width, depth), tile_threads < 0 the storage-generic kernel.  No reference
+ * counterpart: the reference has no batched signature entry point. */
+int rpp_hip_signature_tiles(int32_t dtype_bytes, int32_t* out, int32_t capacity);
 
 /* ------------------------------------------------------------------ misc */
 

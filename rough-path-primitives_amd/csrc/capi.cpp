@@ -13,6 +13,7 @@
 #include "ft_ops.hpp"
 #include "lie.hpp"
 #include "signature.hpp"
+#include "signature_reg.hpp"
 
 namespace rpp_hip {
 
@@ -348,6 +349,19 @@ const char* rpp_hip_version(void) { return "rpp_hip 0.1.0 (gfx950)"; }
 namespace rpp_hip {
 namespace {
 
+// The compiled register tile for (width, depth): the requested one
+// (threads > 0), else the first listed; backward tiles fix at most one letter.
+template <typename T>
+const RegTile<T>* pick_tile(const rpp_hip_basis& b, int threads, int split, bool backward) {
+    for (const auto& t : reg_tiles<T>()) {
+        if (t.width != b.width || t.depth != b.depth) continue;
+        if (backward && (t.backward == nullptr || t.split > 1)) continue;
+        if (threads > 0 && (t.threads != threads || t.split != split)) continue;
+        return &t;
+    }
+    return nullptr;
+}
+
 template <typename T>
 int signature_impl(const rpp_hip_signature_desc* d, void* s) {
     if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
@@ -366,6 +380,13 @@ int signature_impl(const rpp_hip_signature_desc* d, void* s) {
     std::vector<int64_t> lengths;
     int64_t* order = nullptr;
     RPP_TRY(path_order(d->offsets, d->num_paths, lengths, &order, stream));
+    if (d->tile_threads < 0) return signature_forward_generic<T>(B, *d, order, stream);
+    if (const RegTile<T>* tile = pick_tile<T>(d->basis, d->tile_threads, d->tile_split, false)) {
+        void* cnt = nullptr;
+        RPP_TRY(workspace(kSlotCounter, 256, &cnt));
+        return tile->forward(B, *d, order, static_cast<unsigned*>(cnt), stream);
+    }
+    if (d->tile_threads > 0) return fail(RPP_HIP_NOT_IMPLEMENTED, "no compiled tile matches the requested one");
     return signature_forward_generic<T>(B, *d, order, stream);
 }
 
@@ -390,6 +411,16 @@ int signature_backward_impl(const rpp_hip_signature_backward_desc* d, void* s) {
     std::vector<int64_t> lengths;
     int64_t* order = nullptr;
     RPP_TRY(path_order(d->offsets, d->num_paths, lengths, &order, stream));
+    if (d->scratch == nullptr && d->tile_threads >= 0) {
+        if (const RegTile<T>* tile = pick_tile<T>(d->basis, d->tile_threads, d->tile_split, true)) {
+            int64_t total = 0;
+            for (int64_t L : lengths) total += L;
+            void* cnt = nullptr;
+            RPP_TRY(workspace(kSlotCounter, 256, &cnt));
+            return tile->backward(B, *d, order, total, static_cast<unsigned*>(cnt), stream);
+        }
+        if (d->tile_threads > 0) return fail(RPP_HIP_NOT_IMPLEMENTED, "no compiled backward tile matches");
+    }
     return signature_backward_generic<T>(B, *d, order, stream);
 }
 
@@ -415,3 +446,22 @@ int64_t rpp_hip_signature_backward_scratch_bytes(const rpp_hip_signature_backwar
 }
 
 }  // extern "C"
+
+extern "C" int rpp_hip_signature_tiles(int32_t dtype_bytes, int32_t* out, int32_t capacity) {
+    int n = 0;
+    auto emit = [&](const auto& tiles) {
+        for (const auto& t : tiles) {
+            if (n < capacity && out) {
+                out[4 * n + 0] = t.width;
+                out[4 * n + 1] = t.depth;
+                out[4 * n + 2] = t.threads;
+                out[4 * n + 3] = t.split;
+            }
+            ++n;
+        }
+    };
+    if (dtype_bytes == 4) emit(rpp_hip::reg_tiles<float>());
+    else if (dtype_bytes == 8) emit(rpp_hip::reg_tiles<double>());
+    else return rpp_hip::fail(RPP_HIP_INVALID_ARGUMENT, "dtype_bytes must be 4 or 8");
+    return n;
+}

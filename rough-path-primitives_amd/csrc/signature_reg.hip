@@ -1,0 +1,434 @@
+// Register-resident batched Chen recursion: the same Horner scheme as
+// signature.hip, with each path's state held in VGPRs across one work-group
+// and the Horner intermediates exchanged through LDS.
+//
+// Tile (compile time): width W, depth N, NT threads, and M leading letters
+// fixed per work-group.  A work item is (path, prefix q of M letters); it
+// owns every word of degree > M that starts with q (the "subtree"), plus the
+// single word q of degree M (the "root") and S_0.  A Chen step never mixes
+// subtrees: A_i[q u] depends only on A_{i-1}[q u'] and the chain
+// A_0 .. A_M along q, so the subtrees of one path advance independently.
+//
+// Ownership inside a subtree: the words of level i > M are grouped by their
+// parent (a word of level i-1); parent p is owned by thread p mod NT, which
+// holds all W children p*W + b in consecutive registers.  One LDS read of
+// A_{i-1}[p] feeds W fused multiply-adds; A_i goes back to LDS for the next
+// level, except at i = k where it replaces S_k in place.
+//
+// Backward (M <= 1): S and the cotangent G share the layout.  For each target
+// level k (ascending) the forward Horner values A_{i-1}[p] each thread needs
+// are kept in registers, and the reverse sweep exchanges Abar through LDS.
+#include "common.hpp"
+#include "signature.hpp"
+#include "signature_reg.hpp"
+
+#include <algorithm>
+#include <utility>
+
+namespace rpp_hip {
+
+namespace {
+
+constexpr int ipow(int b, int e) {
+    int r = 1;
+    for (int i = 0; i < e; ++i) r *= b;
+    return r;
+}
+
+template <int W, int N, int NT, int M>
+struct Geo {
+    // parents of level i (> M) inside one subtree: words of level i-1 under q
+    static constexpr int parents(int i) { return ipow(W, i - 1 - M); }
+    static constexpr int slots(int i) { return (parents(i) + NT - 1) / NT; }
+    // register offset of level i's first slot
+    static constexpr int offset(int i) {
+        int o = 0;
+        for (int j = M + 1; j < i; ++j) o += slots(j) * W;
+        return o;
+    }
+    static constexpr int total = offset(N + 1);
+    static constexpr int apar_offset(int i) {
+        int o = 0;
+        for (int j = M + 1; j < i; ++j) o += slots(j);
+        return o;
+    }
+    static constexpr int apar_total = apar_offset(N + 1);
+    // largest LDS exchange: the parents of level N
+    static constexpr int lds_elems = N > M ? parents(N) : 1;
+    static constexpr int P = ipow(W, M);
+    // global word index of subtree word (level i, local r) for prefix q
+    __device__ __forceinline__ static int64_t global_index(const DevBasis& B, int i, int64_t q, int64_t r) {
+        return B.start[i] + q * static_cast<int64_t>(ipow(W, i - M)) + r;
+    }
+};
+
+template <typename T, int W, int N, int NT, int M>
+struct Tile {
+    using G = Geo<W, N, NT, M>;
+    T reg[G::total > 0 ? G::total : 1];
+    T root;  // S_M[q] (for M = 0 this is S_0)
+    T s0;    // S_0
+
+    __device__ __forceinline__ void set_identity() {
+#pragma unroll
+        for (int i = 0; i < G::total; ++i) reg[i] = T(0);
+        root = M == 0 ? T(1) : T(0);
+        s0 = T(1);
+    }
+
+    // Store (out = true) or load the subtree words and the chain words this
+    // work item owns.  Object base `o`, coefficient stride cs.
+    template <bool STORE>
+    __device__ __forceinline__ void move(const DevBasis& B, T* o, int64_t cs, int q, bool owns_chain) {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int i = M + 1; i <= N; ++i) {
+#pragma unroll
+            for (int s = 0; s < G::slots(i); ++s) {
+                const int p = t + s * NT;
+                if (p < G::parents(i)) {
+#pragma unroll
+                    for (int b = 0; b < W; ++b) {
+                        const int64_t gi = G::global_index(B, i, q, static_cast<int64_t>(p) * W + b);
+                        if constexpr (STORE) {
+                            o[gi * cs] = reg[G::offset(i) + s * W + b];
+                        } else {
+                            reg[G::offset(i) + s * W + b] = o[gi * cs];
+                        }
+                    }
+                }
+            }
+        }
+        if constexpr (STORE) {
+            if (t == 0) {
+                if (M == 0) {
+                    if (owns_chain) o[0] = root;
+                } else {
+                    o[G::global_index(B, M, q, 0) * cs] = root;
+                    if (owns_chain) o[0] = s0;
+                }
+            }
+        } else {
+            root = M == 0 ? o[0] : o[G::global_index(B, M, q, 0) * cs];
+            s0 = o[0];
+        }
+    }
+};
+
+// One in-place Chen step S <- S (x) exp(x) on the tile.  xs: W increments
+// (uniform), inv[m] = 1/m, lds: two buffers of G::lds_elems.
+template <typename T, int W, int N, int NT, int M>
+__device__ __forceinline__ void chen_step_reg(Tile<T, W, N, NT, M>& S, const T* xs, const T* inv, T* lds0,
+                                              T* lds1, int q) {
+    using G = Geo<W, N, NT, M>;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = N; k >= M + 1; --k) {
+        // chain A_0 .. A_M along q (uniform); M <= 1 here
+        T a = S.root;
+        if (M == 1) a = S.s0 * (xs[q] * inv[k]) + S.root;
+#pragma unroll
+        for (int i = M + 1; i <= k; ++i) {
+            const T c = inv[k - i + 1];
+            T xc[W];
+#pragma unroll
+            for (int b = 0; b < W; ++b) xc[b] = xs[b] * c;
+            const T* prev = ((i - M) & 1) ? lds1 : lds0;
+            T* cur = ((i - M) & 1) ? lds0 : lds1;
+#pragma unroll
+            for (int s = 0; s < G::slots(i); ++s) {
+                const int p = t + s * NT;
+                if (p < G::parents(i)) {
+                    const T ap = (i == M + 1) ? a : prev[p];
+#pragma unroll
+                    for (int b = 0; b < W; ++b) {
+                        T& sv = S.reg[G::offset(i) + s * W + b];
+                        const T v = ap * xc[b] + sv;
+                        if (i == k) {
+                            sv = v;
+                        } else {
+                            cur[p * W + b] = v;
+                        }
+                    }
+                }
+            }
+            if (i < k) __syncthreads();
+        }
+        __syncthreads();
+    }
+    // target level M: the root (chain) word
+    if (M == 1) S.root = S.s0 * xs[q] + S.root;
+}
+
+template <typename T, int W, int N, int NT, int M>
+__global__ void __launch_bounds__(NT)
+k_sig_fwd_reg(DevBasis B, int64_t n_items, const int64_t* order, const T* incr, const int64_t* offs,
+              View<T> out, int ck_every, T* ckpts, const int64_t* ck_offs, unsigned* counter) {
+    using G = Geo<W, N, NT, M>;
+    __shared__ T lds0[G::lds_elems];
+    __shared__ T lds1[G::lds_elems];
+    __shared__ T xs[W];
+    __shared__ T inv[N + 2];
+    __shared__ unsigned slot;
+    if (threadIdx.x <= N + 1) inv[threadIdx.x] = threadIdx.x == 0 ? T(1) : T(1) / T(threadIdx.x);
+    const int64_t dim = B.start[N + 1];
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) slot = atomicAdd(counter, 1u);
+        __syncthreads();
+        const int64_t item = slot;
+        if (item >= n_items) break;
+        const int64_t path = order ? order[item / G::P] : item / G::P;
+        const int q = static_cast<int>(item % G::P);
+        Tile<T, W, N, NT, M> S;
+        S.set_identity();
+        const int64_t t0 = offs[path], t1 = offs[path + 1];
+        for (int64_t t = t0; t < t1; ++t) {
+            if (threadIdx.x < W) xs[threadIdx.x] = incr[t * W + threadIdx.x];
+            __syncthreads();
+            chen_step_reg<T, W, N, NT, M>(S, xs, inv, lds0, lds1, q);
+            if (ck_every > 0 && ((t - t0 + 1) % ck_every) == 0) {
+                T* dst = ckpts + (ck_offs[path] + (t - t0 + 1) / ck_every - 1) * dim;
+                S.template move<true>(B, dst, 1, q, q == 0);
+            }
+        }
+        S.template move<true>(B, out.obj(path), out.cs, q, q == 0);
+    }
+}
+
+// ------------------------------------------------------------------ backward
+
+// Reverse-mode of the Horner scheme of one target level K (see the file
+// comment): recompute A_{i-1}[p] for the owned parents, then sweep i = K..M+1.
+template <int K, typename T, int W, int N, int NT, int M>
+__device__ __forceinline__ void bwd_target(Tile<T, W, N, NT, M>& S, Tile<T, W, N, NT, M>& Gt, const T* xs,
+                                           const T* inv, T* lds0, T* lds1, int q, T (&xbar)[W], T& xq) {
+    using G = Geo<W, N, NT, M>;
+    const int t = threadIdx.x;
+    constexpr int k = K;
+
+    // forward Horner of target k, keeping A_{i-1}[p] per owned parent
+    T apar[G::apar_total > 0 ? G::apar_total : 1];
+    const T a_chain = (M == 1) ? S.s0 * (xs[q] * inv[k]) + S.root : S.root;
+#pragma unroll
+    for (int i = M + 1; i <= k; ++i) {
+        const T c = inv[k - i + 1];
+        const T* prev = ((i - M) & 1) ? lds1 : lds0;
+        T* cur = ((i - M) & 1) ? lds0 : lds1;
+#pragma unroll
+        for (int s = 0; s < G::slots(i); ++s) {
+            const int p = t + s * NT;
+            if (p < G::parents(i)) {
+                const T ap = (i == M + 1) ? a_chain : prev[p];
+                apar[G::apar_offset(i) + s] = ap;
+                if (i < k) {
+#pragma unroll
+                    for (int b = 0; b < W; ++b)
+                        cur[p * W + b] = ap * (xs[b] * c) + S.reg[G::offset(i) + s * W + b];
+                }
+            }
+        }
+        if (i < k) __syncthreads();
+    }
+    __syncthreads();
+    // reverse sweep: Abar_k = G_k; Abar_{i-1}[p] = c sum_b Abar_i[pb] x[b]
+#pragma unroll
+    for (int i = k; i >= M + 1; --i) {
+        const T c = inv[k - i + 1];
+        const T* src = ((k - i) & 1) ? lds1 : lds0;  // Abar_i, when i < k
+        T* dst = ((k - i) & 1) ? lds0 : lds1;        // Abar_{i-1}
+#pragma unroll
+        for (int s = 0; s < G::slots(i); ++s) {
+            const int p = t + s * NT;
+            if (p < G::parents(i)) {
+                const T apc = apar[G::apar_offset(i) + s] * c;
+                T acc = T(0);
+#pragma unroll
+                for (int b = 0; b < W; ++b) {
+                    T g;
+                    if (i == k) {
+                        g = Gt.reg[G::offset(i) + s * W + b];
+                    } else {
+                        g = src[p * W + b];
+                        Gt.reg[G::offset(i) + s * W + b] += g;
+                    }
+                    xbar[b] += g * apc;
+                    acc += g * xs[b];
+                }
+                acc *= c;
+                if (i > M + 1) {
+                    dst[p] = acc;
+                } else if (M == 1) {
+                    // Abar of the root word q (level 1)
+                    const T ab = acc;
+                    if (k > 1) Gt.root += ab;
+                    xq += ab * S.s0 * inv[k];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int W, int N, int NT, int M, int... Ks>
+__device__ __forceinline__ void bwd_targets(Tile<T, W, N, NT, M>& S, Tile<T, W, N, NT, M>& Gt, const T* xs,
+                                            const T* inv, T* lds0, T* lds1, int q, T (&xbar)[W], T& xq,
+                                            std::integer_sequence<int, Ks...>) {
+    (bwd_target<M + 1 + Ks, T, W, N, NT, M>(S, Gt, xs, inv, lds0, lds1, q, xbar, xq), ...);
+}
+
+template <typename T, int W, int N, int NT, int M>
+__global__ void __launch_bounds__(NT)
+k_sig_bwd_reg(DevBasis B, int64_t n_items, const int64_t* order, const T* incr, const int64_t* offs,
+              View<const T> sigs, View<const T> cot, T* partial, int ck_every, const T* ckpts,
+              const int64_t* ck_offs, unsigned* counter) {
+    static_assert(M <= 1, "the backward pass keeps chain cotangents inside one work item");
+    using G = Geo<W, N, NT, M>;
+    __shared__ T lds0[G::lds_elems];
+    __shared__ T lds1[G::lds_elems];
+    __shared__ T xs[W];
+    __shared__ T xneg[W];
+    __shared__ T inv[N + 2];
+    __shared__ T red[NT / 64][W];
+    __shared__ unsigned slot;
+    if (threadIdx.x <= N + 1) inv[threadIdx.x] = threadIdx.x == 0 ? T(1) : T(1) / T(threadIdx.x);
+    const int t = threadIdx.x;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) slot = atomicAdd(counter, 1u);
+        __syncthreads();
+        const int64_t item = slot;
+        if (item >= n_items) break;
+        const int64_t path = order ? order[item / G::P] : item / G::P;
+        const int q = static_cast<int>(item % G::P);
+        Tile<T, W, N, NT, M> S, Gt;
+        S.template move<false>(B, const_cast<T*>(sigs.obj(path)), sigs.cs, q, true);
+        Gt.template move<false>(B, const_cast<T*>(cot.obj(path)), cot.cs, q, true);
+        const int64_t t0 = offs[path], t1 = offs[path + 1];
+        for (int64_t tt = t1 - 1; tt >= t0; --tt) {
+            if (threadIdx.x < W) {
+                const T v = incr[tt * W + threadIdx.x];
+                xs[threadIdx.x] = v;
+                xneg[threadIdx.x] = -v;
+            }
+            __syncthreads();
+            const int64_t tl = tt - t0;
+            if (tl == 0) {
+                S.set_identity();
+            } else if (ck_every > 0 && (tl % ck_every) == 0) {
+                const T* src = ckpts + (ck_offs[path] + tl / ck_every - 1) * B.start[N + 1];
+                S.template move<false>(B, const_cast<T*>(src), 1, q, true);
+            } else {
+                chen_step_reg<T, W, N, NT, M>(S, xneg, inv, lds0, lds1, q);
+            }
+            T xbar[W];
+#pragma unroll
+            for (int b = 0; b < W; ++b) xbar[b] = T(0);
+            // Chain contribution to x[q] (M = 1), thread 0 only.  Target level
+            // 1 first: new S_1[q] = S_0 x[q] + S_1[q], seeded by the old G_1[q]
+            // before the higher targets add into it.
+            T xq = (M == 1 && t == 0) ? Gt.root * S.s0 : T(0);
+            bwd_targets<T, W, N, NT, M>(S, Gt, xs, inv, lds0, lds1, q, xbar, xq,
+                                        std::make_integer_sequence<int, (N > M ? N - M : 0)>{});
+            // target level M = 1: root A_1 = S_0 x[q] + S_1[q], seeded by G_1[q]
+            if (M == 1 && t == 0) {
+#pragma unroll
+                for (int b = 0; b < W; ++b)
+                    if (b == q) xbar[b] += xq;
+            }
+            // work-group reduction of xbar (deterministic order)
+#pragma unroll
+            for (int b = 0; b < W; ++b) {
+                T v = xbar[b];
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+                if ((t & 63) == 0) red[t >> 6][b] = v;
+            }
+            __syncthreads();
+            if (t < W) {
+                T acc = T(0);
+                for (int w = 0; w < NT / 64; ++w) acc += red[w][t];
+                partial[(tt * G::P + q) * W + t] = acc;
+            }
+        }
+    }
+}
+
+__global__ void k_reset(unsigned* c) {
+    if (threadIdx.x == 0) *c = 0u;
+}
+
+template <typename T, int W>
+__global__ void k_sum_partials(const T* partial, T* grad, int64_t total, int P) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= total * W) return;
+    const int64_t t = i / W;
+    const int b = static_cast<int>(i - t * W);
+    T acc = T(0);
+    for (int q = 0; q < P; ++q) acc += partial[(t * P + q) * W + b];
+    grad[i] = acc;
+}
+
+template <typename T, int W, int N, int NT, int M>
+int fwd_launch(const DevBasis& B, const rpp_hip_signature_desc& d, const int64_t* order, unsigned* counter,
+               hipStream_t stream) {
+    using G = Geo<W, N, NT, M>;
+    const int64_t items = d.num_paths * G::P;
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>(items, static_cast<int64_t>(device_cu_count()) * 2));
+    k_reset<<<1, 64, 0, stream>>>(counter);
+    k_sig_fwd_reg<T, W, N, NT, M><<<grid, NT, 0, stream>>>(
+        B, items, order, static_cast<const T*>(d.increments), d.offsets, make_view<T>(d.out), d.checkpoint_every,
+        static_cast<T*>(d.checkpoints), d.ckpt_offsets, counter);
+    return check_hip(hipGetLastError(), "k_sig_fwd_reg launch");
+}
+
+template <typename T, int W, int N, int NT, int M>
+int bwd_launch(const DevBasis& B, const rpp_hip_signature_backward_desc& d, const int64_t* order,
+               int64_t total_incr, unsigned* counter, hipStream_t stream) {
+    using G = Geo<W, N, NT, M>;
+    const int64_t items = d.num_paths * G::P;
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>(items, static_cast<int64_t>(device_cu_count()) * 2));
+    void* part = nullptr;
+    if (int rc = workspace(kSlotTmp, static_cast<size_t>(std::max<int64_t>(total_incr, 1)) * G::P * W * sizeof(T), &part))
+        return rc;
+    k_reset<<<1, 64, 0, stream>>>(counter);
+    k_sig_bwd_reg<T, W, N, NT, M><<<grid, NT, 0, stream>>>(
+        B, items, order, static_cast<const T*>(d.increments), d.offsets, make_cview<T>(d.signatures),
+        make_cview<T>(d.cotangent), static_cast<T*>(part), d.checkpoint_every, static_cast<const T*>(d.checkpoints),
+        d.ckpt_offsets, counter);
+    if (int rc = check_hip(hipGetLastError(), "k_sig_bwd_reg launch")) return rc;
+    if (total_incr > 0) {
+        const int64_t n = total_incr * W;
+        k_sum_partials<T, W><<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(
+            static_cast<const T*>(part), static_cast<T*>(d.grad_increments), total_incr, G::P);
+    }
+    return check_hip(hipGetLastError(), "k_sum_partials launch");
+}
+
+}  // namespace
+
+// The compiled tiles.  Each entry: dtype bytes, width, depth, threads,
+// letters fixed per work-group, forward/backward launchers.
+#define RPP_TILE(T, W, N, NT, M)                                                                   \
+    {sizeof(T), W, N, NT, M, &fwd_launch<T, W, N, NT, M>, M <= 1 ? &bwd_launch<T, W, N, NT, (M <= 1 ? M : 1)> : nullptr}
+
+template <typename T>
+const std::vector<RegTile<T>>& reg_tiles() {
+    static const std::vector<RegTile<T>> tiles = {
+        // BASELINE C2: width 5, depth 6
+        RPP_TILE(T, 5, 6, 512, 0), RPP_TILE(T, 5, 6, 1024, 0), RPP_TILE(T, 5, 6, 640, 1), RPP_TILE(T, 5, 6, 320, 1),
+        // BASELINE C5: width 6, depth 6
+        RPP_TILE(T, 6, 6, 448, 1), RPP_TILE(T, 6, 6, 704, 1), RPP_TILE(T, 6, 6, 1024, 1),
+        // BASELINE C4: width 8, depth 5
+        RPP_TILE(T, 8, 5, 512, 1), RPP_TILE(T, 8, 5, 256, 1),
+        // BASELINE C3: width 3, depth 8
+        RPP_TILE(T, 3, 8, 768, 0), RPP_TILE(T, 3, 8, 256, 1), RPP_TILE(T, 3, 8, 512, 1),
+        // BASELINE C1 shape: width 4, depth 4
+        RPP_TILE(T, 4, 4, 64, 0), RPP_TILE(T, 4, 4, 128, 0),
+    };
+    return tiles;
+}
+
+template const std::vector<RegTile<float>>& reg_tiles<float>();
+template const std::vector<RegTile<double>>& reg_tiles<double>();
+
+}  // namespace rpp_hip

@@ -98,7 +98,7 @@ class SignatureBackwardDesc(C.Structure):
     _fields_ = [("basis", Basis), ("num_paths", _i64), ("increments", _vp), ("offsets", _vp),
                 ("signatures", TensorBatch), ("cotangent", TensorBatch), ("grad_increments", _vp),
                 ("checkpoint_every", _i32), ("checkpoints", _vp), ("ckpt_offsets", _vp),
-                ("scratch", _vp)]
+                ("scratch", _vp), ("tile_threads", _i32), ("tile_split", _i32)]
 
 
 # entry point -> descriptor type; every one also takes a stream pointer.
@@ -121,7 +121,7 @@ ENTRY_POINTS = {
 }
 
 OTHER_SYMBOLS = ["rpp_hip_lie_basis_sizes", "rpp_hip_lie_basis_copy", "rpp_hip_last_error",
-                 "rpp_hip_version", "rpp_hip_signature_backward_scratch_bytes"]
+                 "rpp_hip_version", "rpp_hip_signature_backward_scratch_bytes", "rpp_hip_signature_tiles"]
 
 
 def exported_symbols() -> list[str]:
@@ -153,5 +153,7 @@ def lib() -> C.CDLL:
         lib_.rpp_hip_lie_basis_copy.argtypes = [_i32, _i32] + [_vp] * 8
         lib_.rpp_hip_signature_backward_scratch_bytes.argtypes = [C.POINTER(SignatureBackwardDesc), _i32]
         lib_.rpp_hip_signature_backward_scratch_bytes.restype = _i64
+        lib_.rpp_hip_signature_tiles.argtypes = [_i32, _vp, _i32]
+        lib_.rpp_hip_signature_tiles.restype = C.c_int
         _lib = lib_
     return _lib

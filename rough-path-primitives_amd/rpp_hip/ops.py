@@ -311,6 +311,17 @@ def lie_basis(width: int, depth: int) -> dict:
 
 # -------------------------------------------------- batched Chen recursion
 
+def signature_tiles(dtype) -> list:
+    """Compiled register tiles (width, depth, threads, split) for a dtype."""
+    import numpy as np
+    nbytes = torch.tensor([], dtype=dtype).element_size()
+    lib = A.lib()
+    n = lib.rpp_hip_signature_tiles(nbytes, None, 0)
+    buf = np.zeros(4 * max(n, 1), dtype=np.int32)
+    lib.rpp_hip_signature_tiles(nbytes, buf.ctypes.data, n)
+    return [tuple(int(v) for v in buf[4 * i: 4 * i + 4]) for i in range(n)]
+
+
 def _offsets_of(lengths, device) -> torch.Tensor:
     lengths = torch.as_tensor(lengths, dtype=torch.int64)
     off = torch.zeros(lengths.numel() + 1, dtype=torch.int64)
@@ -344,7 +355,8 @@ def signature_backward(increments: torch.Tensor, offsets: torch.Tensor, basis: T
                        signatures: torch.Tensor, cotangent: torch.Tensor,
                        grad: Optional[torch.Tensor] = None, checkpoint_every: int = 0,
                        checkpoints: Optional[torch.Tensor] = None,
-                       ckpt_offsets: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
+                       ckpt_offsets: Optional[torch.Tensor] = None, tile_threads: int = 0,
+                       tile_split: int = 0, stream=None) -> torch.Tensor:
     """d loss / d increments given d loss / d signatures (cotangent may be 1-D,
     one functional for every path)."""
     n = offsets.numel() - 1
@@ -355,6 +367,7 @@ def signature_backward(increments: torch.Tensor, offsets: torch.Tensor, basis: T
                                 Batch(signatures, 0, basis.depth)._c(),
                                 Batch(cotangent, 0, basis.depth)._c(), grad.data_ptr(), checkpoint_every,
                                 checkpoints.data_ptr() if checkpoints is not None else None,
-                                ckpt_offsets.data_ptr() if ckpt_offsets is not None else None, None)
+                                ckpt_offsets.data_ptr() if ckpt_offsets is not None else None, None,
+                                tile_threads, tile_split)
     _call("rpp_hip_signature_backward" + _sfx(inc, signatures, cotangent), d, stream)
     return grad

@@ -104,3 +104,54 @@ def test_signature_empty_paths(dev):
     assert np.array_equal(sig[0], unit) and np.array_equal(sig[2], unit)
     expected = O.signature(w, N, inc.cpu().numpy(), off.cpu().numpy())
     assert_close(sig, expected, np.float64, "empty paths")
+
+
+def _tiles(dtype):
+    return rpp_hip.signature_tiles(dtype)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
+def test_every_register_tile_forward(dev, dtype):
+    """Each compiled tile (and the storage-generic kernel) against the FP64
+    oracle on uneven paths, lengths 0..6."""
+    npd = np.float64 if dtype == torch.float64 else np.float32
+    for (w, N, nt, split) in _tiles(dtype):
+        rng = np.random.default_rng(w * 1000 + N * 10 + nt + split)
+        inc, off = paths(rng, 7, w, 0, 6)
+        basis = rpp_hip.TensorBasis(w, N)
+        expected = O.signature(w, N, inc, off)
+        inc_d = torch.tensor(inc, dtype=dtype, device=dev)
+        off_d = torch.tensor(off, device=dev)
+        got = rpp_hip.signature(inc_d, off_d, basis, tile_threads=nt, tile_split=split)
+        assert_close(got.double().cpu().numpy(), expected, npd, f"tile {(w, N, nt, split)}")
+        gen = rpp_hip.signature(inc_d, off_d, basis, tile_threads=-1)
+        assert_close(gen.double().cpu().numpy(), expected, npd, f"generic {(w, N)}")
+
+
+@pytest.mark.parametrize("ckpt", [0, 2])
+def test_every_register_tile_backward_fp64(dev, ckpt):
+    for (w, N, nt, split) in _tiles(torch.float64):
+        if split > 1:
+            continue
+        rng = np.random.default_rng(w * 100 + N + nt + split)
+        n = 4
+        inc, off = paths(rng, n, w, 1, 5)
+        basis = rpp_hip.TensorBasis(w, N)
+        dim = basis.size()
+        ell = rng.normal(size=dim)
+        expected = O.signature_vjp(w, N, inc, off, ell)
+        inc_d = torch.tensor(inc, device=dev)
+        off_d = torch.tensor(off, device=dev)
+        kw = {}
+        if ckpt:
+            nck = np.diff(off) // ckpt
+            ck_off = np.zeros(n + 1, dtype=np.int64)
+            ck_off[1:] = np.cumsum(nck)
+            kw = dict(checkpoint_every=ckpt,
+                      checkpoints=torch.zeros((max(int(ck_off[-1]), 1), dim), dtype=torch.float64, device=dev),
+                      ckpt_offsets=torch.tensor(ck_off, device=dev))
+        sig = rpp_hip.signature(inc_d, off_d, basis, tile_threads=nt, tile_split=split, **kw)
+        grad = rpp_hip.signature_backward(inc_d, off_d, basis, sig, torch.tensor(ell, device=dev),
+                                          tile_threads=nt, tile_split=split, **kw)
+        err = rel_err(grad.cpu().numpy(), expected)
+        assert err < 1e-10, f"tile {(w, N, nt, split)} ckpt={ckpt}: {err:.3e}"
