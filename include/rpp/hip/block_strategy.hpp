@@ -1,0 +1,402 @@
+// rpp::hip::strategies::BlockStrategy -- the MI355X (gfx950) execution
+// strategy of RoughPathPrimitives' operation API, beside
+// rpp::cpu::strategies::SingleThreadStrategy
+// (cpu/single_thread/strategy.hpp:64-128) and in place of the CUDA
+// rpp::gpu::strategies::BlockStrategy (gpu/block/strategy.hpp:212-365).
+//
+// Host-only header.  It is included together with the reference's own
+// headers (<rpp/operations.hpp>, <rpp/views/views.hpp>, ...): callers keep
+// calling rpp::ops::ft_mul(strategy, config, out, lhs, rhs, basis, n, beta)
+// and friends unchanged, and every op below turns the batches into the
+// descriptors of include/rpp_hip.h and calls librpp_hip.so.  Batch data are
+// device pointers (hipMalloc'd by the caller); layouts::StrideLayout and
+// layouts::NoStrideLayout batches are accepted; Scalar must equal Accum
+// (float or double).  launch() is asynchronous on LaunchConfig::stream and
+// returns rpp::Error<const char*> like the reference GPU strategy.
+#ifndef RPP_HIP_BLOCK_STRATEGY_HPP
+#define RPP_HIP_BLOCK_STRATEGY_HPP
+
+#include <cstdint>
+#include <tuple>
+#include <type_traits>
+#include <utility>
+
+#include <rpp/architecture.hpp>
+#include <rpp/basis/basis_pack.hpp>
+#include <rpp/basis/tensor_basis.hpp>
+#include <rpp/operations/basic/ft_adj_lmul.hpp>
+#include <rpp/operations/basic/ft_adj_rmul.hpp>
+#include <rpp/operations/basic/ft_fma.hpp>
+#include <rpp/operations/basic/ft_inplace_fma.hpp>
+#include <rpp/operations/basic/ft_inplace_mul.hpp>
+#include <rpp/operations/basic/ft_mul.hpp>
+#include <rpp/operations/basic/lie_to_tensor.hpp>
+#include <rpp/operations/basic/tensor_pairing.hpp>
+#include <rpp/operations/basic/tensor_to_lie.hpp>
+#include <rpp/operations/intermediate/ft_exp.hpp>
+#include <rpp/operations/intermediate/ft_fmexp.hpp>
+#include <rpp/operations/intermediate/ft_log.hpp>
+#include <rpp/operations/linalg/sparse_matrix_vector.hpp>
+#include <rpp/sparse/matrix.hpp>
+#include <rpp/support/error.hpp>
+#include <rpp/views/batch.hpp>
+
+#include "rpp_hip.h"
+
+namespace rpp::hip {
+
+namespace strategies {
+
+struct LaunchConfig {
+    void* stream = nullptr;  // hipStream_t
+};
+
+template <typename Accum_, typename Architecture_ = arch::DefaultArchitecture>
+struct BlockStrategy {
+    static_assert(std::is_same_v<Accum_, float> || std::is_same_v<Accum_, double>,
+                  "hip::block computes in float or double");
+    using Accum = Accum_;
+    using Architecture = Architecture_;
+    using Size = typename Architecture::Size;
+    using Degree = typename Architecture::Degree;
+    using Index = typename Architecture::Index;
+    using Letter = typename Architecture::Letter;
+    using Bitmask = typename Architecture::Bitmask;
+    using LaunchConfig = hip::strategies::LaunchConfig;
+    struct Context {};  // the device work-groups carry their own context
+
+    // Same contract as SingleThreadStrategy::launch (cpu/single_thread/
+    // strategy.hpp:64-90): each op specialisation below implements
+    // hip_launch from the batches, bases and extras.
+    template <typename Op, typename Batches, typename Bases, typename... Extras>
+    RPP_NODISCARD Error<const char*> launch(LaunchConfig config, Batches batches, Bases bases,
+                                            Index batch_size, Extras&&... extras) const noexcept {
+        return Op::hip_launch(config, batches, bases, static_cast<int64_t>(batch_size),
+                              std::forward<Extras>(extras)...);
+    }
+};
+
+}  // namespace strategies
+
+namespace detail {
+
+template <typename S>
+struct is_block : std::false_type {};
+template <typename A, typename Arch>
+struct is_block<strategies::BlockStrategy<A, Arch>> : std::true_type {};
+
+inline Error<const char*> to_error(int rc) {
+    if (rc == 0) return Error<const char*>::success();
+    return Error<const char*>(static_cast<ErrorCode>(rc), rpp_hip_last_error());
+}
+
+template <typename Layout>
+int64_t batch_stride_of(Layout const& layout) {
+    if constexpr (std::is_same_v<Layout, layouts::NoStrideLayout>) {
+        return 1;
+    } else {
+        return static_cast<int64_t>(layout.stride_);
+    }
+}
+
+template <typename Accum, typename Batch>
+rpp_hip_tensor_batch tensor_of(Batch const& b) {
+    using Data = std::decay_t<decltype(b.data())>;
+    static_assert(std::is_pointer_v<Data>, "hip::block batches hold raw device pointers");
+    static_assert(std::is_same_v<std::remove_cv_t<std::remove_pointer_t<Data>>, Accum>,
+                  "hip::block needs Scalar == Accum");
+    auto const& md = b.metadata();
+    return rpp_hip_tensor_batch{const_cast<void*>(static_cast<const void*>(b.data())),
+                                batch_stride_of(std::decay_t<decltype(b.layout())>(b.layout())), 1,
+                                static_cast<int32_t>(std::get<1>(md)), static_cast<int32_t>(std::get<2>(md))};
+}
+
+template <typename Bases>
+rpp_hip_basis tensor_basis_of(Bases const& bases) {
+    auto const& tb = basis::get_basis(basis::TensorBasisTag{}, bases);
+    return rpp_hip_basis{static_cast<int32_t>(tb.width), static_cast<int32_t>(tb.depth)};
+}
+
+template <typename Accum>
+constexpr bool is_f64 = std::is_same_v<Accum, double>;
+
+#define RPP_HIP_CALL(NAME, DESC, STREAM) \
+    (is_f64<Accum> ? NAME##_f64(&(DESC), (STREAM)) : NAME##_f32(&(DESC), (STREAM)))
+
+}  // namespace detail
+}  // namespace rpp::hip
+
+namespace rpp::ops {
+
+#define RPP_HIP_OP_COMMON                                                  \
+    using Strategy = hip::strategies::BlockStrategy<Accum_, Arch_>;       \
+    using Accum = Accum_;                                                  \
+    using Context = typename Strategy::Context;                            \
+    using LaunchConfig = typename Strategy::LaunchConfig;                  \
+    using BaseOperation<Strategy>::BaseOperation;                          \
+    static constexpr bool is_implemented = true
+
+// ft_mul (operations/basic/ft_mul.hpp:53-79)
+template <typename Accum_, typename Arch_>
+class FTMul<hip::strategies::BlockStrategy<Accum_, Arch_>>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases, int64_t n,
+                                         Accum beta = Accum{1}) noexcept {
+        using namespace hip::detail;
+        rpp_hip_ft_mul_desc d{tensor_basis_of(bases), n, tensor_of<Accum>(std::get<0>(b)),
+                              tensor_of<Accum>(std::get<1>(b)), tensor_of<Accum>(std::get<2>(b)),
+                              static_cast<double>(beta)};
+        return to_error(RPP_HIP_CALL(rpp_hip_ft_mul, d, cfg.stream));
+    }
+};
+
+// ft_fma (operations/basic/ft_fma.hpp:54-83)
+template <typename Accum_, typename Arch_>
+class FTFma<hip::strategies::BlockStrategy<Accum_, Arch_>>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases, int64_t n,
+                                         Accum alpha = Accum{1}, Accum beta = Accum{1}) noexcept {
+        using namespace hip::detail;
+        rpp_hip_ft_fma_desc d{tensor_basis_of(bases),           n,
+                              tensor_of<Accum>(std::get<0>(b)), tensor_of<Accum>(std::get<1>(b)),
+                              tensor_of<Accum>(std::get<2>(b)), tensor_of<Accum>(std::get<3>(b)),
+                              static_cast<double>(alpha),       static_cast<double>(beta)};
+        return to_error(RPP_HIP_CALL(rpp_hip_ft_fma, d, cfg.stream));
+    }
+};
+
+// ft_inplace_fma<FMAType> (operations/basic/ft_inplace_fma.hpp:64-100)
+template <typename Accum_, typename Arch_, FTInplaceFMAType FMAType>
+class FTInplaceFma<hip::strategies::BlockStrategy<Accum_, Arch_>, FMAType>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases, int64_t n,
+                                         Accum alpha = Accum{1}, Accum beta = Accum{1}) noexcept {
+        using namespace hip::detail;
+        constexpr int32_t variant = FMAType == FTInplaceFMAType::AEqualsBCPlusA   ? 231
+                                    : FMAType == FTInplaceFMAType::AEqualsABPlusC ? 123
+                                                                                  : 213;
+        rpp_hip_ft_inplace_fma_desc d{tensor_basis_of(bases),           n,
+                                      variant,                          tensor_of<Accum>(std::get<0>(b)),
+                                      tensor_of<Accum>(std::get<1>(b)), tensor_of<Accum>(std::get<2>(b)),
+                                      static_cast<double>(alpha),       static_cast<double>(beta)};
+        return to_error(RPP_HIP_CALL(rpp_hip_ft_inplace_fma, d, cfg.stream));
+    }
+};
+
+// ft_inplace_mul (operations/basic/ft_inplace_mul.hpp:38-69)
+template <typename Accum_, typename Arch_>
+class FTInplaceMul<hip::strategies::BlockStrategy<Accum_, Arch_>>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases, int64_t n,
+                                         Accum beta = Accum{1}) noexcept {
+        using namespace hip::detail;
+        rpp_hip_ft_inplace_mul_desc d{tensor_basis_of(bases), n, tensor_of<Accum>(std::get<0>(b)),
+                                      tensor_of<Accum>(std::get<1>(b)), static_cast<double>(beta)};
+        return to_error(RPP_HIP_CALL(rpp_hip_ft_inplace_mul, d, cfg.stream));
+    }
+};
+
+// ft_exp / ft_log (operations/intermediate/ft_exp.hpp:89-112, ft_log.hpp:91-114)
+template <typename Accum_, typename Arch_>
+class FTExp<hip::strategies::BlockStrategy<Accum_, Arch_>>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases,
+                                         int64_t n) noexcept {
+        using namespace hip::detail;
+        rpp_hip_ft_unary_desc d{tensor_basis_of(bases), n, tensor_of<Accum>(std::get<0>(b)),
+                                tensor_of<Accum>(std::get<1>(b))};
+        return to_error(RPP_HIP_CALL(rpp_hip_ft_exp, d, cfg.stream));
+    }
+};
+
+template <typename Accum_, typename Arch_>
+class FTLog<hip::strategies::BlockStrategy<Accum_, Arch_>>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases,
+                                         int64_t n) noexcept {
+        using namespace hip::detail;
+        rpp_hip_ft_unary_desc d{tensor_basis_of(bases), n, tensor_of<Accum>(std::get<0>(b)),
+                                tensor_of<Accum>(std::get<1>(b))};
+        return to_error(RPP_HIP_CALL(rpp_hip_ft_log, d, cfg.stream));
+    }
+};
+
+// ft_fmexp (operations/intermediate/ft_fmexp.hpp:96-121)
+template <typename Accum_, typename Arch_>
+class FTFMExp<hip::strategies::BlockStrategy<Accum_, Arch_>>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases,
+                                         int64_t n) noexcept {
+        using namespace hip::detail;
+        rpp_hip_ft_fmexp_desc d{tensor_basis_of(bases), n, tensor_of<Accum>(std::get<0>(b)),
+                                tensor_of<Accum>(std::get<1>(b)), tensor_of<Accum>(std::get<2>(b))};
+        return to_error(RPP_HIP_CALL(rpp_hip_ft_fmexp, d, cfg.stream));
+    }
+};
+
+// tensor_pairing (operations/basic/tensor_pairing.hpp:47-73); the result
+// batch is make_scalar_batch(device pointer).
+template <typename Accum_, typename Arch_>
+class TensorPairing<hip::strategies::BlockStrategy<Accum_, Arch_>>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases,
+                                         int64_t n) noexcept {
+        using namespace hip::detail;
+        auto const& out = std::get<0>(b);
+        rpp_hip_scalar_batch s{const_cast<void*>(static_cast<const void*>(out.data())),
+                               batch_stride_of(std::decay_t<decltype(out.layout())>(out.layout()))};
+        rpp_hip_tensor_pairing_desc d{tensor_basis_of(bases), n, s, tensor_of<Accum>(std::get<1>(b)),
+                                      tensor_of<Accum>(std::get<2>(b))};
+        return to_error(RPP_HIP_CALL(rpp_hip_tensor_pairing, d, cfg.stream));
+    }
+};
+
+// ft_adj_lmul / ft_adj_rmul (operations/basic/ft_adj_lmul.hpp:49-73,
+// ft_adj_rmul.hpp:47-71)
+template <typename Accum_, typename Arch_>
+class FTAdjLMul<hip::strategies::BlockStrategy<Accum_, Arch_>>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases,
+                                         int64_t n) noexcept {
+        using namespace hip::detail;
+        rpp_hip_ft_adjoint_desc d{tensor_basis_of(bases), n, tensor_of<Accum>(std::get<0>(b)),
+                                  tensor_of<Accum>(std::get<1>(b)), tensor_of<Accum>(std::get<2>(b))};
+        return to_error(RPP_HIP_CALL(rpp_hip_ft_adj_lmul, d, cfg.stream));
+    }
+};
+
+template <typename Accum_, typename Arch_>
+class FTAdjRMul<hip::strategies::BlockStrategy<Accum_, Arch_>>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases,
+                                         int64_t n) noexcept {
+        using namespace hip::detail;
+        rpp_hip_ft_adjoint_desc d{tensor_basis_of(bases), n, tensor_of<Accum>(std::get<0>(b)),
+                                  tensor_of<Accum>(std::get<1>(b)), tensor_of<Accum>(std::get<2>(b))};
+        return to_error(RPP_HIP_CALL(rpp_hip_ft_adj_rmul, d, cfg.stream));
+    }
+};
+
+namespace hip_detail {
+
+// Graded vector batch -> SpMV vector batch: view-relative indexing (R9):
+// entry 0 is view.begin() = degree_begin[min_degree] of the view's basis.
+template <typename Accum, typename Batch, typename Basis>
+rpp_hip_vector_batch vector_of(Batch const& b, Basis const& basis) {
+    auto const& md = b.metadata();
+    const auto mn = std::get<1>(md);
+    const auto mx = std::get<2>(md);
+    const int64_t begin = static_cast<int64_t>(basis.start_of_degree(mn));
+    const int64_t end = static_cast<int64_t>(basis.end_of_degree(mx));
+    return rpp_hip_vector_batch{const_cast<void*>(static_cast<const void*>(b.data())),
+                                hip::detail::batch_stride_of(std::decay_t<decltype(b.layout())>(b.layout())), 1,
+                                begin, end - begin};
+}
+
+template <typename Accum, typename Matrix>
+rpp_hip_sparse_matrix matrix_of(Matrix const& m) {
+    static_assert(sizeof(*m.indices()) == 8 && sizeof(*m.offsets()) == 8,
+                  "hip::block sparse matrices use 64-bit indices and offsets");
+    constexpr bool csc = sparse::matrix_format_v<Matrix> == sparse::MatrixFormat::CSC;
+    return rpp_hip_sparse_matrix{csc ? RPP_HIP_CSC : RPP_HIP_CSR,
+                                 static_cast<const void*>(&*m.data()),
+                                 reinterpret_cast<const int64_t*>(&*m.indices()),
+                                 reinterpret_cast<const int64_t*>(&*m.offsets()),
+                                 static_cast<int64_t>(m.nnz()),
+                                 static_cast<int64_t>(m.rows()),
+                                 static_cast<int64_t>(m.cols())};
+}
+
+}  // namespace hip_detail
+
+// sparse_matrix_vector_product (operations/linalg/sparse_matrix_vector.hpp
+// :57-96), matrix arrays in device memory.
+template <typename Accum_, typename Arch_, sparse::MatrixFormat Format>
+class SparseMatrixVectorProduct<hip::strategies::BlockStrategy<Accum_, Arch_>, Format>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases, typename Matrix>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases, int64_t n,
+                                         Matrix const& matrix, Accum alpha = Accum{1}) noexcept {
+        using namespace hip::detail;
+        auto const& ob = basis::get_basis(std::get<0>(std::get<0>(b).metadata()), bases);
+        auto const& ab = basis::get_basis(std::get<0>(std::get<1>(b).metadata()), bases);
+        rpp_hip_spmv_desc d{n, hip_detail::vector_of<Accum>(std::get<0>(b), ob),
+                            hip_detail::vector_of<Accum>(std::get<1>(b), ab),
+                            hip_detail::matrix_of<Accum>(matrix), static_cast<double>(alpha)};
+        return to_error(RPP_HIP_CALL(rpp_hip_sparse_matrix_vector_product, d, cfg.stream));
+    }
+};
+
+// tensor_to_lie / lie_to_tensor (operations/basic/tensor_to_lie.hpp:65-103,
+// lie_to_tensor.hpp:70-108): the caller's matrix through the SpMV above.
+template <typename Accum_, typename Arch_, T2LImplementationType Impl>
+class TensorToLie<hip::strategies::BlockStrategy<Accum_, Arch_>, Impl>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+    using SpMV = SparseMatrixVectorProduct<
+        hip::strategies::BlockStrategy<Accum_, Arch_>,
+        Impl == T2LImplementationType::CSRSparseMatrix ? sparse::MatrixFormat::CSR : sparse::MatrixFormat::CSC>;
+
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases, typename Matrix>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases, int64_t n,
+                                         Matrix const& matrix) noexcept {
+        return SpMV::hip_launch(cfg, b, bases, n, matrix);
+    }
+};
+
+template <typename Accum_, typename Arch_, L2TImplementationType Impl>
+class LieToTensor<hip::strategies::BlockStrategy<Accum_, Arch_>, Impl>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+    using SpMV = SparseMatrixVectorProduct<
+        hip::strategies::BlockStrategy<Accum_, Arch_>,
+        Impl == L2TImplementationType::CSRSparseMatrix ? sparse::MatrixFormat::CSR : sparse::MatrixFormat::CSC>;
+
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases, typename Matrix>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases, int64_t n,
+                                         Matrix const& matrix) noexcept {
+        return SpMV::hip_launch(cfg, b, bases, n, matrix);
+    }
+};
+
+#undef RPP_HIP_OP_COMMON
+
+}  // namespace rpp::ops
+
+#undef RPP_HIP_CALL
+
+#endif  // RPP_HIP_BLOCK_STRATEGY_HPP

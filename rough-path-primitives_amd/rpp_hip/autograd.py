@@ -1,0 +1,85 @@
+"""torch.autograd bindings over the C-ABI: the batched signature (forward and
+backward kernels) and ft_mul (backward through ft_adj_rmul / ft_adj_lmul,
+operations/basic/ft_adj_rmul.hpp:47-71, ft_adj_lmul.hpp:49-73)."""
+from __future__ import annotations
+
+from typing import Optional
+
+import torch
+
+from . import ops
+
+
+def _checkpoint_layout(offsets: torch.Tensor, every: int):
+    lengths = (offsets[1:] - offsets[:-1]).cpu()
+    counts = torch.div(lengths, every, rounding_mode="floor")
+    ck_off = torch.zeros(counts.numel() + 1, dtype=torch.int64)
+    ck_off[1:] = torch.cumsum(counts, 0)
+    return ck_off.to(offsets.device), int(ck_off[-1])
+
+
+class _Signature(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, increments, offsets, width: int, depth: int, checkpoint_every: int, tile):
+        basis = ops.TensorBasis(width, depth)
+        ckpts = ck_off = None
+        if checkpoint_every > 0:
+            ck_off, total = _checkpoint_layout(offsets, checkpoint_every)
+            ckpts = torch.empty((max(total, 1), basis.size()), dtype=increments.dtype, device=increments.device)
+        sig = ops.signature(increments, offsets, basis, checkpoint_every=checkpoint_every, checkpoints=ckpts,
+                            ckpt_offsets=ck_off, tile_threads=tile[0], tile_split=tile[1])
+        ctx.basis = basis
+        ctx.every = checkpoint_every
+        ctx.tile = tile
+        ctx.save_for_backward(increments, offsets, sig, ckpts if ckpts is not None else sig,
+                              ck_off if ck_off is not None else offsets)
+        return sig
+
+    @staticmethod
+    def backward(ctx, grad_sig):
+        increments, offsets, sig, ckpts, ck_off = ctx.saved_tensors
+        kw = {}
+        if ctx.every > 0:
+            kw = dict(checkpoint_every=ctx.every, checkpoints=ckpts, ckpt_offsets=ck_off)
+        tile = ctx.tile if ctx.tile[1] <= 1 else (0, 0)
+        grad = ops.signature_backward(increments, offsets, ctx.basis, sig, grad_sig.contiguous(),
+                                      tile_threads=tile[0], tile_split=tile[1], **kw)
+        return grad, None, None, None, None, None
+
+
+def signature(increments: torch.Tensor, offsets: torch.Tensor, depth: int, checkpoint_every: int = 0,
+              tile: Optional[tuple] = None) -> torch.Tensor:
+    """Differentiable signatures of paths of uneven length.
+
+    increments: [total, width]; offsets: int64 [n+1] on the same device.
+    checkpoint_every > 0 stores the state every that many increments so the
+    backward pass re-anchors its exp(-x) reconstruction there.
+    """
+    return _Signature.apply(increments, offsets, increments.shape[1], depth, checkpoint_every, tile or (0, 0))
+
+
+class _FTMul(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, lhs, rhs, width: int, depth: int):
+        basis = ops.TensorBasis(width, depth)
+        out = torch.empty_like(lhs)
+        ops.ft_mul(out, lhs, rhs, basis)
+        ctx.basis = basis
+        ctx.save_for_backward(lhs, rhs)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        lhs, rhs = ctx.saved_tensors
+        g = g.contiguous()
+        gl = torch.empty_like(lhs)
+        gr = torch.empty_like(rhs)
+        # <g, lhs rhs>:  d/dlhs = R_rhs^* g,  d/drhs = L_lhs^* g
+        ops.ft_adj_rmul(gl, rhs, g, ctx.basis)
+        ops.ft_adj_lmul(gr, lhs, g, ctx.basis)
+        return gl, gr, None, None
+
+
+def ft_mul(lhs: torch.Tensor, rhs: torch.Tensor, width: int, depth: int) -> torch.Tensor:
+    """Differentiable batched concatenation product (full windows)."""
+    return _FTMul.apply(lhs.contiguous(), rhs.contiguous(), width, depth)

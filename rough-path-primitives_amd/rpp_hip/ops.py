@@ -339,6 +339,9 @@ def signature(increments: torch.Tensor, offsets: torch.Tensor, basis: TensorBasi
     """
     n = offsets.numel() - 1
     dim = basis.size()
+    if tile_threads == 0:
+        from . import autotune
+        tile_threads, tile_split = autotune.best(basis.width, basis.depth, increments.dtype, "forward") or (0, 0)
     if out is None:
         out = torch.empty((n, dim), dtype=increments.dtype, device=increments.device)
     inc = increments.contiguous()
@@ -361,6 +364,9 @@ def signature_backward(increments: torch.Tensor, offsets: torch.Tensor, basis: T
     one functional for every path)."""
     n = offsets.numel() - 1
     inc = increments.contiguous()
+    if tile_threads == 0:
+        from . import autotune
+        tile_threads, tile_split = autotune.best(basis.width, basis.depth, increments.dtype, "backward") or (0, 0)
     if grad is None:
         grad = torch.empty_like(inc)
     d = A.SignatureBackwardDesc(basis._c(), n, inc.data_ptr(), offsets.data_ptr(),

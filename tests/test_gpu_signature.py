@@ -155,3 +155,23 @@ def test_every_register_tile_backward_fp64(dev, ckpt):
                                           tile_threads=nt, tile_split=split, **kw)
         err = rel_err(grad.cpu().numpy(), expected)
         assert err < 1e-10, f"tile {(w, N, nt, split)} ckpt={ckpt}: {err:.3e}"
+
+
+def test_autograd_signature_and_ft_mul(dev):
+    from rpp_hip import autograd as AG
+    w, N, n = 3, 4, 5
+    rng = np.random.default_rng(12)
+    inc, off = paths(rng, n, w, 1, 7)
+    dim = rpp_hip.tensor_dimension(w, N)
+    ell = rng.normal(size=dim)
+    expected = O.signature_vjp(w, N, inc, off, ell)
+    for every in (0, 2):
+        x = torch.tensor(inc, device=dev, requires_grad=True)
+        sig = AG.signature(x, torch.tensor(off, device=dev), N, checkpoint_every=every)
+        loss = (sig @ torch.tensor(ell, device=dev)).sum()
+        loss.backward()
+        assert rel_err(x.grad.cpu().numpy(), expected) < 1e-10
+    # ft_mul: torch.autograd.gradcheck in FP64
+    a = torch.tensor(rng.uniform(-1, 1, (2, dim)), device=dev, requires_grad=True)
+    b = torch.tensor(rng.uniform(-1, 1, (2, dim)), device=dev, requires_grad=True)
+    assert torch.autograd.gradcheck(lambda p, q: AG.ft_mul(p, q, w, N), (a, b), eps=1e-6, atol=1e-8)
