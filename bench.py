@@ -10,7 +10,7 @@ ping-pong buffers, exponent view (1,1)); FP64.  One step is the whole chain
 value = path-increments per second of that plain per-operation path.  The
 same line carries the fused path (all increments in one persistent launch),
 FP32 variants, the error of each against the FP64 oracle, the roofline of
-the dominant kernel (k_fmexp_deg1, HBM-bound) and the reference CPU path
+the dominant kernel (k_fmexp_tile, HBM-bound) and the reference CPU path
 (oracle/_ref, the real cpu::single_thread) timed on a bounded sample.
 """
 from __future__ import annotations
@@ -129,7 +129,7 @@ def cpu_baseline(inc):
 
 
 def load_traffic():
-    p = os.path.join(ROOT, "profiles", "traffic_k_fmexp_deg1_c2_fp64.json")
+    p = os.path.join(ROOT, "profiles", "traffic_k_fmexp_tile_c2_fp64.json")
     if os.path.exists(p):
         with open(p) as f:
             return json.load(f).get("hbm_bytes_per_launch")
@@ -142,6 +142,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default=None,
+                    help="run one variant (plain_fp64, fused_fp64, plain_fp32, fused_fp32) and print its timing")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -163,9 +167,25 @@ def main():
     ref64 = O.signature(WIDTH, DEPTH, inc[:n_chk].reshape(-1, WIDTH),
                         np.arange(n_chk + 1, dtype=np.int64) * LENGTH)
 
+    if args.copy_mib:
+        src = torch.ones(args.copy_mib * (1 << 20) // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    variants = (("plain_fp64", Plain, torch.float64), ("fused_fp64", Fused, torch.float64),
+                ("plain_fp32", Plain, torch.float32), ("fused_fp32", Fused, torch.float32))
+    if args.only:
+        variants = [v for v in variants if v[0] == args.only]
+        name, cls, dt = variants[0]
+        runner = cls(rpp_hip, torch, inc, dt, dev)
+        ms_ev, _, _ = time_steps(torch, runner, args.steps, args.warmup)
+        print(json.dumps({"variant": name, "ms_per_step": ms_ev, "path_increments_per_s": units / (ms_ev * 1e-3)}))
+        return
+
     results = {}
-    for name, cls, dt in (("plain_fp64", Plain, torch.float64), ("fused_fp64", Fused, torch.float64),
-                          ("plain_fp32", Plain, torch.float32), ("fused_fp32", Fused, torch.float32)):
+    for name, cls, dt in variants:
         runner = cls(rpp_hip, torch, inc, dt, dev)
         ms_ev, s_wall, out = time_steps(torch, runner, args.steps, args.warmup)
         err = rel_err(out[:n_chk].double().cpu().numpy(), ref64)
@@ -199,7 +219,7 @@ def main():
                    "increments_per_path": LENGTH},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(),
-                     "kernel": "k_fmexp_deg1", "avg_launch_ms": launch_ms,
+                     "kernel": "k_fmexp_tile<double,5,6,640,1>", "avg_launch_ms": launch_ms,
                      "algorithmic_bytes_per_launch": alg_bytes},
         "max_rel_err_vs_fp64_oracle": plain["max_rel_err_vs_fp64_oracle"],
         "fused_vs_plain": {

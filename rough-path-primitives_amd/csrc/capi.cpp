@@ -80,7 +80,7 @@ DevBasis make_basis(int width, int depth) {
 namespace {
 
 int check_basis(const rpp_hip_basis& b) {
-    if (b.width < 1 || b.width > 255) return fail(RPP_HIP_INVALID_ARGUMENT, "width must be in [1, 255]");
+    if (b.width < 1 || b.width > 65535) return fail(RPP_HIP_INVALID_ARGUMENT, "width must be in [1, 65535]");
     if (b.depth < 0 || b.depth > kMaxDepth)
         return fail(RPP_HIP_INVALID_ARGUMENT, "depth must be in [0, 30]");
     uint64_t p = 1;
@@ -375,6 +375,7 @@ int signature_impl(const rpp_hip_signature_desc* d, void* s) {
     if (d->checkpoint_every < 0) return fail(RPP_HIP_INVALID_ARGUMENT, "negative checkpoint_every");
     if (d->checkpoint_every > 0 && (!d->checkpoints || !d->ckpt_offsets))
         return fail(RPP_HIP_INVALID_ARGUMENT, "checkpoints requested without buffers");
+    if (d->basis.width > 256) return fail(RPP_HIP_NOT_IMPLEMENTED, "signature supports width <= 256");
     const DevBasis B = make_basis(d->basis.width, d->basis.depth);
     hipStream_t stream = as_stream(s);
     std::vector<int64_t> lengths;

@@ -16,6 +16,7 @@
 // division by the width, compile-time for the common widths.
 #include "common.hpp"
 #include "ft_ops.hpp"
+#include "signature_reg.hpp"
 
 #include <algorithm>
 
@@ -24,6 +25,9 @@ namespace rpp_hip {
 namespace {
 
 constexpr int kBlock = 256;
+// LDS budget of the staged small-tensor kernels (three 48 KiB work-groups
+// per 160 KiB compute unit).
+constexpr int64_t kLdsStageBytes = 48 * 1024;
 
 template <bool CS1>
 __device__ __forceinline__ int64_t at(int64_t idx, int64_t cs) {
@@ -99,6 +103,93 @@ k_product(DevBasis B, View<T> out, View<const T> lhs, View<const T> rhs, View<co
                                 lhs.mx, rhs.obj(b), rhs.cs, rhs.mn, rhs.mx,
                                 with_add ? add.obj(b) : nullptr, add.cs, add.mn, add.mx,
                                 with_add != 0, alpha, beta);
+    }
+}
+
+// Division by a per-launch constant d >= 1 (n < 2^31): the multiply-shift of
+// DevBasis::div_width, with the magic number formed on the device.
+struct Divider {
+    uint64_t magic;
+    uint32_t shift;
+    __device__ explicit Divider(uint32_t d) {
+        uint32_t L = 0;
+        while ((uint64_t{1} << L) < d) ++L;
+        const uint64_t num = uint64_t{1} << (32 + L);
+        magic = num / d + ((num % d) ? 1 : 0);
+        shift = 32 + L;
+    }
+    __device__ __forceinline__ uint32_t div(uint32_t n) const {
+        return static_cast<uint32_t>((static_cast<uint64_t>(n) * magic) >> shift);
+    }
+};
+
+// Small tensors: a work-group stages the operand windows of G objects in LDS
+// with contiguous loads (object-major batches with stride = dim are one
+// contiguous block), computes every output word of the G objects from LDS and
+// writes each word once.  All operand reads precede the first write, so out
+// may alias an operand; the addend is read at its own index.
+template <typename T, int W>
+__global__ void __launch_bounds__(kBlock)
+k_product_lds(DevBasis B, View<T> out, View<const T> lhs, View<const T> rhs, View<const T> add,
+              int with_add, T alpha, T beta, int64_t n, int G) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    using L = Letters<W>;
+    const uint32_t width = L::width(B);
+    T* sl = reinterpret_cast<T*>(smem_raw);
+    const int64_t dim = B.start[B.depth + 1];
+    T* sr = sl + G * dim;
+    const bool lw = lhs.mn <= lhs.mx, rw = rhs.mn <= rhs.mx;
+    const uint32_t lb = lw ? static_cast<uint32_t>(B.start[lhs.mn]) : 0;
+    const uint32_t llen = lw ? static_cast<uint32_t>(B.start[lhs.mx + 1]) - lb : 0;
+    const uint32_t rb = rw ? static_cast<uint32_t>(B.start[rhs.mn]) : 0;
+    const uint32_t rlen = rw ? static_cast<uint32_t>(B.start[rhs.mx + 1]) - rb : 0;
+    const uint32_t ob = static_cast<uint32_t>(B.start[out.mn]);
+    const uint32_t nout = static_cast<uint32_t>(B.start[out.mx + 1]) - ob;
+    const Divider dl(llen > 0 ? llen : 1), dr(rlen > 0 ? rlen : 1), dout(nout > 0 ? nout : 1);
+    for (int64_t g0 = static_cast<int64_t>(blockIdx.x) * G; g0 < n; g0 += static_cast<int64_t>(gridDim.x) * G) {
+        const uint32_t gn = static_cast<uint32_t>(min<int64_t>(G, n - g0));
+        __syncthreads();
+        for (uint32_t f = threadIdx.x; f < gn * llen; f += blockDim.x) {
+            const uint32_t g = dl.div(f);
+            const uint32_t i = lb + (f - g * llen);
+            sl[g * dim + i] = lhs.obj(g0 + g)[i];
+        }
+        for (uint32_t f = threadIdx.x; f < gn * rlen; f += blockDim.x) {
+            const uint32_t g = dr.div(f);
+            const uint32_t i = rb + (f - g * rlen);
+            sr[g * dim + i] = rhs.obj(g0 + g)[i];
+        }
+        __syncthreads();
+        for (uint32_t f = threadIdx.x; f < gn * nout; f += blockDim.x) {
+            const uint32_t g = dout.div(f);
+            const uint32_t i = ob + (f - g * nout);
+            int k = out.mn;
+            while (i >= static_cast<uint32_t>(B.start[k + 1])) ++k;
+            const uint32_t w = i - static_cast<uint32_t>(B.start[k]);
+            const int jlo = max(max(rhs.mn, k - lhs.mx), 0);
+            const int jhi = min(min(rhs.mx, k - lhs.mn), k);
+            const T* Lg = sl + g * dim;
+            const T* Rg = sr + g * dim;
+            T acc = T(0);
+            if (jlo <= jhi) {
+                uint32_t p = w, sfx = 0;
+                for (int j = 0; j < jlo; ++j) {
+                    const uint32_t q = L::div(B, p);
+                    sfx += (p - q * width) * B.pow[j];
+                    p = q;
+                }
+                for (int j = jlo;; ++j) {
+                    acc += Lg[B.start[k - j] + p] * Rg[B.start[j] + sfx];
+                    if (j == jhi) break;
+                    const uint32_t q = L::div(B, p);
+                    sfx += (p - q * width) * B.pow[j];
+                    p = q;
+                }
+            }
+            T v = beta * acc;
+            if (with_add && add.mn <= k && k <= add.mx) v = alpha * add.obj(g0 + g)[i] + v;
+            out.obj(g0 + g)[i] = v;
+        }
     }
 }
 
@@ -197,6 +288,81 @@ k_fmexp_deg1(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int6
             }
         }
         __syncthreads();
+    }
+}
+
+// ft_fmexp with a degree-one exponent x and out from degree 0, in Horner
+// form per target level k (out must not alias the multiplier, R7):
+//     A_0 = mult_0,  A_i[p a] = A_{i-1}[p] x[a] / (k-i+1) + mult_i[p a],
+//     out_k = A_k,
+// i.e. sum_i mult_i (x) x^{k-i}/(k-i)!  -- the same truncated series as the
+// reference recursion, at sum_k sum_{i<=k} d^i multiply-adds per object.
+// A work item is (object, first letter q) when split = 1: the words under q
+// evolve independently given the chain value A_1[q].  Thread t owns parents
+// p = t, t + blockDim, ... of a level and computes their W children; the
+// intermediates A_i (i < k) go through two LDS buffers.  mult and x are read
+// from HBM once per item (lower levels again from cache), out written once.
+template <typename T, int W>
+__global__ void __launch_bounds__(kBlock)
+k_fmexp_horner(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int64_t n, int split) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    using L = Letters<W>;
+    const uint32_t width = L::width(B);
+    const int N = B.depth;
+    const int M = split;
+    const uint32_t P = M == 1 ? width : 1u;
+    const uint32_t cap = N - 1 - M >= 0 ? B.pow[N - 1 - M] : 1u;
+    T* buf0 = reinterpret_cast<T*>(smem_raw);
+    T* buf1 = buf0 + cap;
+    __shared__ T xs[256];
+    __shared__ T inv[kMaxDepth + 2];
+    if (threadIdx.x <= static_cast<unsigned>(N + 1))
+        inv[threadIdx.x] = threadIdx.x == 0 ? T(1) : T(1) / T(threadIdx.x);
+    const int omx = out.mx;
+    for (int64_t item = blockIdx.x; item < n * P; item += gridDim.x) {
+        const int64_t b = item / P;
+        const uint32_t q = static_cast<uint32_t>(item - b * P);
+        T* o = out.obj(b);
+        const T* m = mult.obj(b);
+        const T* x = ex.obj(b);
+        __syncthreads();
+        for (uint32_t a = threadIdx.x; a < width; a += blockDim.x) xs[a] = x[1 + a];
+        __syncthreads();
+        const T m0 = mult.has(0) ? m[0] : T(0);
+        // chain value A_1[q] of target k (M = 1): m0 x[q] / k + mult_1[q]
+        const T m1q = (M == 1 && mult.has(1)) ? m[1 + q] : T(0);
+        if (threadIdx.x == 0) {
+            if (q == 0 && omx >= 0) o[0] = m0;
+            if (M == 1 && omx >= 1) o[1 + q] = m0 * xs[q] + m1q;
+        }
+        for (int k = omx; k >= M + 1; --k) {
+            const T chain = M == 1 ? m0 * (xs[q] * inv[k]) + m1q : m0;
+            for (int i = M + 1; i <= k; ++i) {
+                const T c = inv[k - i + 1];
+                const uint32_t np = B.pow[i - 1 - M];  // parents of level i in the item's subtree
+                const int64_t base = B.start[i] + static_cast<int64_t>(q) * B.pow[i - M];
+                const bool mi = mult.has(i);
+                const T* prev = ((i - M) & 1) ? buf1 : buf0;
+                T* cur = ((i - M) & 1) ? buf0 : buf1;
+                for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) {
+                    const T ap = i == M + 1 ? chain : prev[p];
+                    const int64_t wb = base + static_cast<int64_t>(p) * width;
+                    if constexpr (W > 0) {
+#pragma unroll
+                        for (int a = 0; a < W; ++a) {
+                            const T v = ap * (xs[a] * c) + (mi ? m[wb + a] : T(0));
+                            if (i == k) o[wb + a] = v; else cur[p * W + a] = v;
+                        }
+                    } else {
+                        for (uint32_t a = 0; a < width; ++a) {
+                            const T v = ap * (xs[a] * c) + (mi ? m[wb + a] : T(0));
+                            if (i == k) o[wb + a] = v; else cur[p * width + a] = v;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
     }
 }
 
@@ -379,6 +545,18 @@ int launch_product(const DevBasis& B, int64_t n, const rpp_hip_tensor_batch& out
     const View<const T> a = add ? make_cview<T>(*add) : View<const T>{nullptr, 0, 1, 1, 0};
     const int with_add = add ? 1 : 0;
     const bool contig = all_contig({&out, &lhs, &rhs, add});
+    const int64_t dim = B.start[B.depth + 1];
+    const int64_t per_object = 2 * dim * static_cast<int64_t>(sizeof(T));
+    if (contig && out.min_degree <= out.max_degree && per_object <= kLdsStageBytes) {
+        const int G = static_cast<int>(std::min<int64_t>(64, kLdsStageBytes / per_object));
+        const int64_t groups = (n + G - 1) / G;
+        const unsigned grid = grid_for(groups);
+        const size_t shmem = static_cast<size_t>(G) * per_object;
+        RPP_DISPATCH_W(B.width, true,
+                       (k_product_lds<T, W_><<<grid, kBlock, shmem, stream>>>(B, o, l, r, a, with_add, T(alpha),
+                                                                             T(beta), n, G)));
+        return check_hip(hipGetLastError(), "k_product_lds launch");
+    }
     const unsigned grid = grid_for(n);
     RPP_DISPATCH_W(B.width, contig,
                    (k_product<T, W_, C_><<<grid, kBlock, 0, stream>>>(B, o, l, r, a, with_add, T(alpha),
@@ -399,6 +577,24 @@ int launch_horner(int op, const DevBasis& B, int64_t n, const rpp_hip_tensor_bat
     // Shortcuts that compute the same truncated series in one pass.
     const bool deg1 = B.depth >= 1 && arg.max_degree == 1 && arg.min_degree <= 1 &&
                       out.min_degree == 0 && out.max_degree >= 0;
+    if (op == kOpFmexp && deg1 && contig && out.max_degree == B.depth && mult && B.depth >= 1) {
+        if (const RegTile<T>* tile = fmexp_tile<T>(B.width, B.depth))
+            return tile->fmexp(B, out, *mult, arg, n, stream);
+    }
+    if (op == kOpFmexp && deg1 && contig && B.width <= 256 && out.max_degree >= 1) {
+        // split by first letter when the top-level exchange would not fit
+        const int N = B.depth;
+        const int split = (N >= 2 && static_cast<int64_t>(B.pow[N - 1]) * 2 * sizeof(T) > 16 * 1024) ? 1 : 0;
+        const int64_t cap = N - 1 - split >= 0 ? B.pow[N - 1 - split] : 1;
+        const size_t shmem = static_cast<size_t>(2 * cap) * sizeof(T);
+        if (shmem <= static_cast<size_t>(kLdsStageBytes)) {
+            const int64_t items = n * (split ? B.width : 1);
+            const unsigned g = grid_for(items);
+            RPP_DISPATCH_W(B.width, true,
+                           (k_fmexp_horner<T, W_><<<g, kBlock, shmem, stream>>>(B, o, m, x, n, split)));
+            return check_hip(hipGetLastError(), "k_fmexp_horner launch");
+        }
+    }
     if (op == kOpFmexp && deg1 && B.width <= 256) {
         RPP_DISPATCH_W(B.width, contig,
                        (k_fmexp_deg1<T, W_, C_><<<grid, kBlock, 0, stream>>>(B, o, m, x, n)));

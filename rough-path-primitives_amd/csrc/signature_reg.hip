@@ -113,6 +113,27 @@ struct Tile {
             s0 = o[0];
         }
     }
+
+    // Load the words of degree [mn, mx] (zero elsewhere), object-major.
+    __device__ __forceinline__ void load_window(const DevBasis& B, const T* o, int q, int mn, int mx) {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int i = M + 1; i <= N; ++i) {
+            const bool in = mn <= i && i <= mx;
+#pragma unroll
+            for (int s = 0; s < G::slots(i); ++s) {
+                const int p = t + s * NT;
+                if (p < G::parents(i)) {
+#pragma unroll
+                    for (int b = 0; b < W; ++b)
+                        reg[G::offset(i) + s * W + b] =
+                            in ? o[G::global_index(B, i, q, static_cast<int64_t>(p) * W + b)] : T(0);
+                }
+            }
+        }
+        root = (mn <= M && M <= mx) ? o[M == 0 ? 0 : G::global_index(B, M, q, 0)] : T(0);
+        s0 = (mn <= 0 && 0 <= mx) ? o[0] : T(0);
+    }
 };
 
 // One in-place Chen step S <- S (x) exp(x) on the tile.  xs: W increments
@@ -194,6 +215,43 @@ k_sig_fwd_reg(DevBasis B, int64_t n_items, const int64_t* order, const T* incr, 
         }
         S.template move<true>(B, out.obj(path), out.cs, q, q == 0);
     }
+}
+
+// Plain ft_fmexp (one increment) on the tile: out = mult (x) exp(x) for a
+// degree-one exponent, full output window.  All loads of the multiplier are
+// issued up front into registers, one in-register Chen step, one store: the
+// per-call path reads mult once and writes out once.
+template <typename T, int W, int N, int NT, int M>
+__global__ void __launch_bounds__(NT)
+k_fmexp_tile(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int64_t n) {
+    using G = Geo<W, N, NT, M>;
+    __shared__ T lds0[G::lds_elems];
+    __shared__ T lds1[G::lds_elems];
+    __shared__ T xs[W];
+    __shared__ T inv[N + 2];
+    if (threadIdx.x <= N + 1) inv[threadIdx.x] = threadIdx.x == 0 ? T(1) : T(1) / T(threadIdx.x);
+    for (int64_t item = blockIdx.x; item < n * G::P; item += gridDim.x) {
+        const int64_t b = item / G::P;
+        const int q = static_cast<int>(item - b * G::P);
+        Tile<T, W, N, NT, M> S;
+        S.load_window(B, mult.obj(b), q, mult.mn, mult.mx);
+        __syncthreads();
+        if (threadIdx.x < W) xs[threadIdx.x] = ex.obj(b)[1 + threadIdx.x];
+        __syncthreads();
+        chen_step_reg<T, W, N, NT, M>(S, xs, inv, lds0, lds1, q);
+        S.template move<true>(B, out.obj(b), 1, q, q == 0);
+    }
+}
+
+template <typename T, int W, int N, int NT, int M>
+int fmexp_launch(const DevBasis& B, const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& mult,
+                 const rpp_hip_tensor_batch& ex, int64_t n, hipStream_t stream) {
+    using G = Geo<W, N, NT, M>;
+    const int64_t items = n * G::P;
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>(items, 65535LL * 16));
+    k_fmexp_tile<T, W, N, NT, M><<<grid, NT, 0, stream>>>(B, make_view<T>(out), make_cview<T>(mult),
+                                                          make_cview<T>(ex), n);
+    return check_hip(hipGetLastError(), "k_fmexp_tile launch");
 }
 
 // ------------------------------------------------------------------ backward
@@ -409,7 +467,8 @@ int bwd_launch(const DevBasis& B, const rpp_hip_signature_backward_desc& d, cons
 // The compiled tiles.  Each entry: dtype bytes, width, depth, threads,
 // letters fixed per work-group, forward/backward launchers.
 #define RPP_TILE(T, W, N, NT, M)                                                                   \
-    {sizeof(T), W, N, NT, M, &fwd_launch<T, W, N, NT, M>, M <= 1 ? &bwd_launch<T, W, N, NT, (M <= 1 ? M : 1)> : nullptr}
+    {sizeof(T), W, N, NT, M, &fwd_launch<T, W, N, NT, M>,                                          \
+     M <= 1 ? &bwd_launch<T, W, N, NT, (M <= 1 ? M : 1)> : nullptr, &fmexp_launch<T, W, N, NT, M>}
 
 template <typename T>
 const std::vector<RegTile<T>>& reg_tiles() {
@@ -428,6 +487,21 @@ const std::vector<RegTile<T>>& reg_tiles() {
     return tiles;
 }
 
+// Plain ft_fmexp prefers a tile that fixes one letter: d work-groups per
+// object keep more of them in flight for this single-step, HBM-bound call.
+template <typename T>
+const RegTile<T>* fmexp_tile(int width, int depth) {
+    const RegTile<T>* any = nullptr;
+    for (const auto& t : reg_tiles<T>()) {
+        if (t.width != width || t.depth != depth) continue;
+        if (t.split == 1) return &t;
+        if (!any) any = &t;
+    }
+    return any;
+}
+
+template const RegTile<float>* fmexp_tile<float>(int, int);
+template const RegTile<double>* fmexp_tile<double>(int, int);
 template const std::vector<RegTile<float>>& reg_tiles<float>();
 template const std::vector<RegTile<double>>& reg_tiles<double>();
 

@@ -17,11 +17,18 @@ struct RegTile {
     int (*forward)(const DevBasis&, const rpp_hip_signature_desc&, const int64_t*, unsigned*, hipStream_t);
     int (*backward)(const DevBasis&, const rpp_hip_signature_backward_desc&, const int64_t*, int64_t, unsigned*,
                     hipStream_t);
+    // one plain ft_fmexp call (degree-one exponent, full output window)
+    int (*fmexp)(const DevBasis&, const rpp_hip_tensor_batch&, const rpp_hip_tensor_batch&,
+                 const rpp_hip_tensor_batch&, int64_t, hipStream_t);
 };
 
 // Every compiled tile for dtype T; the first match of a (width, depth) is the
 // default when a descriptor names no tile.
 template <typename T>
 const std::vector<RegTile<T>>& reg_tiles();
+
+// The tile used by the plain ft_fmexp call for (width, depth), or nullptr.
+template <typename T>
+const RegTile<T>* fmexp_tile(int width, int depth);
 
 }  // namespace rpp_hip

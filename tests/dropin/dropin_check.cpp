@@ -43,7 +43,15 @@ struct Dev {
         if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) std::abort();
         if (n && hipMemcpy(p, h.data(), n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) std::abort();
     }
-    ~Dev() { (void)hipFree(p); }
+    Dev(const Dev&) = delete;
+    Dev& operator=(const Dev&) = delete;
+    Dev(Dev&& o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    ~Dev() {
+        if (p) (void)hipFree(p);
+    }
     std::vector<T> host() const {
         std::vector<T> h(n);
         if (hipDeviceSynchronize() != hipSuccess) std::abort();

@@ -180,3 +180,28 @@ def test_chained_fmexp_c2_shape(dev, dtype):
         rpp_hip.ft_fmexp(nxt, cur, rpp_hip.Batch(pad_d[:, t * w: t * w + 1 + w], 1, 1), basis)
         cur, nxt = nxt, cur
     assert_close(cur.cpu().numpy(), expected, dtype, "C2 chain")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("shape", [(3, 4), (5, 6), (2, 7)])
+def test_ft_fmexp_interleaved_and_broadcast(dev, shape, dtype):
+    """Batch-interleaved multiplier/out with a broadcast degree-one exponent
+    (the strided kernels) against the oracle."""
+    w, N = shape
+    rng = np.random.default_rng(50 + w)
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    n = 9
+    mult = damped_batch(rng, n, w, N, dtype)
+    ex = np.zeros(dim, dtype=dtype)
+    ex[1:1 + w] = rng.normal(0, 0.4, size=w)
+    expected = np.zeros((n, dim), dtype=dtype)
+    O.orc.ft_fmexp(w, N, expected, mult, ex, ew=(1, 1))
+    out = torch.zeros((dim, n), dtype=getattr(torch, np.dtype(dtype).name), device=dev)
+    rpp_hip.ft_fmexp(rpp_hip.interleaved(out, basis), rpp_hip.interleaved(to_dev(mult.T, dev), basis),
+                     rpp_hip.Batch(to_dev(ex, dev), 1, 1), basis)
+    assert_close(out.cpu().numpy().T, expected, dtype, f"interleaved fmexp {shape}")
+    # object-major multiplier, broadcast exponent (Horner kernel)
+    out2 = torch.zeros((n, dim), dtype=out.dtype, device=dev)
+    rpp_hip.ft_fmexp(out2, to_dev(mult, dev), rpp_hip.Batch(to_dev(ex, dev), 1, 1), basis)
+    assert_close(out2.cpu().numpy(), expected, dtype, f"broadcast fmexp {shape}")
