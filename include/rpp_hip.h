@@ -260,6 +260,8 @@ typedef struct rpp_hip_signature_desc {
     const int64_t* ckpt_offsets;
     int32_t tile_threads;
     int32_t tile_split;
+    int32_t tile_kind;    /* 0: Horner register tile, 1: two-barrier chain tile */
+    int32_t tile_expand;  /* chain tiles: levels expanded per job */
 } rpp_hip_signature_desc;
 int rpp_hip_signature_f32(const rpp_hip_signature_desc* desc, void* stream);
 int rpp_hip_signature_f64(const rpp_hip_signature_desc* desc, void* stream);
@@ -293,12 +295,15 @@ int64_t rpp_hip_signature_backward_scratch_bytes(const rpp_hip_signature_backwar
 int rpp_hip_signature_backward_f32(const rpp_hip_signature_backward_desc* desc, void* stream);
 int rpp_hip_signature_backward_f64(const rpp_hip_signature_backward_desc* desc, void* stream);
 
-/* The compiled register tiles of rpp_hip_signature_*: writes up to
- * `capacity` quadruples (width, depth, threads, split) for the dtype (4 or 8
- * bytes) and returns how many exist.  A descriptor selects one with
- * tile_threads/tile_split; tile_threads = 0 takes the library default for
- * (width, depth), tile_threads < 0 the storage-generic kernel.  No reference
- * counterpart: the reference has no batched signature entry point. */
+/* The compiled tiles of rpp_hip_signature_*: writes up to `capacity`
+ * 6-tuples (width, depth, threads, split, kind, expand) for the dtype (4 or 8
+ * bytes) and returns how many exist.  kind 0 tiles (Horner, register state)
+ * serve forward and backward (split <= 1); kind 1 tiles (two barriers per
+ * increment) serve the forward pass.  A descriptor selects one with
+ * tile_threads/tile_split/tile_kind/tile_expand; tile_threads = 0 takes the
+ * library default for (width, depth), tile_threads < 0 the storage-generic
+ * kernel.  No reference counterpart: the reference has no batched signature
+ * entry point. */
 int rpp_hip_signature_tiles(int32_t dtype_bytes, int32_t* out, int32_t capacity);
 
 /* ------------------------------------------------------------------ misc */

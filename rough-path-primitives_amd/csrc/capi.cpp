@@ -14,6 +14,7 @@
 #include "lie.hpp"
 #include "signature.hpp"
 #include "signature_reg.hpp"
+#include "signature_chain.hpp"
 
 namespace rpp_hip {
 
@@ -382,6 +383,18 @@ int signature_impl(const rpp_hip_signature_desc* d, void* s) {
     int64_t* order = nullptr;
     RPP_TRY(path_order(d->offsets, d->num_paths, lengths, &order, stream));
     if (d->tile_threads < 0) return signature_forward_generic<T>(B, *d, order, stream);
+    if (d->tile_kind == 1 || d->tile_threads == 0) {
+        for (const auto& c : chain_tiles<T>()) {
+            if (c.width != d->basis.width || c.depth != d->basis.depth) continue;
+            if (d->tile_threads > 0 && (c.threads != d->tile_threads || c.split != d->tile_split ||
+                                        c.expand != d->tile_expand))
+                continue;
+            void* cnt = nullptr;
+            RPP_TRY(workspace(kSlotCounter, 256, &cnt));
+            return c.forward(B, *d, order, static_cast<unsigned*>(cnt), stream);
+        }
+        if (d->tile_kind == 1) return fail(RPP_HIP_NOT_IMPLEMENTED, "no compiled chain tile matches");
+    }
     if (const RegTile<T>* tile = pick_tile<T>(d->basis, d->tile_threads, d->tile_split, false)) {
         void* cnt = nullptr;
         RPP_TRY(workspace(kSlotCounter, 256, &cnt));
@@ -450,19 +463,19 @@ int64_t rpp_hip_signature_backward_scratch_bytes(const rpp_hip_signature_backwar
 
 extern "C" int rpp_hip_signature_tiles(int32_t dtype_bytes, int32_t* out, int32_t capacity) {
     int n = 0;
-    auto emit = [&](const auto& tiles) {
-        for (const auto& t : tiles) {
-            if (n < capacity && out) {
-                out[4 * n + 0] = t.width;
-                out[4 * n + 1] = t.depth;
-                out[4 * n + 2] = t.threads;
-                out[4 * n + 3] = t.split;
-            }
-            ++n;
+    auto put = [&](int w, int d, int th, int sp, int kind, int ex) {
+        if (n < capacity && out) {
+            int32_t* o = out + 6 * n;
+            o[0] = w, o[1] = d, o[2] = th, o[3] = sp, o[4] = kind, o[5] = ex;
         }
+        ++n;
     };
-    if (dtype_bytes == 4) emit(rpp_hip::reg_tiles<float>());
-    else if (dtype_bytes == 8) emit(rpp_hip::reg_tiles<double>());
+    auto emit = [&](const auto& reg, const auto& chain) {
+        for (const auto& t : chain) put(t.width, t.depth, t.threads, t.split, 1, t.expand);
+        for (const auto& t : reg) put(t.width, t.depth, t.threads, t.split, 0, 0);
+    };
+    if (dtype_bytes == 4) emit(rpp_hip::reg_tiles<float>(), rpp_hip::chain_tiles<float>());
+    else if (dtype_bytes == 8) emit(rpp_hip::reg_tiles<double>(), rpp_hip::chain_tiles<double>());
     else return rpp_hip::fail(RPP_HIP_INVALID_ARGUMENT, "dtype_bytes must be 4 or 8");
     return n;
 }

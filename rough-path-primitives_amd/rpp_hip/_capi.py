@@ -91,7 +91,8 @@ class LieConvertDesc(C.Structure):
 class SignatureDesc(C.Structure):
     _fields_ = [("basis", Basis), ("num_paths", _i64), ("increments", _vp), ("offsets", _vp),
                 ("out", TensorBatch), ("checkpoint_every", _i32), ("checkpoints", _vp),
-                ("ckpt_offsets", _vp), ("tile_threads", _i32), ("tile_split", _i32)]
+                ("ckpt_offsets", _vp), ("tile_threads", _i32), ("tile_split", _i32),
+                ("tile_kind", _i32), ("tile_expand", _i32)]
 
 
 class SignatureBackwardDesc(C.Structure):

@@ -40,7 +40,7 @@ def load() -> dict:
 
 
 def best(width: int, depth: int, dtype, kind: str):
-    """(threads, split) recorded for kind 'forward' or 'backward', or None."""
+    """(threads, split, kind, expand) recorded for 'forward' or 'backward', or None."""
     e = load().get(_key(width, depth, dtype), {}).get(kind)
     return tuple(e) if e else None
 
@@ -63,15 +63,18 @@ def tune(width: int, depth: int, dtype=torch.float64, paths: int = 1024, length:
     g = torch.Generator(device="cpu").manual_seed(1234)
     inc = (torch.randn(paths * length, width, generator=g, dtype=torch.float64) / length ** 0.5).to(dev, dtype)
     off = torch.arange(paths + 1, dtype=torch.int64, device=dev) * length
-    cands = [(t, s) for (w, n, t, s) in ops.signature_tiles(dtype) if (w, n) == (width, depth)] + [(-1, 0)]
+    cands = [(t, s, k, e) for (w, n, t, s, k, e) in ops.signature_tiles(dtype) if (w, n) == (width, depth)]
+    cands.append((-1, 0, 0, 0))
     result = {"forward": {}, "backward": {}}
     sig = ops.signature(inc, off, basis)
     ell = torch.randn(basis.size(), generator=g, dtype=torch.float64).to(dev, dtype)
-    for t, s in cands:
-        result["forward"][f"{t},{s}"] = _time(
-            lambda: ops.signature(inc, off, basis, out=sig, tile_threads=t, tile_split=s), reps)
-        if backward and s <= 1:
-            result["backward"][f"{t},{s}"] = _time(
+    for t, s, k, e in cands:
+        key = f"{t},{s},{k},{e}"
+        result["forward"][key] = _time(
+            lambda: ops.signature(inc, off, basis, out=sig, tile_threads=t, tile_split=s, tile_kind=k,
+                                  tile_expand=e), reps)
+        if backward and k == 0 and s <= 1:
+            result["backward"][key] = _time(
                 lambda: ops.signature_backward(inc, off, basis, sig, ell, tile_threads=t, tile_split=s), reps)
     entry = {}
     for kind in ("forward", "backward"):

@@ -312,14 +312,16 @@ def lie_basis(width: int, depth: int) -> dict:
 # -------------------------------------------------- batched Chen recursion
 
 def signature_tiles(dtype) -> list:
-    """Compiled register tiles (width, depth, threads, split) for a dtype."""
+    """Compiled tiles (width, depth, threads, split, kind, expand) for a dtype:
+    kind 1 = two-barrier forward tile, kind 0 = Horner register tile (forward
+    and, for split <= 1, backward)."""
     import numpy as np
     nbytes = torch.tensor([], dtype=dtype).element_size()
     lib = A.lib()
     n = lib.rpp_hip_signature_tiles(nbytes, None, 0)
-    buf = np.zeros(4 * max(n, 1), dtype=np.int32)
+    buf = np.zeros(6 * max(n, 1), dtype=np.int32)
     lib.rpp_hip_signature_tiles(nbytes, buf.ctypes.data, n)
-    return [tuple(int(v) for v in buf[4 * i: 4 * i + 4]) for i in range(n)]
+    return [tuple(int(v) for v in buf[6 * i: 6 * i + 6]) for i in range(n)]
 
 
 def _offsets_of(lengths, device) -> torch.Tensor:
@@ -332,7 +334,8 @@ def _offsets_of(lengths, device) -> torch.Tensor:
 def signature(increments: torch.Tensor, offsets: torch.Tensor, basis: TensorBasis,
               out: Optional[torch.Tensor] = None, checkpoint_every: int = 0,
               checkpoints: Optional[torch.Tensor] = None, ckpt_offsets: Optional[torch.Tensor] = None,
-              tile_threads: int = 0, tile_split: int = 0, stream=None) -> torch.Tensor:
+              tile_threads: int = 0, tile_split: int = 0, tile_kind: int = 0, tile_expand: int = 0,
+              stream=None) -> torch.Tensor:
     """Signatures of many paths of uneven length in one launch.
 
     increments: [total, width] device tensor; offsets: int64 device tensor [n+1].
@@ -341,7 +344,8 @@ def signature(increments: torch.Tensor, offsets: torch.Tensor, basis: TensorBasi
     dim = basis.size()
     if tile_threads == 0:
         from . import autotune
-        tile_threads, tile_split = autotune.best(basis.width, basis.depth, increments.dtype, "forward") or (0, 0)
+        tile_threads, tile_split, tile_kind, tile_expand = (
+            autotune.best(basis.width, basis.depth, increments.dtype, "forward") or (0, 0, 0, 0))
     if out is None:
         out = torch.empty((n, dim), dtype=increments.dtype, device=increments.device)
     inc = increments.contiguous()
@@ -349,7 +353,7 @@ def signature(increments: torch.Tensor, offsets: torch.Tensor, basis: TensorBasi
                         Batch(out, 0, basis.depth)._c(), checkpoint_every,
                         checkpoints.data_ptr() if checkpoints is not None else None,
                         ckpt_offsets.data_ptr() if ckpt_offsets is not None else None,
-                        tile_threads, tile_split)
+                        tile_threads, tile_split, tile_kind, tile_expand)
     _call("rpp_hip_signature" + _sfx(inc, out), d, stream)
     return out
 
@@ -366,7 +370,8 @@ def signature_backward(increments: torch.Tensor, offsets: torch.Tensor, basis: T
     inc = increments.contiguous()
     if tile_threads == 0:
         from . import autotune
-        tile_threads, tile_split = autotune.best(basis.width, basis.depth, increments.dtype, "backward") or (0, 0)
+        tile_threads, tile_split = (autotune.best(basis.width, basis.depth, increments.dtype, "backward")
+                                    or (0, 0, 0, 0))[:2]
     if grad is None:
         grad = torch.empty_like(inc)
     d = A.SignatureBackwardDesc(basis._c(), n, inc.data_ptr(), offsets.data_ptr(),

@@ -115,23 +115,24 @@ def test_every_register_tile_forward(dev, dtype):
     """Each compiled tile (and the storage-generic kernel) against the FP64
     oracle on uneven paths, lengths 0..6."""
     npd = np.float64 if dtype == torch.float64 else np.float32
-    for (w, N, nt, split) in _tiles(dtype):
-        rng = np.random.default_rng(w * 1000 + N * 10 + nt + split)
+    for (w, N, nt, split, kind, expand) in _tiles(dtype):
+        rng = np.random.default_rng(w * 1000 + N * 10 + nt + split + kind)
         inc, off = paths(rng, 7, w, 0, 6)
         basis = rpp_hip.TensorBasis(w, N)
         expected = O.signature(w, N, inc, off)
         inc_d = torch.tensor(inc, dtype=dtype, device=dev)
         off_d = torch.tensor(off, device=dev)
-        got = rpp_hip.signature(inc_d, off_d, basis, tile_threads=nt, tile_split=split)
-        assert_close(got.double().cpu().numpy(), expected, npd, f"tile {(w, N, nt, split)}")
+        got = rpp_hip.signature(inc_d, off_d, basis, tile_threads=nt, tile_split=split, tile_kind=kind,
+                                tile_expand=expand)
+        assert_close(got.double().cpu().numpy(), expected, npd, f"tile {(w, N, nt, split, kind, expand)}")
         gen = rpp_hip.signature(inc_d, off_d, basis, tile_threads=-1)
         assert_close(gen.double().cpu().numpy(), expected, npd, f"generic {(w, N)}")
 
 
 @pytest.mark.parametrize("ckpt", [0, 2])
 def test_every_register_tile_backward_fp64(dev, ckpt):
-    for (w, N, nt, split) in _tiles(torch.float64):
-        if split > 1:
+    for (w, N, nt, split, kind, _) in _tiles(torch.float64):
+        if split > 1 or kind != 0:
             continue
         rng = np.random.default_rng(w * 100 + N + nt + split)
         n = 4
@@ -150,7 +151,7 @@ def test_every_register_tile_backward_fp64(dev, ckpt):
             kw = dict(checkpoint_every=ckpt,
                       checkpoints=torch.zeros((max(int(ck_off[-1]), 1), dim), dtype=torch.float64, device=dev),
                       ckpt_offsets=torch.tensor(ck_off, device=dev))
-        sig = rpp_hip.signature(inc_d, off_d, basis, tile_threads=nt, tile_split=split, **kw)
+        sig = rpp_hip.signature(inc_d, off_d, basis, **kw)
         grad = rpp_hip.signature_backward(inc_d, off_d, basis, sig, torch.tensor(ell, device=dev),
                                           tile_threads=nt, tile_split=split, **kw)
         err = rel_err(grad.cpu().numpy(), expected)
@@ -175,3 +176,28 @@ def test_autograd_signature_and_ft_mul(dev):
     a = torch.tensor(rng.uniform(-1, 1, (2, dim)), device=dev, requires_grad=True)
     b = torch.tensor(rng.uniform(-1, 1, (2, dim)), device=dev, requires_grad=True)
     assert torch.autograd.gradcheck(lambda p, q: AG.ft_mul(p, q, w, N), (a, b), eps=1e-6, atol=1e-8)
+
+
+def test_chain_tiles_checkpoints_match(dev):
+    """Checkpoints written by the two-barrier tiles equal the signatures of
+    the path prefixes."""
+    for (w, N, nt, split, kind, expand) in _tiles(torch.float64):
+        if kind != 1:
+            continue
+        rng = np.random.default_rng(nt + split)
+        inc, off = paths(rng, 3, w, 4, 7)
+        basis = rpp_hip.TensorBasis(w, N)
+        dim = basis.size()
+        nck = np.diff(off) // 2
+        ck_off = np.zeros(4, dtype=np.int64)
+        ck_off[1:] = np.cumsum(nck)
+        ckpts = torch.zeros((int(ck_off[-1]), dim), dtype=torch.float64, device=dev)
+        rpp_hip.signature(torch.tensor(inc, device=dev), torch.tensor(off, device=dev), basis,
+                          checkpoint_every=2, checkpoints=ckpts, ckpt_offsets=torch.tensor(ck_off, device=dev),
+                          tile_threads=nt, tile_split=split, tile_kind=kind, tile_expand=expand)
+        got = ckpts.cpu().numpy()
+        for p in range(3):
+            for j in range(int(nck[p])):
+                pre = inc[off[p]: off[p] + 2 * (j + 1)]
+                expected = O.signature(w, N, pre, np.array([0, len(pre)]))[0]
+                assert_close(got[ck_off[p] + j], expected, np.float64, f"checkpoint tile {(w, N, nt, split)}")
