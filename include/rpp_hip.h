@@ -165,6 +165,25 @@ int rpp_hip_ft_adj_lmul_f64(const rpp_hip_ft_adjoint_desc* desc, void* stream);
 int rpp_hip_ft_adj_rmul_f32(const rpp_hip_ft_adjoint_desc* desc, void* stream);
 int rpp_hip_ft_adj_rmul_f64(const rpp_hip_ft_adjoint_desc* desc, void* stream);
 
+/* Adjoint of rpp::ops::ft_fmexp for a degree-one exponent (one Chen step
+ * S' = S (x) exp(x)): given the multiplier S, the exponent x (degree 1 of a
+ * tensor batch) and the cotangent G = d loss / d out, write
+ *   grad_multiplier = R_{exp(x)}^* G   (the transpose of right multiplication,
+ *                                       as rpp::ops::ft_adj_rmul with
+ *                                       op = exp(x), ft_adj_rmul.hpp:47-71)
+ *   grad_exponent   = the x-gradient, into degree 1 of a tensor batch.
+ * Full windows [0, depth] for multiplier, cotangent and grad_multiplier; the
+ * exponent and grad_exponent windows must hold degree 1.  No reference
+ * counterpart (the reference has no backward for ft_fmexp). */
+typedef struct rpp_hip_ft_fmexp_adjoint_desc {
+    rpp_hip_basis basis;
+    int64_t num_batches;
+    rpp_hip_tensor_batch multiplier, exponent, cotangent;
+    rpp_hip_tensor_batch grad_multiplier, grad_exponent;
+} rpp_hip_ft_fmexp_adjoint_desc;
+int rpp_hip_ft_fmexp_adjoint_f32(const rpp_hip_ft_fmexp_adjoint_desc* desc, void* stream);
+int rpp_hip_ft_fmexp_adjoint_f64(const rpp_hip_ft_fmexp_adjoint_desc* desc, void* stream);
+
 /* --------------------------------------------------------- sparse and Lie */
 
 enum rpp_hip_matrix_format { RPP_HIP_CSR = 0, RPP_HIP_CSC = 1 };

@@ -253,6 +253,21 @@ def signature_vjp(width, depth, increments, offsets, ell):
     return grad
 
 
+def fmexp_vjp(width, depth, mult, x, cot):
+    """Adjoint of one ft_fmexp with a degree-one exponent, composed from the
+    reference adjoints (FP64): mult, cot [n, dim]; x [n, width].  Returns
+    (grad_mult [n, dim], grad_x [n, width])."""
+    mult = np.ascontiguousarray(mult, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    cot = np.ascontiguousarray(cot, dtype=np.float64)
+    n = mult.shape[0]
+    gm = np.zeros_like(mult)
+    gx = np.zeros_like(x)
+    _call(lib(), "orc_fmexp_vjp_f64", _i(width), _i(depth), _ll(n), mult.ctypes.data_as(_p),
+          x.ctypes.data_as(_p), cot.ctypes.data_as(_p), gm.ctypes.data_as(_p), gx.ctypes.data_as(_p))
+    return gm, gx
+
+
 def ref_signature(width, depth, increments, threads=1):
     """Reference Chen recursion (one ft_fmexp per increment over the batch);
     increments [n, L, width]."""

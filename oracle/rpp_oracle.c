@@ -83,14 +83,85 @@ static void trunc_window(int mn, int mx, int a, int b, int* omn, int* omx)
 #undef T
 #undef SFX
 
-/* Gradient of loss_p = <ell, S_p> with respect to the increments of each path,
- * composed from the reference adjoints (operations/basic/ft_adj_lmul.hpp,
+/* Reverse mode of one Chen step S' = S (x) exp(x) for a degree-one x, composed
+ * from the reference adjoints (operations/basic/ft_adj_lmul.hpp,
  * ft_adj_rmul.hpp) as the reference leaves it to the caller:
- *   G_L = ell;  for t = L..1:  E_t = exp(x_t)
- *     Ebar    = ft_adj_lmul(op = S_{t-1}, arg = G_t)
- *     xbar_t  = J_exp(x_t)^T Ebar,   exp(x)_w = prod_i x_{w_i} / |w|!
- *     G_{t-1} = ft_adj_rmul(op = E_t, arg = G_t)
- * FP64 only.  ell is one functional (dim) shared by every path. */
+ *   E     = exp(x)
+ *   Ebar  = ft_adj_lmul(op = S, arg = G)
+ *   xbar  = J_exp(x)^T Ebar,   exp(x)_w = prod_i x_{w_i} / |w|!
+ *   Gprev = ft_adj_rmul(op = E, arg = G)
+ * Work arrays: scratch (width^depth + 1), e (dim, zero outside degree 1), E,
+ * Eb (dim), fact (depth + 1), letters (depth + 1). */
+static void orc_step_vjp_f64(const orc_basis* B, double* scratch, double* e, double* E, double* Eb,
+                             const double* fact, int* letters, const double* S, const double* x,
+                             const double* G, double* xb, double* Gprev)
+{
+    const int width = B->width, depth = B->depth;
+    for (int l = 0; l < width; ++l) e[1 + l] = x[l];
+    orc_ft_exp_f64(B, scratch, E, 0, depth, e, 1, 1);
+    orc_ft_adj_lmul_f64(B, Eb, 0, depth, S, 0, depth, G, 0, depth);
+    for (int l = 0; l < width; ++l) xb[l] = 0.0;
+    for (int k = 1; k <= depth; ++k) {
+        for (long long w = 0; w < B->pow[k]; ++w) {
+            const double g = Eb[B->start[k] + w] / fact[k];
+            if (g == 0.0) continue;
+            long long r = w;
+            for (int i = k - 1; i >= 0; --i) { letters[i] = (int)(r % width); r /= width; }
+            for (int i = 0; i < k; ++i) {
+                double prod = 1.0;
+                for (int j = 0; j < k; ++j)
+                    if (j != i) prod *= x[letters[j]];
+                xb[letters[i]] += g * prod;
+            }
+        }
+    }
+    orc_ft_adj_rmul_f64(B, Gprev, 0, depth, E, 0, depth, G, 0, depth);
+}
+
+typedef struct {
+    double *scratch, *e, *E, *Eb, *fact;
+    int* letters;
+} orc_vjp_work;
+
+static void orc_vjp_work_init(orc_vjp_work* w, const orc_basis* B)
+{
+    const long long dim = B->start[B->depth + 1];
+    w->scratch = (double*)malloc(sizeof(double) * (size_t)(B->pow[B->depth] + 1));
+    w->e = (double*)calloc((size_t)dim, sizeof(double));
+    w->E = (double*)malloc(sizeof(double) * (size_t)dim);
+    w->Eb = (double*)malloc(sizeof(double) * (size_t)dim);
+    w->fact = (double*)malloc(sizeof(double) * (size_t)(B->depth + 1));
+    w->letters = (int*)malloc(sizeof(int) * (size_t)(B->depth + 1));
+    w->fact[0] = 1.0;
+    for (int k = 1; k <= B->depth; ++k) w->fact[k] = w->fact[k - 1] * k;
+}
+
+static void orc_vjp_work_free(orc_vjp_work* w)
+{
+    free(w->scratch); free(w->e); free(w->E); free(w->Eb); free(w->fact); free(w->letters);
+}
+
+/* Adjoint of one ft_fmexp with a degree-one exponent over a batch of n
+ * objects: mult, cot, grad_mult are [n, dim], x and grad_x [n, width].
+ * FP64 only. */
+int orc_fmexp_vjp_f64(int width, int depth, long long n, const double* mult, const double* x,
+                      const double* cot, double* grad_mult, double* grad_x)
+{
+    orc_basis B;
+    if (orc_basis_init(&B, width, depth)) return 3;
+    const long long dim = B.start[depth + 1];
+    orc_vjp_work w;
+    orc_vjp_work_init(&w, &B);
+    for (long long b = 0; b < n; ++b)
+        orc_step_vjp_f64(&B, w.scratch, w.e, w.E, w.Eb, w.fact, w.letters, mult + b * dim, x + b * width,
+                         cot + b * dim, grad_x + b * width, grad_mult + b * dim);
+    orc_vjp_work_free(&w);
+    return 0;
+}
+
+/* Gradient of loss_p = <ell, S_p> with respect to the increments of each path:
+ * the forward chain of ft_fmexp, then orc_step_vjp_f64 for t = L..1 with
+ * G_L = ell.  FP64 only.  ell is one functional (dim) shared by every path. */
 int orc_signature_vjp_f64(int width, int depth, long long n_paths,
                           const long long* offsets, const double* increments,
                           const double* ell, double* grad)
@@ -98,16 +169,10 @@ int orc_signature_vjp_f64(int width, int depth, long long n_paths,
     orc_basis B;
     if (orc_basis_init(&B, width, depth)) return 3;
     const long long dim = B.start[depth + 1];
-    double* scratch = (double*)malloc(sizeof(double) * (size_t)(B.pow[depth] + 1));
-    double* e = (double*)calloc((size_t)dim, sizeof(double));
-    double* E = (double*)malloc(sizeof(double) * (size_t)dim);
+    orc_vjp_work w;
+    orc_vjp_work_init(&w, &B);
     double* G = (double*)malloc(sizeof(double) * (size_t)dim);
     double* G2 = (double*)malloc(sizeof(double) * (size_t)dim);
-    double* Eb = (double*)malloc(sizeof(double) * (size_t)dim);
-    double* fact = (double*)malloc(sizeof(double) * (size_t)(depth + 1));
-    int* letters = (int*)malloc(sizeof(int) * (size_t)(depth + 1));
-    fact[0] = 1.0;
-    for (int k = 1; k <= depth; ++k) fact[k] = fact[k - 1] * k;
     for (long long p = 0; p < n_paths; ++p) {
         const long long L = offsets[p + 1] - offsets[p];
         double* S = (double*)malloc(sizeof(double) * (size_t)(dim * (L + 1)));
@@ -115,37 +180,20 @@ int orc_signature_vjp_f64(int width, int depth, long long n_paths,
         S[0] = 1.0;
         for (long long t = 0; t < L; ++t) {
             const double* x = increments + (offsets[p] + t) * width;
-            for (int l = 0; l < width; ++l) e[1 + l] = x[l];
-            orc_ft_fmexp_f64(&B, scratch, S + (t + 1) * dim, 0, depth,
-                             S + t * dim, 0, depth, e, 1, 1);
+            for (int l = 0; l < width; ++l) w.e[1 + l] = x[l];
+            orc_ft_fmexp_f64(&B, w.scratch, S + (t + 1) * dim, 0, depth,
+                             S + t * dim, 0, depth, w.e, 1, 1);
         }
         for (long long i = 0; i < dim; ++i) G[i] = ell[i];
         for (long long t = L - 1; t >= 0; --t) {
             const double* x = increments + (offsets[p] + t) * width;
-            double* xb = grad + (offsets[p] + t) * width;
-            for (int l = 0; l < width; ++l) e[1 + l] = x[l];
-            orc_ft_exp_f64(&B, scratch, E, 0, depth, e, 1, 1);
-            orc_ft_adj_lmul_f64(&B, Eb, 0, depth, S + t * dim, 0, depth, G, 0, depth);
-            for (int l = 0; l < width; ++l) xb[l] = 0.0;
-            for (int k = 1; k <= depth; ++k) {
-                for (long long w = 0; w < B.pow[k]; ++w) {
-                    const double g = Eb[B.start[k] + w] / fact[k];
-                    if (g == 0.0) continue;
-                    long long r = w;
-                    for (int i = k - 1; i >= 0; --i) { letters[i] = (int)(r % width); r /= width; }
-                    for (int i = 0; i < k; ++i) {
-                        double prod = 1.0;
-                        for (int j = 0; j < k; ++j)
-                            if (j != i) prod *= x[letters[j]];
-                        xb[letters[i]] += g * prod;
-                    }
-                }
-            }
-            orc_ft_adj_rmul_f64(&B, G2, 0, depth, E, 0, depth, G, 0, depth);
+            orc_step_vjp_f64(&B, w.scratch, w.e, w.E, w.Eb, w.fact, w.letters, S + t * dim, x, G,
+                             grad + (offsets[p] + t) * width, G2);
             double* tmp = G; G = G2; G2 = tmp;
         }
         free(S);
     }
-    free(scratch); free(e); free(E); free(G); free(G2); free(Eb); free(fact); free(letters);
+    orc_vjp_work_free(&w);
+    free(G); free(G2);
     return 0;
 }

@@ -405,6 +405,42 @@ int signature_impl(const rpp_hip_signature_desc* d, void* s) {
 }
 
 template <typename T>
+int fmexp_adjoint_impl(const rpp_hip_ft_fmexp_adjoint_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    RPP_TRY(check_basis(d->basis));
+    const int64_t n = d->num_batches;
+    if (n < 0) return fail(RPP_HIP_INVALID_ARGUMENT, "negative num_batches");
+    if (n == 0) return 0;
+    RPP_TRY(check_view(d->multiplier, d->basis, n, "multiplier", false));
+    RPP_TRY(check_view(d->exponent, d->basis, n, "exponent", false));
+    RPP_TRY(check_view(d->cotangent, d->basis, n, "cotangent", false));
+    RPP_TRY(check_view(d->grad_multiplier, d->basis, n, "grad_multiplier", true));
+    RPP_TRY(check_view(d->grad_exponent, d->basis, n, "grad_exponent", true));
+    if (d->basis.depth < 1) return fail(RPP_HIP_INVALID_ARGUMENT, "ft_fmexp adjoint: depth must be >= 1");
+    if (d->exponent.max_degree != 1 || d->exponent.min_degree > 1 || d->grad_exponent.max_degree != 1 ||
+        d->grad_exponent.min_degree > 1)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "ft_fmexp adjoint: exponent windows must end at degree 1");
+    if (d->grad_multiplier.min_degree != 0 || d->grad_multiplier.max_degree != d->basis.depth)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "ft_fmexp adjoint: grad_multiplier must hold [0, depth]");
+    const void* outs[] = {d->grad_multiplier.data, d->grad_exponent.data};
+    const void* ins[] = {d->multiplier.data, d->exponent.data, d->cotangent.data};
+    for (const void* o : outs)
+        for (const void* i : ins)
+            if (o == i) return fail(RPP_HIP_INVALID_ARGUMENT, "ft_fmexp adjoint: gradients must not alias inputs");
+    if (d->grad_multiplier.data == d->grad_exponent.data)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "ft_fmexp adjoint: the two gradients must not alias");
+    const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    hipStream_t stream = as_stream(s);
+    const bool contig = d->multiplier.coeff_stride == 1 && d->cotangent.coeff_stride == 1 &&
+                        d->grad_multiplier.coeff_stride == 1 && d->exponent.coeff_stride == 1;
+    if (contig) {
+        if (const RegTile<T>* tile = fmexp_adj_tile<T>(d->basis.width, d->basis.depth))
+            return tile->fmexp_adjoint(B, *d, stream);
+    }
+    return fmexp_adjoint_generic<T>(B, *d, stream);
+}
+
+template <typename T>
 int signature_backward_impl(const rpp_hip_signature_backward_desc* d, void* s) {
     if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
     RPP_TRY(check_basis(d->basis));
@@ -443,6 +479,12 @@ int signature_backward_impl(const rpp_hip_signature_backward_desc* d, void* s) {
 
 extern "C" {
 
+int rpp_hip_ft_fmexp_adjoint_f32(const rpp_hip_ft_fmexp_adjoint_desc* d, void* s) {
+    return fmexp_adjoint_impl<float>(d, s);
+}
+int rpp_hip_ft_fmexp_adjoint_f64(const rpp_hip_ft_fmexp_adjoint_desc* d, void* s) {
+    return fmexp_adjoint_impl<double>(d, s);
+}
 int rpp_hip_signature_f32(const rpp_hip_signature_desc* d, void* s) { return signature_impl<float>(d, s); }
 int rpp_hip_signature_f64(const rpp_hip_signature_desc* d, void* s) { return signature_impl<double>(d, s); }
 int rpp_hip_signature_backward_f32(const rpp_hip_signature_backward_desc* d, void* s) {

@@ -77,6 +77,63 @@ __device__ void chen_step(const DevBasis& B, T* S, const T* xs, const T* inv, T*
     }
 }
 
+// Reverse mode of one Chen step taken at state S (the state before the
+// step, x in LDS): G, the cotangent of the new state, becomes in place the
+// cotangent of S (degree 0 included), and part[thread * 16 + a] accumulates
+// this thread's share of xbar[a] (cleared here).  Ast: Horner values of
+// levels 0..depth-1; Q0/Q1: reverse ping-pong of width^(depth-1) each.
+template <typename T, int W>
+__device__ void chen_step_adjoint(const DevBasis& B, const T* S, T* G, const T* xs, const T* inv, T* Ast, T* Q0,
+                                  T* Q1, T* part) {
+    using L = Letters<W>;
+    const uint32_t width = L::width(B);
+    const int N = B.depth;
+    for (uint32_t a = 0; a < width; ++a) part[threadIdx.x * kMaxBackwardWidth + a] = T(0);
+    __syncthreads();
+    for (int k = 1; k <= N; ++k) {
+        // forward Horner values A_0..A_{k-1} of target k
+        if (threadIdx.x == 0) Ast[0] = S[0];
+        __syncthreads();
+        for (int i = 1; i < k; ++i) {
+            const T c = inv[k - i + 1];
+            const uint32_t ni = B.pow[i];
+            T* Ai = Ast + B.start[i];
+            const T* Ap = Ast + B.start[i - 1];
+            const T* Si = S + B.start[i];
+            for (uint32_t w = threadIdx.x; w < ni; w += blockDim.x) {
+                const uint32_t q = L::div(B, w);
+                Ai[w] = Ap[q] * (xs[w - q * width] * c) + Si[w];
+            }
+            __syncthreads();
+        }
+        // reverse: children p*d + a of every prefix p, one thread each.  G_k
+        // is read before any later target changes it (targets ascend and
+        // target k only adds into levels below k).
+        const T* src = G + B.start[k];
+        for (int i = k; i >= 1; --i) {
+            const T c = inv[k - i + 1];
+            const uint32_t np = B.pow[i - 1];
+            T* dst = (i & 1) ? Q0 : Q1;
+            const T* Ap = Ast + B.start[i - 1];
+            T* Gp = G + B.start[i - 1];
+            for (uint32_t q = threadIdx.x; q < np; q += blockDim.x) {
+                const T apc = Ap[q] * c;
+                T acc = T(0);
+                for (uint32_t a = 0; a < width; ++a) {
+                    const T g = src[q * width + a];
+                    part[threadIdx.x * kMaxBackwardWidth + a] += g * apc;
+                    acc += g * xs[a];
+                }
+                acc *= c;
+                dst[q] = acc;
+                Gp[q] += acc;
+            }
+            __syncthreads();
+            src = dst;
+        }
+    }
+}
+
 template <typename T, int W>
 __global__ void __launch_bounds__(kSigBlock)
 k_sig_fwd_generic(DevBasis B, int64_t n_paths, const int64_t* order, const T* incr,
@@ -169,48 +226,7 @@ k_sig_bwd_generic(DevBasis B, int64_t n_paths, const int64_t* order, const T* in
             } else {
                 chen_step<T, W>(B, S, xneg, inv, P0, P1);
             }
-            for (uint32_t a = 0; a < width; ++a) part[threadIdx.x * kMaxBackwardWidth + a] = T(0);
-            __syncthreads();
-            for (int k = 1; k <= N; ++k) {
-                // forward Horner values A_0..A_{k-1} of target k
-                if (threadIdx.x == 0) Ast[0] = S[0];
-                __syncthreads();
-                for (int i = 1; i < k; ++i) {
-                    const T c = inv[k - i + 1];
-                    const uint32_t ni = B.pow[i];
-                    T* Ai = Ast + B.start[i];
-                    const T* Ap = Ast + B.start[i - 1];
-                    const T* Si = S + B.start[i];
-                    for (uint32_t w = threadIdx.x; w < ni; w += blockDim.x) {
-                        const uint32_t q = L::div(B, w);
-                        Ai[w] = Ap[q] * (xs[w - q * width] * c) + Si[w];
-                    }
-                    __syncthreads();
-                }
-                // reverse: children p*d + a of every prefix p, one thread each
-                const T* src = G + B.start[k];
-                for (int i = k; i >= 1; --i) {
-                    const T c = inv[k - i + 1];
-                    const uint32_t np = B.pow[i - 1];
-                    T* dst = (i & 1) ? Q0 : Q1;
-                    const T* Ap = Ast + B.start[i - 1];
-                    T* Gp = G + B.start[i - 1];
-                    for (uint32_t q = threadIdx.x; q < np; q += blockDim.x) {
-                        const T apc = Ap[q] * c;
-                        T acc = T(0);
-                        for (uint32_t a = 0; a < width; ++a) {
-                            const T g = src[q * width + a];
-                            part[threadIdx.x * kMaxBackwardWidth + a] += g * apc;
-                            acc += g * xs[a];
-                        }
-                        acc *= c;
-                        dst[q] = acc;
-                        if (i >= 2) Gp[q] += acc;
-                    }
-                    __syncthreads();
-                    src = dst;
-                }
-            }
+            chen_step_adjoint<T, W>(B, S, G, xs, inv, Ast, Q0, Q1, part);
             // deterministic reduction of the per-thread partials
             for (uint32_t a = threadIdx.x; a < width; a += blockDim.x) {
                 T acc = T(0);
@@ -218,6 +234,53 @@ k_sig_bwd_generic(DevBasis B, int64_t n_paths, const int64_t* order, const T* in
                 grad[t * width + a] = acc;
             }
         }
+    }
+}
+
+// Per-operation adjoint of ft_fmexp with a degree-one exponent, any tile
+// shape: one object per work-group iteration, S and G staged in the
+// work-group's workspace slice (windows zero-extended).
+template <typename T, int W>
+__global__ void __launch_bounds__(kSigBlock)
+k_fmexp_adj_generic(DevBasis B, int64_t n, View<const T> mult, View<const T> ex, View<const T> cot, View<T> gmult,
+                    View<T> gx, T* scratch, int64_t slot_elems) {
+    __shared__ T xs[kMaxBackwardWidth];
+    __shared__ T inv[kMaxDepth + 2];
+    __shared__ T part[kSigBlock * kMaxBackwardWidth];
+    using L = Letters<W>;
+    const uint32_t width = L::width(B);
+    const int N = B.depth;
+    const int64_t dim = B.start[N + 1];
+    const uint32_t top = N >= 1 ? B.pow[N - 1] : 1;
+    if (threadIdx.x <= static_cast<unsigned>(N + 1))
+        inv[threadIdx.x] = threadIdx.x == 0 ? T(1) : T(1) / T(threadIdx.x);
+    T* S = scratch + static_cast<int64_t>(blockIdx.x) * slot_elems;
+    T* G = S + dim;
+    T* Ast = G + dim;
+    T* Q0 = Ast + B.start[N];
+    T* Q1 = Q0 + top;
+    const int64_t m0 = B.start[mult.mn], m1 = B.start[mult.mx + 1];
+    const int64_t c0 = B.start[cot.mn], c1 = B.start[cot.mx + 1];
+    for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
+        __syncthreads();
+        const T* so = mult.obj(b);
+        const T* go = cot.obj(b);
+        for (int64_t i = threadIdx.x; i < dim; i += blockDim.x) {
+            S[i] = (m0 <= i && i < m1) ? so[i * mult.cs] : T(0);
+            G[i] = (c0 <= i && i < c1) ? go[i * cot.cs] : T(0);
+        }
+        for (uint32_t a = threadIdx.x; a < width; a += blockDim.x) xs[a] = ex.obj(b)[(1 + a) * ex.cs];
+        __syncthreads();
+        chen_step_adjoint<T, W>(B, S, G, xs, inv, Ast, Q0, Q1, part);
+        T* xo = gx.obj(b);
+        for (uint32_t a = threadIdx.x; a < width; a += blockDim.x) {
+            T acc = T(0);
+            for (unsigned j = 0; j < blockDim.x; ++j) acc += part[j * kMaxBackwardWidth + a];
+            xo[(1 + a) * gx.cs] = acc;
+        }
+        if (threadIdx.x == 0 && gx.mn == 0) xo[0] = T(0);
+        T* o = gmult.obj(b);
+        for (int64_t i = threadIdx.x; i < dim; i += blockDim.x) o[i * gmult.cs] = G[i];
     }
 }
 
@@ -312,6 +375,24 @@ int signature_backward_generic(const DevBasis& B, const rpp_hip_signature_backwa
     return check_hip(hipGetLastError(), "k_sig_bwd_generic launch");
 }
 
+template <typename T>
+int fmexp_adjoint_generic(const DevBasis& B, const rpp_hip_ft_fmexp_adjoint_desc& d, hipStream_t stream) {
+    if (B.width > kMaxBackwardWidth)
+        return fail(RPP_HIP_NOT_IMPLEMENTED, "ft_fmexp adjoint supports width <= 16 without a compiled tile");
+    if (d.num_batches <= 0) return 0;
+    const int64_t slot = backward_slot_elems(B);
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>(d.num_batches, device_cu_count() * 4));
+    void* ws = nullptr;
+    if (int rc = workspace(kSlotState, static_cast<size_t>(slot) * grid * sizeof(T) + 256, &ws)) return rc;
+    RPP_SIG_DISPATCH(B.width, (k_fmexp_adj_generic<T, W_><<<grid, kSigBlock, 0, stream>>>(
+                                  B, d.num_batches, make_cview<T>(d.multiplier), make_cview<T>(d.exponent),
+                                  make_cview<T>(d.cotangent), make_view<T>(d.grad_multiplier),
+                                  make_view<T>(d.grad_exponent), static_cast<T*>(ws), slot)));
+    return check_hip(hipGetLastError(), "k_fmexp_adj_generic launch");
+}
+
+template int fmexp_adjoint_generic<float>(const DevBasis&, const rpp_hip_ft_fmexp_adjoint_desc&, hipStream_t);
+template int fmexp_adjoint_generic<double>(const DevBasis&, const rpp_hip_ft_fmexp_adjoint_desc&, hipStream_t);
 template int signature_forward_generic<float>(const DevBasis&, const rpp_hip_signature_desc&, const int64_t*,
                                               hipStream_t);
 template int signature_forward_generic<double>(const DevBasis&, const rpp_hip_signature_desc&, const int64_t*,

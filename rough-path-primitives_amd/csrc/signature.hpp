@@ -23,4 +23,7 @@ int signature_backward_generic(const DevBasis& B, const rpp_hip_signature_backwa
 
 int64_t backward_slot_elems(const DevBasis& B);
 
+template <typename T>
+int fmexp_adjoint_generic(const DevBasis& B, const rpp_hip_ft_fmexp_adjoint_desc& d, hipStream_t stream);
+
 }  // namespace rpp_hip

@@ -68,12 +68,27 @@ struct Tile {
     T reg[G::total > 0 ? G::total : 1];
     T root;  // S_M[q] (for M = 0 this is S_0)
     T s0;    // S_0
+    T mid[M > 1 ? M - 1 : 1];  // S_1[q_1] .. S_{M-1}[q_1..q_{M-1}] when M >= 2
+
+    // prefix of q of length i (a word of level i), i <= M
+    __device__ __forceinline__ static int prefix(int q, int i) { return q / ipow(W, M - i); }
+    __device__ __forceinline__ static int letter(int q, int i) { return prefix(q, i) % W; }
+    // chain value of level i (0 <= i <= M)
+    // (for M = 0, root and s0 both hold S_0)
+    __device__ __forceinline__ T chain(int i) const { return i == M ? root : (i == 0 ? s0 : mid[i - 1]); }
+    __device__ __forceinline__ void set_chain(int i, T v) {
+        if (i == M) root = v;
+        if (i == 0) s0 = v;
+        if (i != 0 && i != M) mid[i - 1] = v;
+    }
 
     __device__ __forceinline__ void set_identity() {
 #pragma unroll
         for (int i = 0; i < G::total; ++i) reg[i] = T(0);
         root = M == 0 ? T(1) : T(0);
         s0 = T(1);
+#pragma unroll
+        for (int i = 0; i < (M > 1 ? M - 1 : 1); ++i) mid[i] = T(0);
     }
 
     // Store (out = true) or load the subtree words and the chain words this
@@ -99,18 +114,17 @@ struct Tile {
                 }
             }
         }
+        // chain words: level i <= M holds the prefix of q of length i; it is
+        // stored by the item whose remaining letters are all zero
         if constexpr (STORE) {
             if (t == 0) {
-                if (M == 0) {
-                    if (owns_chain) o[0] = root;
-                } else {
-                    o[G::global_index(B, M, q, 0) * cs] = root;
-                    if (owns_chain) o[0] = s0;
-                }
+#pragma unroll
+                for (int i = 0; i <= M; ++i)
+                    if (q % ipow(W, M - i) == 0) o[(B.start[i] + prefix(q, i)) * cs] = chain(i);
             }
         } else {
-            root = M == 0 ? o[0] : o[G::global_index(B, M, q, 0) * cs];
-            s0 = o[0];
+#pragma unroll
+            for (int i = 0; i <= M; ++i) set_chain(i, o[(B.start[i] + prefix(q, i)) * cs]);
         }
     }
 
@@ -131,9 +145,10 @@ struct Tile {
                 }
             }
         }
-        root = (mn <= M && M <= mx) ? o[M == 0 ? 0 : G::global_index(B, M, q, 0)] : T(0);
-        s0 = (mn <= 0 && 0 <= mx) ? o[0] : T(0);
+#pragma unroll
+        for (int i = 0; i <= M; ++i) set_chain(i, (mn <= i && i <= mx) ? o[B.start[i] + prefix(q, i)] : T(0));
     }
+
 };
 
 // One in-place Chen step S <- S (x) exp(x) on the tile.  xs: W increments
@@ -145,9 +160,11 @@ __device__ __forceinline__ void chen_step_reg(Tile<T, W, N, NT, M>& S, const T* 
     const int t = threadIdx.x;
 #pragma unroll
     for (int k = N; k >= M + 1; --k) {
-        // chain A_0 .. A_M along q (uniform); M <= 1 here
-        T a = S.root;
-        if (M == 1) a = S.s0 * (xs[q] * inv[k]) + S.root;
+        // chain A_0 .. A_M along q (uniform)
+        T a = S.s0;
+#pragma unroll
+        for (int i = 1; i <= M; ++i) a = a * (xs[S.letter(q, i)] * inv[k - i + 1]) + S.chain(i);
+        if (M == 0) a = S.root;
 #pragma unroll
         for (int i = M + 1; i <= k; ++i) {
             const T c = inv[k - i + 1];
@@ -177,8 +194,15 @@ __device__ __forceinline__ void chen_step_reg(Tile<T, W, N, NT, M>& S, const T* 
         }
         __syncthreads();
     }
-    // target level M: the root (chain) word
-    if (M == 1) S.root = S.s0 * xs[q] + S.root;
+    // target levels M .. 1: the chain words, descending so that each Horner
+    // reads old lower chain values
+#pragma unroll
+    for (int k = M; k >= 1; --k) {
+        T a = S.s0;
+#pragma unroll
+        for (int i = 1; i <= k; ++i) a = a * (xs[S.letter(q, i)] * inv[k - i + 1]) + S.chain(i);
+        S.set_chain(k, a);
+    }
 }
 
 template <typename T, int W, int N, int NT, int M>
@@ -260,7 +284,8 @@ int fmexp_launch(const DevBasis& B, const rpp_hip_tensor_batch& out, const rpp_h
 // comment): recompute A_{i-1}[p] for the owned parents, then sweep i = K..M+1.
 template <int K, typename T, int W, int N, int NT, int M>
 __device__ __forceinline__ void bwd_target(Tile<T, W, N, NT, M>& S, Tile<T, W, N, NT, M>& Gt, const T* xs,
-                                           const T* inv, T* lds0, T* lds1, int q, T (&xbar)[W], T& xq) {
+                                           const T* inv, T* lds0, T* lds1, int q, T (&xbar)[W], T& xq,
+                                           T& s0bar) {
     using G = Geo<W, N, NT, M>;
     const int t = threadIdx.x;
     constexpr int k = K;
@@ -317,10 +342,14 @@ __device__ __forceinline__ void bwd_target(Tile<T, W, N, NT, M>& S, Tile<T, W, N
                 if (i > M + 1) {
                     dst[p] = acc;
                 } else if (M == 1) {
-                    // Abar of the root word q (level 1)
+                    // Abar of the root word q (level 1): A_1 = S_0 x[q] / k + S_1[q]
                     const T ab = acc;
                     if (k > 1) Gt.root += ab;
                     xq += ab * S.s0 * inv[k];
+                    s0bar += ab * xs[q] * inv[k];
+                } else {
+                    // M = 0: Abar_0 of target k, the cotangent of S_0
+                    s0bar += acc;
                 }
             }
         }
@@ -331,8 +360,8 @@ __device__ __forceinline__ void bwd_target(Tile<T, W, N, NT, M>& S, Tile<T, W, N
 template <typename T, int W, int N, int NT, int M, int... Ks>
 __device__ __forceinline__ void bwd_targets(Tile<T, W, N, NT, M>& S, Tile<T, W, N, NT, M>& Gt, const T* xs,
                                             const T* inv, T* lds0, T* lds1, int q, T (&xbar)[W], T& xq,
-                                            std::integer_sequence<int, Ks...>) {
-    (bwd_target<M + 1 + Ks, T, W, N, NT, M>(S, Gt, xs, inv, lds0, lds1, q, xbar, xq), ...);
+                                            T& s0bar, std::integer_sequence<int, Ks...>) {
+    (bwd_target<M + 1 + Ks, T, W, N, NT, M>(S, Gt, xs, inv, lds0, lds1, q, xbar, xq, s0bar), ...);
 }
 
 template <typename T, int W, int N, int NT, int M>
@@ -386,7 +415,8 @@ k_sig_bwd_reg(DevBasis B, int64_t n_items, const int64_t* order, const T* incr, 
             // 1 first: new S_1[q] = S_0 x[q] + S_1[q], seeded by the old G_1[q]
             // before the higher targets add into it.
             T xq = (M == 1 && t == 0) ? Gt.root * S.s0 : T(0);
-            bwd_targets<T, W, N, NT, M>(S, Gt, xs, inv, lds0, lds1, q, xbar, xq,
+            T s0bar = T(0);  // unused: S_0 is the constant 1 along a path
+            bwd_targets<T, W, N, NT, M>(S, Gt, xs, inv, lds0, lds1, q, xbar, xq, s0bar,
                                         std::make_integer_sequence<int, (N > M ? N - M : 0)>{});
             // target level M = 1: root A_1 = S_0 x[q] + S_1[q], seeded by G_1[q]
             if (M == 1 && t == 0) {
@@ -408,6 +438,109 @@ k_sig_bwd_reg(DevBasis B, int64_t n_items, const int64_t* order, const T* incr, 
                 partial[(tt * G::P + q) * W + t] = acc;
             }
         }
+    }
+}
+
+// Sum the per-prefix partials of k_fmexp_adj_tile in prefix order:
+// grad_exponent[b] degree 1 (and 0 at degree 0 when its window holds it),
+// and the S_0 cotangent added to grad_mult[b][0].
+template <typename T, int W>
+__global__ void k_scatter_partials(const T* partial, View<T> gx, View<T> gm, int64_t n, int P) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n * (W + 1)) return;
+    const int64_t b = i / (W + 1);
+    const int a = static_cast<int>(i - b * (W + 1));
+    T acc = T(0);
+    for (int q = 0; q < P; ++q) acc += partial[(b * P + q) * (W + 1) + a];
+    if (a < W) {
+        gx.obj(b)[(1 + a) * gx.cs] = acc;
+    } else {
+        gm.obj(b)[0] += acc;
+        if (gx.mn == 0) gx.obj(b)[0] = T(0);
+    }
+}
+
+// Per-operation adjoint of ft_fmexp with a degree-one exponent (one Chen
+// step): given S = multiplier, x and G = d loss / d out, write
+//   grad_mult = R_{exp(x)}^* G  and  grad_x = J(x)^T L_S^* G
+// in one pass: the reverse sweep of k_sig_bwd_reg without the reconstruction.
+// Each work item (object b, prefix q) writes W partial x sums and its share of
+// the S_0 cotangent to partial[(b P + q)(W + 1) ..]; k_scatter_partials adds
+// them up in a fixed order.
+template <typename T, int W, int N, int NT, int M>
+__global__ void __launch_bounds__(NT)
+k_fmexp_adj_tile(DevBasis B, View<const T> mult, View<const T> ex, View<const T> cot, View<T> gmult, T* partial,
+                 int64_t n) {
+    static_assert(M <= 1, "chain cotangents stay inside one work item");
+    using G = Geo<W, N, NT, M>;
+    __shared__ T lds0[G::lds_elems];
+    __shared__ T lds1[G::lds_elems];
+    __shared__ T xs[W];
+    __shared__ T inv[N + 2];
+    __shared__ T red[NT / 64][W];
+    const int t = threadIdx.x;
+    if (t <= N + 1) inv[t] = t == 0 ? T(1) : T(1) / T(t);
+    for (int64_t item = blockIdx.x; item < n * G::P; item += gridDim.x) {
+        const int64_t b = item / G::P;
+        const int q = static_cast<int>(item - b * G::P);
+        Tile<T, W, N, NT, M> S, Gt;
+        S.load_window(B, mult.obj(b), q, mult.mn, mult.mx);
+        Gt.load_window(B, cot.obj(b), q, cot.mn, cot.mx);
+        __syncthreads();
+        if (t < W) xs[t] = ex.obj(b)[1 + t];
+        __syncthreads();
+        T xbar[W];
+#pragma unroll
+        for (int a = 0; a < W; ++a) xbar[a] = T(0);
+        // target level 1 when M = 1: new S_1[q] = S_0 x[q] + S_1[q]
+        T xq = (M == 1 && t == 0) ? Gt.root * S.s0 : T(0);
+        T s0bar = (M == 1 && t == 0) ? Gt.root * xs[q] : T(0);
+        bwd_targets<T, W, N, NT, M>(S, Gt, xs, inv, lds0, lds1, q, xbar, xq, s0bar,
+                                    std::make_integer_sequence<int, (N > M ? N - M : 0)>{});
+        if (M == 1 && t == 0) {
+#pragma unroll
+            for (int a = 0; a < W; ++a)
+                if (a == q) xbar[a] += xq;
+        }
+#pragma unroll
+        for (int a = 0; a < W; ++a) {
+            T v = xbar[a];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if ((t & 63) == 0) red[t >> 6][a] = v;
+        }
+        __syncthreads();
+        if (t < W) {
+            T acc = T(0);
+            for (int w = 0; w < NT / 64; ++w) acc += red[w][t];
+            partial[(b * G::P + q) * (W + 1) + t] = acc;
+        }
+        if (t == 0) partial[(b * G::P + q) * (W + 1) + W] = s0bar;
+        // grad_mult: the cotangent of the old state; degree 0 holds G_0 here and
+        // receives the summed S_0 cotangent from k_scatter_partials
+        Gt.template move<true>(B, gmult.obj(b), 1, q, q == 0);
+    }
+}
+
+template <typename T, int W, int N, int NT, int M>
+int fmexp_adj_launch(const DevBasis& B, const rpp_hip_ft_fmexp_adjoint_desc& d, hipStream_t stream) {
+    using G = Geo<W, N, NT, M>;
+    if constexpr (M > 1) {
+        return fail(RPP_HIP_NOT_IMPLEMENTED, "adjoint tiles fix at most one letter");
+    } else {
+        const int64_t n = d.num_batches;
+        void* part = nullptr;
+        if (int rc = workspace(kSlotTmp, static_cast<size_t>(std::max<int64_t>(n, 1)) * G::P * (W + 1) * sizeof(T),
+                               &part))
+            return rc;
+        const int64_t items = n * G::P;
+        const unsigned grid = static_cast<unsigned>(std::min<int64_t>(items, 65535LL * 16));
+        k_fmexp_adj_tile<T, W, N, NT, M><<<grid, NT, 0, stream>>>(
+            B, make_cview<T>(d.multiplier), make_cview<T>(d.exponent), make_cview<T>(d.cotangent),
+            make_view<T>(d.grad_multiplier), static_cast<T*>(part), n);
+        if (int rc = check_hip(hipGetLastError(), "k_fmexp_adj_tile launch")) return rc;
+        k_scatter_partials<T, W><<<static_cast<unsigned>((n * (W + 1) + 255) / 256), 256, 0, stream>>>(
+            static_cast<const T*>(part), make_view<T>(d.grad_exponent), make_view<T>(d.grad_multiplier), n, G::P);
+        return check_hip(hipGetLastError(), "k_scatter_partials launch");
     }
 }
 
@@ -468,38 +601,54 @@ int bwd_launch(const DevBasis& B, const rpp_hip_signature_backward_desc& d, cons
 // letters fixed per work-group, forward/backward launchers.
 #define RPP_TILE(T, W, N, NT, M)                                                                   \
     {sizeof(T), W, N, NT, M, &fwd_launch<T, W, N, NT, M>,                                          \
-     M <= 1 ? &bwd_launch<T, W, N, NT, (M <= 1 ? M : 1)> : nullptr, &fmexp_launch<T, W, N, NT, M>}
+     M <= 1 ? &bwd_launch<T, W, N, NT, (M <= 1 ? M : 1)> : nullptr, &fmexp_launch<T, W, N, NT, M>,      \
+     M <= 1 ? &fmexp_adj_launch<T, W, N, NT, M> : nullptr}
 
 template <typename T>
 const std::vector<RegTile<T>>& reg_tiles() {
     static const std::vector<RegTile<T>> tiles = {
         // BASELINE C2: width 5, depth 6
         RPP_TILE(T, 5, 6, 512, 0), RPP_TILE(T, 5, 6, 1024, 0), RPP_TILE(T, 5, 6, 640, 1), RPP_TILE(T, 5, 6, 320, 1),
+        RPP_TILE(T, 5, 6, 128, 2),
         // BASELINE C5: width 6, depth 6
-        RPP_TILE(T, 6, 6, 448, 1), RPP_TILE(T, 6, 6, 704, 1), RPP_TILE(T, 6, 6, 1024, 1),
+        RPP_TILE(T, 6, 6, 448, 1), RPP_TILE(T, 6, 6, 704, 1), RPP_TILE(T, 6, 6, 1024, 1), RPP_TILE(T, 6, 6, 256, 2),
         // BASELINE C4: width 8, depth 5
-        RPP_TILE(T, 8, 5, 512, 1), RPP_TILE(T, 8, 5, 256, 1),
+        RPP_TILE(T, 8, 5, 512, 1), RPP_TILE(T, 8, 5, 256, 1), RPP_TILE(T, 8, 5, 64, 2),
         // BASELINE C3: width 3, depth 8
-        RPP_TILE(T, 3, 8, 768, 0), RPP_TILE(T, 3, 8, 256, 1), RPP_TILE(T, 3, 8, 512, 1),
+        RPP_TILE(T, 3, 8, 768, 0), RPP_TILE(T, 3, 8, 256, 1), RPP_TILE(T, 3, 8, 512, 1), RPP_TILE(T, 3, 8, 256, 2),
         // BASELINE C1 shape: width 4, depth 4
         RPP_TILE(T, 4, 4, 64, 0), RPP_TILE(T, 4, 4, 128, 0),
     };
     return tiles;
 }
 
-// Plain ft_fmexp prefers a tile that fixes one letter: d work-groups per
-// object keep more of them in flight for this single-step, HBM-bound call.
+// Plain ft_fmexp prefers the tile that fixes the most letters, then the
+// smallest work-group: more, smaller work-groups per compute unit overlap
+// the loads, the Chen step and the stores of this single-step, HBM-bound call.
 template <typename T>
 const RegTile<T>* fmexp_tile(int width, int depth) {
-    const RegTile<T>* any = nullptr;
+    const RegTile<T>* best = nullptr;
     for (const auto& t : reg_tiles<T>()) {
         if (t.width != width || t.depth != depth) continue;
-        if (t.split == 1) return &t;
-        if (!any) any = &t;
+        if (!best || t.split > best->split || (t.split == best->split && t.threads < best->threads)) best = &t;
     }
-    return any;
+    return best;
 }
 
+// The per-operation adjoint needs a tile fixing at most one letter; prefer
+// one letter and the smallest work-group.
+template <typename T>
+const RegTile<T>* fmexp_adj_tile(int width, int depth) {
+    const RegTile<T>* best = nullptr;
+    for (const auto& t : reg_tiles<T>()) {
+        if (t.width != width || t.depth != depth || t.split > 1) continue;
+        if (!best || t.split > best->split || (t.split == best->split && t.threads < best->threads)) best = &t;
+    }
+    return best;
+}
+
+template const RegTile<float>* fmexp_adj_tile<float>(int, int);
+template const RegTile<double>* fmexp_adj_tile<double>(int, int);
 template const RegTile<float>* fmexp_tile<float>(int, int);
 template const RegTile<double>* fmexp_tile<double>(int, int);
 template const std::vector<RegTile<float>>& reg_tiles<float>();

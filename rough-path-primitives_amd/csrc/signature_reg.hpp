@@ -20,6 +20,8 @@ struct RegTile {
     // one plain ft_fmexp call (degree-one exponent, full output window)
     int (*fmexp)(const DevBasis&, const rpp_hip_tensor_batch&, const rpp_hip_tensor_batch&,
                  const rpp_hip_tensor_batch&, int64_t, hipStream_t);
+    // per-operation adjoint of that call (split <= 1)
+    int (*fmexp_adjoint)(const DevBasis&, const rpp_hip_ft_fmexp_adjoint_desc&, hipStream_t);
 };
 
 // Every compiled tile for dtype T; the first match of a (width, depth) is the
@@ -30,5 +32,8 @@ const std::vector<RegTile<T>>& reg_tiles();
 // The tile used by the plain ft_fmexp call for (width, depth), or nullptr.
 template <typename T>
 const RegTile<T>* fmexp_tile(int width, int depth);
+
+template <typename T>
+const RegTile<T>* fmexp_adj_tile(int width, int depth);
 
 }  // namespace rpp_hip

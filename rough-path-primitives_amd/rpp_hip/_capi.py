@@ -59,6 +59,12 @@ class FtFmexpDesc(C.Structure):
                 ("multiplier", TensorBatch), ("exponent", TensorBatch)]
 
 
+class FtFmexpAdjointDesc(C.Structure):
+    _fields_ = [("basis", Basis), ("num_batches", _i64), ("multiplier", TensorBatch),
+                ("exponent", TensorBatch), ("cotangent", TensorBatch), ("grad_multiplier", TensorBatch),
+                ("grad_exponent", TensorBatch)]
+
+
 class PairingDesc(C.Structure):
     _fields_ = [("basis", Basis), ("num_batches", _i64), ("out", ScalarBatch),
                 ("functional", TensorBatch), ("arg", TensorBatch)]
@@ -111,6 +117,7 @@ ENTRY_POINTS = {
     "rpp_hip_ft_exp": FtUnaryDesc,
     "rpp_hip_ft_log": FtUnaryDesc,
     "rpp_hip_ft_fmexp": FtFmexpDesc,
+    "rpp_hip_ft_fmexp_adjoint": FtFmexpAdjointDesc,
     "rpp_hip_tensor_pairing": PairingDesc,
     "rpp_hip_ft_adj_lmul": AdjointDesc,
     "rpp_hip_ft_adj_rmul": AdjointDesc,

@@ -209,6 +209,19 @@ def ft_fmexp(out, multiplier, exponent, basis: TensorBasis, num_batches=None, st
     _call("rpp_hip_ft_fmexp" + _sfx(out, multiplier, exponent), d, stream)
 
 
+def ft_fmexp_adjoint(grad_multiplier, grad_exponent, multiplier, exponent, cotangent, basis: TensorBasis,
+                     num_batches=None, stream=None):
+    """Adjoint of ft_fmexp for a degree-one exponent (one Chen step
+    out = multiplier (x) exp(x)).  Writes grad_multiplier = R_{exp(x)}^* cotangent
+    (ft_adj_rmul with op = exp(x), operations/basic/ft_adj_rmul.hpp:47-71) and
+    the exponent gradient into degree 1 of grad_exponent.  The exponent windows
+    end at degree 1; grad_multiplier holds the full window."""
+    gm, gx, m, x, g = (_as_batch(t, basis) for t in (grad_multiplier, grad_exponent, multiplier, exponent,
+                                                      cotangent))
+    d = A.FtFmexpAdjointDesc(basis._c(), _n(num_batches, gm), m._c(), x._c(), g._c(), gm._c(), gx._c())
+    _call("rpp_hip_ft_fmexp_adjoint" + _sfx(gm, gx, m, x, g), d, stream)
+
+
 def tensor_pairing(out: torch.Tensor, functional, arg, basis: TensorBasis, num_batches=None, stream=None):
     """rpp::ops::tensor_pairing (operations/basic/tensor_pairing.hpp:47-73);
     ``out`` is a 1-D device tensor (make_scalar_batch)."""

@@ -64,3 +64,22 @@ def test_signature_rejects_partial_windows():
     t = A.TensorBatch(8, 13, 1, 1, 2)
     d = A.SignatureDesc(A.Basis(3, 2), 2, 8, 8, t, 0, None, None, 0, 0)
     assert _call("rpp_hip_signature_f64", d) == 3
+
+
+def test_fmexp_adjoint_validation_on_host():
+    """Windows and aliasing of the ft_fmexp adjoint are checked before any HIP
+    call (fake non-null pointers never reach a kernel)."""
+    full = A.TensorBatch(0x1000, 31, 1, 0, 2)
+    other = A.TensorBatch(0x9000, 31, 1, 0, 2)
+    cot = A.TensorBatch(0x5000, 31, 1, 0, 2)
+    x = A.TensorBatch(0x2000, 6, 1, 1, 1)
+    gx = A.TensorBatch(0x3000, 6, 1, 1, 1)
+    basis = A.Basis(5, 2)
+    bad_x = A.TensorBatch(0x2000, 6, 1, 1, 2)
+    assert _call("rpp_hip_ft_fmexp_adjoint_f64", A.FtFmexpAdjointDesc(basis, 2, full, bad_x, cot, other, gx)) == 3
+    partial = A.TensorBatch(0x9000, 31, 1, 1, 2)
+    assert _call("rpp_hip_ft_fmexp_adjoint_f64", A.FtFmexpAdjointDesc(basis, 2, full, x, cot, partial, gx)) == 3
+    assert _call("rpp_hip_ft_fmexp_adjoint_f64", A.FtFmexpAdjointDesc(basis, 2, full, x, cot, cot, gx)) == 3
+    assert _call("rpp_hip_ft_fmexp_adjoint_f64", A.FtFmexpAdjointDesc(A.Basis(5, 0), 2, full, x, cot, other,
+                                                                      gx)) == 3
+    assert _call("rpp_hip_ft_fmexp_adjoint_f64", A.FtFmexpAdjointDesc(basis, 0, full, x, cot, other, gx)) == 0

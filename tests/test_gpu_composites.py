@@ -205,3 +205,50 @@ def test_ft_fmexp_interleaved_and_broadcast(dev, shape, dtype):
     out2 = torch.zeros((n, dim), dtype=out.dtype, device=dev)
     rpp_hip.ft_fmexp(out2, to_dev(mult, dev), rpp_hip.Batch(to_dev(ex, dev), 1, 1), basis)
     assert_close(out2.cpu().numpy(), expected, dtype, f"broadcast fmexp {shape}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("shape", [(5, 6), (6, 6), (8, 5), (3, 8), (4, 4), (2, 6), (3, 4), (7, 2), (16, 2)])
+@pytest.mark.parametrize("xw", [(1, 1), (0, 1)], ids=["x11", "x01"])
+def test_ft_fmexp_adjoint(dev, shape, xw, dtype):
+    """Adjoint of one ft_fmexp step (register tiles at the compiled shapes,
+    the staged kernel elsewhere) against the oracle composed from the reference
+    adjoints (ft_adj_lmul/ft_adj_rmul and the exp Jacobian)."""
+    w, N = shape
+    rng = np.random.default_rng(300 + 10 * w + N)
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    n = 5
+    S = damped_batch(rng, n, w, N, dtype)
+    G = damped_batch(rng, n, w, N, dtype)
+    x = rng.normal(0, 0.5, size=(n, w)).astype(dtype)
+    gm_ref, gx_ref = O.fmexp_vjp(w, N, S, x, G)
+    ex = np.zeros((n, 1 + w), dtype=dtype)
+    ex[:, 0] = 7.0  # junk at degree 0: the adjoint reads degree 1 only
+    ex[:, 1:] = x
+    gm = to_dev(np.full((n, dim), np.nan, dtype=dtype), dev)
+    gx = to_dev(np.full((n, 1 + w), np.nan, dtype=dtype), dev)
+    rpp_hip.ft_fmexp_adjoint(gm, rpp_hip.Batch(gx, *xw), to_dev(S, dev), rpp_hip.Batch(to_dev(ex, dev), *xw),
+                             to_dev(G, dev), basis)
+    got_x = gx.cpu().numpy()
+    if xw[0] == 0:
+        assert np.all(got_x[:, 0] == 0)
+    else:
+        assert np.all(np.isnan(got_x[:, 0]))
+    assert_close(gm.cpu().numpy(), gm_ref, dtype, f"fmexp adjoint grad_mult {shape}")
+    assert_close(got_x[:, 1:], gx_ref, dtype, f"fmexp adjoint grad_x {shape}")
+
+
+def test_ft_fmexp_adjoint_rejects_aliasing_and_windows(dev):
+    w, N = 3, 4
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    S = torch.zeros((2, dim), dtype=torch.float64, device=dev)
+    G = torch.zeros_like(S)
+    gm = torch.zeros_like(S)
+    ex = torch.zeros((2, 1 + w), dtype=torch.float64, device=dev)
+    gx = torch.zeros_like(ex)
+    with pytest.raises(rpp_hip.RppHipError):
+        rpp_hip.ft_fmexp_adjoint(G, rpp_hip.Batch(gx, 1, 1), S, rpp_hip.Batch(ex, 1, 1), G, basis)
+    with pytest.raises(rpp_hip.RppHipError):
+        rpp_hip.ft_fmexp_adjoint(gm, rpp_hip.Batch(gx, 1, 1), S, rpp_hip.Batch(S, 1, 2), G, basis)

@@ -173,3 +173,38 @@ def test_signature_vjp_finite_differences():
             m[t, a] -= h
             fd[t, a] = (O.signature(w, N, p, off)[0] @ ell - O.signature(w, N, m, off)[0] @ ell) / (2 * h)
     assert rel_err(g, fd) < 1e-7
+
+
+@pytest.mark.parametrize("shape", [(2, 5), (3, 4), (5, 3)])
+def test_fmexp_vjp_pairing_and_finite_differences(shape):
+    """Adjoint of one ft_fmexp step: out = S (x) exp(x) is linear in S, so
+    <grad_S, V> = <G, fmexp(V, x)> for every V (degree 0 included); the x part
+    against central differences."""
+    w, N = shape
+    dim = O.tensor_dimension(w, N)
+    rng = np.random.default_rng(70 + w)
+    n = 3
+    S = rng.normal(size=(n, dim))
+    x = rng.normal(0, 0.5, size=(n, w))
+    G = rng.normal(size=(n, dim))
+    gm, gx = O.fmexp_vjp(w, N, S, x, G)
+    e = np.zeros((n, dim))
+    e[:, 1:1 + w] = x
+
+    def fmexp(mult, ex):
+        out = np.zeros((n, dim))
+        O.orc.ft_fmexp(w, N, out, mult, ex, ew=(1, 1))
+        return out
+
+    V = rng.normal(size=(n, dim))
+    lhs = np.einsum("bi,bi->b", gm, V)
+    rhs = np.einsum("bi,bi->b", G, fmexp(V, e))
+    assert rel_err(lhs, rhs) < 1e-12
+    h = 1e-5
+    fd = np.zeros_like(x)
+    for a in range(w):
+        p, m = e.copy(), e.copy()
+        p[:, 1 + a] += h
+        m[:, 1 + a] -= h
+        fd[:, a] = np.einsum("bi,bi->b", G, fmexp(S, p) - fmexp(S, m)) / (2 * h)
+    assert rel_err(gx, fd) < 1e-7
