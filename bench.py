@@ -1,17 +1,32 @@
 """Benchmark of the truncated tensor algebra hot path on one MI355X.
 
-Workload (the configuration the headline metric is quoted on): signatures of
-4096 piecewise-linear paths, width 5, depth 6 (19531 coefficients), 128
-increments per path drawn iid N(0, 1/128) per channel, by the chained
-ft_fmexp of the reference's usage pattern (one call per increment,
+Headline workload (the configuration the headline metric is quoted on):
+signatures of 4096 piecewise-linear paths, width 5, depth 6 (19531
+coefficients), 128 increments per path drawn iid N(0, 1/128) per channel, by
+the chained ft_fmexp of the reference's usage pattern (one call per increment,
 ping-pong buffers, exponent view (1,1)); FP64.  One step is the whole chain
 (128 launches) over the batch, inputs resident in HBM.
 
 value = path-increments per second of that plain per-operation path.  The
-same line carries the fused path (all increments in one persistent launch),
-FP32 variants, the error of each against the FP64 oracle, the roofline of
-the dominant kernel (k_fmexp_tile, HBM-bound) and the reference CPU path
-(oracle/_ref, the real cpu::single_thread) timed on a bounded sample.
+same line carries, each under its own key:
+  plain_vs_fused  the headline shape through the fused persistent kernel, and
+                  FP32, with the error of each against the FP64 oracle;
+  fwd_bwd         forward + backward of <ell, S(path)>: 2048 paths, width 6,
+                  depth 6, 256 increments N(0, 1/256): the plain per-op path
+                  (ft_fmexp chain, then per increment ft_fmexp(S, -x) to
+                  reconstruct and the ft_fmexp adjoint) against the fused
+                  kernels (exp(-x) reconstruction, and sqrt(L) checkpoints);
+  ft_mul          1024 pairs of width 4 depth 4 tensors (GPU and reference
+                  CPU) and 2^17 pairs for the HBM fraction;
+  log_signature   16384 paths, width 3, depth 8, 64 increments: fused
+                  signature, ft_log, tensor_to_lie;
+  uneven          8192 paths, width 8, depth 5, lengths uniform in [16, 1024]:
+                  fused forward replayed from a hipGraph, FP32 and FP64;
+  roofline        the dominant kernel of the headline path (the ft_fmexp
+                  register tile, HBM-bound) from HIP events around its
+                  launches;
+  cpu_baseline    the reference cpu::single_thread chain (oracle/_ref) timed
+                  on a bounded sample of the headline workload.
 """
 from __future__ import annotations
 
@@ -31,10 +46,18 @@ WIDTH, DEPTH, PATHS, LENGTH = 5, 6, 4096, 128
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
+SECTIONS = ("fwd_bwd", "ft_mul", "log_signature", "uneven")
 
 
 def dim_of(w, n):
     return (w ** (n + 1) - 1) // (w - 1)
+
+
+def chen_flops(w, n):
+    """Floating-point operations of one Chen step S <- S (x) exp(x) in the
+    Horner form: one multiply-add per (target k, level i <= k, word of level i),
+    plus the scaled increments: 2 * sum_k sum_{i<=k} w^i."""
+    return 2 * sum(sum(w ** i for i in range(1, k + 1)) for k in range(1, n + 1))
 
 
 def make_increments(seed, paths, length, width):
@@ -42,70 +65,107 @@ def make_increments(seed, paths, length, width):
     return rng.normal(0.0, np.sqrt(1.0 / length), size=(paths, length, width))
 
 
+def rel_err(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(1.0, np.maximum(np.abs(a), np.abs(b)))))
+
+
+class EventTimer:
+    """HIP events on torch's current stream (where every rpp_hip call of this
+    script is launched), summed over the bracketed regions."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.pairs = []
+
+    def start(self):
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record()
+        self.pairs.append([e, None])
+
+    def stop(self):
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record()
+        self.pairs[-1][1] = e
+
+    def total_ms(self):
+        self.torch.cuda.synchronize()
+        return sum(a.elapsed_time(b) for a, b in self.pairs)
+
+
+def timed(torch, step, steps, warmup):
+    """Run `step` warmup times, then time exactly `steps` calls (device events
+    and host wall clock, synchronised on both sides)."""
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record()
+    out = None
+    for _ in range(steps):
+        out = step()
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return start.elapsed_time(stop) / steps, wall * 1e3 / steps, out
+
+
+# ------------------------------------------------------------ headline (C2)
+
 class Plain:
     """Chained ft_fmexp, one launch per increment (the reference usage)."""
 
-    def __init__(self, rpp_hip, torch, inc, dtype, dev):
-        self.rpp, self.torch = rpp_hip, torch
+    def __init__(self, rpp, torch, inc, dtype, dev, depth=DEPTH):
+        self.rpp, self.torch = rpp, torch
         n, L, w = inc.shape
         self.n, self.L, self.w = n, L, w
-        self.basis = rpp_hip.TensorBasis(w, DEPTH)
+        self.basis = rpp.TensorBasis(w, depth)
         dim = self.basis.size()
         padded = np.zeros((n, 1 + L * w))
         padded[:, 1:] = inc.reshape(n, L * w)
         self.pad = torch.tensor(padded, dtype=dtype, device=dev)
         self.a = torch.zeros((n, dim), dtype=dtype, device=dev)
         self.b = torch.empty_like(self.a)
-        self.ex = [rpp_hip.Batch(self.pad[:, t * w: t * w + 1 + w], 1, 1) for t in range(L)]
+        self.ex = [rpp.Batch(self.pad[:, t * w: t * w + 1 + w], 1, 1) for t in range(L)]
+        self.timer = None
 
-    def step(self):
+    def chain(self):
         self.a.zero_()
         self.a[:, 0] = 1.0
         cur, nxt = self.a, self.b
+        if self.timer:
+            self.timer.start()
         for t in range(self.L):
             self.rpp.ft_fmexp(nxt, cur, self.ex[t], self.basis)
             cur, nxt = nxt, cur
+        if self.timer:
+            self.timer.stop()
         return cur
+
+    step = chain
 
 
 class Fused:
     """All increments of every path in one persistent launch."""
 
-    def __init__(self, rpp_hip, torch, inc, dtype, dev):
-        self.rpp = rpp_hip
+    def __init__(self, rpp, torch, inc, dtype, dev, depth=DEPTH):
+        self.rpp = rpp
         n, L, w = inc.shape
-        self.basis = rpp_hip.TensorBasis(w, DEPTH)
+        self.basis = rpp.TensorBasis(w, depth)
         self.inc = torch.tensor(inc.reshape(n * L, w), dtype=dtype, device=dev)
         self.off = torch.arange(n + 1, dtype=torch.int64, device=dev) * L
+        self.order = rpp.path_order(self.off)
         self.out = torch.empty((n, self.basis.size()), dtype=dtype, device=dev)
 
     def step(self):
-        return self.rpp.signature(self.inc, self.off, self.basis, out=self.out)
-
-
-def time_steps(torch, runner, steps, warmup):
-    for _ in range(warmup):
-        runner.step()
-    torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    start.record()
-    for _ in range(steps):
-        out = runner.step()
-    stop.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    return start.elapsed_time(stop) / steps, wall / steps, out
-
-
-def rel_err(a, b):
-    return float(np.max(np.abs(a - b) / np.maximum(1.0, np.maximum(np.abs(a), np.abs(b)))))
+        return self.rpp.signature(self.inc, self.off, self.basis, out=self.out, order=self.order)
 
 
 def cpu_baseline(inc):
     """The reference cpu::single_thread chain (oracle/_ref), its batch split
-    over std::threads, on a bounded sample of the same workload."""
+    over std::threads, on a bounded sample of the headline workload."""
     import oracle as O
     threads = max(1, min(16, os.cpu_count() or 1))
     if O.has_ref():
@@ -124,16 +184,243 @@ def cpu_baseline(inc):
     dt = time.perf_counter() - t0
     units = sample.shape[0] * sample.shape[1]
     return {"value": units / dt, "unit": "path-increments/s", "cores": threads, "kind": kind,
-            "sample": f"{sample.shape[0]} paths x {sample.shape[1]} increments of the same workload "
+            "sample": f"{sample.shape[0]} paths x {sample.shape[1]} increments of the headline workload "
                       f"(width 5, depth 6, FP64), {dt:.2f} s"}
 
 
-def load_traffic():
-    p = os.path.join(ROOT, "profiles", "traffic_k_fmexp_tile_c2_fp64.json")
+def load_traffic(name):
+    p = os.path.join(ROOT, "profiles", name)
     if os.path.exists(p):
         with open(p) as f:
             return json.load(f).get("hbm_bytes_per_launch")
     return None
+
+
+def headline(rpp, torch, O, dev, steps, warmup):
+    inc = make_increments(0xC2, PATHS, LENGTH, WIDTH)
+    n_chk = 8
+    ref64 = O.signature(WIDTH, DEPTH, inc[:n_chk].reshape(-1, WIDTH),
+                        np.arange(n_chk + 1, dtype=np.int64) * LENGTH)
+    results = {}
+    roof = None
+    for name, cls, dt in (("plain_fp64", Plain, torch.float64), ("fused_fp64", Fused, torch.float64),
+                          ("plain_fp32", Plain, torch.float32), ("fused_fp32", Fused, torch.float32)):
+        runner = cls(rpp, torch, inc, dt, dev)
+        if name == "plain_fp64":
+            for _ in range(warmup):
+                runner.step()
+            runner.timer = EventTimer(torch)
+            ms, wall, out = timed(torch, runner.step, steps, 0)
+            chain_ms = runner.timer.total_ms() / steps
+            roof = chain_ms / LENGTH
+        else:
+            ms, wall, out = timed(torch, runner.step, steps, warmup)
+        results[name] = {"ms_per_step": ms, "wall_ms_per_step": wall,
+                         "path_increments_per_s": PATHS * LENGTH / (ms * 1e-3),
+                         "max_rel_err_vs_fp64_oracle": rel_err(out[:n_chk].double().cpu().numpy(), ref64)}
+        del runner
+        torch.cuda.empty_cache()
+    return inc, results, roof
+
+
+# ----------------------------------------------------------- fwd+bwd (C5)
+
+def fwd_bwd(rpp, torch, O, dev, steps=2, warmup=1):
+    w, N, n, L = 6, 6, 2048, 256
+    inc = make_increments(0xC5, n, L, w)
+    basis = rpp.TensorBasis(w, N)
+    dim = basis.size()
+    rng = np.random.default_rng(0xC5 + 1)
+    ell = torch.tensor(rng.normal(size=dim), device=dev)
+    flat = torch.tensor(inc.reshape(n * L, w), device=dev)
+    off = torch.arange(n + 1, dtype=torch.int64, device=dev) * L
+    order = rpp.path_order(off)
+    loss = torch.empty(n, dtype=torch.float64, device=dev)
+
+    # plain per-op path: buffers for two states and two cotangents
+    padded = torch.zeros((n, 1 + L * w), dtype=torch.float64, device=dev)
+    padded[:, 1:] = flat.view(n, L * w)
+    gpad = torch.zeros_like(padded)
+    S0 = torch.zeros((n, dim), dtype=torch.float64, device=dev)
+    S1 = torch.empty_like(S0)
+    G0 = torch.empty_like(S0)
+    G1 = torch.empty_like(S0)
+    ex = [rpp.Batch(padded[:, t * w: t * w + 1 + w], 1, 1) for t in range(L)]
+    gx = [rpp.Batch(gpad[:, t * w: t * w + 1 + w], 1, 1) for t in range(L)]
+
+    def plain():
+        S0.zero_()
+        S0[:, 0] = 1.0
+        cur, nxt = S0, S1
+        for t in range(L):
+            rpp.ft_fmexp(nxt, cur, ex[t], basis)
+            cur, nxt = nxt, cur
+        rpp.tensor_pairing(loss, rpp.Batch(ell, 0, N), cur, basis, num_batches=n)
+        neg = -padded
+        nex = [rpp.Batch(neg[:, t * w: t * w + 1 + w], 1, 1) for t in range(L)]
+        Gc, Gn = G0, G1
+        Gc.copy_(ell.expand(n, dim))
+        for t in range(L - 1, -1, -1):
+            rpp.ft_fmexp(nxt, cur, nex[t], basis)  # S_{t-1} = S_t exp(-x_t)
+            rpp.ft_fmexp_adjoint(Gn, gx[t], nxt, ex[t], Gc, basis)
+            cur, nxt = nxt, cur
+            Gc, Gn = Gn, Gc
+        return gpad
+
+    sig = torch.empty((n, dim), dtype=torch.float64, device=dev)
+    grad = torch.empty_like(flat)
+
+    def fused(every=0, ck=None, ck_off=None):
+        kw = dict(checkpoint_every=every, checkpoints=ck, ckpt_offsets=ck_off) if every else {}
+
+        def step():
+            rpp.signature(flat, off, basis, out=sig, order=order, **kw)
+            rpp.tensor_pairing(loss, rpp.Batch(ell, 0, N), sig, basis, num_batches=n)
+            rpp.signature_backward(flat, off, basis, sig, ell, grad=grad, order=order, **kw)
+            return grad
+        return step
+
+    every = 16
+    ck = torch.empty((n * (L // every), dim), dtype=torch.float64, device=dev)
+    ck_off = torch.arange(n + 1, dtype=torch.int64, device=dev) * (L // every)
+    # FP64 oracle gradient of one path
+    g_ref = O.signature_vjp(w, N, inc[0], np.array([0, L], dtype=np.int64), ell.cpu().numpy())
+    out = {"config": {"paths": n, "width": w, "depth": N, "coefficients": dim, "increments_per_path": L,
+                      "loss": "tensor_pairing with a seeded N(0,1) functional", "dtype": "f64", "steps": steps,
+                      "warmup": warmup}}
+    for name, fn in (("plain", plain), ("fused_reconstruct", fused()),
+                     ("fused_ckpt16", fused(every, ck, ck_off))):
+        ms, wall, g = timed(torch, fn, steps, warmup)
+        if name == "plain":
+            g0 = g[0, 1:].view(L, w)
+        else:
+            g0 = g[:L]
+        out[name] = {"ms_per_step": ms, "wall_ms_per_step": wall, "paths_per_s": n / (ms * 1e-3),
+                     "grad_max_rel_err_vs_fp64_oracle_path0": rel_err(g0.cpu().numpy(), g_ref)}
+        torch.cuda.empty_cache()
+    out["fused_speedup"] = out["plain"]["ms_per_step"] / out["fused_reconstruct"]["ms_per_step"]
+    out["fused_ckpt_speedup"] = out["plain"]["ms_per_step"] / out["fused_ckpt16"]["ms_per_step"]
+    del S0, S1, G0, G1, ck, padded, gpad
+    torch.cuda.empty_cache()
+    return out
+
+
+# ------------------------------------------------------------- ft_mul (C1)
+
+def ft_mul_section(rpp, torch, O, dev):
+    w, N = 4, 4
+    basis = rpp.TensorBasis(w, N)
+    dim = basis.size()
+    res = {}
+    for label, n, reps in (("c1_1024_pairs", 1024, 200), ("hbm_131072_pairs", 1 << 17, 50)):
+        rng = np.random.default_rng(0xC1)
+        lhs = rng.uniform(-1, 1, size=(n, dim))
+        rhs = rng.uniform(-1, 1, size=(n, dim))
+        lhs[:, 0] = 1.0
+        rhs[:, 0] = 1.0
+        L = torch.tensor(lhs, device=dev)
+        R = torch.tensor(rhs, device=dev)
+        O_ = torch.empty_like(L)
+        ms, _, _ = timed(torch, lambda: rpp.ft_mul(O_, L, R, basis), reps, 5)
+        k = min(n, 256)
+        expected = np.zeros((k, dim))
+        O.orc.ft_mul(w, N, expected, lhs[:k], rhs[:k])
+        entry = {"pairs": n, "us_per_call": ms * 1e3,
+                 "max_rel_err_vs_fp64_oracle": rel_err(O_[:k].cpu().numpy(), expected)}
+        alg = 3 * n * dim * 8
+        entry["algorithmic_bytes"] = alg
+        entry["achieved_gbs"] = alg / (ms * 1e-3) / 1e9
+        entry["hbm_frac"] = entry["achieved_gbs"] / HBM_PEAK_GBS
+        if n == 1024 and O.has_ref():
+            out_cpu = np.zeros((n, dim))
+            t0 = time.perf_counter()
+            r = 20
+            for _ in range(r):
+                O.refimpl.ft_mul(w, N, out_cpu, lhs, rhs)
+            entry["cpu_reference_us_per_call"] = (time.perf_counter() - t0) / r * 1e6
+            entry["cpu_reference_cores"] = 1
+            entry["cpu_reference_max_rel_err_vs_gpu"] = rel_err(O_.cpu().numpy(), out_cpu)
+        res[label] = entry
+        del L, R, O_
+        torch.cuda.empty_cache()
+    res["config"] = {"width": w, "depth": N, "coefficients": dim, "dtype": "f64",
+                     "inputs": "Uniform[-1,1], degree 0 set to 1"}
+    return res
+
+
+# ---------------------------------------------------------- log-signature (C3)
+
+def log_signature(rpp, torch, O, dev, steps=3, warmup=1):
+    import scipy.sparse as sp
+    w, N, n, L = 3, 8, 16384, 64
+    inc = make_increments(0xC3, n, L, w)
+    basis = rpp.TensorBasis(w, N)
+    dim = basis.size()
+    lie = rpp.lie_basis(w, N)
+    ts = lie["true_size"]
+    flat = torch.tensor(inc.reshape(n * L, w), device=dev)
+    off = torch.arange(n + 1, dtype=torch.int64, device=dev) * L
+    order = rpp.path_order(off)
+    sig = torch.empty((n, dim), dtype=torch.float64, device=dev)
+    lg = torch.empty_like(sig)
+    lie_out = torch.empty((n, ts), dtype=torch.float64, device=dev)
+    rpp.tensor_to_lie(lie_out, lg, basis)  # uploads the Hall metadata once
+
+    def step():
+        rpp.signature(flat, off, basis, out=sig, order=order)
+        rpp.ft_log(lg, sig, basis)
+        rpp.tensor_to_lie(lie_out, lg, basis)
+        return lie_out
+
+    ms, wall, got = timed(torch, step, steps, warmup)
+    k = 4
+    s_ref = O.signature(w, N, inc[:k].reshape(-1, w), np.arange(k + 1, dtype=np.int64) * L)
+    log_ref = np.zeros_like(s_ref)
+    O.orc.ft_log(w, N, log_ref, s_ref)
+    t2l = lie["t2l"]
+    T2L = sp.csc_matrix((t2l["data"], t2l["indices"], t2l["offsets"]), shape=(ts - 1, dim))
+    lie_ref = (T2L @ log_ref.T).T
+    res = {"config": {"paths": n, "width": w, "depth": N, "coefficients": dim, "lie_dimension": ts - 1,
+                      "increments_per_path": L, "dtype": "f64", "steps": steps, "warmup": warmup},
+           "ms_per_step": ms, "wall_ms_per_step": wall, "paths_per_s": n / (ms * 1e-3),
+           "log_max_rel_err_vs_fp64_oracle": rel_err(lg[:k].cpu().numpy(), log_ref),
+           "lie_max_rel_err_vs_fp64_oracle": rel_err(got[:k, 1:].cpu().numpy(), lie_ref)}
+    del sig, lg, lie_out
+    torch.cuda.empty_cache()
+    return res
+
+
+# ------------------------------------------------------------- uneven (C4)
+
+def uneven(rpp, torch, O, dev, steps=3, warmup=1):
+    w, N, n = 8, 5, 8192
+    rng = np.random.default_rng(0xC4)
+    lengths = rng.integers(16, 1025, size=n)
+    off_np = np.zeros(n + 1, dtype=np.int64)
+    off_np[1:] = np.cumsum(lengths)
+    inc = np.concatenate([rng.normal(0.0, np.sqrt(1.0 / Lp), size=(Lp, w)) for Lp in lengths])
+    basis = rpp.TensorBasis(w, N)
+    off = torch.tensor(off_np, device=dev)
+    total = int(off_np[-1])
+    chk = np.argsort(lengths)[:2]
+    ref = np.concatenate([O.signature(w, N, inc[off_np[p]:off_np[p + 1]], np.array([0, lengths[p]]))
+                          for p in chk])
+    res = {"config": {"paths": n, "width": w, "depth": N, "coefficients": basis.size(),
+                      "lengths": "uniform integers in [16, 1024]", "increments": total,
+                      "launch": "hipGraph replay of the fused persistent kernel", "steps": steps,
+                      "warmup": warmup}}
+    for name, dt, peak in (("fp32", torch.float32, FP32_PEAK_TFLOPS), ("fp64", torch.float64, FP64_PEAK_TFLOPS)):
+        g = rpp.SignatureGraph(torch.tensor(inc, dtype=dt, device=dev), off, basis)
+        ms, wall, out = timed(torch, g.replay, steps, warmup)
+        tflops = total * chen_flops(w, N) / (ms * 1e-3) / 1e12
+        res[name] = {"ms_per_step": ms, "wall_ms_per_step": wall,
+                     "path_increments_per_s": total / (ms * 1e-3), "paths_per_s": n / (ms * 1e-3),
+                     "vector_tflops": tflops, "vector_frac": tflops / peak,
+                     "max_rel_err_vs_fp64_oracle": rel_err(out[torch.tensor(chk, device=dev)].double().cpu().numpy(),
+                                                           ref)}
+        del g, out
+        torch.cuda.empty_cache()
+    return res
 
 
 def main():
@@ -142,31 +429,26 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sections", default=",".join(SECTIONS),
+                    help="comma list of extra sections to run (" + ",".join(SECTIONS) + "), or 'none'")
     ap.add_argument("--only", default=None,
-                    help="run one variant (plain_fp64, fused_fp64, plain_fp32, fused_fp32) and print its timing")
+                    help="run one headline variant (plain_fp64, fused_fp64, plain_fp32, fused_fp32) and print "
+                         "its timing only")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(paths are independent; see DESIGN.md for how the batch shards)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: paths are independent, "
+              "so a multi-GPU run would shard the batch by paths with no communication in the hot path",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
 
     import oracle as O
-    import rpp_hip
+    import rpp_hip as rpp
 
     dev = torch.device("cuda:0")
-    inc = make_increments(0xC2, PATHS, LENGTH, WIDTH)
-    dim = dim_of(WIDTH, DEPTH)
-    units = PATHS * LENGTH
-
-    # FP64 oracle on a few paths for the error statements
-    n_chk = 8
-    ref64 = O.signature(WIDTH, DEPTH, inc[:n_chk].reshape(-1, WIDTH),
-                        np.arange(n_chk + 1, dtype=np.int64) * LENGTH)
-
     if args.copy_mib:
         src = torch.ones(args.copy_mib * (1 << 20) // 8, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)
@@ -174,33 +456,25 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    variants = (("plain_fp64", Plain, torch.float64), ("fused_fp64", Fused, torch.float64),
-                ("plain_fp32", Plain, torch.float32), ("fused_fp32", Fused, torch.float32))
     if args.only:
-        variants = [v for v in variants if v[0] == args.only]
-        name, cls, dt = variants[0]
-        runner = cls(rpp_hip, torch, inc, dt, dev)
-        ms_ev, _, _ = time_steps(torch, runner, args.steps, args.warmup)
-        print(json.dumps({"variant": name, "ms_per_step": ms_ev, "path_increments_per_s": units / (ms_ev * 1e-3)}))
+        inc = make_increments(0xC2, PATHS, LENGTH, WIDTH)
+        cls = Plain if args.only.startswith("plain") else Fused
+        dt = torch.float64 if args.only.endswith("fp64") else torch.float32
+        runner = cls(rpp, torch, inc, dt, dev)
+        ms, _, _ = timed(torch, runner.step, args.steps, args.warmup)
+        print(json.dumps({"variant": args.only, "ms_per_step": ms,
+                          "path_increments_per_s": PATHS * LENGTH / (ms * 1e-3)}))
         return
 
-    results = {}
-    for name, cls, dt in variants:
-        runner = cls(rpp_hip, torch, inc, dt, dev)
-        ms_ev, s_wall, out = time_steps(torch, runner, args.steps, args.warmup)
-        err = rel_err(out[:n_chk].double().cpu().numpy(), ref64)
-        results[name] = {"ms_per_step": ms_ev, "wall_ms_per_step": s_wall * 1e3,
-                         "path_increments_per_s": units / (ms_ev * 1e-3), "max_rel_err_vs_fp64_oracle": err}
-        del runner
-        torch.cuda.empty_cache()
-
+    inc, results, launch_ms = headline(rpp, torch, O, dev, args.steps, args.warmup)
     plain = results["plain_fp64"]
-    # roofline of the dominant kernel: one k_fmexp_deg1 launch per increment
-    launch_ms = plain["ms_per_step"] / LENGTH
+    dim = dim_of(WIDTH, DEPTH)
     alg_bytes = PATHS * (2 * dim + WIDTH) * 8
     achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
+    tile = rpp.fmexp_tile(WIDTH, DEPTH, torch.float64)
+    kernel = f"k_fmexp_tile<double,{WIDTH},{DEPTH},{tile[0]},{tile[1]}>" if tile else "k_fmexp_horner"
     fused = results["fused_fp64"]
-    fused_flops = PATHS * LENGTH * 2 * sum((WIDTH ** (k + 1) - WIDTH) // (WIDTH - 1) for k in range(1, DEPTH + 1))
+    fused_tflops = PATHS * LENGTH * chen_flops(WIDTH, DEPTH) / (fused["ms_per_step"] * 1e-3) / 1e12
     line = {
         "metric": "ft_fmexp increments/s + fwd+bwd paths/s, fused vs plain GPU, err vs FP64",
         "value": plain["path_increments_per_s"],
@@ -218,19 +492,24 @@ def main():
                    "paths": PATHS, "width": WIDTH, "depth": DEPTH, "coefficients": dim,
                    "increments_per_path": LENGTH},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(),
-                     "kernel": "k_fmexp_tile<double,5,6,640,1>", "avg_launch_ms": launch_ms,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": load_traffic("traffic_k_fmexp_tile_c2_fp64.json"),
+                     "kernel": kernel, "avg_launch_ms": launch_ms,
                      "algorithmic_bytes_per_launch": alg_bytes},
         "max_rel_err_vs_fp64_oracle": plain["max_rel_err_vs_fp64_oracle"],
-        "fused_vs_plain": {
-            name: r for name, r in results.items()
-        },
+        "plain_vs_fused": results,
         "fused_speedup_fp64": fused["path_increments_per_s"] / plain["path_increments_per_s"],
-        "fused_fp64_vector_tflops": fused_flops / (fused["ms_per_step"] * 1e-3) / 1e12,
-        "fused_fp64_vector_frac": fused_flops / (fused["ms_per_step"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+        "fused_fp64_vector_tflops": fused_tflops,
+        "fused_fp64_vector_frac": fused_tflops / FP64_PEAK_TFLOPS,
     }
+    del inc
+    wanted = [] if args.sections == "none" else [s for s in args.sections.split(",") if s]
+    runners = {"fwd_bwd": fwd_bwd, "ft_mul": ft_mul_section, "log_signature": log_signature, "uneven": uneven}
+    for name in wanted:
+        print(f"section {name}", file=sys.stderr, flush=True)
+        line[name] = runners[name](rpp, torch, O, dev)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(inc)
+        line["cpu_baseline"] = cpu_baseline(make_increments(0xC2, PATHS, LENGTH, WIDTH))
     print(json.dumps(line))
 
 

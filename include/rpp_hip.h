@@ -184,6 +184,22 @@ typedef struct rpp_hip_ft_fmexp_adjoint_desc {
 int rpp_hip_ft_fmexp_adjoint_f32(const rpp_hip_ft_fmexp_adjoint_desc* desc, void* stream);
 int rpp_hip_ft_fmexp_adjoint_f64(const rpp_hip_ft_fmexp_adjoint_desc* desc, void* stream);
 
+/* Adjoint of rpp::ops::ft_log (operations/intermediate/ft_log.hpp:66-85):
+ * given arg and the cotangent G = d loss / d log(arg) (log taken over the
+ * full window [0, depth]; G's window is zero-extended), write
+ * grad_arg = d loss / d arg on arg's window (degree 0 gets 0: ft_log reads
+ * arg from degree 1).  grad_arg's window must equal arg's and it must not
+ * alias arg or the cotangent.  Uses a library workspace of about
+ * (3 + 1/(width-1)) * dim coefficients per object.  No reference counterpart
+ * (the reference has no backward for ft_log). */
+typedef struct rpp_hip_ft_log_adjoint_desc {
+    rpp_hip_basis basis;
+    int64_t num_batches;
+    rpp_hip_tensor_batch arg, cotangent, grad_arg;
+} rpp_hip_ft_log_adjoint_desc;
+int rpp_hip_ft_log_adjoint_f32(const rpp_hip_ft_log_adjoint_desc* desc, void* stream);
+int rpp_hip_ft_log_adjoint_f64(const rpp_hip_ft_log_adjoint_desc* desc, void* stream);
+
 /* --------------------------------------------------------- sparse and Lie */
 
 enum rpp_hip_matrix_format { RPP_HIP_CSR = 0, RPP_HIP_CSC = 1 };
@@ -257,8 +273,8 @@ int rpp_hip_lie_basis_copy(int32_t width, int32_t depth, int64_t* pairs,
 /* ------------------------------------------------- batched Chen recursion */
 
 /* Signatures of many piecewise-linear paths of uneven length in ONE
- * persistent launch (the caller-side chain of ft_fmexp, SURVEYed as the
- * reference's usage: S_0 = 1, S_t = S_{t-1} (x) exp(dX_t)).
+ * persistent launch (the caller-side chain of ft_fmexp of the reference's
+ * usage pattern: S_0 = 1, S_t = S_{t-1} (x) exp(dX_t)).
  *   increments: device [total, width] row-major; path p owns rows
  *               offsets[p] .. offsets[p+1]-1 (offsets: device int64[n+1]).
  *   out:        tensor batch with the full window [0, depth].
@@ -266,8 +282,15 @@ int rpp_hip_lie_basis_copy(int32_t width, int32_t depth, int64_t* pairs,
  *               state after every checkpoint_every-th increment of path p is
  *               stored at checkpoints + (ckpt_offsets[p] + j) * dim, used by
  *               the backward pass to re-anchor its reconstruction.
- * The tile shape (threads per path, letters fixed per work-group) comes from
- * the autotune table for (width, depth, dtype) unless tile_threads > 0. */
+ *   order / num_increments: optional.  NULL order: the library reads the
+ *               offsets back (one host synchronisation) and queues paths by
+ *               decreasing length.  Otherwise order is a device permutation
+ *               of 0..n-1 (the queue order, longest paths first for balance)
+ *               and num_increments = offsets[n]: the call then issues kernel
+ *               launches only, so it can be captured in a hipGraph once the
+ *               library workspace has been sized by an uncaptured call.
+ * tile_threads > 0 selects a compiled tile (rpp_hip_signature_tiles),
+ * 0 the library default for (width, depth), < 0 the storage-generic kernel. */
 typedef struct rpp_hip_signature_desc {
     rpp_hip_basis basis;
     int64_t num_paths;
@@ -281,6 +304,8 @@ typedef struct rpp_hip_signature_desc {
     int32_t tile_split;
     int32_t tile_kind;    /* 0: Horner register tile, 1: two-barrier chain tile */
     int32_t tile_expand;  /* chain tiles: levels expanded per job */
+    const int64_t* order;
+    int64_t num_increments;
 } rpp_hip_signature_desc;
 int rpp_hip_signature_f32(const rpp_hip_signature_desc* desc, void* stream);
 int rpp_hip_signature_f64(const rpp_hip_signature_desc* desc, void* stream);
@@ -293,7 +318,8 @@ int rpp_hip_signature_f64(const rpp_hip_signature_desc* desc, void* stream);
  * the reconstruction is re-anchored at each of them (sqrt(L) checkpointing).
  * scratch: device workspace of rpp_hip_signature_backward_scratch_bytes() for
  * the storage-generic kernel (NULL: library workspace).  tile_threads /
- * tile_split select a register tile as for the forward pass (split <= 1). */
+ * tile_split select a register tile as for the forward pass (split <= 1);
+ * order / num_increments as for the forward pass. */
 typedef struct rpp_hip_signature_backward_desc {
     rpp_hip_basis basis;
     int64_t num_paths;
@@ -308,6 +334,8 @@ typedef struct rpp_hip_signature_backward_desc {
     void* scratch;
     int32_t tile_threads;
     int32_t tile_split;
+    const int64_t* order;
+    int64_t num_increments;
 } rpp_hip_signature_backward_desc;
 int64_t rpp_hip_signature_backward_scratch_bytes(const rpp_hip_signature_backward_desc* desc,
                                                  int32_t dtype_bytes);
@@ -324,6 +352,12 @@ int rpp_hip_signature_backward_f64(const rpp_hip_signature_backward_desc* desc, 
  * kernel.  No reference counterpart: the reference has no batched signature
  * entry point. */
 int rpp_hip_signature_tiles(int32_t dtype_bytes, int32_t* out, int32_t capacity);
+
+/* The register tile that serves plain ft_fmexp (degree-one exponent, full
+ * output window, unit coefficient stride) for (width, depth) and the dtype:
+ * returns 1 and writes threads and split, or 0 when another kernel serves it.
+ * No reference counterpart (introspection of this strategy). */
+int rpp_hip_fmexp_tile(int32_t dtype_bytes, int32_t width, int32_t depth, int32_t* threads, int32_t* split);
 
 /* ------------------------------------------------------------------ misc */
 

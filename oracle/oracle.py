@@ -268,6 +268,18 @@ def fmexp_vjp(width, depth, mult, x, cot):
     return gm, gx
 
 
+def log_vjp(width, depth, arg, cot):
+    """Gradient of ft_log (full output window) with respect to arg, summed term
+    by term from the log series with the reference adjoints (FP64):
+    arg, cot [n, dim] -> grad [n, dim] (degree 0 is 0)."""
+    arg = np.ascontiguousarray(arg, dtype=np.float64)
+    cot = np.ascontiguousarray(cot, dtype=np.float64)
+    grad = np.zeros_like(arg)
+    _call(lib(), "orc_log_vjp_f64", _i(width), _i(depth), _ll(arg.shape[0]), arg.ctypes.data_as(_p),
+          cot.ctypes.data_as(_p), grad.ctypes.data_as(_p))
+    return grad
+
+
 def ref_signature(width, depth, increments, threads=1):
     """Reference Chen recursion (one ft_fmexp per increment over the batch);
     increments [n, L, width]."""

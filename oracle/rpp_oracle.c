@@ -197,3 +197,47 @@ int orc_signature_vjp_f64(int width, int depth, long long n_paths,
     free(G); free(G2);
     return 0;
 }
+
+/* Gradient of ft_log (output over the full window [0, depth]) with respect to
+ * its argument, from the series log(1 + Y) = sum_k c_k Y^k, c_k = (-1)^(k+1)/k,
+ * Y = arg without its degree-0 term (ft_log reads degrees >= 1,
+ * operations/intermediate/ft_log.hpp:66-85):
+ *   d(Y^k)[delta] = sum_{j<k} Y^j delta Y^(k-1-j), so
+ *   Ybar = sum_k c_k sum_{j<k} R^*_{Y^(k-1-j)} L^*_{Y^j} G
+ * with L^*, R^* the reference adjoints (ft_adj_lmul.hpp, ft_adj_rmul.hpp) and
+ * every product truncated at depth.  This sums the derivative term by term
+ * instead of reversing the Horner scheme.  grad[0] = 0.  FP64 only;
+ * arg, cot, grad are [n, dim]. */
+int orc_log_vjp_f64(int width, int depth, long long n, const double* arg, const double* cot, double* grad)
+{
+    orc_basis B;
+    if (orc_basis_init(&B, width, depth)) return 3;
+    const long long dim = B.start[depth + 1];
+    double* scratch = (double*)malloc(sizeof(double) * (size_t)(B.pow[depth] + 1));
+    double* pw = (double*)calloc((size_t)(dim * (depth + 1)), sizeof(double)); /* Y^0 .. Y^depth */
+    double* t1 = (double*)malloc(sizeof(double) * (size_t)dim);
+    double* t2 = (double*)malloc(sizeof(double) * (size_t)dim);
+    for (long long b = 0; b < n; ++b) {
+        const double* a = arg + b * dim;
+        const double* G = cot + b * dim;
+        double* g = grad + b * dim;
+        for (long long i = 0; i < dim * (depth + 1); ++i) pw[i] = 0.0;
+        pw[0] = 1.0;
+        for (long long i = 1; i < dim; ++i) pw[dim + i] = a[i];
+        for (int k = 2; k <= depth; ++k)
+            orc_ft_mul_f64(&B, scratch, pw + k * dim, 0, depth, pw + (k - 1) * dim, 0, depth, pw + dim, 0, depth,
+                           1.0);
+        for (long long i = 0; i < dim; ++i) g[i] = 0.0;
+        for (int k = 1; k <= depth; ++k) {
+            const double ck = (k % 2 == 0 ? -1.0 : 1.0) / k;
+            for (int j = 0; j < k; ++j) {
+                orc_ft_adj_lmul_f64(&B, t1, 0, depth, pw + j * dim, 0, depth, G, 0, depth);
+                orc_ft_adj_rmul_f64(&B, t2, 0, depth, pw + (k - 1 - j) * dim, 0, depth, t1, 0, depth);
+                for (long long i = 1; i < dim; ++i) g[i] += ck * t2[i];
+            }
+        }
+        g[0] = 0.0;
+    }
+    free(scratch); free(pw); free(t1); free(t2);
+    return 0;
+}

@@ -10,6 +10,7 @@
 #include <string>
 
 #include "common.hpp"
+#include "fmexp_coal.hpp"
 #include "ft_ops.hpp"
 #include "lie.hpp"
 #include "signature.hpp"
@@ -197,6 +198,24 @@ int fmexp_impl(const rpp_hip_ft_fmexp_desc* d, void* s) {
 }
 
 template <typename T>
+int log_adjoint_impl(const rpp_hip_ft_log_adjoint_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    RPP_TRY(check_basis(d->basis));
+    const int64_t n = d->num_batches;
+    if (n < 0) return fail(RPP_HIP_INVALID_ARGUMENT, "negative num_batches");
+    if (n == 0) return 0;
+    RPP_TRY(check_view(d->arg, d->basis, n, "arg", false));
+    RPP_TRY(check_view(d->cotangent, d->basis, n, "cotangent", false));
+    RPP_TRY(check_view(d->grad_arg, d->basis, n, "grad_arg", true));
+    if (d->grad_arg.min_degree != d->arg.min_degree || d->grad_arg.max_degree != d->arg.max_degree)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "ft_log adjoint: grad_arg must have arg's window");
+    if (d->grad_arg.data == d->arg.data || d->grad_arg.data == d->cotangent.data)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "ft_log adjoint: grad_arg must not alias an input");
+    const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    return launch_log_adjoint<T>(B, n, d->grad_arg, d->arg, d->cotangent, as_stream(s));
+}
+
+template <typename T>
 int pairing_impl(const rpp_hip_tensor_pairing_desc* d, void* s) {
     if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
     RPP_TRY(check_basis(d->basis));
@@ -289,6 +308,10 @@ int rpp_hip_ft_exp_f64(const rpp_hip_ft_unary_desc* d, void* s) { return unary_i
 int rpp_hip_ft_log_f32(const rpp_hip_ft_unary_desc* d, void* s) { return unary_impl<float>(kOpLog, d, s); }
 int rpp_hip_ft_log_f64(const rpp_hip_ft_unary_desc* d, void* s) { return unary_impl<double>(kOpLog, d, s); }
 
+int rpp_hip_ft_log_adjoint_f32(const rpp_hip_ft_log_adjoint_desc* d, void* s) { return log_adjoint_impl<float>(d, s); }
+int rpp_hip_ft_log_adjoint_f64(const rpp_hip_ft_log_adjoint_desc* d, void* s) {
+    return log_adjoint_impl<double>(d, s);
+}
 int rpp_hip_ft_adj_lmul_f32(const rpp_hip_ft_adjoint_desc* d, void* s) { return adjoint_impl<float>(true, d, s); }
 int rpp_hip_ft_adj_lmul_f64(const rpp_hip_ft_adjoint_desc* d, void* s) { return adjoint_impl<double>(true, d, s); }
 int rpp_hip_ft_adj_rmul_f32(const rpp_hip_ft_adjoint_desc* d, void* s) { return adjoint_impl<float>(false, d, s); }
@@ -379,11 +402,15 @@ int signature_impl(const rpp_hip_signature_desc* d, void* s) {
     if (d->basis.width > 256) return fail(RPP_HIP_NOT_IMPLEMENTED, "signature supports width <= 256");
     const DevBasis B = make_basis(d->basis.width, d->basis.depth);
     hipStream_t stream = as_stream(s);
-    std::vector<int64_t> lengths;
-    int64_t* order = nullptr;
-    RPP_TRY(path_order(d->offsets, d->num_paths, lengths, &order, stream));
+    const int64_t* order = d->order;
+    if (order == nullptr) {
+        std::vector<int64_t> lengths;
+        int64_t* host_order = nullptr;
+        RPP_TRY(path_order(d->offsets, d->num_paths, lengths, &host_order, stream));
+        order = host_order;
+    }
     if (d->tile_threads < 0) return signature_forward_generic<T>(B, *d, order, stream);
-    if (d->tile_kind == 1 || d->tile_threads == 0) {
+    if (d->tile_kind == 1) {
         for (const auto& c : chain_tiles<T>()) {
             if (c.width != d->basis.width || c.depth != d->basis.depth) continue;
             if (d->tile_threads > 0 && (c.threads != d->tile_threads || c.split != d->tile_split ||
@@ -393,7 +420,7 @@ int signature_impl(const rpp_hip_signature_desc* d, void* s) {
             RPP_TRY(workspace(kSlotCounter, 256, &cnt));
             return c.forward(B, *d, order, static_cast<unsigned*>(cnt), stream);
         }
-        if (d->tile_kind == 1) return fail(RPP_HIP_NOT_IMPLEMENTED, "no compiled chain tile matches");
+        return fail(RPP_HIP_NOT_IMPLEMENTED, "no compiled chain tile matches");
     }
     if (const RegTile<T>* tile = pick_tile<T>(d->basis, d->tile_threads, d->tile_split, false)) {
         void* cnt = nullptr;
@@ -458,13 +485,20 @@ int signature_backward_impl(const rpp_hip_signature_backward_desc* d, void* s) {
         return fail(RPP_HIP_INVALID_ARGUMENT, "checkpoints requested without buffers");
     const DevBasis B = make_basis(d->basis.width, d->basis.depth);
     hipStream_t stream = as_stream(s);
-    std::vector<int64_t> lengths;
-    int64_t* order = nullptr;
-    RPP_TRY(path_order(d->offsets, d->num_paths, lengths, &order, stream));
+    const int64_t* order = d->order;
+    int64_t total = d->num_increments;
+    if (order == nullptr) {
+        std::vector<int64_t> lengths;
+        int64_t* host_order = nullptr;
+        RPP_TRY(path_order(d->offsets, d->num_paths, lengths, &host_order, stream));
+        order = host_order;
+        total = 0;
+        for (int64_t L : lengths) total += L;
+    } else if (total < 0) {
+        return fail(RPP_HIP_INVALID_ARGUMENT, "num_increments must be offsets[num_paths] when order is given");
+    }
     if (d->scratch == nullptr && d->tile_threads >= 0) {
         if (const RegTile<T>* tile = pick_tile<T>(d->basis, d->tile_threads, d->tile_split, true)) {
-            int64_t total = 0;
-            for (int64_t L : lengths) total += L;
             void* cnt = nullptr;
             RPP_TRY(workspace(kSlotCounter, 256, &cnt));
             return tile->backward(B, *d, order, total, static_cast<unsigned*>(cnt), stream);
@@ -502,6 +536,25 @@ int64_t rpp_hip_signature_backward_scratch_bytes(const rpp_hip_signature_backwar
 }
 
 }  // extern "C"
+
+extern "C" int rpp_hip_fmexp_tile(int32_t dtype_bytes, int32_t width, int32_t depth, int32_t* threads,
+                                  int32_t* split) {
+    auto report = [&](const auto* t) {
+        if (!t) return 0;
+        if (threads) *threads = t->threads;
+        if (split) *split = t->split;
+        return 1;
+    };
+    if (dtype_bytes == 4) {
+        if (const auto* c = rpp_hip::fmexp_coal_tile<float>(width, depth)) return report(c);
+        return report(rpp_hip::fmexp_tile<float>(width, depth));
+    }
+    if (dtype_bytes == 8) {
+        if (const auto* c = rpp_hip::fmexp_coal_tile<double>(width, depth)) return report(c);
+        return report(rpp_hip::fmexp_tile<double>(width, depth));
+    }
+    return rpp_hip::fail(RPP_HIP_INVALID_ARGUMENT, "dtype_bytes must be 4 or 8");
+}
 
 extern "C" int rpp_hip_signature_tiles(int32_t dtype_bytes, int32_t* out, int32_t capacity) {
     int n = 0;

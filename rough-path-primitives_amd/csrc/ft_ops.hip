@@ -16,6 +16,7 @@
 // division by the width, compile-time for the common widths.
 #include "common.hpp"
 #include "ft_ops.hpp"
+#include "fmexp_coal.hpp"
 #include "signature_reg.hpp"
 
 #include <algorithm>
@@ -433,7 +434,7 @@ k_pairing(DevBasis B, T* out, int64_t os, View<const T> f, View<const T> x, int6
 // reference's argument-degree-major loop (cpu/.../ft_adj_lmul.hpp:69-114).
 template <typename T, bool LEFT, bool CS1>
 __global__ void __launch_bounds__(kBlock)
-k_adjoint(DevBasis B, View<T> out, View<const T> op, View<const T> arg, int64_t n) {
+k_adjoint(DevBasis B, View<T> out, View<const T> op, View<const T> arg, int64_t n, bool accumulate) {
     for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
         T* o = out.obj(b);
         const T* P = op.obj(b);
@@ -458,7 +459,8 @@ k_adjoint(DevBasis B, View<T> out, View<const T> op, View<const T> arg, int64_t 
                             acc += A[at<CS1>(base + u, arg.cs)] * P[at<CS1>(pb + u, op.cs)];
                     }
                 }
-                o[at<CS1>(B.start[od] + w, out.cs)] = acc;
+                T& dst = o[at<CS1>(B.start[od] + w, out.cs)];
+                dst = accumulate ? dst + acc : acc;
             }
         }
     }
@@ -578,6 +580,8 @@ int launch_horner(int op, const DevBasis& B, int64_t n, const rpp_hip_tensor_bat
     const bool deg1 = B.depth >= 1 && arg.max_degree == 1 && arg.min_degree <= 1 &&
                       out.min_degree == 0 && out.max_degree >= 0;
     if (op == kOpFmexp && deg1 && contig && out.max_degree == B.depth && mult && B.depth >= 1) {
+        if (const FmexpTile<T>* tile = fmexp_coal_tile<T>(B.width, B.depth))
+            return tile->launch(B, out, *mult, arg, n, stream);
         if (const RegTile<T>* tile = fmexp_tile<T>(B.width, B.depth))
             return tile->fmexp(B, out, *mult, arg, n, stream);
     }
@@ -626,18 +630,18 @@ int launch_pairing(const DevBasis& B, int64_t n, const rpp_hip_scalar_batch& out
 template <typename T>
 int launch_adjoint(bool left, const DevBasis& B, int64_t n, const rpp_hip_tensor_batch& out,
                    const rpp_hip_tensor_batch& op, const rpp_hip_tensor_batch& arg,
-                   hipStream_t stream) {
+                   hipStream_t stream, bool accumulate) {
     if (n <= 0) return 0;
     const unsigned grid = grid_for(n);
     const bool contig = all_contig({&out, &op, &arg});
     const View<T> o = make_view<T>(out);
     const View<const T> p = make_cview<T>(op), g = make_cview<T>(arg);
     if (left) {
-        if (contig) k_adjoint<T, true, true><<<grid, kBlock, 0, stream>>>(B, o, p, g, n);
-        else k_adjoint<T, true, false><<<grid, kBlock, 0, stream>>>(B, o, p, g, n);
+        if (contig) k_adjoint<T, true, true><<<grid, kBlock, 0, stream>>>(B, o, p, g, n, accumulate);
+        else k_adjoint<T, true, false><<<grid, kBlock, 0, stream>>>(B, o, p, g, n, accumulate);
     } else {
-        if (contig) k_adjoint<T, false, true><<<grid, kBlock, 0, stream>>>(B, o, p, g, n);
-        else k_adjoint<T, false, false><<<grid, kBlock, 0, stream>>>(B, o, p, g, n);
+        if (contig) k_adjoint<T, false, true><<<grid, kBlock, 0, stream>>>(B, o, p, g, n, accumulate);
+        else k_adjoint<T, false, false><<<grid, kBlock, 0, stream>>>(B, o, p, g, n, accumulate);
     }
     return check_hip(hipGetLastError(), "k_adjoint launch");
 }
@@ -684,7 +688,7 @@ int launch_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector
                                    hipStream_t);                                                  \
     template int launch_adjoint<T>(bool, const DevBasis&, int64_t, const rpp_hip_tensor_batch&,   \
                                    const rpp_hip_tensor_batch&, const rpp_hip_tensor_batch&,      \
-                                   hipStream_t);                                                  \
+                                   hipStream_t, bool);                                            \
     template int launch_spmv<T>(int64_t, const rpp_hip_vector_batch&, const rpp_hip_vector_batch&, \
                                 const rpp_hip_sparse_matrix&, double, hipStream_t);
 

@@ -21,10 +21,17 @@ template <typename T>
 int launch_pairing(const DevBasis& B, int64_t n, const rpp_hip_scalar_batch& out,
                    const rpp_hip_tensor_batch& f, const rpp_hip_tensor_batch& x, hipStream_t stream);
 
+// out = L_op^* arg (left) or R_op^* arg over out's window; with accumulate the
+// result is added to what out holds.
 template <typename T>
 int launch_adjoint(bool left, const DevBasis& B, int64_t n, const rpp_hip_tensor_batch& out,
                    const rpp_hip_tensor_batch& op, const rpp_hip_tensor_batch& arg,
-                   hipStream_t stream);
+                   hipStream_t stream, bool accumulate = false);
+
+// Adjoint of ft_log (log_adjoint.hip).
+template <typename T>
+int launch_log_adjoint(const DevBasis& B, int64_t n, const rpp_hip_tensor_batch& grad_arg,
+                       const rpp_hip_tensor_batch& arg, const rpp_hip_tensor_batch& cotangent, hipStream_t stream);
 
 template <typename T>
 int launch_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg,

@@ -613,7 +613,7 @@ const std::vector<RegTile<T>>& reg_tiles() {
         // BASELINE C5: width 6, depth 6
         RPP_TILE(T, 6, 6, 448, 1), RPP_TILE(T, 6, 6, 704, 1), RPP_TILE(T, 6, 6, 1024, 1), RPP_TILE(T, 6, 6, 256, 2),
         // BASELINE C4: width 8, depth 5
-        RPP_TILE(T, 8, 5, 512, 1), RPP_TILE(T, 8, 5, 256, 1), RPP_TILE(T, 8, 5, 64, 2),
+        RPP_TILE(T, 8, 5, 256, 1), RPP_TILE(T, 8, 5, 512, 1), RPP_TILE(T, 8, 5, 64, 2),
         // BASELINE C3: width 3, depth 8
         RPP_TILE(T, 3, 8, 768, 0), RPP_TILE(T, 3, 8, 256, 1), RPP_TILE(T, 3, 8, 512, 1), RPP_TILE(T, 3, 8, 256, 2),
         // BASELINE C1 shape: width 4, depth 4

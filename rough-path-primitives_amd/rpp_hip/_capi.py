@@ -65,6 +65,11 @@ class FtFmexpAdjointDesc(C.Structure):
                 ("grad_exponent", TensorBatch)]
 
 
+class FtLogAdjointDesc(C.Structure):
+    _fields_ = [("basis", Basis), ("num_batches", _i64), ("arg", TensorBatch), ("cotangent", TensorBatch),
+                ("grad_arg", TensorBatch)]
+
+
 class PairingDesc(C.Structure):
     _fields_ = [("basis", Basis), ("num_batches", _i64), ("out", ScalarBatch),
                 ("functional", TensorBatch), ("arg", TensorBatch)]
@@ -98,14 +103,15 @@ class SignatureDesc(C.Structure):
     _fields_ = [("basis", Basis), ("num_paths", _i64), ("increments", _vp), ("offsets", _vp),
                 ("out", TensorBatch), ("checkpoint_every", _i32), ("checkpoints", _vp),
                 ("ckpt_offsets", _vp), ("tile_threads", _i32), ("tile_split", _i32),
-                ("tile_kind", _i32), ("tile_expand", _i32)]
+                ("tile_kind", _i32), ("tile_expand", _i32), ("order", _vp), ("num_increments", _i64)]
 
 
 class SignatureBackwardDesc(C.Structure):
     _fields_ = [("basis", Basis), ("num_paths", _i64), ("increments", _vp), ("offsets", _vp),
                 ("signatures", TensorBatch), ("cotangent", TensorBatch), ("grad_increments", _vp),
                 ("checkpoint_every", _i32), ("checkpoints", _vp), ("ckpt_offsets", _vp),
-                ("scratch", _vp), ("tile_threads", _i32), ("tile_split", _i32)]
+                ("scratch", _vp), ("tile_threads", _i32), ("tile_split", _i32), ("order", _vp),
+                ("num_increments", _i64)]
 
 
 # entry point -> descriptor type; every one also takes a stream pointer.
@@ -116,6 +122,7 @@ ENTRY_POINTS = {
     "rpp_hip_ft_inplace_mul": FtInplaceMulDesc,
     "rpp_hip_ft_exp": FtUnaryDesc,
     "rpp_hip_ft_log": FtUnaryDesc,
+    "rpp_hip_ft_log_adjoint": FtLogAdjointDesc,
     "rpp_hip_ft_fmexp": FtFmexpDesc,
     "rpp_hip_ft_fmexp_adjoint": FtFmexpAdjointDesc,
     "rpp_hip_tensor_pairing": PairingDesc,
@@ -129,7 +136,8 @@ ENTRY_POINTS = {
 }
 
 OTHER_SYMBOLS = ["rpp_hip_lie_basis_sizes", "rpp_hip_lie_basis_copy", "rpp_hip_last_error",
-                 "rpp_hip_version", "rpp_hip_signature_backward_scratch_bytes", "rpp_hip_signature_tiles"]
+                 "rpp_hip_version", "rpp_hip_signature_backward_scratch_bytes", "rpp_hip_signature_tiles",
+                 "rpp_hip_fmexp_tile"]
 
 
 def exported_symbols() -> list[str]:

@@ -1,6 +1,8 @@
 """torch.autograd bindings over the C-ABI: the batched signature (forward and
-backward kernels) and ft_mul (backward through ft_adj_rmul / ft_adj_lmul,
-operations/basic/ft_adj_rmul.hpp:47-71, ft_adj_lmul.hpp:49-73)."""
+backward kernels), ft_mul (backward through ft_adj_rmul / ft_adj_lmul,
+operations/basic/ft_adj_rmul.hpp:47-71, ft_adj_lmul.hpp:49-73), ft_fmexp with
+a degree-one exponent (the ft_fmexp adjoint kernel) and ft_log (the ft_log
+adjoint kernel)."""
 from __future__ import annotations
 
 from typing import Optional
@@ -83,3 +85,55 @@ class _FTMul(torch.autograd.Function):
 def ft_mul(lhs: torch.Tensor, rhs: torch.Tensor, width: int, depth: int) -> torch.Tensor:
     """Differentiable batched concatenation product (full windows)."""
     return _FTMul.apply(lhs.contiguous(), rhs.contiguous(), width, depth)
+
+
+class _FTFmexp(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, mult, x, width: int, depth: int):
+        basis = ops.TensorBasis(width, depth)
+        n = mult.shape[0]
+        ex = torch.zeros((n, 1 + width), dtype=mult.dtype, device=mult.device)
+        ex[:, 1:] = x
+        out = torch.empty_like(mult)
+        ops.ft_fmexp(out, mult.contiguous(), ops.Batch(ex, 1, 1), basis)
+        ctx.basis = basis
+        ctx.save_for_backward(mult, ex)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        mult, ex = ctx.saved_tensors
+        gm = torch.empty_like(mult)
+        gx = torch.empty_like(ex)
+        ops.ft_fmexp_adjoint(gm, ops.Batch(gx, 1, 1), mult.contiguous(), ops.Batch(ex, 1, 1), g.contiguous(),
+                             ctx.basis)
+        return gm, gx[:, 1:], None, None
+
+
+def ft_fmexp(mult: torch.Tensor, x: torch.Tensor, depth: int) -> torch.Tensor:
+    """Differentiable out = mult (x) exp(x) for a batch of tensors mult [n, dim]
+    and degree-one exponents x [n, width]."""
+    return _FTFmexp.apply(mult, x, x.shape[1], depth)
+
+
+class _FTLog(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, arg, width: int, depth: int):
+        basis = ops.TensorBasis(width, depth)
+        out = torch.empty_like(arg)
+        ops.ft_log(out, arg.contiguous(), basis)
+        ctx.basis = basis
+        ctx.save_for_backward(arg)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        (arg,) = ctx.saved_tensors
+        ga = torch.empty_like(arg)
+        ops.ft_log_adjoint(ga, arg.contiguous(), g.contiguous(), ctx.basis)
+        return ga, None, None
+
+
+def ft_log(arg: torch.Tensor, width: int, depth: int) -> torch.Tensor:
+    """Differentiable ft_log over a batch [n, dim] (full windows)."""
+    return _FTLog.apply(arg, width, depth)

@@ -202,6 +202,14 @@ def ft_log(out, arg, basis: TensorBasis, num_batches=None, stream=None):
     _call("rpp_hip_ft_log" + _sfx(out, arg), d, stream)
 
 
+def ft_log_adjoint(grad_arg, arg, cotangent, basis: TensorBasis, num_batches=None, stream=None):
+    """Adjoint of ft_log (operations/intermediate/ft_log.hpp:66-85): grad_arg =
+    d loss / d arg on arg's window, given cotangent = d loss / d log(arg)."""
+    ga, a, g = (_as_batch(t, basis) for t in (grad_arg, arg, cotangent))
+    d = A.FtLogAdjointDesc(basis._c(), _n(num_batches, ga), a._c(), g._c(), ga._c())
+    _call("rpp_hip_ft_log_adjoint" + _sfx(ga, a, g), d, stream)
+
+
 def ft_fmexp(out, multiplier, exponent, basis: TensorBasis, num_batches=None, stream=None):
     """rpp::ops::ft_fmexp (operations/intermediate/ft_fmexp.hpp:96-121)."""
     out, multiplier, exponent = (_as_batch(x, basis) for x in (out, multiplier, exponent))
@@ -324,6 +332,16 @@ def lie_basis(width: int, depth: int) -> dict:
 
 # -------------------------------------------------- batched Chen recursion
 
+def fmexp_tile(width: int, depth: int, dtype):
+    """(threads, split) of the register tile serving plain ft_fmexp at
+    (width, depth), or None."""
+    nbytes = torch.tensor([], dtype=dtype).element_size()
+    th, sp = C.c_int32(0), C.c_int32(0)
+    if A.lib().rpp_hip_fmexp_tile(nbytes, width, depth, C.byref(th), C.byref(sp)) == 1:
+        return th.value, sp.value
+    return None
+
+
 def signature_tiles(dtype) -> list:
     """Compiled tiles (width, depth, threads, split, kind, expand) for a dtype:
     kind 1 = two-barrier forward tile, kind 0 = Horner register tile (forward
@@ -344,14 +362,23 @@ def _offsets_of(lengths, device) -> torch.Tensor:
     return off.to(device)
 
 
+def path_order(offsets: torch.Tensor) -> torch.Tensor:
+    """Queue order of the batched Chen recursion computed on the device: path
+    indices by decreasing length.  Passing it to signature/signature_backward
+    removes their host synchronisation (graph-capturable calls)."""
+    lengths = offsets[1:] - offsets[:-1]
+    return torch.argsort(lengths, descending=True, stable=True).to(torch.int64)
+
+
 def signature(increments: torch.Tensor, offsets: torch.Tensor, basis: TensorBasis,
               out: Optional[torch.Tensor] = None, checkpoint_every: int = 0,
               checkpoints: Optional[torch.Tensor] = None, ckpt_offsets: Optional[torch.Tensor] = None,
               tile_threads: int = 0, tile_split: int = 0, tile_kind: int = 0, tile_expand: int = 0,
-              stream=None) -> torch.Tensor:
+              order: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
     """Signatures of many paths of uneven length in one launch.
 
-    increments: [total, width] device tensor; offsets: int64 device tensor [n+1].
+    increments: [total, width] device tensor; offsets: int64 device tensor [n+1]
+    with offsets[n] = total; order: optional device queue order (path_order).
     """
     n = offsets.numel() - 1
     dim = basis.size()
@@ -366,7 +393,8 @@ def signature(increments: torch.Tensor, offsets: torch.Tensor, basis: TensorBasi
                         Batch(out, 0, basis.depth)._c(), checkpoint_every,
                         checkpoints.data_ptr() if checkpoints is not None else None,
                         ckpt_offsets.data_ptr() if ckpt_offsets is not None else None,
-                        tile_threads, tile_split, tile_kind, tile_expand)
+                        tile_threads, tile_split, tile_kind, tile_expand,
+                        order.data_ptr() if order is not None else None, inc.shape[0])
     _call("rpp_hip_signature" + _sfx(inc, out), d, stream)
     return out
 
@@ -376,7 +404,7 @@ def signature_backward(increments: torch.Tensor, offsets: torch.Tensor, basis: T
                        grad: Optional[torch.Tensor] = None, checkpoint_every: int = 0,
                        checkpoints: Optional[torch.Tensor] = None,
                        ckpt_offsets: Optional[torch.Tensor] = None, tile_threads: int = 0,
-                       tile_split: int = 0, stream=None) -> torch.Tensor:
+                       tile_split: int = 0, order: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
     """d loss / d increments given d loss / d signatures (cotangent may be 1-D,
     one functional for every path)."""
     n = offsets.numel() - 1
@@ -392,6 +420,7 @@ def signature_backward(increments: torch.Tensor, offsets: torch.Tensor, basis: T
                                 Batch(cotangent, 0, basis.depth)._c(), grad.data_ptr(), checkpoint_every,
                                 checkpoints.data_ptr() if checkpoints is not None else None,
                                 ckpt_offsets.data_ptr() if ckpt_offsets is not None else None, None,
-                                tile_threads, tile_split)
+                                tile_threads, tile_split, order.data_ptr() if order is not None else None,
+                                inc.shape[0])
     _call("rpp_hip_signature_backward" + _sfx(inc, signatures, cotangent), d, stream)
     return grad

@@ -252,3 +252,34 @@ def test_ft_fmexp_adjoint_rejects_aliasing_and_windows(dev):
         rpp_hip.ft_fmexp_adjoint(G, rpp_hip.Batch(gx, 1, 1), S, rpp_hip.Batch(ex, 1, 1), G, basis)
     with pytest.raises(rpp_hip.RppHipError):
         rpp_hip.ft_fmexp_adjoint(gm, rpp_hip.Batch(gx, 1, 1), S, rpp_hip.Batch(S, 1, 2), G, basis)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("shape", [(3, 4), (2, 6), (4, 4), (3, 8), (5, 3), (16, 2)])
+@pytest.mark.parametrize("windows", [((1, 99), (0, 99)), ((0, 99), (0, 99)), ((2, 3), (0, 99)),
+                                     ((1, 99), (1, 3)), ((1, 99), (0, 2))],
+                         ids=["a1N", "a0N", "a23", "g13", "g02"])
+def test_ft_log_adjoint(dev, shape, windows, dtype):
+    """Adjoint of ft_log (reverse Horner on the GPU) against the oracle's
+    term-by-term series derivative built from the reference adjoints."""
+    w, N = shape
+    aw, gw = windows
+    aw = (aw[0], min(aw[1], N))
+    gw = (gw[0], min(gw[1], N))
+    if aw[0] > N or gw[0] > N:
+        pytest.skip("window beyond depth")
+    rng = np.random.default_rng(500 + 10 * w + N)
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    n = 4
+    X = damped_batch(rng, n, w, N, dtype)
+    X[:, 0] = 1
+    G = damped_batch(rng, n, w, N, dtype)
+    am, gm = window_mask(w, N, aw), window_mask(w, N, gw)
+    expected = O.log_vjp(w, N, np.where(am, X, 0), np.where(gm, G, 0))
+    got = to_dev(np.full((n, dim), np.nan, dtype=dtype), dev)
+    rpp_hip.ft_log_adjoint(rpp_hip.Batch(got, *aw), rpp_hip.Batch(to_dev(X, dev), *aw),
+                           rpp_hip.Batch(to_dev(G, dev), *gw), basis)
+    g = got.cpu().numpy()
+    assert np.all(np.isnan(g[:, ~am]))
+    assert_close(g[:, am], expected[:, am], dtype, f"log adjoint {shape} {windows}")

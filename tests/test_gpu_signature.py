@@ -178,6 +178,37 @@ def test_autograd_signature_and_ft_mul(dev):
     assert torch.autograd.gradcheck(lambda p, q: AG.ft_mul(p, q, w, N), (a, b), eps=1e-6, atol=1e-8)
 
 
+@pytest.mark.parametrize("shape", [(3, 4), (5, 6), (2, 5)])
+def test_autograd_fmexp_and_log_gradcheck(dev, shape):
+    """torch.autograd.gradcheck (FP64) of the ft_fmexp and ft_log bindings,
+    i.e. of their adjoint kernels, against finite differences of the forward
+    kernels, plus a chained fmexp against the signature gradient."""
+    from rpp_hip import autograd as AG
+    w, N = shape
+    dim = rpp_hip.tensor_dimension(w, N)
+    rng = np.random.default_rng(77 + w)
+    m = torch.tensor(rng.normal(0, 0.3, (2, dim)), device=dev, requires_grad=True)
+    x = torch.tensor(rng.normal(0, 0.5, (2, w)), device=dev, requires_grad=True)
+    assert torch.autograd.gradcheck(lambda p, q: AG.ft_fmexp(p, q, N), (m, x), eps=1e-6, atol=1e-7)
+    X = rng.normal(0, 0.2, (2, dim))
+    X[:, 0] = 1
+    a = torch.tensor(X, device=dev, requires_grad=True)
+    assert torch.autograd.gradcheck(lambda p: AG.ft_log(p, w, N), (a,), eps=1e-6, atol=1e-7)
+    # chained per-op fmexp == the fused signature gradient
+    L = 6
+    inc = rng.normal(0, 0.4, (L, w))
+    ell = torch.tensor(rng.normal(size=dim), device=dev)
+    xs = [torch.tensor(inc[t:t + 1], device=dev, requires_grad=True) for t in range(L)]
+    S = torch.zeros((1, dim), dtype=torch.float64, device=dev)
+    S[0, 0] = 1
+    for t in range(L):
+        S = AG.ft_fmexp(S, xs[t], N)
+    (S[0] @ ell).backward()
+    got = torch.cat([v.grad for v in xs]).cpu().numpy()
+    expected = O.signature_vjp(w, N, inc, np.array([0, L]), ell.cpu().numpy())
+    assert rel_err(got, expected) < 1e-10
+
+
 def test_chain_tiles_checkpoints_match(dev):
     """Checkpoints written by the two-barrier tiles equal the signatures of
     the path prefixes."""
@@ -201,3 +232,95 @@ def test_chain_tiles_checkpoints_match(dev):
                 pre = inc[off[p]: off[p] + 2 * (j + 1)]
                 expected = O.signature(w, N, pre, np.array([0, len(pre)]))[0]
                 assert_close(got[ck_off[p] + j], expected, np.float64, f"checkpoint tile {(w, N, nt, split)}")
+
+
+@pytest.mark.parametrize("shape", [(4, 4), (3, 4), (5, 6)])
+def test_signature_device_order(dev, shape):
+    """A queue order computed on the device gives the same signatures and
+    gradients as the library's own (host) ordering."""
+    w, N = shape
+    rng = np.random.default_rng(40 + w)
+    inc, off = paths(rng, 11, w, 0, 12)
+    basis = rpp_hip.TensorBasis(w, N)
+    inc_d, off_d = torch.tensor(inc, device=dev), torch.tensor(off, device=dev)
+    order = rpp_hip.path_order(off_d)
+    assert sorted(order.cpu().tolist()) == list(range(11))
+    a = rpp_hip.signature(inc_d, off_d, basis)
+    b = rpp_hip.signature(inc_d, off_d, basis, order=order)
+    assert_close(b.cpu().numpy(), a.cpu().numpy(), np.float64, "device order forward")
+    ell = torch.tensor(rng.normal(size=basis.size()), device=dev)
+    ga = rpp_hip.signature_backward(inc_d, off_d, basis, a, ell)
+    gb = rpp_hip.signature_backward(inc_d, off_d, basis, a, ell, order=order)
+    assert_close(gb.cpu().numpy(), ga.cpu().numpy(), np.float64, "device order backward")
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("shape,ckpt", [((4, 4), 0), ((3, 4), 0), ((5, 6), 0), ((4, 4), 4)])
+def test_signature_graph_replays(dev, shape, ckpt, dtype):
+    """Forward + backward captured in one hipGraph: every replay, with new
+    increments and cotangents written into the static buffers and eager work
+    (and synchronisations) in between, matches the eager calls."""
+    w, N = shape
+    rng = np.random.default_rng(60 + w * 3 + N)
+    inc, off = paths(rng, 9, w, 1, 14)
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    off_d = torch.tensor(off, device=dev)
+    inc_d = torch.tensor(inc, device=dev, dtype=dtype)
+    ell = torch.tensor(rng.normal(size=dim), device=dev, dtype=dtype)
+    g = rpp_hip.SignatureGraph(inc_d, off_d, basis, backward=True, cotangent=ell, checkpoint_every=ckpt)
+    npdt = np.float64 if dtype == torch.float64 else np.float32
+    for rep in range(3):
+        noise = np.random.default_rng(rep).normal(0, 0.05, size=inc.shape)
+        new_inc = torch.tensor(inc * (1 + 0.1 * rep) + (noise if rep else 0), device=dev, dtype=dtype)
+        g.increments.copy_(new_inc)
+        g.cotangent.copy_(ell * (rep + 1))
+        junk = torch.randn(1 << 16, device=dev).sum()  # eager work between replays
+        torch.cuda.synchronize()
+        out, grad = g.replay()
+        torch.cuda.synchronize()
+        ref = rpp_hip.signature(new_inc, off_d, basis)
+        ref_g = rpp_hip.signature_backward(new_inc, off_d, basis, ref, ell * (rep + 1))
+        assert_close(out.cpu().numpy(), ref.cpu().numpy(), npdt, f"graph forward replay {rep}")
+        tol = 1e-10 if dtype == torch.float64 else 1e-4
+        assert rel_err(grad.cpu().numpy(), ref_g.cpu().numpy()) < tol, f"graph backward replay {rep}"
+        del junk
+    exp64 = O.signature(w, N, g.increments.double().cpu().numpy(), off)
+    assert_close(out.cpu().numpy(), exp64, npdt, "graph vs oracle")
+
+
+@pytest.mark.parametrize("ckpt", [0, 16])
+def test_signature_backward_fd_c5_shape(dev, ckpt):
+    """Width 6, depth 6, 32 paths of 256 increments N(0, 1/256), loss = <ell, S>
+    with ell ~ N(0, 1): the backward pass (exp(-x) reconstruction, and re-anchored
+    at sqrt(L) checkpoints) against FP64 central differences of the GPU forward
+    at four random coordinates per path."""
+    w, N, n, L = 6, 6, 32, 256
+    rng = np.random.default_rng(0xC5)
+    inc = rng.normal(0, np.sqrt(1.0 / L), size=(n * L, w))
+    off = np.arange(n + 1, dtype=np.int64) * L
+    basis = rpp_hip.TensorBasis(w, N)
+    ell = torch.tensor(rng.normal(size=basis.size()), device=dev)
+    inc_d, off_d = torch.tensor(inc, device=dev), torch.tensor(off, device=dev)
+    kw = {}
+    if ckpt:
+        ck_off = torch.arange(n + 1, dtype=torch.int64, device=dev) * (L // ckpt)
+        kw = dict(checkpoint_every=ckpt, checkpoints=torch.empty((n * (L // ckpt), basis.size()),
+                                                                 dtype=torch.float64, device=dev),
+                  ckpt_offsets=ck_off)
+    sig = rpp_hip.signature(inc_d, off_d, basis, **kw)
+    grad = rpp_hip.signature_backward(inc_d, off_d, basis, sig, ell, **kw).cpu().numpy()
+    h = 1e-5
+    worst = 0.0
+    for _ in range(4):
+        t = rng.integers(0, L, size=n)
+        a = rng.integers(0, w, size=n)
+        rows = np.arange(n) * L + t
+        plus, minus = inc.copy(), inc.copy()
+        plus[rows, a] += h
+        minus[rows, a] -= h
+        lp = rpp_hip.signature(torch.tensor(plus, device=dev), off_d, basis) @ ell
+        lm = rpp_hip.signature(torch.tensor(minus, device=dev), off_d, basis) @ ell
+        fd = ((lp - lm) / (2 * h)).cpu().numpy()
+        worst = max(worst, rel_err(grad[rows, a], fd))
+    assert worst < 1e-6, f"C5 shape finite differences: {worst:.3e}"

@@ -208,3 +208,33 @@ def test_fmexp_vjp_pairing_and_finite_differences(shape):
         m[:, 1 + a] -= h
         fd[:, a] = np.einsum("bi,bi->b", G, fmexp(S, p) - fmexp(S, m)) / (2 * h)
     assert rel_err(gx, fd) < 1e-7
+
+
+@pytest.mark.parametrize("shape", [(2, 5), (3, 4), (4, 3)])
+def test_log_vjp_finite_differences(shape):
+    """Gradient of <G, log(X)> against central differences of the oracle's
+    ft_log, and degree 0 of the argument (ignored by ft_log) gets 0."""
+    w, N = shape
+    dim = O.tensor_dimension(w, N)
+    rng = np.random.default_rng(90 + w)
+    n = 2
+    X = rng.normal(0, 0.3, size=(n, dim))
+    X[:, 0] = 1.0
+    G = rng.normal(size=(n, dim))
+    g = O.log_vjp(w, N, X, G)
+    assert np.all(g[:, 0] == 0)
+
+    def loss(Z):
+        out = np.zeros((n, dim))
+        O.orc.ft_log(w, N, out, Z)
+        return np.einsum("bi,bi->b", G, out)
+
+    h = 1e-6
+    idx = rng.choice(np.arange(1, dim), size=min(dim - 1, 25), replace=False)
+    fd = np.zeros((n, idx.size))
+    for c, i in enumerate(idx):
+        p, m = X.copy(), X.copy()
+        p[:, i] += h
+        m[:, i] -= h
+        fd[:, c] = (loss(p) - loss(m)) / (2 * h)
+    assert rel_err(g[:, idx], fd) < 1e-7
