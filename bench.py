@@ -432,8 +432,8 @@ def main():
     ap.add_argument("--sections", default=",".join(SECTIONS),
                     help="comma list of extra sections to run (" + ",".join(SECTIONS) + "), or 'none'")
     ap.add_argument("--only", default=None,
-                    help="run one headline variant (plain_fp64, fused_fp64, plain_fp32, fused_fp32) and print "
-                         "its timing only")
+                    help="run one headline variant (plain_fp64, fused_fp64, plain_fp32, fused_fp32) or "
+                         "ft_mul_hbm (2^17 width-4 depth-4 products) and print its timing only")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -456,6 +456,17 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    if args.only == "ft_mul_hbm":
+        w, N, n = 4, 4, 1 << 17
+        basis = rpp.TensorBasis(w, N)
+        rng = np.random.default_rng(0xC1)
+        L = torch.tensor(rng.uniform(-1, 1, size=(n, basis.size())), device=dev)
+        R = torch.tensor(rng.uniform(-1, 1, size=(n, basis.size())), device=dev)
+        out = torch.empty_like(L)
+        ms, _, _ = timed(torch, lambda: rpp.ft_mul(out, L, R, basis), args.steps, args.warmup)
+        print(json.dumps({"variant": args.only, "us_per_call": ms * 1e3,
+                          "achieved_gbs": 3 * n * basis.size() * 8 / (ms * 1e-3) / 1e9}))
+        return
     if args.only:
         inc = make_increments(0xC2, PATHS, LENGTH, WIDTH)
         cls = Plain if args.only.startswith("plain") else Fused

@@ -303,7 +303,8 @@ typedef struct rpp_hip_signature_desc {
     int32_t tile_threads;
     int32_t tile_split;
     int32_t tile_kind;    /* 0: Horner register tile, 1: two-barrier chain tile */
-    int32_t tile_expand;  /* chain tiles: levels expanded per job */
+    int32_t tile_expand;  /* chain tiles: levels expanded per job; register
+                             tiles: top levels held in registers (0: any) */
     const int64_t* order;
     int64_t num_increments;
 } rpp_hip_signature_desc;
@@ -336,6 +337,7 @@ typedef struct rpp_hip_signature_backward_desc {
     int32_t tile_split;
     const int64_t* order;
     int64_t num_increments;
+    int32_t tile_expand;  /* register levels of the tile (0: any) */
 } rpp_hip_signature_backward_desc;
 int64_t rpp_hip_signature_backward_scratch_bytes(const rpp_hip_signature_backward_desc* desc,
                                                  int32_t dtype_bytes);
@@ -344,9 +346,10 @@ int rpp_hip_signature_backward_f64(const rpp_hip_signature_backward_desc* desc, 
 
 /* The compiled tiles of rpp_hip_signature_*: writes up to `capacity`
  * 6-tuples (width, depth, threads, split, kind, expand) for the dtype (4 or 8
- * bytes) and returns how many exist.  kind 0 tiles (Horner, register state)
- * serve forward and backward (split <= 1); kind 1 tiles (two barriers per
- * increment) serve the forward pass.  A descriptor selects one with
+ * bytes) and returns how many exist.  kind 0 tiles (level-synchronous Horner,
+ * the top `expand` subtree levels in registers and the rest in LDS) serve
+ * forward and backward (split <= 1); kind 1 tiles (two barriers per
+ * increment, `expand` levels per job) serve the forward pass.  A descriptor selects one with
  * tile_threads/tile_split/tile_kind/tile_expand; tile_threads = 0 takes the
  * library default for (width, depth), tile_threads < 0 the storage-generic
  * kernel.  No reference counterpart: the reference has no batched signature

@@ -376,11 +376,12 @@ namespace {
 // The compiled register tile for (width, depth): the requested one
 // (threads > 0), else the first listed; backward tiles fix at most one letter.
 template <typename T>
-const RegTile<T>* pick_tile(const rpp_hip_basis& b, int threads, int split, bool backward) {
+const RegTile<T>* pick_tile(const rpp_hip_basis& b, int threads, int split, int reg_levels, bool backward) {
     for (const auto& t : reg_tiles<T>()) {
         if (t.width != b.width || t.depth != b.depth) continue;
         if (backward && (t.backward == nullptr || t.split > 1)) continue;
         if (threads > 0 && (t.threads != threads || t.split != split)) continue;
+        if (threads > 0 && reg_levels > 0 && t.reg_levels != reg_levels) continue;
         return &t;
     }
     return nullptr;
@@ -422,7 +423,7 @@ int signature_impl(const rpp_hip_signature_desc* d, void* s) {
         }
         return fail(RPP_HIP_NOT_IMPLEMENTED, "no compiled chain tile matches");
     }
-    if (const RegTile<T>* tile = pick_tile<T>(d->basis, d->tile_threads, d->tile_split, false)) {
+    if (const RegTile<T>* tile = pick_tile<T>(d->basis, d->tile_threads, d->tile_split, d->tile_expand, false)) {
         void* cnt = nullptr;
         RPP_TRY(workspace(kSlotCounter, 256, &cnt));
         return tile->forward(B, *d, order, static_cast<unsigned*>(cnt), stream);
@@ -498,7 +499,8 @@ int signature_backward_impl(const rpp_hip_signature_backward_desc* d, void* s) {
         return fail(RPP_HIP_INVALID_ARGUMENT, "num_increments must be offsets[num_paths] when order is given");
     }
     if (d->scratch == nullptr && d->tile_threads >= 0) {
-        if (const RegTile<T>* tile = pick_tile<T>(d->basis, d->tile_threads, d->tile_split, true)) {
+        if (const RegTile<T>* tile =
+                pick_tile<T>(d->basis, d->tile_threads, d->tile_split, d->tile_expand, true)) {
             void* cnt = nullptr;
             RPP_TRY(workspace(kSlotCounter, 256, &cnt));
             return tile->backward(B, *d, order, total, static_cast<unsigned*>(cnt), stream);
@@ -567,7 +569,7 @@ extern "C" int rpp_hip_signature_tiles(int32_t dtype_bytes, int32_t* out, int32_
     };
     auto emit = [&](const auto& reg, const auto& chain) {
         for (const auto& t : chain) put(t.width, t.depth, t.threads, t.split, 1, t.expand);
-        for (const auto& t : reg) put(t.width, t.depth, t.threads, t.split, 0, 0);
+        for (const auto& t : reg) put(t.width, t.depth, t.threads, t.split, 0, t.reg_levels);
     };
     if (dtype_bytes == 4) emit(rpp_hip::reg_tiles<float>(), rpp_hip::chain_tiles<float>());
     else if (dtype_bytes == 8) emit(rpp_hip::reg_tiles<double>(), rpp_hip::chain_tiles<double>());

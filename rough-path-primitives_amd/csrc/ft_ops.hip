@@ -17,6 +17,7 @@
 #include "common.hpp"
 #include "ft_ops.hpp"
 #include "fmexp_coal.hpp"
+#include "product_small.hpp"
 #include "signature_reg.hpp"
 
 #include <algorithm>
@@ -466,6 +467,40 @@ k_adjoint(DevBasis B, View<T> out, View<const T> op, View<const T> arg, int64_t 
     }
 }
 
+// CSR rows with the argument staged in LDS: a work-group streams objects; it
+// copies the object's argument segment (the matrix columns) into LDS with
+// coalesced loads, then groups of 8 lanes take one row each, read the row's
+// entries consecutively and gather from LDS, and reduce in a fixed order.
+// The sparse matrix itself is shared by every object and stays in L2.
+constexpr int kSpmvLanes = 8;
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_spmv_csr_lds(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t abs_, int64_t abeg,
+               int64_t count, const T* values, const int64_t* indices, const int64_t* offsets, int64_t rows,
+               T alpha, int64_t n) {
+    extern __shared__ __align__(16) unsigned char spmv_raw[];
+    T* xs = reinterpret_cast<T*>(spmv_raw);
+    const int t = threadIdx.x;
+    const int lane = t % kSpmvLanes;
+    const int grp = t / kSpmvLanes;
+    constexpr int groups = kBlock / kSpmvLanes;
+    for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
+        const T* a = arg + b * abs_ + abeg;
+        for (int64_t i = t; i < count; i += kBlock) xs[i] = a[i];
+        __syncthreads();
+        T* o = out + b * obs;
+        for (int64_t row = grp; row < rows; row += groups) {
+            const int64_t e0 = offsets[row], e1 = offsets[row + 1];
+            T acc = T(0);
+            for (int64_t e = e0 + lane; e < e1; e += kSpmvLanes) acc += values[e] * xs[indices[e]];
+#pragma unroll
+            for (int off = kSpmvLanes / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, kSpmvLanes);
+            if (lane == 0) o[(obeg + row) * ocs] = alpha * acc;
+        }
+        __syncthreads();
+    }
+}
+
 // CSR rows (cpu/.../linalg/sparse_matrix_vector.hpp:45-73): one thread per
 // row, grid.y over objects.
 template <typename T>
@@ -549,6 +584,8 @@ int launch_product(const DevBasis& B, int64_t n, const rpp_hip_tensor_batch& out
     const bool contig = all_contig({&out, &lhs, &rhs, add});
     const int64_t dim = B.start[B.depth + 1];
     const int64_t per_object = 2 * dim * static_cast<int64_t>(sizeof(T));
+    if (contig && out.min_degree <= out.max_degree && product_small_available<T>(B.width, B.depth))
+        return launch_product_small<T>(B, n, out, lhs, rhs, add, alpha, beta, stream);
     if (contig && out.min_degree <= out.max_degree && per_object <= kLdsStageBytes) {
         const int G = static_cast<int>(std::min<int64_t>(64, kLdsStageBytes / per_object));
         const int64_t groups = (n + G - 1) / G;
@@ -656,6 +693,22 @@ int launch_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector
     const T* v = static_cast<const T*>(M.values);
     if (M.format == RPP_HIP_CSR) {
         if (M.rows == 0) return 0;
+        const int64_t count = std::min<int64_t>(M.cols, arg.size);
+        const size_t lds = static_cast<size_t>(std::max<int64_t>(count, 1)) * sizeof(T);
+        constexpr size_t kSpmvLdsMax = 128 * 1024;
+        if (arg.coeff_stride == 1 && M.cols <= arg.size && lds <= kSpmvLdsMax && M.rows >= 64) {
+            static const bool raised = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spmv_csr_lds<T>),
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                           static_cast<int>(kSpmvLdsMax)) == hipSuccess;
+            if (raised || lds <= 64 * 1024) {
+                const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(8, (160 * 1024) / lds));
+                const unsigned grid = static_cast<unsigned>(std::min<int64_t>(n, device_cu_count() * per_cu));
+                k_spmv_csr_lds<T><<<grid, kBlock, lds, stream>>>(o, out.batch_stride, out.coeff_stride, out.begin, a,
+                                                               arg.batch_stride, arg.begin, count, v, M.indices,
+                                                               M.offsets, M.rows, T(alpha), n);
+                return check_hip(hipGetLastError(), "k_spmv_csr_lds launch");
+            }
+        }
         dim3 grid(static_cast<unsigned>((M.rows + kBlock - 1) / kBlock), gy);
         k_spmv_csr<T><<<grid, kBlock, 0, stream>>>(o, out.batch_stride, out.coeff_stride, out.begin, a,
                                                    arg.batch_stride, arg.coeff_stride, arg.begin, v,

@@ -1,0 +1,152 @@
+// Truncated tensor products of small tensors (at most 2048 coefficients) with
+// the shape fixed at compile time.
+//
+//     out_k[w] = beta * sum_j lhs_{k-j}[w / width^j] * rhs_j[w % width^j]
+//                (+ alpha * add_k[w] on add's window)
+//
+// over the window-restricted j range, as product_pass in ft_ops.hip computes
+// it (same terms, same ascending-j order).  With width and depth known to the
+// compiler every degree offset, word count and division by width^j is a
+// constant, so the inner loops unroll into LDS reads and fused multiply-adds.
+// One work-group streams objects: it stages the lhs and rhs windows of one
+// object in LDS with coalesced loads, then writes every output word once with
+// coalesced stores.  All operand reads of an object precede its first write,
+// so out may alias lhs or rhs; the addend is read at the index being written.
+#include "common.hpp"
+#include "product_small.hpp"
+
+#include <algorithm>
+
+namespace rpp_hip {
+
+namespace {
+
+constexpr int spow(int b, int e) {
+    int r = 1;
+    for (int i = 0; i < e; ++i) r *= b;
+    return r;
+}
+
+template <int W, int N>
+struct SmallGeo {
+    static constexpr int words(int k) { return spow(W, k); }
+    static constexpr int start(int k) {
+        int s = 0;
+        for (int i = 0; i < k; ++i) s += spow(W, i);
+        return s;
+    }
+    static constexpr int dim = start(N + 1);
+};
+
+__device__ __forceinline__ int window_begin(const int* st, int mn, int mx) { return mn <= mx ? st[mn] : 0; }
+__device__ __forceinline__ int window_end(const int* st, int mn, int mx) { return mn <= mx ? st[mx + 1] : 0; }
+
+template <typename T, int W, int N, int NT>
+__global__ void __launch_bounds__(NT)
+k_product_small(View<T> out, View<const T> lhs, View<const T> rhs, View<const T> add, int with_add, T alpha,
+                T beta, int64_t n) {
+    using G = SmallGeo<W, N>;
+    __shared__ T sl[G::dim];
+    __shared__ T sr[G::dim];
+    int st[N + 2];
+#pragma unroll
+    for (int k = 0; k <= N + 1; ++k) st[k] = G::start(k);
+    const int t = threadIdx.x;
+    const int lb = window_begin(st, lhs.mn, lhs.mx), le = window_end(st, lhs.mn, lhs.mx);
+    const int rb = window_begin(st, rhs.mn, rhs.mx), re = window_end(st, rhs.mn, rhs.mx);
+    for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
+        const T* L = lhs.obj(b);
+        const T* R = rhs.obj(b);
+        for (int i = lb + t; i < le; i += NT) sl[i] = L[i];
+        for (int i = rb + t; i < re; i += NT) sr[i] = R[i];
+        __syncthreads();
+        T* O = out.obj(b);
+        const T* A = with_add ? add.obj(b) : nullptr;
+#pragma unroll
+        for (int k = N; k >= 0; --k) {
+            if (k < out.mn || k > out.mx) continue;
+            const int jlo = max(max(rhs.mn, k - lhs.mx), 0);
+            const int jhi = min(min(rhs.mx, k - lhs.mn), k);
+            const bool use_add = with_add && add.mn <= k && k <= add.mx;
+#pragma unroll
+            for (int s = 0; s < (G::words(k) + NT - 1) / NT; ++s) {
+                const int w = t + s * NT;
+                if (G::words(k) % NT == 0 || w < G::words(k)) {
+                    T acc = T(0);
+#pragma unroll
+                    for (int j = 0; j <= k; ++j) {
+                        if (j >= jlo && j <= jhi) {
+                            const int p = w / G::words(j);
+                            const int sfx = w - p * G::words(j);
+                            acc += sl[G::start(k - j) + p] * sr[G::start(j) + sfx];
+                        }
+                    }
+                    T v = beta * acc;
+                    if (use_add) v = alpha * A[G::start(k) + w] + v;
+                    O[G::start(k) + w] = v;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int W, int N>
+int small_launch(int64_t n, const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& lhs,
+                 const rpp_hip_tensor_batch& rhs, const rpp_hip_tensor_batch* add, double alpha, double beta,
+                 hipStream_t stream) {
+    constexpr int dim = SmallGeo<W, N>::dim;
+    constexpr int NT = dim <= 512 ? 64 : (dim <= 1024 ? 128 : 256);
+    const int64_t cap = static_cast<int64_t>(device_cu_count()) * (2048 / NT);
+    const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min(n, cap)));
+    const View<const T> a = add ? make_cview<T>(*add) : View<const T>{nullptr, 0, 1, 1, 0};
+    k_product_small<T, W, N, NT><<<grid, NT, 0, stream>>>(make_view<T>(out), make_cview<T>(lhs), make_cview<T>(rhs),
+                                                          a, add ? 1 : 0, T(alpha), T(beta), n);
+    return check_hip(hipGetLastError(), "k_product_small launch");
+}
+
+template <typename T>
+using SmallFn = int (*)(int64_t, const rpp_hip_tensor_batch&, const rpp_hip_tensor_batch&,
+                        const rpp_hip_tensor_batch&, const rpp_hip_tensor_batch*, double, double, hipStream_t);
+
+template <typename T>
+SmallFn<T> small_kernel(int width, int depth) {
+#define RPP_SMALL(W, N) \
+    if (width == W && depth == N) return &small_launch<T, W, N>;
+    RPP_SMALL(2, 1) RPP_SMALL(2, 2) RPP_SMALL(2, 3) RPP_SMALL(2, 4) RPP_SMALL(2, 5) RPP_SMALL(2, 6)
+    RPP_SMALL(2, 7) RPP_SMALL(2, 8) RPP_SMALL(2, 9)
+    RPP_SMALL(3, 1) RPP_SMALL(3, 2) RPP_SMALL(3, 3) RPP_SMALL(3, 4) RPP_SMALL(3, 5) RPP_SMALL(3, 6)
+    RPP_SMALL(4, 1) RPP_SMALL(4, 2) RPP_SMALL(4, 3) RPP_SMALL(4, 4) RPP_SMALL(4, 5)
+    RPP_SMALL(5, 1) RPP_SMALL(5, 2) RPP_SMALL(5, 3) RPP_SMALL(5, 4)
+    RPP_SMALL(6, 1) RPP_SMALL(6, 2) RPP_SMALL(6, 3) RPP_SMALL(6, 4)
+    RPP_SMALL(8, 1) RPP_SMALL(8, 2) RPP_SMALL(8, 3)
+#undef RPP_SMALL
+    return nullptr;
+}
+
+}  // namespace
+
+template <typename T>
+bool product_small_available(int width, int depth) {
+    return small_kernel<T>(width, depth) != nullptr;
+}
+
+template <typename T>
+int launch_product_small(const DevBasis& B, int64_t n, const rpp_hip_tensor_batch& out,
+                         const rpp_hip_tensor_batch& lhs, const rpp_hip_tensor_batch& rhs,
+                         const rpp_hip_tensor_batch* add, double alpha, double beta, hipStream_t stream) {
+    const SmallFn<T> fn = small_kernel<T>(B.width, B.depth);
+    if (!fn) return fail(RPP_HIP_NOT_IMPLEMENTED, "no compiled small-tensor product for this shape");
+    return fn(n, out, lhs, rhs, add, alpha, beta, stream);
+}
+
+template bool product_small_available<float>(int, int);
+template bool product_small_available<double>(int, int);
+template int launch_product_small<float>(const DevBasis&, int64_t, const rpp_hip_tensor_batch&,
+                                         const rpp_hip_tensor_batch&, const rpp_hip_tensor_batch&,
+                                         const rpp_hip_tensor_batch*, double, double, hipStream_t);
+template int launch_product_small<double>(const DevBasis&, int64_t, const rpp_hip_tensor_batch&,
+                                          const rpp_hip_tensor_batch&, const rpp_hip_tensor_batch&,
+                                          const rpp_hip_tensor_batch*, double, double, hipStream_t);
+
+}  // namespace rpp_hip

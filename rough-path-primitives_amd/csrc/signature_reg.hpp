@@ -14,6 +14,7 @@ struct RegTile {
     int depth;
     int threads;  // work-group size
     int split;    // leading letters fixed per work-group
+    int reg_levels;  // top subtree levels held in registers (the rest in LDS)
     int (*forward)(const DevBasis&, const rpp_hip_signature_desc&, const int64_t*, unsigned*, hipStream_t);
     int (*backward)(const DevBasis&, const rpp_hip_signature_backward_desc&, const int64_t*, int64_t, unsigned*,
                     hipStream_t);

@@ -28,8 +28,10 @@ class _Signature(torch.autograd.Function):
         if checkpoint_every > 0:
             ck_off, total = _checkpoint_layout(offsets, checkpoint_every)
             ckpts = torch.empty((max(total, 1), basis.size()), dtype=increments.dtype, device=increments.device)
+        tile = tuple(tile) + (0, 0, 0, 0)
         sig = ops.signature(increments, offsets, basis, checkpoint_every=checkpoint_every, checkpoints=ckpts,
-                            ckpt_offsets=ck_off, tile_threads=tile[0], tile_split=tile[1])
+                            ckpt_offsets=ck_off, tile_threads=tile[0], tile_split=tile[1], tile_kind=tile[2],
+                            tile_expand=tile[3])
         ctx.basis = basis
         ctx.every = checkpoint_every
         ctx.tile = tile
@@ -43,9 +45,10 @@ class _Signature(torch.autograd.Function):
         kw = {}
         if ctx.every > 0:
             kw = dict(checkpoint_every=ctx.every, checkpoints=ckpts, ckpt_offsets=ck_off)
-        tile = ctx.tile if ctx.tile[1] <= 1 else (0, 0)
+        t = ctx.tile
+        th, sp, ex = (t[0], t[1], t[3]) if t[0] > 0 and t[1] <= 1 and t[2] == 0 else (0, 0, 0)
         grad = ops.signature_backward(increments, offsets, ctx.basis, sig, grad_sig.contiguous(),
-                                      tile_threads=tile[0], tile_split=tile[1], **kw)
+                                      tile_threads=th, tile_split=sp, tile_expand=ex, **kw)
         return grad, None, None, None, None, None
 
 

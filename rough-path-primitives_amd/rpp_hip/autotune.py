@@ -64,7 +64,7 @@ def tune(width: int, depth: int, dtype=torch.float64, paths: int = 1024, length:
     inc = (torch.randn(paths * length, width, generator=g, dtype=torch.float64) / length ** 0.5).to(dev, dtype)
     off = torch.arange(paths + 1, dtype=torch.int64, device=dev) * length
     cands = [(t, s, k, e) for (w, n, t, s, k, e) in ops.signature_tiles(dtype) if (w, n) == (width, depth)]
-    cands.append((-1, 0, 0, 0))
+    cands.append((-1, 0, 0, 0))  # the storage-generic kernel
     result = {"forward": {}, "backward": {}}
     sig = ops.signature(inc, off, basis)
     ell = torch.randn(basis.size(), generator=g, dtype=torch.float64).to(dev, dtype)
@@ -75,7 +75,8 @@ def tune(width: int, depth: int, dtype=torch.float64, paths: int = 1024, length:
                                   tile_expand=e), reps)
         if backward and k == 0 and s <= 1:
             result["backward"][key] = _time(
-                lambda: ops.signature_backward(inc, off, basis, sig, ell, tile_threads=t, tile_split=s), reps)
+                lambda: ops.signature_backward(inc, off, basis, sig, ell, tile_threads=t, tile_split=s,
+                                               tile_expand=e), reps)
     entry = {}
     for kind in ("forward", "backward"):
         if result[kind]:

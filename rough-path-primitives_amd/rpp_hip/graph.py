@@ -71,9 +71,10 @@ class SignatureGraph:
         ops.signature(self.increments, self.offsets, self.basis, out=self.out, tile_threads=th, tile_split=sp,
                       tile_kind=kind, tile_expand=ex, order=self.order, **kw)
         if self.backward:
-            bth, bsp = (th, sp) if kind == 0 and sp <= 1 else (0, 0)
+            bth, bsp, bex = (th, sp, ex) if kind == 0 and sp <= 1 and th > 0 else (0, 0, 0)
             ops.signature_backward(self.increments, self.offsets, self.basis, self.out, self.cotangent,
-                                   grad=self.grad, tile_threads=bth, tile_split=bsp, order=self.order, **kw)
+                                   grad=self.grad, tile_threads=bth, tile_split=bsp, tile_expand=bex,
+                                   order=self.order, **kw)
 
     def replay(self):
         """Re-run the captured launches; returns out (and grad with backward)."""

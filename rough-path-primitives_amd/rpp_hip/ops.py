@@ -404,15 +404,16 @@ def signature_backward(increments: torch.Tensor, offsets: torch.Tensor, basis: T
                        grad: Optional[torch.Tensor] = None, checkpoint_every: int = 0,
                        checkpoints: Optional[torch.Tensor] = None,
                        ckpt_offsets: Optional[torch.Tensor] = None, tile_threads: int = 0,
-                       tile_split: int = 0, order: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
+                       tile_split: int = 0, order: Optional[torch.Tensor] = None, tile_expand: int = 0,
+                       stream=None) -> torch.Tensor:
     """d loss / d increments given d loss / d signatures (cotangent may be 1-D,
     one functional for every path)."""
     n = offsets.numel() - 1
     inc = increments.contiguous()
     if tile_threads == 0:
         from . import autotune
-        tile_threads, tile_split = (autotune.best(basis.width, basis.depth, increments.dtype, "backward")
-                                    or (0, 0, 0, 0))[:2]
+        best = autotune.best(basis.width, basis.depth, increments.dtype, "backward") or (0, 0, 0, 0)
+        tile_threads, tile_split, tile_expand = best[0], best[1], best[3]
     if grad is None:
         grad = torch.empty_like(inc)
     d = A.SignatureBackwardDesc(basis._c(), n, inc.data_ptr(), offsets.data_ptr(),
@@ -421,6 +422,6 @@ def signature_backward(increments: torch.Tensor, offsets: torch.Tensor, basis: T
                                 checkpoints.data_ptr() if checkpoints is not None else None,
                                 ckpt_offsets.data_ptr() if ckpt_offsets is not None else None, None,
                                 tile_threads, tile_split, order.data_ptr() if order is not None else None,
-                                inc.shape[0])
+                                inc.shape[0], tile_expand)
     _call("rpp_hip_signature_backward" + _sfx(inc, signatures, cotangent), d, stream)
     return grad

@@ -162,7 +162,7 @@ def test_adjoint_multiplications(dev, side, windows, dtype):
 
 @pytest.mark.parametrize("dtype", DTYPES)
 def test_chained_fmexp_c2_shape(dev, dtype):
-    """BASELINE C2 shape (width 5, depth 6), 64 paths x 16 increments of the
+    """The headline shape (width 5, depth 6), 64 paths x 16 increments of the
     plain chain, against the oracle chain."""
     w, N, n, L = 5, 6, 64, 16
     rng = np.random.default_rng(0xC2)

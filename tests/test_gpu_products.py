@@ -157,7 +157,7 @@ def test_ft_mul_layouts(dev, dtype):
 
 
 def test_c1_shape_ft_mul(dev):
-    """BASELINE C1: 1024 pairs, width 4, depth 4, U[-1,1] with coefficient 0 = 1, FP64."""
+    """1024 pairs, width 4, depth 4, U[-1,1] with coefficient 0 = 1, FP64."""
     rng = np.random.default_rng(0xC1)
     basis = rpp_hip.TensorBasis(4, 4)
     lhs = random_batch(rng, 1024, 341, np.float64)

@@ -131,7 +131,7 @@ def test_every_register_tile_forward(dev, dtype):
 
 @pytest.mark.parametrize("ckpt", [0, 2])
 def test_every_register_tile_backward_fp64(dev, ckpt):
-    for (w, N, nt, split, kind, _) in _tiles(torch.float64):
+    for (w, N, nt, split, kind, expand) in _tiles(torch.float64):
         if split > 1 or kind != 0:
             continue
         rng = np.random.default_rng(w * 100 + N + nt + split)
@@ -153,9 +153,9 @@ def test_every_register_tile_backward_fp64(dev, ckpt):
                       ckpt_offsets=torch.tensor(ck_off, device=dev))
         sig = rpp_hip.signature(inc_d, off_d, basis, **kw)
         grad = rpp_hip.signature_backward(inc_d, off_d, basis, sig, torch.tensor(ell, device=dev),
-                                          tile_threads=nt, tile_split=split, **kw)
+                                          tile_threads=nt, tile_split=split, tile_expand=expand, **kw)
         err = rel_err(grad.cpu().numpy(), expected)
-        assert err < 1e-10, f"tile {(w, N, nt, split)} ckpt={ckpt}: {err:.3e}"
+        assert err < 1e-10, f"tile {(w, N, nt, split, expand)} ckpt={ckpt}: {err:.3e}"
 
 
 def test_autograd_signature_and_ft_mul(dev):
@@ -189,11 +189,15 @@ def test_autograd_fmexp_and_log_gradcheck(dev, shape):
     rng = np.random.default_rng(77 + w)
     m = torch.tensor(rng.normal(0, 0.3, (2, dim)), device=dev, requires_grad=True)
     x = torch.tensor(rng.normal(0, 0.5, (2, w)), device=dev, requires_grad=True)
-    assert torch.autograd.gradcheck(lambda p, q: AG.ft_fmexp(p, q, N), (m, x), eps=1e-6, atol=1e-7)
+    # fast_mode: random projections of the Jacobian (two forward calls and
+    # one backward call per check), so large tensors stay cheap
+    fast = dim > 200
+    assert torch.autograd.gradcheck(lambda p, q: AG.ft_fmexp(p, q, N), (m, x), eps=1e-6, atol=1e-7,
+                                    fast_mode=fast)
     X = rng.normal(0, 0.2, (2, dim))
     X[:, 0] = 1
     a = torch.tensor(X, device=dev, requires_grad=True)
-    assert torch.autograd.gradcheck(lambda p: AG.ft_log(p, w, N), (a,), eps=1e-6, atol=1e-7)
+    assert torch.autograd.gradcheck(lambda p: AG.ft_log(p, w, N), (a,), eps=1e-6, atol=1e-7, fast_mode=fast)
     # chained per-op fmexp == the fused signature gradient
     L = 6
     inc = rng.normal(0, 0.4, (L, w))
