@@ -483,7 +483,7 @@ def main():
     alg_bytes = PATHS * (2 * dim + WIDTH) * 8
     achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
     tile = rpp.fmexp_tile(WIDTH, DEPTH, torch.float64)
-    kernel = f"k_fmexp_tile<double,{WIDTH},{DEPTH},{tile[0]},{tile[1]}>" if tile else "k_fmexp_horner"
+    kernel = f"{tile[0]}<double,{WIDTH},{DEPTH},{tile[1]},{tile[2]}>" if tile else "k_fmexp_horner"
     fused = results["fused_fp64"]
     fused_tflops = PATHS * LENGTH * chen_flops(WIDTH, DEPTH) / (fused["ms_per_step"] * 1e-3) / 1e12
     line = {
@@ -504,7 +504,7 @@ def main():
                    "increments_per_path": LENGTH},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": load_traffic("traffic_k_fmexp_tile_c2_fp64.json"),
+                     "traffic": load_traffic("k_fmexp_coal_w5d6_4096objects_fp64_traffic.json"),
                      "kernel": kernel, "avg_launch_ms": launch_ms,
                      "algorithmic_bytes_per_launch": alg_bytes},
         "max_rel_err_vs_fp64_oracle": plain["max_rel_err_vs_fp64_oracle"],

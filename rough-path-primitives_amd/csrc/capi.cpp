@@ -278,6 +278,9 @@ int lie_convert_impl(bool to_lie, const rpp_hip_lie_convert_desc* d, void* s) {
     const int64_t lbeg = md->degree_begin[static_cast<size_t>(d->lie.min_degree)];
     rpp_hip_vector_batch lv{d->lie.data, d->lie.batch_stride, d->lie.coeff_stride, lbeg, md->true_size - lbeg};
     const DeviceCsr* m = to_lie ? t2l : l2t;
+    if (lie_spmv_fits<T>(*m, to_lie ? tv : lv))
+        return to_lie ? launch_lie_spmv<T>(d->num_batches, lv, tv, *m, as_stream(s))
+                      : launch_lie_spmv<T>(d->num_batches, tv, lv, *m, as_stream(s));
     rpp_hip_sparse_matrix M{RPP_HIP_CSR, m->values, m->indices, m->offsets, m->nnz, m->rows, m->cols};
     // R9: matrix row 0 <-> out.begin(), column 0 <-> arg.begin().
     return to_lie ? launch_spmv<T>(d->num_batches, lv, tv, M, 1.0, as_stream(s))
@@ -541,19 +544,19 @@ int64_t rpp_hip_signature_backward_scratch_bytes(const rpp_hip_signature_backwar
 
 extern "C" int rpp_hip_fmexp_tile(int32_t dtype_bytes, int32_t width, int32_t depth, int32_t* threads,
                                   int32_t* split) {
-    auto report = [&](const auto* t) {
+    auto report = [&](const auto* t, int kind) {
         if (!t) return 0;
         if (threads) *threads = t->threads;
         if (split) *split = t->split;
-        return 1;
+        return kind;
     };
     if (dtype_bytes == 4) {
-        if (const auto* c = rpp_hip::fmexp_coal_tile<float>(width, depth)) return report(c);
-        return report(rpp_hip::fmexp_tile<float>(width, depth));
+        if (const auto* c = rpp_hip::fmexp_coal_tile<float>(width, depth)) return report(c, 2);
+        return report(rpp_hip::fmexp_tile<float>(width, depth), 1);
     }
     if (dtype_bytes == 8) {
-        if (const auto* c = rpp_hip::fmexp_coal_tile<double>(width, depth)) return report(c);
-        return report(rpp_hip::fmexp_tile<double>(width, depth));
+        if (const auto* c = rpp_hip::fmexp_coal_tile<double>(width, depth)) return report(c, 2);
+        return report(rpp_hip::fmexp_tile<double>(width, depth), 1);
     }
     return rpp_hip::fail(RPP_HIP_INVALID_ARGUMENT, "dtype_bytes must be 4 or 8");
 }

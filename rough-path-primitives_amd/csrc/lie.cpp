@@ -284,6 +284,35 @@ int upload(const Csr& h, DeviceCsr& d) {
     if (hipMemcpy(d.offsets, h.offsets.data(), h.offsets.size() * sizeof(int64_t), hipMemcpyHostToDevice) !=
         hipSuccess)
         return 9;
+    // packed form: value table + (table index, column) words
+    std::vector<T> table;
+    std::map<T, uint32_t> slot;
+    bool packable = h.cols < (int64_t{1} << 24) && d.nnz < (int64_t{1} << 31) && !vals.empty();
+    for (size_t e = 0; packable && e < vals.size(); ++e) {
+        if (slot.count(vals[e])) continue;
+        if (table.size() == 256) {
+            packable = false;
+            break;
+        }
+        slot.emplace(vals[e], static_cast<uint32_t>(table.size()));
+        table.push_back(vals[e]);
+    }
+    if (packable) {
+        std::vector<uint32_t> packed(vals.size());
+        for (size_t e = 0; e < vals.size(); ++e)
+            packed[e] = (slot[vals[e]] << 24) | static_cast<uint32_t>(h.indices[e]);
+        std::vector<int32_t> off32(h.offsets.begin(), h.offsets.end());
+        if (hipMalloc(reinterpret_cast<void**>(&d.packed), packed.size() * sizeof(uint32_t)) != hipSuccess) return 5;
+        if (hipMalloc(reinterpret_cast<void**>(&d.offsets32), off32.size() * sizeof(int32_t)) != hipSuccess) return 5;
+        if (hipMalloc(&d.table, table.size() * sizeof(T)) != hipSuccess) return 5;
+        if (hipMemcpy(d.packed, packed.data(), packed.size() * sizeof(uint32_t), hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(d.offsets32, off32.data(), off32.size() * sizeof(int32_t), hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(d.table, table.data(), table.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            return 9;
+        d.table_size = static_cast<int32_t>(table.size());
+    }
     return 0;
 }
 

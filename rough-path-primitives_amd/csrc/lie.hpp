@@ -1,9 +1,13 @@
 // Hall-basis metadata and its per-device CSR cache (lie.cpp).
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 #include <memory>
 #include <vector>
+
+#include "rpp_hip.h"
 
 namespace rpp_hip {
 
@@ -38,11 +42,28 @@ struct DeviceCsr {
     int64_t* indices = nullptr;
     int64_t* offsets = nullptr;
     int64_t rows = 0, cols = 0, nnz = 0;
+    // Packed form, present when the matrix has at most 256 distinct values,
+    // fewer than 2^24 columns and fewer than 2^31 entries: each entry is
+    // (index into `table` << 24) | column, rows delimited by 32-bit offsets.
+    // A quarter of the bytes of the CSR arrays.
+    uint32_t* packed = nullptr;
+    int32_t* offsets32 = nullptr;
+    void* table = nullptr;
+    int32_t table_size = 0;  // 0: no packed form
 };
 
 // Device copies for the current device and dtype, uploaded on first use and
 // kept for the life of the process.
 template <typename T>
 int device_lie_matrices(int width, int depth, const DeviceCsr** l2t, const DeviceCsr** t2l);
+
+// out = M arg over a batch with the packed form of M (lie_spmv.hip); matrix
+// row 0 is out.begin, column 0 is arg.begin.  lie_spmv_fits tells whether the
+// packed kernel applies (packed form present, one argument segment fits in LDS).
+template <typename T>
+int launch_lie_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg, const DeviceCsr& m,
+                    hipStream_t stream);
+template <typename T>
+bool lie_spmv_fits(const DeviceCsr& m, const rpp_hip_vector_batch& arg);
 
 }  // namespace rpp_hip

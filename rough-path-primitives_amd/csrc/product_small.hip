@@ -8,9 +8,9 @@
 // it (same terms, same ascending-j order).  With width and depth known to the
 // compiler every degree offset, word count and division by width^j is a
 // constant, so the inner loops unroll into LDS reads and fused multiply-adds.
-// One work-group streams objects: it stages the lhs and rhs windows of one
-// object in LDS with coalesced loads, then writes every output word once with
-// coalesced stores.  All operand reads of an object precede its first write,
+// A work-group takes one object (a grid-stride loop covers batches beyond the
+// grid limit): it stages the object's lhs and rhs windows in LDS with
+// coalesced loads, then writes every output word once with coalesced stores.  All operand reads of an object precede its first write,
 // so out may alias lhs or rhs; the addend is read at the index being written.
 #include "common.hpp"
 #include "product_small.hpp"
@@ -38,8 +38,15 @@ struct SmallGeo {
     static constexpr int dim = start(N + 1);
 };
 
-__device__ __forceinline__ int window_begin(const int* st, int mn, int mx) { return mn <= mx ? st[mn] : 0; }
-__device__ __forceinline__ int window_end(const int* st, int mn, int mx) { return mn <= mx ? st[mx + 1] : 0; }
+// first word of degree k (k <= N + 1), by a loop over the compile-time degrees
+template <int W, int N>
+__device__ __forceinline__ int degree_start(int k) {
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i <= N; ++i)
+        if (i < k) s += SmallGeo<W, N>::words(i);
+    return s;
+}
 
 template <typename T, int W, int N, int NT>
 __global__ void __launch_bounds__(NT)
@@ -48,17 +55,23 @@ k_product_small(View<T> out, View<const T> lhs, View<const T> rhs, View<const T>
     using G = SmallGeo<W, N>;
     __shared__ T sl[G::dim];
     __shared__ T sr[G::dim];
-    int st[N + 2];
-#pragma unroll
-    for (int k = 0; k <= N + 1; ++k) st[k] = G::start(k);
     const int t = threadIdx.x;
-    const int lb = window_begin(st, lhs.mn, lhs.mx), le = window_end(st, lhs.mn, lhs.mx);
-    const int rb = window_begin(st, rhs.mn, rhs.mx), re = window_end(st, rhs.mn, rhs.mx);
+    const bool lw = lhs.mn <= lhs.mx, rw = rhs.mn <= rhs.mx;
+    const int lb = lw ? degree_start<W, N>(lhs.mn) : 0, le = lw ? degree_start<W, N>(lhs.mx + 1) : 0;
+    const int rb = rw ? degree_start<W, N>(rhs.mn) : 0, re = rw ? degree_start<W, N>(rhs.mx + 1) : 0;
     for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
         const T* L = lhs.obj(b);
         const T* R = rhs.obj(b);
-        for (int i = lb + t; i < le; i += NT) sl[i] = L[i];
-        for (int i = rb + t; i < re; i += NT) sr[i] = R[i];
+        // compile-time trip count: every load of the object is issued before
+        // the first LDS store waits on one
+#pragma unroll
+        for (int s = 0; s < (G::dim + NT - 1) / NT; ++s) {
+            const int i = t + s * NT;
+            if (G::dim % NT == 0 || i < G::dim) {
+                if (lb <= i && i < le) sl[i] = L[i];
+                if (rb <= i && i < re) sr[i] = R[i];
+            }
+        }
         __syncthreads();
         T* O = out.obj(b);
         const T* A = with_add ? add.obj(b) : nullptr;
@@ -97,8 +110,9 @@ int small_launch(int64_t n, const rpp_hip_tensor_batch& out, const rpp_hip_tenso
                  hipStream_t stream) {
     constexpr int dim = SmallGeo<W, N>::dim;
     constexpr int NT = dim <= 512 ? 64 : (dim <= 1024 ? 128 : 256);
-    const int64_t cap = static_cast<int64_t>(device_cu_count()) * (2048 / NT);
-    const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min(n, cap)));
+    // one object per work-group: the hardware scheduler keeps every compute
+    // unit full and no work-group waits on a second object's loads
+    const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(n, 0x7fffffff)));
     const View<const T> a = add ? make_cview<T>(*add) : View<const T>{nullptr, 0, 1, 1, 0};
     k_product_small<T, W, N, NT><<<grid, NT, 0, stream>>>(make_view<T>(out), make_cview<T>(lhs), make_cview<T>(rhs),
                                                           a, add ? 1 : 0, T(alpha), T(beta), n);

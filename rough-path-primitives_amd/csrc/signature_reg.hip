@@ -34,6 +34,7 @@
 #include "signature_reg.hpp"
 
 #include <algorithm>
+#include <tuple>
 #include <utility>
 
 namespace rpp_hip {
@@ -763,15 +764,16 @@ const RegTile<T>* fmexp_tile(int width, int depth) {
 }
 
 // The per-operation adjoint needs a tile fixing at most one letter; prefer
-// one letter and the smallest work-group.
+// one letter, then all levels in registers (one step per call leaves no
+// recurrence for LDS-resident levels to pay off), then the smallest
+// work-group.
 template <typename T>
 const RegTile<T>* fmexp_adj_tile(int width, int depth) {
     const RegTile<T>* best = nullptr;
+    auto key = [](const RegTile<T>& t) { return std::make_tuple(t.split, t.reg_levels, -t.threads); };
     for (const auto& t : reg_tiles<T>()) {
         if (t.width != width || t.depth != depth || t.split > 1) continue;
-        if (!best || t.split > best->split || (t.split == best->split && t.threads < best->threads) ||
-            (t.split == best->split && t.threads == best->threads && t.reg_levels > best->reg_levels))
-            best = &t;
+        if (!best || key(t) > key(*best)) best = &t;
     }
     return best;
 }

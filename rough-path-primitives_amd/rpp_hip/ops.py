@@ -333,12 +333,14 @@ def lie_basis(width: int, depth: int) -> dict:
 # -------------------------------------------------- batched Chen recursion
 
 def fmexp_tile(width: int, depth: int, dtype):
-    """(threads, split) of the register tile serving plain ft_fmexp at
-    (width, depth), or None."""
+    """(kernel, threads, split) of the tile serving plain ft_fmexp at
+    (width, depth): kernel "k_fmexp_coal" or "k_fmexp_tile"; None when a
+    generic kernel serves it."""
     nbytes = torch.tensor([], dtype=dtype).element_size()
     th, sp = C.c_int32(0), C.c_int32(0)
-    if A.lib().rpp_hip_fmexp_tile(nbytes, width, depth, C.byref(th), C.byref(sp)) == 1:
-        return th.value, sp.value
+    kind = A.lib().rpp_hip_fmexp_tile(nbytes, width, depth, C.byref(th), C.byref(sp))
+    if kind in (1, 2):
+        return ("k_fmexp_coal" if kind == 2 else "k_fmexp_tile"), th.value, sp.value
     return None
 
 
