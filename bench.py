@@ -19,7 +19,7 @@ same line carries, each under its own key:
   ft_mul          1024 pairs of width 4 depth 4 tensors (GPU and reference
                   CPU) and 2^17 pairs for the HBM fraction;
   log_signature   16384 paths, width 3, depth 8, 64 increments: fused
-                  signature, ft_log, tensor_to_lie;
+                  signature, ft_log, tensor_to_lie, lie_to_tensor;
   uneven          8192 paths, width 8, depth 5, lengths uniform in [16, 1024]:
                   fused forward replayed from a hipGraph, FP32 and FP64;
   roofline        the dominant kernel of the headline path (the ft_fmexp
@@ -350,7 +350,16 @@ def ft_mul_section(rpp, torch, O, dev):
 
 # ---------------------------------------------------------- log-signature (C3)
 
+def lie_l2t(lie, dim, ts):
+    import scipy.sparse as sp
+    l2t = lie["l2t"]
+    return sp.csc_matrix((l2t["data"], l2t["indices"], l2t["offsets"]), shape=(dim, ts - 1))
+
+
 def log_signature(rpp, torch, O, dev, steps=3, warmup=1):
+    """Signature -> ft_log -> tensor_to_lie -> lie_to_tensor, each stage timed
+    with HIP events; the round trip lie_to_tensor(tensor_to_lie(log S)) returns
+    log S (a Lie element), a size-independent check on all 16384 paths."""
     import scipy.sparse as sp
     w, N, n, L = 3, 8, 16384, 64
     inc = make_increments(0xC3, n, L, w)
@@ -363,16 +372,29 @@ def log_signature(rpp, torch, O, dev, steps=3, warmup=1):
     order = rpp.path_order(off)
     sig = torch.empty((n, dim), dtype=torch.float64, device=dev)
     lg = torch.empty_like(sig)
+    back = torch.empty_like(sig)
     lie_out = torch.empty((n, ts), dtype=torch.float64, device=dev)
     rpp.tensor_to_lie(lie_out, lg, basis)  # uploads the Hall metadata once
+    stages = ("signature", "ft_log", "tensor_to_lie", "lie_to_tensor")
+    timers = {k: EventTimer(torch) for k in stages}
 
-    def step():
-        rpp.signature(flat, off, basis, out=sig, order=order)
-        rpp.ft_log(lg, sig, basis)
-        rpp.tensor_to_lie(lie_out, lg, basis)
+    def step(timed_stages=False):
+        calls = ((lambda: rpp.signature(flat, off, basis, out=sig, order=order)),
+                 (lambda: rpp.ft_log(lg, sig, basis)),
+                 (lambda: rpp.tensor_to_lie(lie_out, lg, basis)),
+                 (lambda: rpp.lie_to_tensor(back, lie_out, basis)))
+        for name, call in zip(stages, calls):
+            if timed_stages:
+                timers[name].start()
+            call()
+            if timed_stages:
+                timers[name].stop()
         return lie_out
 
     ms, wall, got = timed(torch, step, steps, warmup)
+    for _ in range(steps):
+        step(True)
+    stage_ms = {k: timers[k].total_ms() / steps for k in stages}
     k = 4
     s_ref = O.signature(w, N, inc[:k].reshape(-1, w), np.arange(k + 1, dtype=np.int64) * L)
     log_ref = np.zeros_like(s_ref)
@@ -380,12 +402,28 @@ def log_signature(rpp, torch, O, dev, steps=3, warmup=1):
     t2l = lie["t2l"]
     T2L = sp.csc_matrix((t2l["data"], t2l["indices"], t2l["offsets"]), shape=(ts - 1, dim))
     lie_ref = (T2L @ log_ref.T).T
+    lg_all = lg.cpu().numpy()
+    back_all = back.cpu().numpy()
+    rt = np.max(np.abs(back_all - lg_all) / np.maximum(1.0, np.maximum(np.abs(back_all), np.abs(lg_all))), axis=1)
+    worst = int(np.argmax(rt))
+    # the worst round-trip path against the oracle: a large round-trip error
+    # with a small error against the oracle is rounding in the log itself
+    s_w = O.signature(w, N, inc[worst], np.array([0, L], dtype=np.int64))
+    log_w = np.zeros_like(s_w)
+    O.orc.ft_log(w, N, log_w, s_w)
+    back_w = np.asarray((lie_l2t(lie, dim, ts) @ (T2L @ log_w.T)).T)
     res = {"config": {"paths": n, "width": w, "depth": N, "coefficients": dim, "lie_dimension": ts - 1,
-                      "increments_per_path": L, "dtype": "f64", "steps": steps, "warmup": warmup},
+                      "increments_per_path": L, "dtype": "f64", "steps": steps, "warmup": warmup,
+                      "pipeline": "signature -> ft_log -> tensor_to_lie -> lie_to_tensor"},
            "ms_per_step": ms, "wall_ms_per_step": wall, "paths_per_s": n / (ms * 1e-3),
-           "log_max_rel_err_vs_fp64_oracle": rel_err(lg[:k].cpu().numpy(), log_ref),
-           "lie_max_rel_err_vs_fp64_oracle": rel_err(got[:k, 1:].cpu().numpy(), lie_ref)}
-    del sig, lg, lie_out
+           "stage_ms": stage_ms,
+           "log_max_rel_err_vs_fp64_oracle": rel_err(lg_all[:k], log_ref),
+           "lie_max_rel_err_vs_fp64_oracle": rel_err(got[:k, 1:].cpu().numpy(), lie_ref),
+           "round_trip_max_rel_err_all_paths": float(rt[worst]),
+           "round_trip_worst_path": {"path": worst, "gpu_log_rel_err_vs_oracle": rel_err(lg_all[worst], log_w[0]),
+                                     "oracle_round_trip_rel_err": rel_err(back_w, log_w),
+                                     "max_abs_log": float(np.abs(log_w).max())}}
+    del sig, lg, lie_out, back
     torch.cuda.empty_cache()
     return res
 

@@ -31,7 +31,10 @@
 #include <rpp/operations/basic/ft_inplace_mul.hpp>
 #include <rpp/operations/basic/ft_mul.hpp>
 #include <rpp/operations/basic/lie_to_tensor.hpp>
+#include <rpp/operations/basic/tensor_antipode.hpp>
+#include <rpp/operations/basic/tensor_generalised_antipode.hpp>
 #include <rpp/operations/basic/tensor_pairing.hpp>
+#include <rpp/operations/basic/tensor_reflect.hpp>
 #include <rpp/operations/basic/tensor_to_lie.hpp>
 #include <rpp/operations/intermediate/ft_exp.hpp>
 #include <rpp/operations/intermediate/ft_fmexp.hpp>
@@ -252,6 +255,27 @@ public:
         rpp_hip_ft_fmexp_desc d{tensor_basis_of(bases), n, tensor_of<Accum>(std::get<0>(b)),
                                 tensor_of<Accum>(std::get<1>(b)), tensor_of<Accum>(std::get<2>(b))};
         return to_error(RPP_HIP_CALL(rpp_hip_ft_fmexp, d, cfg.stream));
+    }
+};
+
+// tensor_antipode / tensor_reflect (operations/basic/tensor_antipode.hpp:15-39,
+// tensor_reflect.hpp:15-39; TensorGeneralisedAntipode,
+// tensor_generalised_antipode.hpp:17-81)
+template <typename Accum_, typename Arch_, TensorAntipodeSigningPolicy Policy>
+class TensorGeneralisedAntipode<hip::strategies::BlockStrategy<Accum_, Arch_>, Policy>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases,
+                                         int64_t n) noexcept {
+        using namespace hip::detail;
+        rpp_hip_ft_unary_desc d{tensor_basis_of(bases), n, tensor_of<Accum>(std::get<0>(b)),
+                                tensor_of<Accum>(std::get<1>(b))};
+        if constexpr (Policy == TensorAntipodeSigningPolicy::SignByDegree)
+            return to_error(RPP_HIP_CALL(rpp_hip_tensor_antipode, d, cfg.stream));
+        else
+            return to_error(RPP_HIP_CALL(rpp_hip_tensor_reflect, d, cfg.stream));
     }
 };
 

@@ -200,6 +200,17 @@ typedef struct rpp_hip_ft_log_adjoint_desc {
 int rpp_hip_ft_log_adjoint_f32(const rpp_hip_ft_log_adjoint_desc* desc, void* stream);
 int rpp_hip_ft_log_adjoint_f64(const rpp_hip_ft_log_adjoint_desc* desc, void* stream);
 
+/* rpp::ops::tensor_antipode and rpp::ops::tensor_reflect
+ * (operations/basic/tensor_antipode.hpp:15-39, tensor_reflect.hpp:15-39;
+ * TensorGeneralisedAntipode, tensor_generalised_antipode.hpp:17-81): on each
+ * degree k of out's window, out[reverse(w)] = (-1)^k arg[w] (antipode) or
+ * arg[w] (reflect); 0 where arg's window lacks degree k.  out must not alias
+ * arg. */
+int rpp_hip_tensor_antipode_f32(const rpp_hip_ft_unary_desc* desc, void* stream);
+int rpp_hip_tensor_antipode_f64(const rpp_hip_ft_unary_desc* desc, void* stream);
+int rpp_hip_tensor_reflect_f32(const rpp_hip_ft_unary_desc* desc, void* stream);
+int rpp_hip_tensor_reflect_f64(const rpp_hip_ft_unary_desc* desc, void* stream);
+
 /* --------------------------------------------------------- sparse and Lie */
 
 enum rpp_hip_matrix_format { RPP_HIP_CSR = 0, RPP_HIP_CSC = 1 };

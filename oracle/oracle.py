@@ -196,6 +196,18 @@ class _Impl:
               *_arg(out), *_win(ow or f), *_arg(op), *_win(pw or f), *_arg(arg), *_win(gw or f))
         return out
 
+    def tensor_antipode(self, width, depth, out, arg, ow=None, aw=None, n=None, sign_by_degree=True):
+        sfx, _ = _sfx(out.dtype)
+        n = out.shape[0] if n is None else n
+        f = _full(depth)
+        name = ("orc_batch_tensor_antipode_" if self.which == "orc" else "ref_tensor_antipode_") + sfx
+        _call(self._lib(), name, _i(1 if sign_by_degree else 0), _i(width), _i(depth), _ll(n),
+              *_arg(out), *_win(ow or f), *_arg(arg), *_win(aw or f))
+        return out
+
+    def tensor_reflect(self, width, depth, out, arg, ow=None, aw=None, n=None):
+        return self.tensor_antipode(width, depth, out, arg, ow, aw, n, sign_by_degree=False)
+
     def ft_adj_lmul(self, width, depth, out, op, arg, ow=None, pw=None, gw=None, n=None):
         return self._adj("ft_adj_lmul", width, depth, out, op, arg, ow, pw, gw, n)
 

@@ -180,6 +180,17 @@ int ft_log(int w, int N, long long n, T* out, long long os, int omin, int omax,
 }
 
 template <typename T>
+int antipode(int sign_by_degree, int w, int N, long long n, T* out, long long os, int omin, int omax, const T* arg,
+             long long as, int amin, int amax) {
+    BasisHolder bh(w, N);
+    auto O = batch<T>(out, os, bh.basis, omin, omax);
+    auto A = batch<T>(arg, as, bh.basis, amin, amax);
+    if (sign_by_degree)
+        return code_of(rpp::ops::tensor_antipode(Strat<T>{}, {}, O, A, bh.basis, static_cast<Index>(n)));
+    return code_of(rpp::ops::tensor_reflect(Strat<T>{}, {}, O, A, bh.basis, static_cast<Index>(n)));
+}
+
+template <typename T>
 int pairing(int w, int N, long long n, T* out, const T* f, long long fs,
             int fmin, int fmax, const T* x, long long xs, int xmin, int xmax) {
     BasisHolder bh(w, N);
@@ -320,6 +331,13 @@ extern "C" {
                               int gmin, int gmax) {                            \
         return adj<T, false>(w, N, n, out, os, omin, omax, op, ps, pmin, pmax, \
                              arg, gs, gmin, gmax);                             \
+    }                                                                          \
+    int ref_tensor_antipode_##SFX(int sign, int w, int N, long long n, T* out, \
+                                  long long os, int omin, int omax,            \
+                                  const T* arg, long long as, int amin,        \
+                                  int amax) {                                  \
+        return antipode<T>(sign, w, N, n, out, os, omin, omax, arg, as, amin,  \
+                           amax);                                              \
     }                                                                          \
     int ref_signature_##SFX(int w, int N, long long n, long long L,            \
                             const T* incr, T* out, int threads) {              \

@@ -422,6 +422,42 @@ int FN(orc_batch_ft_log)(int width, int depth, long long n, T* out,
     return 0;
 }
 
+/* TensorGeneralisedAntipode (cpu/.../basic/tensor_generalised_antipode.hpp:40-70):
+ * on each degree k of out's window, out[reverse(w)] = s_k * arg[w] with
+ * s_k = (-1)^k (antipode) or 1 (reflect); 0 where arg lacks degree k. */
+static void FN(antipode)(const orc_basis* B, T* out, int omin, int omax, const T* arg, int amin, int amax,
+                         int sign_by_degree)
+{
+    for (int k = omin; k <= omax; ++k) {
+        T* o = out + B->start[k];
+        const long long nk = B->pow[k];
+        if (k < amin || k > amax) {
+            for (long long i = 0; i < nk; ++i) o[i] = (T)0;
+            continue;
+        }
+        const T* a = arg + B->start[k];
+        const T s = (sign_by_degree && (k % 2)) ? (T)-1 : (T)1;
+        for (long long i = 0; i < nk; ++i) {
+            long long r = 0, v = i;
+            for (int d = 0; d < k; ++d) {
+                r = r * B->width + v % B->width;
+                v /= B->width;
+            }
+            o[r] = a[i] * s;
+        }
+    }
+}
+
+int FN(orc_batch_tensor_antipode)(int sign_by_degree, int width, int depth, long long n, T* out, long long os,
+                                  int omin, int omax, const T* arg, long long as, int amin, int amax)
+{
+    orc_basis B;
+    if (orc_basis_init(&B, width, depth)) return 3;
+    for (long long i = 0; i < n; ++i)
+        FN(antipode)(&B, OBJ(out, os, i), omin, omax, OBJ(arg, as, i), amin, amax, sign_by_degree);
+    return 0;
+}
+
 int FN(orc_batch_tensor_pairing)(int width, int depth, long long n, T* out,
                                  long long ostride, const T* f, long long fs,
                                  int fmin, int fmax, const T* x, long long xs,

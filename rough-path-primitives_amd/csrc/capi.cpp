@@ -183,6 +183,19 @@ int unary_impl(int op, const rpp_hip_ft_unary_desc* d, void* s) {
 }
 
 template <typename T>
+int antipode_impl(bool sign_by_degree, const rpp_hip_ft_unary_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    RPP_TRY(check_basis(d->basis));
+    RPP_TRY(check_view(d->out, d->basis, d->num_batches, "out", true));
+    RPP_TRY(check_view(d->arg, d->basis, d->num_batches, "arg", false));
+    if (d->out.data == d->arg.data && d->out.min_degree <= d->out.max_degree &&
+        d->arg.min_degree <= d->arg.max_degree)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "out must not alias arg");
+    const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    return launch_antipode<T>(sign_by_degree, B, d->num_batches, d->out, d->arg, as_stream(s));
+}
+
+template <typename T>
 int fmexp_impl(const rpp_hip_ft_fmexp_desc* d, void* s) {
     if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
     RPP_TRY(check_basis(d->basis));
@@ -306,6 +319,14 @@ RPP_C_ENTRY(rpp_hip_ft_fmexp, rpp_hip_ft_fmexp_desc, fmexp_impl)
 RPP_C_ENTRY(rpp_hip_tensor_pairing, rpp_hip_tensor_pairing_desc, pairing_impl)
 RPP_C_ENTRY(rpp_hip_sparse_matrix_vector_product, rpp_hip_spmv_desc, spmv_impl)
 
+int rpp_hip_tensor_antipode_f32(const rpp_hip_ft_unary_desc* d, void* s) { return antipode_impl<float>(true, d, s); }
+int rpp_hip_tensor_antipode_f64(const rpp_hip_ft_unary_desc* d, void* s) {
+    return antipode_impl<double>(true, d, s);
+}
+int rpp_hip_tensor_reflect_f32(const rpp_hip_ft_unary_desc* d, void* s) { return antipode_impl<float>(false, d, s); }
+int rpp_hip_tensor_reflect_f64(const rpp_hip_ft_unary_desc* d, void* s) {
+    return antipode_impl<double>(false, d, s);
+}
 int rpp_hip_ft_exp_f32(const rpp_hip_ft_unary_desc* d, void* s) { return unary_impl<float>(kOpExp, d, s); }
 int rpp_hip_ft_exp_f64(const rpp_hip_ft_unary_desc* d, void* s) { return unary_impl<double>(kOpExp, d, s); }
 int rpp_hip_ft_log_f32(const rpp_hip_ft_unary_desc* d, void* s) { return unary_impl<float>(kOpLog, d, s); }

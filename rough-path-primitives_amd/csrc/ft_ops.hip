@@ -467,6 +467,41 @@ k_adjoint(DevBasis B, View<T> out, View<const T> op, View<const T> arg, int64_t 
     }
 }
 
+// tensor_antipode / tensor_reflect (cpu/.../tensor_generalised_antipode.hpp:
+// 40-70): out[reverse(w)] = s_k arg[w] on each degree k of out's window, with
+// s_k = (-1)^k when SIGN, 0 where arg lacks degree k.  Reversal is an
+// involution, so each output word gathers arg[reverse(w)] and the writes stay
+// coalesced.  One work-group per object.
+template <typename T, int W, bool CS1, bool SIGN>
+__global__ void __launch_bounds__(kBlock)
+k_antipode(DevBasis B, View<T> out, View<const T> arg, int64_t n) {
+    using L = Letters<W>;
+    const uint32_t width = L::width(B);
+    for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
+        T* o = out.obj(b);
+        const T* a = arg.obj(b);
+        for (int k = out.mn; k <= out.mx; ++k) {
+            const bool has = arg.has(k);
+            const T s = (SIGN && (k & 1)) ? T(-1) : T(1);
+            const int64_t base = B.start[k];
+            const uint32_t nk = B.pow[k];
+            for (uint32_t w = threadIdx.x; w < nk; w += blockDim.x) {
+                T v = T(0);
+                if (has) {
+                    uint32_t r = 0, x = w;
+                    for (int d = 0; d < k; ++d) {
+                        const uint32_t q = L::div(B, x);
+                        r = r * width + (x - q * width);
+                        x = q;
+                    }
+                    v = a[at<CS1>(base + r, arg.cs)] * s;
+                }
+                o[at<CS1>(base + w, out.cs)] = v;
+            }
+        }
+    }
+}
+
 // CSR rows with the argument staged in LDS: a work-group streams objects; it
 // copies the object's argument segment (the matrix columns) into LDS with
 // coalesced loads, then groups of 8 lanes take one row each, read the row's
@@ -684,6 +719,22 @@ int launch_adjoint(bool left, const DevBasis& B, int64_t n, const rpp_hip_tensor
 }
 
 template <typename T>
+int launch_antipode(bool sign_by_degree, const DevBasis& B, int64_t n, const rpp_hip_tensor_batch& out,
+                    const rpp_hip_tensor_batch& arg, hipStream_t stream) {
+    if (n <= 0 || out.min_degree > out.max_degree) return 0;
+    const unsigned grid = grid_for(n);
+    const bool contig = all_contig({&out, &arg});
+    const View<T> o = make_view<T>(out);
+    const View<const T> a = make_cview<T>(arg);
+    if (sign_by_degree) {
+        RPP_DISPATCH_W(B.width, contig, (k_antipode<T, W_, C_, true><<<grid, kBlock, 0, stream>>>(B, o, a, n)));
+    } else {
+        RPP_DISPATCH_W(B.width, contig, (k_antipode<T, W_, C_, false><<<grid, kBlock, 0, stream>>>(B, o, a, n)));
+    }
+    return check_hip(hipGetLastError(), "k_antipode launch");
+}
+
+template <typename T>
 int launch_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg,
                 const rpp_hip_sparse_matrix& M, double alpha, hipStream_t stream) {
     if (n <= 0) return 0;
@@ -742,6 +793,8 @@ int launch_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector
     template int launch_adjoint<T>(bool, const DevBasis&, int64_t, const rpp_hip_tensor_batch&,   \
                                    const rpp_hip_tensor_batch&, const rpp_hip_tensor_batch&,      \
                                    hipStream_t, bool);                                            \
+    template int launch_antipode<T>(bool, const DevBasis&, int64_t, const rpp_hip_tensor_batch&,  \
+                                    const rpp_hip_tensor_batch&, hipStream_t);                     \
     template int launch_spmv<T>(int64_t, const rpp_hip_vector_batch&, const rpp_hip_vector_batch&, \
                                 const rpp_hip_sparse_matrix&, double, hipStream_t);
 

@@ -28,6 +28,11 @@ int launch_adjoint(bool left, const DevBasis& B, int64_t n, const rpp_hip_tensor
                    const rpp_hip_tensor_batch& op, const rpp_hip_tensor_batch& arg,
                    hipStream_t stream, bool accumulate = false);
 
+// tensor_antipode (sign_by_degree) / tensor_reflect over out's window.
+template <typename T>
+int launch_antipode(bool sign_by_degree, const DevBasis& B, int64_t n, const rpp_hip_tensor_batch& out,
+                    const rpp_hip_tensor_batch& arg, hipStream_t stream);
+
 // Adjoint of ft_log (log_adjoint.hip).
 template <typename T>
 int launch_log_adjoint(const DevBasis& B, int64_t n, const rpp_hip_tensor_batch& grad_arg,

@@ -4,14 +4,14 @@ from ._capi import LIB_PATH, exported_symbols, lib
 from .ops import (Batch, RppHipError, SparseMatrix, TensorBasis, fmexp_tile, ft_adj_lmul, ft_adj_rmul,
                   ft_exp, ft_fma, ft_fmexp, ft_fmexp_adjoint, ft_inplace_fma, ft_inplace_mul, ft_log, ft_log_adjoint,
                   ft_mul, interleaved, lie_basis, lie_to_tensor, make_tensor_batch, path_order, signature,
-                  signature_backward, signature_tiles, sparse_matrix_vector_product, tensor_dimension,
-                  tensor_pairing, tensor_to_lie)
+                  signature_backward, signature_tiles, sparse_matrix_vector_product, tensor_antipode,
+                  tensor_dimension, tensor_pairing, tensor_reflect, tensor_to_lie)
 from .graph import SignatureGraph
 
 __all__ = [
     "Batch", "RppHipError", "SignatureGraph", "SparseMatrix", "TensorBasis", "fmexp_tile", "ft_adj_lmul",
     "ft_adj_rmul", "ft_exp", "ft_fma", "ft_fmexp", "ft_fmexp_adjoint", "ft_inplace_fma", "ft_inplace_mul",
     "ft_log", "ft_log_adjoint", "ft_mul", "interleaved", "lie_basis", "lie_to_tensor", "make_tensor_batch", "path_order",
-    "signature", "signature_backward", "signature_tiles", "sparse_matrix_vector_product", "tensor_dimension",
-    "tensor_pairing", "tensor_to_lie", "LIB_PATH", "exported_symbols", "lib",
+    "signature", "signature_backward", "signature_tiles", "sparse_matrix_vector_product", "tensor_antipode",
+    "tensor_dimension", "tensor_pairing", "tensor_reflect", "tensor_to_lie", "LIB_PATH", "exported_symbols", "lib",
 ]

@@ -122,6 +122,8 @@ ENTRY_POINTS = {
     "rpp_hip_ft_inplace_mul": FtInplaceMulDesc,
     "rpp_hip_ft_exp": FtUnaryDesc,
     "rpp_hip_ft_log": FtUnaryDesc,
+    "rpp_hip_tensor_antipode": FtUnaryDesc,
+    "rpp_hip_tensor_reflect": FtUnaryDesc,
     "rpp_hip_ft_log_adjoint": FtLogAdjointDesc,
     "rpp_hip_ft_fmexp": FtFmexpDesc,
     "rpp_hip_ft_fmexp_adjoint": FtFmexpAdjointDesc,

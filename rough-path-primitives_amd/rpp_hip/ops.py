@@ -202,6 +202,22 @@ def ft_log(out, arg, basis: TensorBasis, num_batches=None, stream=None):
     _call("rpp_hip_ft_log" + _sfx(out, arg), d, stream)
 
 
+def tensor_antipode(out, arg, basis: TensorBasis, num_batches=None, stream=None):
+    """rpp::ops::tensor_antipode (operations/basic/tensor_antipode.hpp:15-39):
+    out[reverse(w)] = (-1)^|w| arg[w] on out's window."""
+    out, arg = (_as_batch(x, basis) for x in (out, arg))
+    d = A.FtUnaryDesc(basis._c(), _n(num_batches, out), out._c(), arg._c())
+    _call("rpp_hip_tensor_antipode" + _sfx(out, arg), d, stream)
+
+
+def tensor_reflect(out, arg, basis: TensorBasis, num_batches=None, stream=None):
+    """rpp::ops::tensor_reflect (operations/basic/tensor_reflect.hpp:15-39):
+    out[reverse(w)] = arg[w] on out's window."""
+    out, arg = (_as_batch(x, basis) for x in (out, arg))
+    d = A.FtUnaryDesc(basis._c(), _n(num_batches, out), out._c(), arg._c())
+    _call("rpp_hip_tensor_reflect" + _sfx(out, arg), d, stream)
+
+
 def ft_log_adjoint(grad_arg, arg, cotangent, basis: TensorBasis, num_batches=None, stream=None):
     """Adjoint of ft_log (operations/intermediate/ft_log.hpp:66-85): grad_arg =
     d loss / d arg on arg's window, given cotangent = d loss / d log(arg)."""

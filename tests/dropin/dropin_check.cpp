@@ -208,6 +208,16 @@ void run(int width, int depth, Index n) {
         report(tag("ft_adj_lmul"), ol.host(), hl);
         report(tag("ft_adj_rmul"), orr.host(), hr);
     }
+    {  // antipode and reflection, output window from degree 1
+        std::vector<T> ha = nanv, hr = nanv;
+        (void)rpp::ops::tensor_antipode(cpu, {}, hb(ha.data(), 1, D), hcb(rhs.data(), 0, D), basis, n);
+        (void)rpp::ops::tensor_reflect(cpu, {}, hb(hr.data(), 1, D), hcb(rhs.data(), 0, D), basis, n);
+        Dev<T> oa(nanv), orf(nanv), r(rhs);
+        (void)rpp::ops::tensor_antipode(gpu, {}, hb(oa.p, 1, D), hcb(r.p, 0, D), basis, n);
+        (void)rpp::ops::tensor_reflect(gpu, {}, hb(orf.p, 1, D), hcb(r.p, 0, D), basis, n);
+        report(tag("tensor_antipode"), oa.host(), ha);
+        report(tag("tensor_reflect"), orf.host(), hr);
+    }
     if (width >= 2) {  // tensor_to_lie / lie_to_tensor with the reference-built matrices
         rpp::basis::HallBasis<Arch> hall(static_cast<Degree>(width), D);
         auto lb = hall.to_lie_basis();

@@ -283,3 +283,28 @@ def test_ft_log_adjoint(dev, shape, windows, dtype):
     g = got.cpu().numpy()
     assert np.all(np.isnan(g[:, ~am]))
     assert_close(g[:, am], expected[:, am], dtype, f"log adjoint {shape} {windows}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("op", ["tensor_antipode", "tensor_reflect"])
+@pytest.mark.parametrize("shape", [(2, 5), (3, 4), (5, 6), (7, 3), (16, 2)])
+@pytest.mark.parametrize("windows", [((0, 99), (0, 99)), ((1, 3), (0, 99)), ((0, 99), (2, 3))],
+                         ids=["full", "out13", "arg23"])
+def test_antipode_reflect(dev, op, shape, windows, dtype):
+    w, N = shape
+    ow = (windows[0][0], min(windows[0][1], N))
+    aw = (windows[1][0], min(windows[1][1], N))
+    rng = np.random.default_rng(700 + w * 10 + N)
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    n = 3
+    arg = random_batch(rng, n, dim, dtype)
+    init = np.full((n, dim), np.nan, dtype=dtype)
+    expected = init.copy()
+    getattr(O.orc, op)(w, N, expected, arg, ow=ow, aw=aw)
+    out = to_dev(init, dev)
+    getattr(rpp_hip, op)(rpp_hip.Batch(out, *ow), rpp_hip.Batch(to_dev(arg, dev), *aw), basis)
+    got = out.cpu().numpy()
+    m = window_mask(w, N, ow)
+    assert np.all(np.isnan(got[:, ~m]))
+    assert np.array_equal(got[:, m], expected[:, m]), f"{op} {shape} {windows}"

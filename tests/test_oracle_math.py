@@ -238,3 +238,40 @@ def test_log_vjp_finite_differences(shape):
         m[:, i] -= h
         fd[:, c] = (loss(p) - loss(m)) / (2 * h)
     assert rel_err(g[:, idx], fd) < 1e-7
+
+
+@pytest.mark.parametrize("shape", [(2, 5), (3, 4)])
+def test_antipode_identities(shape):
+    """The antipode reverses products, alpha(a b) = alpha(b) alpha(a), maps
+    exp(x) to exp(-x) for a degree-one x, and reflection is an involution."""
+    w, N = shape
+    dim = O.tensor_dimension(w, N)
+    rng = np.random.default_rng(40 + w)
+    a = rng.normal(size=(2, dim))
+    b = rng.normal(size=(2, dim))
+
+    def mul(x, y):
+        out = np.zeros_like(x)
+        O.orc.ft_mul(w, N, out, x, y)
+        return out
+
+    def anti(x):
+        out = np.zeros_like(x)
+        O.orc.tensor_antipode(w, N, out, x)
+        return out
+
+    assert rel_err(anti(mul(a, b)), mul(anti(b), anti(a))) < 1e-13
+    x = np.zeros((2, dim))
+    x[:, 1:1 + w] = rng.normal(size=(2, w))
+    ex = np.zeros_like(x)
+    O.orc.ft_exp(w, N, ex, x)
+    emx = np.zeros_like(x)
+    O.orc.ft_exp(w, N, emx, -x)
+    assert rel_err(anti(ex), emx) < 1e-14
+    r1 = np.zeros_like(a)
+    O.orc.tensor_reflect(w, N, r1, a)
+    r2 = np.zeros_like(a)
+    O.orc.tensor_reflect(w, N, r2, r1)
+    assert np.array_equal(r2, a)
+    # a non-palindromic word moves: reflect is not the identity
+    assert not np.array_equal(r1, a)

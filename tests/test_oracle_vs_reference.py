@@ -163,3 +163,23 @@ def test_signature_chain(shape, dtype):
     a = O.signature(w, N, inc.reshape(n * L, w), np.arange(n + 1) * L, dtype=dtype)
     b = O.ref_signature(w, N, inc)
     check(a, b, dtype)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("op", ["tensor_antipode", "tensor_reflect"])
+@pytest.mark.parametrize("shape", [(2, 5), (3, 4), (4, 3), (5, 2)])
+@pytest.mark.parametrize("windows", [((0, 99), (0, 99)), ((1, 3), (0, 99)), ((0, 99), (2, 3)), ((2, 2), (1, 2))])
+def test_antipode_reflect(op, shape, windows, dtype):
+    """tensor_antipode / tensor_reflect: bit for bit, every window pairing."""
+    w, N = shape
+    ow = (windows[0][0], min(windows[0][1], N))
+    aw = (windows[1][0], min(windows[1][1], N))
+    rng = np.random.default_rng(20 + w)
+    dim = O.tensor_dimension(w, N)
+    arg = random_batch(rng, 3, dim, dtype)
+    a = np.full((3, dim), np.nan, dtype=dtype)
+    b = a.copy()
+    getattr(O.orc, op)(w, N, a, arg, ow=ow, aw=aw)
+    getattr(O.refimpl, op)(w, N, b, arg, ow=ow, aw=aw)
+    assert np.array_equal(np.isnan(a), np.isnan(b))
+    assert np.array_equal(np.nan_to_num(a), np.nan_to_num(b))
