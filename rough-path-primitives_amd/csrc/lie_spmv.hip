@@ -15,7 +15,7 @@ namespace rpp_hip {
 
 namespace {
 
-constexpr int kLieBlock = 512;
+constexpr int kLieBlock = 1024;
 constexpr int kLieLanes = 8;
 constexpr int kLieTable = 256;
 constexpr size_t kLieLdsMax = 160 * 1024;
@@ -46,13 +46,26 @@ k_lie_spmv(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t
             T acc[G];
 #pragma unroll
             for (int g = 0; g < G; ++g) acc[g] = T(0);
-            for (int32_t e = e0 + lane; e < e1; e += kLieLanes) {
-                const uint32_t u = packed[e];
-                const T v = tab[u >> 24];
-                const uint32_t c = u & 0xffffffu;
+            // four entries per lane in flight: the loads of one step are all
+            // issued before the first gather waits on them
+            constexpr int U = 4;
+            for (int32_t e = e0 + lane; e < e1; e += U * kLieLanes) {
+                uint32_t u[U];
 #pragma unroll
-                for (int g = 0; g < G; ++g)
-                    if (g < gn) acc[g] += v * xs[g * cols + c];
+                for (int j = 0; j < U; ++j) {
+                    const int32_t ej = e + j * kLieLanes;
+                    u[j] = ej < e1 ? packed[ej] : 0u;
+                }
+#pragma unroll
+                for (int j = 0; j < U; ++j) {
+                    if (e + j * kLieLanes < e1) {
+                        const T v = tab[u[j] >> 24];
+                        const uint32_t c = u[j] & 0xffffffu;
+#pragma unroll
+                        for (int g = 0; g < G; ++g)
+                            if (g < gn) acc[g] += v * xs[g * cols + c];
+                    }
+                }
             }
 #pragma unroll
             for (int g = 0; g < G; ++g) {
