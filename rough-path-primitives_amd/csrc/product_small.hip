@@ -112,7 +112,7 @@ int small_launch(int64_t n, const rpp_hip_tensor_batch& out, const rpp_hip_tenso
                  const rpp_hip_tensor_batch& rhs, const rpp_hip_tensor_batch* add, double alpha, double beta,
                  hipStream_t stream) {
     constexpr int dim = SmallGeo<W, N>::dim;
-    constexpr int NT = dim <= 512 ? 64 : (dim <= 1024 ? 128 : 256);
+    constexpr int NT = dim <= 128 ? 64 : (dim <= 1024 ? 128 : 256);
     // one object per work-group: the hardware scheduler keeps every compute
     // unit full and no work-group waits on a second object's loads
     const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(n, 0x7fffffff)));
