@@ -17,6 +17,7 @@
 #include "common.hpp"
 #include "ft_ops.hpp"
 #include "fmexp_coal.hpp"
+#include "log_lds.hpp"
 #include "product_small.hpp"
 #include "signature_reg.hpp"
 
@@ -680,6 +681,9 @@ int launch_horner(int op, const DevBasis& B, int64_t n, const rpp_hip_tensor_bat
         RPP_DISPATCH_W(B.width, contig, (k_exp_deg1<T, W_, C_><<<grid, kBlock, 0, stream>>>(B, o, x, n)));
         return check_hip(hipGetLastError(), "k_exp_deg1 launch");
     }
+    if (op == kOpLog && contig && out.min_degree == 0 && out.max_degree == B.depth &&
+        log_lds_available<T>(B.width, B.depth))
+        return launch_log_lds<T>(B, n, out, arg, stream);
     RPP_DISPATCH_W(B.width, contig,
                    (k_horner<T, W_, C_><<<grid, kBlock, 0, stream>>>(op, B, o, x, m, n)));
     return check_hip(hipGetLastError(), "k_horner launch");

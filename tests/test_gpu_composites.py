@@ -308,3 +308,24 @@ def test_antipode_reflect(dev, op, shape, windows, dtype):
     m = window_mask(w, N, ow)
     assert np.all(np.isnan(got[:, ~m]))
     assert np.array_equal(got[:, m], expected[:, m]), f"{op} {shape} {windows}"
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("shape", [(2, 8), (2, 10), (3, 6), (3, 7), (3, 8), (4, 5), (4, 6), (5, 4), (5, 5), (6, 4),
+                                   (8, 4)])
+@pytest.mark.parametrize("aw", [(0, 99), (1, 99), (2, 99), (1, 3)], ids=["a0N", "a1N", "a2N", "a13"])
+def test_ft_log_lds_shapes(dev, shape, aw, dtype):
+    """ft_log at the shapes with an LDS-resident kernel (full out window),
+    argument windows included, against the oracle."""
+    w, N = shape
+    aw = (aw[0], min(aw[1], N))
+    rng = np.random.default_rng(900 + 10 * w + N)
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    n = 5
+    arg = damped_batch(rng, n, w, N, dtype)
+    expected = np.full((n, dim), np.nan, dtype=dtype)
+    O.orc.ft_log(w, N, expected, arg, aw=aw)
+    out = to_dev(np.full((n, dim), np.nan, dtype=dtype), dev)
+    rpp_hip.ft_log(out, rpp_hip.Batch(to_dev(arg, dev), *aw), basis)
+    assert_close(out.cpu().numpy(), expected, dtype, f"log {shape} {aw}")
