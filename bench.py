@@ -165,11 +165,12 @@ class Fused:
 
 def cpu_baseline(inc):
     """The reference cpu::single_thread chain (oracle/_ref), its batch split
-    over std::threads, on a bounded sample of the headline workload."""
+    over std::threads, on the headline workload (all 4096 paths: about 40 s of
+    CPU time, a few seconds of wall time on 16 threads)."""
     import oracle as O
     threads = max(1, min(16, os.cpu_count() or 1))
     if O.has_ref():
-        sample = inc[: 48 * threads]
+        sample = inc
         kind = "reference"
         run = lambda: O.ref_signature(WIDTH, DEPTH, sample, threads=threads)  # noqa: E731
     else:  # the plain-C restatement, one thread
@@ -331,6 +332,8 @@ def ft_mul_section(rpp, torch, O, dev):
         entry["algorithmic_bytes"] = alg
         entry["achieved_gbs"] = alg / (ms * 1e-3) / 1e9
         entry["hbm_frac"] = entry["achieved_gbs"] / HBM_PEAK_GBS
+        if n == 1 << 17:
+            entry["traffic_bytes_per_call"] = load_traffic("k_product_small_w4d4_131072pairs_fp64_traffic.json")
         if n == 1024 and O.has_ref():
             out_cpu = np.zeros((n, dim))
             t0 = time.perf_counter()
