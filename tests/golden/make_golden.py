@@ -1,0 +1,113 @@
+"""Write the golden fixtures of tests/test_golden.py from the real reference.
+
+Run in a container that holds the reference and has built its driver
+(``make -C oracle ref``): every expected output here comes from the
+reference's own cpu::single_thread free functions, called through
+oracle/_ref/librpp_ref.so on seeded inputs.  The fixtures (inputs and outputs,
+FP64, small batches at the shapes of the benchmark workloads) are committed,
+so the parity tests can use them where the reference is absent.
+
+    python3 tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", "..", "oracle"))
+
+import oracle as O  # noqa: E402
+
+
+def dim_of(w, n):
+    return O.tensor_dimension(w, n)
+
+
+def damped(rng, n, w, N):
+    db = [0]
+    p = 1
+    for _ in range(N + 1):
+        db.append(db[-1] + p)
+        p *= w
+    x = rng.uniform(-1.0, 1.0, size=(n, db[-1])) * min(1.0, 16.0 / np.sqrt(db[-1]))
+    for k in range(N + 1):
+        x[:, db[k]:db[k + 1]] *= 0.5 ** k
+    return x
+
+
+def main():
+    if not O.has_ref():
+        raise SystemExit("the reference driver (oracle/_ref) is not built here")
+    R = O.refimpl
+    out = {}
+    # products at the (4,4) shape: 16 pairs, U[-1,1], degree 0 = 1, full windows
+    rng = np.random.default_rng(0xC1)
+    w, N = 4, 4
+    dim = dim_of(w, N)
+    lhs = rng.uniform(-1, 1, (16, dim))
+    rhs = rng.uniform(-1, 1, (16, dim))
+    lhs[:, 0] = 1.0
+    rhs[:, 0] = 1.0
+    o = np.zeros((16, dim))
+    R.ft_mul(w, N, o, lhs, rhs)
+    out["mul_w4d4"] = dict(lhs=lhs, rhs=rhs, out=o)
+    # the same with windows out (1,3), lhs (0,2), rhs (1,4) and beta = 0.5
+    o = np.zeros((16, dim))
+    R.ft_mul(w, N, o, lhs, rhs, ow=(1, 3), lw=(0, 2), rw=(1, 4), beta=0.5)
+    out["mul_w4d4_windows"] = dict(lhs=lhs, rhs=rhs, out=o)
+    # fma, the three in-place variants: a, b, c at (3,4)
+    rng = np.random.default_rng(31)
+    w, N = 3, 4
+    dim = dim_of(w, N)
+    a, b, c = (damped(rng, 4, w, N) for _ in range(3))
+    o = np.zeros((4, dim))
+    R.ft_fma(w, N, o, a, b, c, alpha=0.75, beta=1.25)
+    out["fma_w3d4"] = dict(a=a, b=b, c=c, out=o)
+    for variant in (231, 123, 213):
+        x = a.copy()
+        R.ft_inplace_fma(variant, w, N, x, b, c, alpha=0.5, beta=2.0)
+        out[f"inplace_fma{variant}_w3d4"] = dict(a=a, b=b, c=c, out=x)
+    # exp, log, antipode, reflect, adjoints at (3,4)
+    for name in ("ft_exp", "ft_log"):
+        o = np.zeros((4, dim))
+        getattr(R, name)(w, N, o, a)
+        out[f"{name[3:]}_w3d4"] = dict(arg=a, out=o)
+    for name in ("tensor_antipode", "tensor_reflect"):
+        o = np.zeros((4, dim))
+        getattr(R, name)(w, N, o, a)
+        out[f"{name[7:]}_w3d4"] = dict(arg=a, out=o)
+    for name in ("ft_adj_lmul", "ft_adj_rmul"):
+        o = np.zeros((4, dim))
+        getattr(R, name)(w, N, o, a, b)
+        out[f"{name[3:]}_w3d4"] = dict(op=a, arg=b, out=o)
+    # pairing at (6,4): 4 objects against their functionals
+    rng = np.random.default_rng(0xC5F)
+    w, N = 6, 4
+    dim = dim_of(w, N)
+    f = rng.normal(size=(4, dim))
+    x = damped(rng, 4, w, N)
+    o = np.zeros(4)
+    R.tensor_pairing(w, N, o, f, x)
+    out["pairing_w6d4"] = dict(functional=f, arg=x, out=o)
+    # the chained ft_fmexp of the headline workload: 2 paths x 16 increments at (5,6)
+    rng = np.random.default_rng(0xC2)
+    inc = rng.normal(0.0, np.sqrt(1.0 / 128), size=(2, 16, 5))
+    out["signature_w5d6"] = dict(increments=inc, out=O.ref_signature(5, 6, inc))
+    # the log-signature pipeline at (3,8): 2 paths x 64 increments, then ft_log,
+    # tensor_to_lie (T2L) and lie_to_tensor (L2T) through the reference ops
+    rng = np.random.default_rng(0xC3)
+    inc = rng.normal(0.0, np.sqrt(1.0 / 64), size=(2, 64, 3))
+    sig = O.ref_signature(3, 8, inc)
+    lg = np.zeros_like(sig)
+    R.ft_log(3, 8, lg, sig)
+    lie = O.ref_lie_convert(True, True, 3, 8, lg)
+    back = O.ref_lie_convert(False, True, 3, 8, lie)
+    out["logsig_w3d8"] = dict(increments=inc, signature=sig, log=lg, lie=lie, back=back)
+    for name, arrays in out.items():
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+    print("wrote", len(out), "fixtures to", HERE)
+
+
+if __name__ == "__main__":
+    main()

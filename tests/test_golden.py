@@ -1,0 +1,161 @@
+"""Parity against the golden fixtures written from the real reference
+(tests/golden/make_golden.py: the reference's own cpu::single_thread free
+functions on seeded inputs).  The CPU tests pin the oracle to them; the GPU
+tests hold the HIP path to them, so both run where the reference is absent."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import assert_close, rel_err
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TOL = 1e-12
+
+
+def fixture(name):
+    with np.load(os.path.join(HERE, "golden", name + ".npz")) as z:
+        return {k: z[k] for k in z.files}
+
+
+def windows_of(name):
+    return dict(ow=(1, 3), lw=(0, 2), rw=(1, 4), beta=0.5) if name == "mul_w4d4_windows" else {}
+
+
+# ------------------------------------------------------------------- oracle
+
+@pytest.mark.parametrize("name", ["mul_w4d4", "mul_w4d4_windows"])
+def test_oracle_products(name):
+    f = fixture(name)
+    o = np.zeros_like(f["out"])
+    O.orc.ft_mul(4, 4, o, f["lhs"], f["rhs"], **windows_of(name))
+    assert rel_err(o, f["out"]) <= TOL
+
+
+def test_oracle_fma_family():
+    f = fixture("fma_w3d4")
+    o = np.zeros_like(f["out"])
+    O.orc.ft_fma(3, 4, o, f["a"], f["b"], f["c"], alpha=0.75, beta=1.25)
+    assert rel_err(o, f["out"]) <= TOL
+    for variant in (231, 123, 213):
+        g = fixture(f"inplace_fma{variant}_w3d4")
+        x = g["a"].copy()
+        O.orc.ft_inplace_fma(variant, 3, 4, x, g["b"], g["c"], alpha=0.5, beta=2.0)
+        assert rel_err(x, g["out"]) <= TOL, variant
+
+
+@pytest.mark.parametrize("name", ["exp", "log", "antipode", "reflect"])
+def test_oracle_unary(name):
+    f = fixture(f"{name}_w3d4")
+    o = np.zeros_like(f["out"])
+    op = {"exp": O.orc.ft_exp, "log": O.orc.ft_log, "antipode": O.orc.tensor_antipode,
+          "reflect": O.orc.tensor_reflect}[name]
+    op(3, 4, o, f["arg"])
+    assert rel_err(o, f["out"]) <= TOL
+
+
+@pytest.mark.parametrize("side", ["lmul", "rmul"])
+def test_oracle_adjoints(side):
+    f = fixture(f"adj_{side}_w3d4")
+    o = np.zeros_like(f["out"])
+    getattr(O.orc, "ft_adj_" + side)(3, 4, o, f["op"], f["arg"])
+    assert rel_err(o, f["out"]) <= TOL
+
+
+def test_oracle_pairing_and_chains():
+    f = fixture("pairing_w6d4")
+    o = np.zeros(f["out"].shape)
+    O.orc.tensor_pairing(6, 4, o, f["functional"], f["arg"])
+    assert rel_err(o, f["out"]) <= TOL
+    s = fixture("signature_w5d6")
+    n, L, w = s["increments"].shape
+    got = O.signature(5, 6, s["increments"].reshape(-1, w), np.arange(n + 1) * L)
+    assert rel_err(got, s["out"]) <= TOL
+    g = fixture("logsig_w3d8")
+    n, L, w = g["increments"].shape
+    sig = O.signature(3, 8, g["increments"].reshape(-1, w), np.arange(n + 1) * L)
+    assert rel_err(sig, g["signature"]) <= TOL
+    lg = np.zeros_like(sig)
+    O.orc.ft_log(3, 8, lg, sig)
+    assert rel_err(lg, g["log"]) <= TOL
+
+
+# ---------------------------------------------------------------------- GPU
+
+@pytest.mark.gpu
+def test_gpu_against_golden(dev):
+    import torch
+
+    import rpp_hip
+
+    def t(x):
+        return torch.tensor(np.ascontiguousarray(x), device=dev)
+
+    for name in ("mul_w4d4", "mul_w4d4_windows"):
+        f = fixture(name)
+        basis = rpp_hip.TensorBasis(4, 4)
+        o = t(np.zeros_like(f["out"]))
+        kw = windows_of(name)
+        if kw:
+            rpp_hip.ft_mul(rpp_hip.Batch(o, *kw["ow"]), rpp_hip.Batch(t(f["lhs"]), *kw["lw"]),
+                           rpp_hip.Batch(t(f["rhs"]), *kw["rw"]), basis, beta=kw["beta"])
+        else:
+            rpp_hip.ft_mul(o, t(f["lhs"]), t(f["rhs"]), basis)
+        assert_close(o.cpu().numpy(), f["out"], np.float64, name)
+    b34 = rpp_hip.TensorBasis(3, 4)
+    f = fixture("fma_w3d4")
+    o = t(np.zeros_like(f["out"]))
+    rpp_hip.ft_fma(o, t(f["a"]), t(f["b"]), t(f["c"]), b34, alpha=0.75, beta=1.25)
+    assert_close(o.cpu().numpy(), f["out"], np.float64, "fma")
+    for variant in (231, 123, 213):
+        g = fixture(f"inplace_fma{variant}_w3d4")
+        x = t(g["a"])
+        rpp_hip.ft_inplace_fma(variant, x, t(g["b"]), t(g["c"]), b34, alpha=0.5, beta=2.0)
+        assert_close(x.cpu().numpy(), g["out"], np.float64, f"inplace_fma {variant}")
+    for name, op in (("exp", rpp_hip.ft_exp), ("log", rpp_hip.ft_log), ("antipode", rpp_hip.tensor_antipode),
+                     ("reflect", rpp_hip.tensor_reflect)):
+        g = fixture(f"{name}_w3d4")
+        o = t(np.zeros_like(g["out"]))
+        op(o, t(g["arg"]), b34)
+        assert_close(o.cpu().numpy(), g["out"], np.float64, name)
+    for side in ("lmul", "rmul"):
+        g = fixture(f"adj_{side}_w3d4")
+        o = t(np.zeros_like(g["out"]))
+        getattr(rpp_hip, "ft_adj_" + side)(o, t(g["op"]), t(g["arg"]), b34)
+        assert_close(o.cpu().numpy(), g["out"], np.float64, "adj_" + side)
+    g = fixture("pairing_w6d4")
+    o = t(np.zeros(g["out"].shape))
+    rpp_hip.tensor_pairing(o, t(g["functional"]), t(g["arg"]), rpp_hip.TensorBasis(6, 4))
+    assert_close(o.cpu().numpy(), g["out"], np.float64, "pairing")
+    # the plain chain and the fused kernel against the reference's chain
+    s = fixture("signature_w5d6")
+    n, L, w = s["increments"].shape
+    b56 = rpp_hip.TensorBasis(5, 6)
+    pad = np.zeros((n, 1 + L * w))
+    pad[:, 1:] = s["increments"].reshape(n, L * w)
+    pad_d = t(pad)
+    cur = t(np.zeros((n, b56.size())))
+    cur[:, 0] = 1
+    nxt = torch.empty_like(cur)
+    for k in range(L):
+        rpp_hip.ft_fmexp(nxt, cur, rpp_hip.Batch(pad_d[:, k * w: k * w + 1 + w], 1, 1), b56)
+        cur, nxt = nxt, cur
+    assert_close(cur.cpu().numpy(), s["out"], np.float64, "plain chain")
+    fused = rpp_hip.signature(t(s["increments"].reshape(-1, w)), t(np.arange(n + 1) * L), b56)
+    assert_close(fused.cpu().numpy(), s["out"], np.float64, "fused chain")
+    # the log-signature pipeline
+    g = fixture("logsig_w3d8")
+    n, L, w = g["increments"].shape
+    b38 = rpp_hip.TensorBasis(3, 8)
+    sig = rpp_hip.signature(t(g["increments"].reshape(-1, w)), t(np.arange(n + 1) * L), b38)
+    assert_close(sig.cpu().numpy(), g["signature"], np.float64, "signature (3,8)")
+    lg = torch.empty_like(sig)
+    rpp_hip.ft_log(lg, sig, b38)
+    assert_close(lg.cpu().numpy(), g["log"], np.float64, "log (3,8)")
+    lie = t(np.zeros(g["lie"].shape))
+    rpp_hip.tensor_to_lie(lie, lg, b38)
+    assert_close(lie.cpu().numpy()[:, 1:], g["lie"][:, 1:], np.float64, "tensor_to_lie (3,8)")
+    back = torch.empty_like(lg)
+    rpp_hip.lie_to_tensor(back, t(g["lie"]), b38)
+    assert_close(back.cpu().numpy(), g["back"], np.float64, "lie_to_tensor (3,8)")
