@@ -22,6 +22,9 @@ same line carries, each under its own key:
                   signature, ft_log, tensor_to_lie, lie_to_tensor;
   uneven          8192 paths, width 8, depth 5, lengths uniform in [16, 1024]:
                   fused forward replayed from a hipGraph, FP32 and FP64;
+  layout          the headline batch moved between the object-major and the
+                  batch-interleaved layouts (vector_assign's tiled
+                  transposition) and vector_add over it, against HBM peak;
   roofline        the dominant kernel of the headline path (the ft_fmexp
                   register tile, HBM-bound) from HIP events around its
                   launches;
@@ -46,7 +49,7 @@ WIDTH, DEPTH, PATHS, LENGTH = 5, 6, 4096, 128
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
-SECTIONS = ("fwd_bwd", "ft_mul", "log_signature", "uneven")
+SECTIONS = ("fwd_bwd", "ft_mul", "log_signature", "uneven", "layout")
 
 
 def dim_of(w, n):
@@ -351,6 +354,51 @@ def ft_mul_section(rpp, torch, O, dev):
     return res
 
 
+# ------------------------------------------------- layouts and linear operations
+
+def layout_section(rpp, torch, O, dev, reps=20):
+    """The wire-format transposition (vector_assign between object-major and
+    batch-interleaved batches) and vector_add, at the headline shape: 4096
+    objects of width 5, depth 6 (19531 coefficients), FP64.  Algorithmic bytes:
+    one read and one write of every coefficient (transposition), two reads
+    and one write (vector_add).  torch's own transpose copy is timed beside
+    it."""
+    B = rpp.TensorBasis(WIDTH, DEPTH)
+    n, dim = PATHS, B.size()
+    g = torch.Generator(device=dev)
+    g.manual_seed(0xC2)
+    x = torch.randn((n, dim), dtype=torch.float64, device=dev, generator=g)
+    y = torch.randn((n, dim), dtype=torch.float64, device=dev, generator=g)
+    il = torch.empty((dim, n), dtype=torch.float64, device=dev)
+    om = torch.empty_like(x)
+    out = torch.empty_like(x)
+    nbytes = n * dim * 8
+    res = {"config": {"objects": n, "width": WIDTH, "depth": DEPTH, "coefficients": dim, "dtype": "f64"}}
+
+    def entry(ms, traffic):
+        gbs = traffic / (ms * 1e-3) / 1e9
+        return {"us_per_call": ms * 1e3, "algorithmic_bytes": traffic, "achieved_gbs": gbs,
+                "hbm_frac": gbs / HBM_PEAK_GBS}
+
+    to_il = lambda: rpp.vector_assign(rpp.interleaved(il, B), x, B, num_batches=n)
+    to_om = lambda: rpp.vector_assign(om, rpp.interleaved(il, B), B, num_batches=n)
+    add = lambda: rpp.vector_add(out, x, y, B, alpha=0.5, beta=-2.0)
+    ms, _, _ = timed(torch, to_il, reps, 3)
+    res["object_major_to_interleaved"] = entry(ms, 2 * nbytes)
+    res["object_major_to_interleaved"]["exact"] = bool(torch.equal(il, x.t()))
+    ms, _, _ = timed(torch, to_om, reps, 3)
+    res["interleaved_to_object_major"] = entry(ms, 2 * nbytes)
+    res["interleaved_to_object_major"]["exact"] = bool(torch.equal(om, x))
+    ms, _, _ = timed(torch, add, reps, 3)
+    res["vector_add"] = entry(ms, 3 * nbytes)
+    res["vector_add"]["exact"] = bool(torch.equal(out, 0.5 * x + (-2.0) * y))
+    ms, _, _ = timed(torch, lambda: il.copy_(x.t()), reps, 3)
+    res["torch_transpose_copy"] = entry(ms, 2 * nbytes)
+    del x, y, il, om, out
+    torch.cuda.empty_cache()
+    return res
+
+
 # ---------------------------------------------------------- log-signature (C3)
 
 def lie_l2t(lie, dim, ts):
@@ -473,8 +521,8 @@ def main():
     ap.add_argument("--sections", default=",".join(SECTIONS),
                     help="comma list of extra sections to run (" + ",".join(SECTIONS) + "), or 'none'")
     ap.add_argument("--only", default=None,
-                    help="run one headline variant (plain_fp64, fused_fp64, plain_fp32, fused_fp32) or "
-                         "ft_mul_hbm (2^17 width-4 depth-4 products) and print its timing only")
+                    help="run one headline variant (plain_fp64, fused_fp64, plain_fp32, fused_fp32), "
+                         "ft_mul_hbm (2^17 width-4 depth-4 products) or layout, and print its timing only")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -507,6 +555,9 @@ def main():
         ms, _, _ = timed(torch, lambda: rpp.ft_mul(out, L, R, basis), args.steps, args.warmup)
         print(json.dumps({"variant": args.only, "us_per_call": ms * 1e3,
                           "achieved_gbs": 3 * n * basis.size() * 8 / (ms * 1e-3) / 1e9}))
+        return
+    if args.only == "layout":
+        print(json.dumps({"variant": args.only, **layout_section(rpp, torch, O, dev, reps=args.steps)}))
         return
     if args.only:
         inc = make_increments(0xC2, PATHS, LENGTH, WIDTH)
@@ -556,7 +607,8 @@ def main():
     }
     del inc
     wanted = [] if args.sections == "none" else [s for s in args.sections.split(",") if s]
-    runners = {"fwd_bwd": fwd_bwd, "ft_mul": ft_mul_section, "log_signature": log_signature, "uneven": uneven}
+    runners = {"fwd_bwd": fwd_bwd, "ft_mul": ft_mul_section, "log_signature": log_signature, "uneven": uneven,
+               "layout": layout_section}
     for name in wanted:
         print(f"section {name}", file=sys.stderr, flush=True)
         line[name] = runners[name](rpp, torch, O, dev)

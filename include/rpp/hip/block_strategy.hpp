@@ -30,7 +30,10 @@
 #include <rpp/operations/basic/ft_inplace_fma.hpp>
 #include <rpp/operations/basic/ft_inplace_mul.hpp>
 #include <rpp/operations/basic/ft_mul.hpp>
+#include <rpp/basis/lie_basis.hpp>
 #include <rpp/operations/basic/lie_to_tensor.hpp>
+#include <rpp/operations/basic/tensor_add_identity.hpp>
+#include <rpp/operations/basic/tensor_set_identity.hpp>
 #include <rpp/operations/basic/tensor_antipode.hpp>
 #include <rpp/operations/basic/tensor_generalised_antipode.hpp>
 #include <rpp/operations/basic/tensor_pairing.hpp>
@@ -40,6 +43,11 @@
 #include <rpp/operations/intermediate/ft_fmexp.hpp>
 #include <rpp/operations/intermediate/ft_log.hpp>
 #include <rpp/operations/linalg/sparse_matrix_vector.hpp>
+#include <rpp/operations/linalg/vector_add.hpp>
+#include <rpp/operations/linalg/vector_assign.hpp>
+#include <rpp/operations/linalg/vector_inplace_add.hpp>
+#include <rpp/operations/linalg/vector_scalar_multiply.hpp>
+#include <rpp/operations/linalg/vector_set_constant.hpp>
 #include <rpp/sparse/matrix.hpp>
 #include <rpp/support/error.hpp>
 #include <rpp/views/batch.hpp>
@@ -416,6 +424,99 @@ public:
         return SpMV::hip_launch(cfg, b, bases, n, matrix);
     }
 };
+
+namespace hip_detail {
+
+// The grading of a batch: the basis its metadata names, looked up in the
+// pack (tensor or Lie; the linear operations use only its degree offsets).
+template <typename Batch, typename Bases>
+rpp_hip_graded_basis graded_of(Batch const& b, Bases const& bases) {
+    auto const& gb = basis::get_basis(std::get<0>(b.metadata()), bases);
+    using Tag = typename std::decay_t<decltype(gb)>::Tag;
+    static_assert(std::is_same_v<Tag, basis::TensorBasisTag> || std::is_same_v<Tag, basis::LieBasisTag>,
+                  "hip::block linear operations take tensor or Lie gradings");
+    return rpp_hip_graded_basis{std::is_same_v<Tag, basis::LieBasisTag> ? RPP_HIP_LIE_BASIS : RPP_HIP_TENSOR_BASIS,
+                                static_cast<int32_t>(gb.width), static_cast<int32_t>(gb.depth)};
+}
+
+}  // namespace hip_detail
+
+// vector_assign (operations/linalg/vector_assign.hpp:36-64)
+template <typename Accum_, typename Arch_>
+class VectorAssign<hip::strategies::BlockStrategy<Accum_, Arch_>>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases,
+                                         int64_t n) noexcept {
+        using namespace hip::detail;
+        rpp_hip_vector_assign_desc d{hip_detail::graded_of(std::get<0>(b), bases), n,
+                                     tensor_of<Accum>(std::get<0>(b)), tensor_of<Accum>(std::get<1>(b))};
+        return to_error(RPP_HIP_CALL(rpp_hip_vector_assign, d, cfg.stream));
+    }
+};
+
+// vector_add (operations/linalg/vector_add.hpp:40-78)
+template <typename Accum_, typename Arch_>
+class VectorAdd<hip::strategies::BlockStrategy<Accum_, Arch_>>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases, int64_t n,
+                                         Accum alpha = Accum{1}, Accum beta = Accum{1}) noexcept {
+        using namespace hip::detail;
+        rpp_hip_vector_add_desc d{hip_detail::graded_of(std::get<0>(b), bases), n,
+                                  tensor_of<Accum>(std::get<0>(b)), tensor_of<Accum>(std::get<1>(b)),
+                                  tensor_of<Accum>(std::get<2>(b)), static_cast<double>(alpha),
+                                  static_cast<double>(beta)};
+        return to_error(RPP_HIP_CALL(rpp_hip_vector_add, d, cfg.stream));
+    }
+};
+
+// vector_inplace_add (operations/linalg/vector_inplace_add.hpp:40-77)
+template <typename Accum_, typename Arch_>
+class VectorInplaceAdd<hip::strategies::BlockStrategy<Accum_, Arch_>>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases, int64_t n,
+                                         Accum alpha = Accum{1}) noexcept {
+        using namespace hip::detail;
+        rpp_hip_vector_inplace_add_desc d{hip_detail::graded_of(std::get<0>(b), bases), n,
+                                          tensor_of<Accum>(std::get<0>(b)), tensor_of<Accum>(std::get<1>(b)),
+                                          static_cast<double>(alpha)};
+        return to_error(RPP_HIP_CALL(rpp_hip_vector_inplace_add, d, cfg.stream));
+    }
+};
+
+// One batch and one scalar: vector_scalar_multiply, vector_set_constant,
+// tensor_set_identity, tensor_add_identity (operations/linalg/
+// vector_scalar_multiply.hpp:37-71, vector_set_constant.hpp:38-72,
+// operations/basic/tensor_set_identity.hpp:38-73, tensor_add_identity.hpp:37-71)
+#define RPP_HIP_SCALAR_OP(CLASS, ENTRY)                                                              \
+    template <typename Accum_, typename Arch_>                                                       \
+    class CLASS<hip::strategies::BlockStrategy<Accum_, Arch_>>                                       \
+        : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {                      \
+    public:                                                                                          \
+        RPP_HIP_OP_COMMON;                                                                           \
+        template <typename Batches, typename Bases>                                                  \
+        static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases, \
+                                             int64_t n, Accum scalar) noexcept {                     \
+            using namespace hip::detail;                                                             \
+            rpp_hip_vector_scalar_desc d{hip_detail::graded_of(std::get<0>(b), bases), n,            \
+                                         tensor_of<Accum>(std::get<0>(b)), static_cast<double>(scalar)}; \
+            return to_error(RPP_HIP_CALL(ENTRY, d, cfg.stream));                                     \
+        }                                                                                            \
+    };
+
+RPP_HIP_SCALAR_OP(VectorScalarMultiply, rpp_hip_vector_scalar_multiply)
+RPP_HIP_SCALAR_OP(VectorSetConstant, rpp_hip_vector_set_constant)
+RPP_HIP_SCALAR_OP(TensorSetIdentity, rpp_hip_tensor_set_identity)
+RPP_HIP_SCALAR_OP(TensorAddIdentity, rpp_hip_tensor_add_identity)
+#undef RPP_HIP_SCALAR_OP
 
 #undef RPP_HIP_OP_COMMON
 

@@ -211,6 +211,84 @@ int rpp_hip_tensor_antipode_f64(const rpp_hip_ft_unary_desc* desc, void* stream)
 int rpp_hip_tensor_reflect_f32(const rpp_hip_ft_unary_desc* desc, void* stream);
 int rpp_hip_tensor_reflect_f64(const rpp_hip_ft_unary_desc* desc, void* stream);
 
+/* ------------------------------------------------------ linear operations */
+
+/* The graded basis of a vector operation: the TensorBasis of (width, depth)
+ * (basis/tensor_basis.hpp:40-130) or the Hall-set LieBasis of (width, depth)
+ * (basis/lie_basis.hpp:42-97: degree 0 holds one slot, degree k the Witt
+ * number of Lie elements).  These operations use only its degree offsets. */
+enum rpp_hip_basis_kind { RPP_HIP_TENSOR_BASIS = 0, RPP_HIP_LIE_BASIS = 1 };
+typedef struct rpp_hip_graded_basis {
+    int32_t kind;
+    int32_t width;
+    int32_t depth;
+} rpp_hip_graded_basis;
+
+/* rpp::ops::vector_assign (operations/linalg/vector_assign.hpp:36-64;
+ * cpu/.../linalg/vector_assign.hpp:32-61): out = arg on the overlap of the two
+ * windows, 0 on the rest of out's window.  out and arg may have different
+ * layouts: between an object-major and a batch-interleaved batch this is the
+ * transposition of the two wire formats (an LDS-tiled kernel).  out may be arg
+ * itself (same data and strides); other overlaps are not supported. */
+typedef struct rpp_hip_vector_assign_desc {
+    rpp_hip_graded_basis basis;
+    int64_t num_batches;
+    rpp_hip_tensor_batch out, arg;
+} rpp_hip_vector_assign_desc;
+int rpp_hip_vector_assign_f32(const rpp_hip_vector_assign_desc* desc, void* stream);
+int rpp_hip_vector_assign_f64(const rpp_hip_vector_assign_desc* desc, void* stream);
+
+/* rpp::ops::vector_add (operations/linalg/vector_add.hpp:40-78;
+ * cpu/.../linalg/vector_add.hpp:36-66): on every degree of out's window,
+ * out = alpha*lhs + beta*rhs, each operand read only on its own window and 0
+ * elsewhere.  out may be lhs or rhs (same data and strides). */
+typedef struct rpp_hip_vector_add_desc {
+    rpp_hip_graded_basis basis;
+    int64_t num_batches;
+    rpp_hip_tensor_batch out, lhs, rhs;
+    double alpha, beta;
+} rpp_hip_vector_add_desc;
+int rpp_hip_vector_add_f32(const rpp_hip_vector_add_desc* desc, void* stream);
+int rpp_hip_vector_add_f64(const rpp_hip_vector_add_desc* desc, void* stream);
+
+/* rpp::ops::vector_inplace_add (operations/linalg/vector_inplace_add.hpp
+ * :40-77; cpu/.../linalg/vector_inplace_add.hpp:36-63): lhs += alpha*rhs on
+ * the overlap of the two windows; the rest of lhs is untouched. */
+typedef struct rpp_hip_vector_inplace_add_desc {
+    rpp_hip_graded_basis basis;
+    int64_t num_batches;
+    rpp_hip_tensor_batch lhs, rhs;
+    double alpha;
+} rpp_hip_vector_inplace_add_desc;
+int rpp_hip_vector_inplace_add_f32(const rpp_hip_vector_inplace_add_desc* desc, void* stream);
+int rpp_hip_vector_inplace_add_f64(const rpp_hip_vector_inplace_add_desc* desc, void* stream);
+
+/* One batch and one scalar, over the batch's window:
+ *   vector_scalar_multiply (operations/linalg/vector_scalar_multiply.hpp
+ *     :28-71; cpu/.../vector_scalar_multiply.hpp:37-49): vec *= scalar;
+ *   vector_set_constant (operations/linalg/vector_set_constant.hpp:27-72;
+ *     cpu/.../vector_set_constant.hpp:34-40): vec = scalar;
+ *   tensor_set_identity (operations/basic/tensor_set_identity.hpp:27-73;
+ *     cpu/.../tensor_set_identity.hpp:34-44): vec = 0, then vec[0] = scalar
+ *     if the window holds degree 0;
+ *   tensor_add_identity (operations/basic/tensor_add_identity.hpp:26-71;
+ *     cpu/.../tensor_add_identity.hpp:34-43): vec[0] += scalar if the window
+ *     holds degree 0. */
+typedef struct rpp_hip_vector_scalar_desc {
+    rpp_hip_graded_basis basis;
+    int64_t num_batches;
+    rpp_hip_tensor_batch vec;
+    double scalar;
+} rpp_hip_vector_scalar_desc;
+int rpp_hip_vector_scalar_multiply_f32(const rpp_hip_vector_scalar_desc* desc, void* stream);
+int rpp_hip_vector_scalar_multiply_f64(const rpp_hip_vector_scalar_desc* desc, void* stream);
+int rpp_hip_vector_set_constant_f32(const rpp_hip_vector_scalar_desc* desc, void* stream);
+int rpp_hip_vector_set_constant_f64(const rpp_hip_vector_scalar_desc* desc, void* stream);
+int rpp_hip_tensor_set_identity_f32(const rpp_hip_vector_scalar_desc* desc, void* stream);
+int rpp_hip_tensor_set_identity_f64(const rpp_hip_vector_scalar_desc* desc, void* stream);
+int rpp_hip_tensor_add_identity_f32(const rpp_hip_vector_scalar_desc* desc, void* stream);
+int rpp_hip_tensor_add_identity_f64(const rpp_hip_vector_scalar_desc* desc, void* stream);
+
 /* --------------------------------------------------------- sparse and Lie */
 
 enum rpp_hip_matrix_format { RPP_HIP_CSR = 0, RPP_HIP_CSC = 1 };

@@ -72,6 +72,25 @@ def degree_begin(width: int, depth: int) -> list[int]:
     return out
 
 
+def lie_degree_begin(width: int, depth: int) -> list[int]:
+    """Degree offsets of the Hall-set Lie basis: one slot for degree 0, then
+    the Witt numbers (1/k) sum_{d|k} mu(d) width^(k/d)."""
+    def mobius(m):
+        mu, p = 1, 2
+        while p * p <= m:
+            if m % p == 0:
+                m //= p
+                if m % p == 0:
+                    return 0
+                mu = -mu
+            p += 1
+        return -mu if m > 1 else mu
+    out = [0, 1]
+    for k in range(1, depth + 1):
+        out.append(out[-1] + sum(mobius(d) * width ** (k // d) for d in range(1, k + 1) if k % d == 0) // k)
+    return out[: depth + 2]
+
+
 def _sfx(dtype):
     dt = np.dtype(dtype)
     if dt == np.float64:
@@ -213,6 +232,61 @@ class _Impl:
 
     def ft_adj_rmul(self, width, depth, out, op, arg, ow=None, pw=None, gw=None, n=None):
         return self._adj("ft_adj_rmul", width, depth, out, op, arg, ow, pw, gw, n)
+
+    # Linear operations over any grading: `start` = degree offsets
+    # start_of_degree(k), k = 0..depth+1 (degree_begin or lie_degree_begin).
+
+    @staticmethod
+    def _start(start):
+        s = np.ascontiguousarray(start, dtype=np.int64)
+        return s, s.ctypes.data_as(_p)
+
+    def vector_assign(self, start, depth, out, arg, ow=None, aw=None, n=None):
+        sfx, _ = _sfx(out.dtype)
+        n = out.shape[0] if n is None else n
+        f = _full(depth)
+        keep, sp = self._start(start)
+        _call(self._lib(), self._name("vector_assign", sfx), sp, _i(depth), _ll(n),
+              *_arg(out), *_win(ow or f), *_arg(arg), *_win(aw or f))
+        return out
+
+    def vector_add(self, start, depth, out, lhs, rhs, ow=None, lw=None, rw=None, alpha=1.0, beta=1.0, n=None):
+        sfx, ct = _sfx(out.dtype)
+        n = out.shape[0] if n is None else n
+        f = _full(depth)
+        keep, sp = self._start(start)
+        _call(self._lib(), self._name("vector_add", sfx), sp, _i(depth), _ll(n), *_arg(out), *_win(ow or f),
+              *_arg(lhs), *_win(lw or f), *_arg(rhs), *_win(rw or f), ct(alpha), ct(beta))
+        return out
+
+    def vector_inplace_add(self, start, depth, lhs, rhs, lw=None, rw=None, alpha=1.0, n=None):
+        sfx, ct = _sfx(lhs.dtype)
+        n = lhs.shape[0] if n is None else n
+        f = _full(depth)
+        keep, sp = self._start(start)
+        _call(self._lib(), self._name("vector_inplace_add", sfx), sp, _i(depth), _ll(n), *_arg(lhs),
+              *_win(lw or f), *_arg(rhs), *_win(rw or f), ct(alpha))
+        return lhs
+
+    def _vector_scalar(self, op, start, depth, vec, vw, scalar, n):
+        sfx, ct = _sfx(vec.dtype)
+        n = vec.shape[0] if n is None else n
+        keep, sp = self._start(start)
+        _call(self._lib(), self._name("vector_scalar", sfx), _i(op), sp, _i(depth), _ll(n), *_arg(vec),
+              *_win(vw or _full(depth)), ct(scalar))
+        return vec
+
+    def vector_scalar_multiply(self, start, depth, vec, scalar, vw=None, n=None):
+        return self._vector_scalar(0, start, depth, vec, vw, scalar, n)
+
+    def vector_set_constant(self, start, depth, vec, value, vw=None, n=None):
+        return self._vector_scalar(1, start, depth, vec, vw, value, n)
+
+    def tensor_set_identity(self, start, depth, vec, scalar=1.0, vw=None, n=None):
+        return self._vector_scalar(2, start, depth, vec, vw, scalar, n)
+
+    def tensor_add_identity(self, start, depth, vec, scalar=1.0, vw=None, n=None):
+        return self._vector_scalar(3, start, depth, vec, vw, scalar, n)
 
 
 orc = _Impl("orc")

@@ -217,6 +217,63 @@ int adj(int w, int N, long long n, T* out, long long os, int omin, int omax,
                                              static_cast<Index>(n)));
 }
 
+// The linear operations over an arbitrary grading: a TensorBasis object
+// built on the given degree offsets (the operations read nothing else of the
+// basis, cpu/.../linalg/*.hpp), so Lie gradings go through the same code.
+struct GradedHolder {
+    std::vector<Index> degree_begin;
+    TBasis basis;
+    GradedHolder(const long long* start, int depth)
+        : degree_begin(start, start + depth + 2),
+          basis(static_cast<Degree>(0), static_cast<Degree>(depth), degree_begin.data()) {}
+};
+
+template <typename T>
+int vector_assign(const long long* start, int N, long long n, T* out, long long os, int omin, int omax,
+                  const T* arg, long long as, int amin, int amax) {
+    GradedHolder g(start, N);
+    return code_of(rpp::ops::vector_assign(Strat<T>{}, {}, batch<T>(out, os, g.basis, omin, omax),
+                                           batch<T>(arg, as, g.basis, amin, amax), g.basis,
+                                           static_cast<Index>(n)));
+}
+
+template <typename T>
+int vector_add(const long long* start, int N, long long n, T* out, long long os, int omin, int omax,
+               const T* lhs, long long ls, int lmin, int lmax, const T* rhs, long long rs, int rmin,
+               int rmax, T alpha, T beta) {
+    GradedHolder g(start, N);
+    return code_of(rpp::ops::vector_add(Strat<T>{}, {}, batch<T>(out, os, g.basis, omin, omax),
+                                        batch<T>(lhs, ls, g.basis, lmin, lmax),
+                                        batch<T>(rhs, rs, g.basis, rmin, rmax), g.basis,
+                                        static_cast<Index>(n), alpha, beta));
+}
+
+template <typename T>
+int vector_inplace_add(const long long* start, int N, long long n, T* lhs, long long ls, int lmin, int lmax,
+                       const T* rhs, long long rs, int rmin, int rmax, T alpha) {
+    GradedHolder g(start, N);
+    return code_of(rpp::ops::vector_inplace_add(Strat<T>{}, {}, batch<T>(lhs, ls, g.basis, lmin, lmax),
+                                                batch<T>(rhs, rs, g.basis, rmin, rmax), g.basis,
+                                                static_cast<Index>(n), alpha));
+}
+
+// op: 0 vector_scalar_multiply, 1 vector_set_constant, 2 tensor_set_identity,
+// 3 tensor_add_identity.
+template <typename T>
+int vector_scalar(int op, const long long* start, int N, long long n, T* vec, long long vs, int vmin, int vmax,
+                  T scalar) {
+    GradedHolder g(start, N);
+    auto V = batch<T>(vec, vs, g.basis, vmin, vmax);
+    const auto count = static_cast<Index>(n);
+    switch (op) {
+        case 0: return code_of(rpp::ops::vector_scalar_multiply(Strat<T>{}, {}, V, g.basis, count, scalar));
+        case 1: return code_of(rpp::ops::vector_set_constant(Strat<T>{}, {}, V, g.basis, count, scalar));
+        case 2: return code_of(rpp::ops::tensor_set_identity(Strat<T>{}, {}, V, g.basis, count, scalar));
+        case 3: return code_of(rpp::ops::tensor_add_identity(Strat<T>{}, {}, V, g.basis, count, scalar));
+        default: return 3;
+    }
+}
+
 // Chen recursion as a reference user writes it: one ft_fmexp per increment
 // over the whole batch, ping-pong buffers, exponent view (1,1) over a
 // width+1 buffer whose slot 0 is unused.  Paths share one length L;
@@ -342,6 +399,34 @@ extern "C" {
     int ref_signature_##SFX(int w, int N, long long n, long long L,            \
                             const T* incr, T* out, int threads) {              \
         return signature<T>(w, N, n, L, incr, out, threads);                   \
+    }                                                                          \
+    int ref_vector_assign_##SFX(const long long* start, int N, long long n,    \
+                                T* out, long long os, int omin, int omax,      \
+                                const T* arg, long long as, int amin,          \
+                                int amax) {                                    \
+        return vector_assign<T>(start, N, n, out, os, omin, omax, arg, as,     \
+                                amin, amax);                                   \
+    }                                                                          \
+    int ref_vector_add_##SFX(const long long* start, int N, long long n,       \
+                             T* out, long long os, int omin, int omax,         \
+                             const T* lhs, long long ls, int lmin, int lmax,   \
+                             const T* rhs, long long rs, int rmin, int rmax,   \
+                             T alpha, T beta) {                                \
+        return vector_add<T>(start, N, n, out, os, omin, omax, lhs, ls, lmin,  \
+                             lmax, rhs, rs, rmin, rmax, alpha, beta);          \
+    }                                                                          \
+    int ref_vector_inplace_add_##SFX(const long long* start, int N,            \
+                                     long long n, T* lhs, long long ls,        \
+                                     int lmin, int lmax, const T* rhs,         \
+                                     long long rs, int rmin, int rmax,         \
+                                     T alpha) {                                \
+        return vector_inplace_add<T>(start, N, n, lhs, ls, lmin, lmax, rhs,    \
+                                     rs, rmin, rmax, alpha);                   \
+    }                                                                          \
+    int ref_vector_scalar_##SFX(int op, const long long* start, int N,         \
+                                long long n, T* vec, long long vs, int vmin,   \
+                                int vmax, T scalar) {                          \
+        return vector_scalar<T>(op, start, N, n, vec, vs, vmin, vmax, scalar); \
     }
 
 REF_FT(double, f64)

@@ -50,6 +50,20 @@ static int orc_basis_init(orc_basis* B, int width, int depth)
     return 0;
 }
 
+/* Any graded basis given by its degree offsets start[0..depth+1] (a Lie
+ * basis, for the linear operations, which read nothing else of it). */
+static int orc_graded_init(orc_basis* G, const long long* start, int depth)
+{
+    if (depth < 0 || depth > ORC_MAX_DEPTH || start == 0) return 1;
+    G->width = 0;
+    G->depth = depth;
+    for (int k = 0; k <= depth + 1; ++k) {
+        G->start[k] = start[k];
+        G->pow[k] = 0;
+    }
+    return 0;
+}
+
 long long orc_tensor_dimension(int width, int depth)
 {
     orc_basis B;

@@ -458,6 +458,93 @@ int FN(orc_batch_tensor_antipode)(int sign_by_degree, int width, int depth, long
     return 0;
 }
 
+/* --- linear operations over any graded basis ------------------------------
+ * `start` holds the degree offsets start_of_degree(k), k = 0..depth+1, of a
+ * tensor or Lie basis. */
+
+/* VectorAdd (cpu/.../linalg/vector_add.hpp:36-66): on every degree of out's
+ * window out = alpha*lhs + beta*rhs, each operand contributing only on its own
+ * window; the sum starts from 0 and adds the two products in that order. */
+static void FN(vector_add)(const orc_basis* G, T* out, int omin, int omax, const T* lhs, int lmin, int lmax,
+                           const T* rhs, int rmin, int rmax, T alpha, T beta)
+{
+    for (int k = omin; k <= omax; ++k) {
+        const int hl = lmin <= k && k <= lmax, hr = rmin <= k && k <= rmax;
+        for (long long i = G->start[k]; i < G->start[k + 1]; ++i) {
+            T v = (T)0;
+            if (hl) v = v + alpha * lhs[i];
+            if (hr) v = v + beta * rhs[i];
+            out[i] = v;
+        }
+    }
+}
+
+/* VectorInplaceAdd (cpu/.../linalg/vector_inplace_add.hpp:36-63): lhs +=
+ * alpha*rhs on the overlap of the two windows. */
+static void FN(vector_inplace_add)(const orc_basis* G, T* lhs, int lmin, int lmax, const T* rhs, int rmin,
+                                   int rmax, T alpha)
+{
+    const int lo = lmin > rmin ? lmin : rmin, hi = lmax < rmax ? lmax : rmax;
+    if (hi < lo) return;
+    for (long long i = G->start[lo]; i < G->start[hi + 1]; ++i) lhs[i] = lhs[i] + alpha * rhs[i];
+}
+
+int FN(orc_batch_vector_assign)(const long long* start, int depth, long long n, T* out, long long os, int omin,
+                                int omax, const T* arg, long long as, int amin, int amax)
+{
+    orc_basis G;
+    if (orc_graded_init(&G, start, depth)) return 3;
+    for (long long i = 0; i < n; ++i)
+        FN(assign)(&G, OBJ(out, os, i), omin, omax, OBJ(arg, as, i), amin, amax);
+    return 0;
+}
+
+int FN(orc_batch_vector_add)(const long long* start, int depth, long long n, T* out, long long os, int omin,
+                             int omax, const T* lhs, long long ls, int lmin, int lmax, const T* rhs,
+                             long long rs, int rmin, int rmax, T alpha, T beta)
+{
+    orc_basis G;
+    if (orc_graded_init(&G, start, depth)) return 3;
+    for (long long i = 0; i < n; ++i)
+        FN(vector_add)(&G, OBJ(out, os, i), omin, omax, OBJ(lhs, ls, i), lmin, lmax, OBJ(rhs, rs, i), rmin,
+                       rmax, alpha, beta);
+    return 0;
+}
+
+int FN(orc_batch_vector_inplace_add)(const long long* start, int depth, long long n, T* lhs, long long ls,
+                                     int lmin, int lmax, const T* rhs, long long rs, int rmin, int rmax,
+                                     T alpha)
+{
+    orc_basis G;
+    if (orc_graded_init(&G, start, depth)) return 3;
+    for (long long i = 0; i < n; ++i)
+        FN(vector_inplace_add)(&G, OBJ(lhs, ls, i), lmin, lmax, OBJ(rhs, rs, i), rmin, rmax, alpha);
+    return 0;
+}
+
+/* op 0: VectorScalarMultiply (cpu/.../linalg/vector_scalar_multiply.hpp:37-49),
+ * 1: VectorSetConstant, 2: TensorSetIdentity, 3: TensorAddIdentity. */
+int FN(orc_batch_vector_scalar)(int op, const long long* start, int depth, long long n, T* vec, long long vs,
+                                int vmin, int vmax, T scalar)
+{
+    orc_basis G;
+    if (orc_graded_init(&G, start, depth)) return 3;
+    for (long long b = 0; b < n; ++b) {
+        T* v = OBJ(vec, vs, b);
+        switch (op) {
+            case 0:
+                if (vmin <= vmax)
+                    for (long long i = G.start[vmin]; i < G.start[vmax + 1]; ++i) v[i] = v[i] * scalar;
+                break;
+            case 1: FN(set_constant)(&G, v, vmin, vmax, scalar); break;
+            case 2: FN(set_identity)(&G, v, vmin, vmax, scalar); break;
+            case 3: FN(add_identity)(v, vmin, vmax, scalar); break;
+            default: return 3;
+        }
+    }
+    return 0;
+}
+
 int FN(orc_batch_tensor_pairing)(int width, int depth, long long n, T* out,
                                  long long ostride, const T* f, long long fs,
                                  int fmin, int fmax, const T* x, long long xs,

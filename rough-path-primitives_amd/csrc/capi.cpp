@@ -16,6 +16,7 @@
 #include "signature.hpp"
 #include "signature_reg.hpp"
 #include "signature_chain.hpp"
+#include "vector_ops.hpp"
 
 namespace rpp_hip {
 
@@ -300,6 +301,136 @@ int lie_convert_impl(bool to_lie, const rpp_hip_lie_convert_desc* d, void* s) {
                   : launch_spmv<T>(d->num_batches, tv, lv, M, 1.0, as_stream(s));
 }
 
+// ---------------------------------------------------------- linear operations
+
+// Number of Lie elements of degree k over `width` letters (Witt's formula:
+// (1/k) sum_{d | k} mu(d) width^(k/d)), the size of degree k of the Hall-set
+// LieBasis (basis/hall_basis.hpp:64-88).
+int64_t witt_dimension(int width, int k) {
+    int64_t sum = 0;
+    for (int d = 1; d <= k; ++d) {
+        if (k % d) continue;
+        int mu = 1, m = d;
+        for (int p = 2; p * p <= m; ++p) {
+            if (m % p) continue;
+            m /= p;
+            if (m % p == 0) {
+                mu = 0;
+                break;
+            }
+            mu = -mu;
+        }
+        if (mu != 0 && m > 1) mu = -mu;
+        if (mu == 0) continue;
+        int64_t pw = 1;
+        for (int e = 0; e < k / d; ++e) pw *= width;
+        sum += mu * pw;
+    }
+    return sum / k;
+}
+
+// Degree offsets start_of_degree(k), k = 0..depth+1, of a graded basis.
+int graded_offsets(const rpp_hip_graded_basis& g, int64_t* start) {
+    const rpp_hip_basis b{g.width, g.depth};
+    RPP_TRY(check_basis(b));
+    if (g.kind == RPP_HIP_TENSOR_BASIS) {
+        const DevBasis B = make_basis(g.width, g.depth);
+        for (int k = 0; k <= g.depth + 1; ++k) start[k] = B.start[k];
+        return 0;
+    }
+    if (g.kind != RPP_HIP_LIE_BASIS) return fail(RPP_HIP_INVALID_ARGUMENT, "basis kind must be tensor or Lie");
+    start[0] = 0;
+    start[1] = 1;
+    for (int k = 1; k <= g.depth; ++k) start[k + 1] = start[k] + witt_dimension(g.width, k);
+    return 0;
+}
+
+IndexRange range_of(const int64_t* start, const rpp_hip_tensor_batch& t) {
+    if (t.min_degree > t.max_degree) return IndexRange{0, 0};
+    return IndexRange{start[t.min_degree], start[t.max_degree + 1]};
+}
+
+IndexRange overlap(IndexRange a, IndexRange b) {
+    return IndexRange{std::max(a.begin, b.begin), std::min(a.end, b.end)};
+}
+
+bool nonempty(const rpp_hip_tensor_batch& t) { return t.min_degree <= t.max_degree; }
+
+// An output may be an input only as the very same batch (same data and
+// strides), which the element-wise kernels serve; any other overlap of the
+// two base pointers is rejected.
+int check_same_or_distinct(const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& in, const char* what) {
+    if (out.data == in.data && nonempty(out) && nonempty(in) &&
+        (out.batch_stride != in.batch_stride || out.coeff_stride != in.coeff_stride))
+        return fail(RPP_HIP_INVALID_ARGUMENT, std::string(what) + ": out may alias an operand only with the same strides");
+    return 0;
+}
+
+int check_count(int64_t n) {
+    if (n < 0) return fail(RPP_HIP_INVALID_ARGUMENT, "negative num_batches");
+    return 0;
+}
+
+template <typename T>
+int vector_assign_impl(const rpp_hip_vector_assign_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    int64_t start[kMaxDepth + 2];
+    RPP_TRY(graded_offsets(d->basis, start));
+    RPP_TRY(check_count(d->num_batches));
+    const rpp_hip_basis b{d->basis.width, d->basis.depth};
+    RPP_TRY(check_view(d->out, b, d->num_batches, "out", true));
+    RPP_TRY(check_view(d->arg, b, d->num_batches, "arg", false));
+    RPP_TRY(check_same_or_distinct(d->out, d->arg, "vector_assign"));
+    const IndexRange o = range_of(start, d->out);
+    return launch_vector_op<T>(kVecAssign, d->num_batches, d->out, o, &d->arg, overlap(o, range_of(start, d->arg)),
+                               nullptr, IndexRange{0, 0}, 0.0, 0.0, as_stream(s));
+}
+
+template <typename T>
+int vector_add_impl(const rpp_hip_vector_add_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    int64_t start[kMaxDepth + 2];
+    RPP_TRY(graded_offsets(d->basis, start));
+    RPP_TRY(check_count(d->num_batches));
+    const rpp_hip_basis b{d->basis.width, d->basis.depth};
+    RPP_TRY(check_view(d->out, b, d->num_batches, "out", true));
+    RPP_TRY(check_view(d->lhs, b, d->num_batches, "lhs", false));
+    RPP_TRY(check_view(d->rhs, b, d->num_batches, "rhs", false));
+    RPP_TRY(check_same_or_distinct(d->out, d->lhs, "vector_add"));
+    RPP_TRY(check_same_or_distinct(d->out, d->rhs, "vector_add"));
+    return launch_vector_op<T>(kVecAdd, d->num_batches, d->out, range_of(start, d->out), &d->lhs,
+                               range_of(start, d->lhs), &d->rhs, range_of(start, d->rhs), d->alpha, d->beta,
+                               as_stream(s));
+}
+
+template <typename T>
+int vector_inplace_add_impl(const rpp_hip_vector_inplace_add_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    int64_t start[kMaxDepth + 2];
+    RPP_TRY(graded_offsets(d->basis, start));
+    RPP_TRY(check_count(d->num_batches));
+    const rpp_hip_basis b{d->basis.width, d->basis.depth};
+    RPP_TRY(check_view(d->lhs, b, d->num_batches, "lhs", true));
+    RPP_TRY(check_view(d->rhs, b, d->num_batches, "rhs", false));
+    RPP_TRY(check_same_or_distinct(d->lhs, d->rhs, "vector_inplace_add"));
+    if (!nonempty(d->lhs) || !nonempty(d->rhs)) return 0;
+    const IndexRange o = overlap(range_of(start, d->lhs), range_of(start, d->rhs));
+    return launch_vector_op<T>(kVecInplaceAdd, d->num_batches, d->lhs, o, &d->rhs, o, nullptr, IndexRange{0, 0},
+                               d->alpha, 0.0, as_stream(s));
+}
+
+template <typename T>
+int vector_scalar_impl(int op, const rpp_hip_vector_scalar_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    int64_t start[kMaxDepth + 2];
+    RPP_TRY(graded_offsets(d->basis, start));
+    RPP_TRY(check_count(d->num_batches));
+    const rpp_hip_basis b{d->basis.width, d->basis.depth};
+    RPP_TRY(check_view(d->vec, b, d->num_batches, "vec", true));
+    return launch_vector_op<T>(op, d->num_batches, d->vec, range_of(start, d->vec), nullptr, IndexRange{0, 0},
+                               nullptr, IndexRange{0, 0}, d->scalar, 0.0, as_stream(s));
+}
+
 }  // namespace
 }  // namespace rpp_hip
 
@@ -318,6 +449,19 @@ RPP_C_ENTRY(rpp_hip_ft_inplace_mul, rpp_hip_ft_inplace_mul_desc, ft_inplace_mul_
 RPP_C_ENTRY(rpp_hip_ft_fmexp, rpp_hip_ft_fmexp_desc, fmexp_impl)
 RPP_C_ENTRY(rpp_hip_tensor_pairing, rpp_hip_tensor_pairing_desc, pairing_impl)
 RPP_C_ENTRY(rpp_hip_sparse_matrix_vector_product, rpp_hip_spmv_desc, spmv_impl)
+RPP_C_ENTRY(rpp_hip_vector_assign, rpp_hip_vector_assign_desc, vector_assign_impl)
+RPP_C_ENTRY(rpp_hip_vector_add, rpp_hip_vector_add_desc, vector_add_impl)
+RPP_C_ENTRY(rpp_hip_vector_inplace_add, rpp_hip_vector_inplace_add_desc, vector_inplace_add_impl)
+
+#define RPP_C_SCALAR_ENTRY(NAME, OP)                                                                     \
+    int NAME##_f32(const rpp_hip_vector_scalar_desc* d, void* s) { return vector_scalar_impl<float>(OP, d, s); } \
+    int NAME##_f64(const rpp_hip_vector_scalar_desc* d, void* s) { return vector_scalar_impl<double>(OP, d, s); }
+
+RPP_C_SCALAR_ENTRY(rpp_hip_vector_scalar_multiply, kVecScale)
+RPP_C_SCALAR_ENTRY(rpp_hip_vector_set_constant, kVecSetConstant)
+RPP_C_SCALAR_ENTRY(rpp_hip_tensor_set_identity, kVecSetIdentity)
+RPP_C_SCALAR_ENTRY(rpp_hip_tensor_add_identity, kVecAddIdentity)
+#undef RPP_C_SCALAR_ENTRY
 
 int rpp_hip_tensor_antipode_f32(const rpp_hip_ft_unary_desc* d, void* s) { return antipode_impl<float>(true, d, s); }
 int rpp_hip_tensor_antipode_f64(const rpp_hip_ft_unary_desc* d, void* s) {

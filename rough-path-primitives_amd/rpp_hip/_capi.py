@@ -99,6 +99,32 @@ class LieConvertDesc(C.Structure):
     _fields_ = [("basis", Basis), ("num_batches", _i64), ("tensor", TensorBatch), ("lie", TensorBatch)]
 
 
+class GradedBasis(C.Structure):
+    _fields_ = [("kind", _i32), ("width", _i32), ("depth", _i32)]
+
+
+TENSOR_BASIS = 0
+LIE_BASIS = 1
+
+
+class VectorAssignDesc(C.Structure):
+    _fields_ = [("basis", GradedBasis), ("num_batches", _i64), ("out", TensorBatch), ("arg", TensorBatch)]
+
+
+class VectorAddDesc(C.Structure):
+    _fields_ = [("basis", GradedBasis), ("num_batches", _i64), ("out", TensorBatch), ("lhs", TensorBatch),
+                ("rhs", TensorBatch), ("alpha", _f64), ("beta", _f64)]
+
+
+class VectorInplaceAddDesc(C.Structure):
+    _fields_ = [("basis", GradedBasis), ("num_batches", _i64), ("lhs", TensorBatch), ("rhs", TensorBatch),
+                ("alpha", _f64)]
+
+
+class VectorScalarDesc(C.Structure):
+    _fields_ = [("basis", GradedBasis), ("num_batches", _i64), ("vec", TensorBatch), ("scalar", _f64)]
+
+
 class SignatureDesc(C.Structure):
     _fields_ = [("basis", Basis), ("num_paths", _i64), ("increments", _vp), ("offsets", _vp),
                 ("out", TensorBatch), ("checkpoint_every", _i32), ("checkpoints", _vp),
@@ -133,6 +159,13 @@ ENTRY_POINTS = {
     "rpp_hip_sparse_matrix_vector_product": SpmvDesc,
     "rpp_hip_tensor_to_lie": LieConvertDesc,
     "rpp_hip_lie_to_tensor": LieConvertDesc,
+    "rpp_hip_vector_assign": VectorAssignDesc,
+    "rpp_hip_vector_add": VectorAddDesc,
+    "rpp_hip_vector_inplace_add": VectorInplaceAddDesc,
+    "rpp_hip_vector_scalar_multiply": VectorScalarDesc,
+    "rpp_hip_vector_set_constant": VectorScalarDesc,
+    "rpp_hip_tensor_set_identity": VectorScalarDesc,
+    "rpp_hip_tensor_add_identity": VectorScalarDesc,
     "rpp_hip_signature": SignatureDesc,
     "rpp_hip_signature_backward": SignatureBackwardDesc,
 }

@@ -65,6 +65,50 @@ def tensor_dimension(width: int, depth: int) -> int:
     return TensorBasis(width, depth).size()
 
 
+def _witt(width: int, k: int) -> int:
+    """Number of Lie elements of degree k (Witt's formula)."""
+    def mobius(m):
+        mu, p = 1, 2
+        while p * p <= m:
+            if m % p == 0:
+                m //= p
+                if m % p == 0:
+                    return 0
+                mu = -mu
+            p += 1
+        return -mu if m > 1 else mu
+    return sum(mobius(d) * width ** (k // d) for d in range(1, k + 1) if k % d == 0) // k
+
+
+@dataclass(frozen=True)
+class LieBasis:
+    """The degree grading of the Hall-set LieBasis(width, depth)
+    (basis/lie_basis.hpp:42-97): degree 0 holds one slot, degree k the Witt
+    number of Lie elements; storage is true_size() per object."""
+    width: int
+    depth: int
+
+    def start_of_degree(self, k: int) -> int:
+        return 0 if k == 0 else 1 + sum(_witt(self.width, j) for j in range(1, k))
+
+    def end_of_degree(self, k: int) -> int:
+        return self.start_of_degree(k + 1)
+
+    def size_of_degree(self, k: int) -> int:
+        return 1 if k == 0 else _witt(self.width, k)
+
+    def true_size(self) -> int:
+        return self.start_of_degree(self.depth + 1)
+
+    def size(self) -> int:
+        return 0 if self.width <= 1 or self.depth == 0 else self.true_size() - 1
+
+
+def _graded(basis) -> A.GradedBasis:
+    kind = A.LIE_BASIS if isinstance(basis, LieBasis) else A.TENSOR_BASIS
+    return A.GradedBasis(kind, basis.width, basis.depth)
+
+
 @dataclass
 class Batch:
     """A batch of graded views: data + window [min_degree, max_degree]."""
@@ -269,6 +313,76 @@ def ft_adj_rmul(out, op, arg, basis: TensorBasis, num_batches=None, stream=None)
     out, op, arg = (_as_batch(x, basis) for x in (out, op, arg))
     d = A.AdjointDesc(basis._c(), _n(num_batches, out), out._c(), op._c(), arg._c())
     _call("rpp_hip_ft_adj_rmul" + _sfx(out, op, arg), d, stream)
+
+
+# -------------------------------------------------------- linear operations
+
+def vector_assign(out, arg, basis, num_batches=None, stream=None):
+    """rpp::ops::vector_assign (operations/linalg/vector_assign.hpp:36-64): out =
+    arg on the overlap of the windows, 0 on the rest of out's window.  ``basis``
+    is a TensorBasis or a LieBasis.  Between an object-major and an
+    interleaved batch this is the layout transposition."""
+    out, arg = (_as_batch(x, basis) for x in (out, arg))
+    d = A.VectorAssignDesc(_graded(basis), _n(num_batches, out), out._c(), arg._c())
+    _call("rpp_hip_vector_assign" + _sfx(out, arg), d, stream)
+
+
+def vector_add(out, lhs, rhs, basis, num_batches=None, alpha: float = 1.0, beta: float = 1.0, stream=None):
+    """rpp::ops::vector_add (operations/linalg/vector_add.hpp:40-78): out =
+    alpha*lhs + beta*rhs on out's window, each operand 0 outside its own."""
+    out, lhs, rhs = (_as_batch(x, basis) for x in (out, lhs, rhs))
+    d = A.VectorAddDesc(_graded(basis), _n(num_batches, out), out._c(), lhs._c(), rhs._c(), alpha, beta)
+    _call("rpp_hip_vector_add" + _sfx(out, lhs, rhs), d, stream)
+
+
+def vector_inplace_add(lhs, rhs, basis, num_batches=None, alpha: float = 1.0, stream=None):
+    """rpp::ops::vector_inplace_add (operations/linalg/vector_inplace_add.hpp:40-77):
+    lhs += alpha*rhs on the overlap of the windows."""
+    lhs, rhs = (_as_batch(x, basis) for x in (lhs, rhs))
+    d = A.VectorInplaceAddDesc(_graded(basis), _n(num_batches, lhs), lhs._c(), rhs._c(), alpha)
+    _call("rpp_hip_vector_inplace_add" + _sfx(lhs, rhs), d, stream)
+
+
+def _scalar_op(name, vec, basis, num_batches, scalar, stream):
+    vec = _as_batch(vec, basis)
+    d = A.VectorScalarDesc(_graded(basis), _n(num_batches, vec), vec._c(), float(scalar))
+    _call(name + _sfx(vec), d, stream)
+
+
+def vector_scalar_multiply(vec, basis, num_batches=None, *, scalar: float, stream=None):
+    """rpp::ops::vector_scalar_multiply (operations/linalg/vector_scalar_multiply.hpp:28-71):
+    vec *= scalar on its window."""
+    _scalar_op("rpp_hip_vector_scalar_multiply", vec, basis, num_batches, scalar, stream)
+
+
+def vector_set_constant(vec, basis, num_batches=None, value: float = 0.0, stream=None):
+    """rpp::ops::vector_set_constant (operations/linalg/vector_set_constant.hpp:27-72)."""
+    _scalar_op("rpp_hip_vector_set_constant", vec, basis, num_batches, value, stream)
+
+
+def tensor_set_identity(tensor, basis, num_batches=None, scalar: float = 1.0, stream=None):
+    """rpp::ops::tensor_set_identity (operations/basic/tensor_set_identity.hpp:27-73):
+    the window is zeroed and degree 0, if held, set to scalar."""
+    _scalar_op("rpp_hip_tensor_set_identity", tensor, basis, num_batches, scalar, stream)
+
+
+def tensor_add_identity(tensor, basis, num_batches=None, scalar: float = 1.0, stream=None):
+    """rpp::ops::tensor_add_identity (operations/basic/tensor_add_identity.hpp:26-71)."""
+    _scalar_op("rpp_hip_tensor_add_identity", tensor, basis, num_batches, scalar, stream)
+
+
+def to_interleaved(x: torch.Tensor, basis, stream=None) -> torch.Tensor:
+    """[n, dim] object-major -> a new [dim, n] batch-interleaved tensor."""
+    out = torch.empty((x.shape[1], x.shape[0]), dtype=x.dtype, device=x.device)
+    vector_assign(interleaved(out, basis), make_tensor_batch(x, basis), basis, x.shape[0], stream)
+    return out
+
+
+def to_object_major(x: torch.Tensor, basis, stream=None) -> torch.Tensor:
+    """[dim, n] batch-interleaved -> a new [n, dim] object-major tensor."""
+    out = torch.empty((x.shape[1], x.shape[0]), dtype=x.dtype, device=x.device)
+    vector_assign(make_tensor_batch(out, basis), interleaved(x, basis), basis, x.shape[1], stream)
+    return out
 
 
 # ------------------------------------------------------------- sparse, Lie

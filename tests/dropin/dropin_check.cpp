@@ -218,6 +218,54 @@ void run(int width, int depth, Index n) {
         report(tag("tensor_antipode"), oa.host(), ha);
         report(tag("tensor_reflect"), orf.host(), hr);
     }
+    {  // linear operations on partial windows
+        const Degree lo = depth >= 2 ? 1 : 0;
+        std::vector<T> hs = lhs, ha = nanv, hadd = nanv, hi = lhs, hc = lhs, hid = lhs, hai = lhs;
+        (void)rpp::ops::vector_scalar_multiply(cpu, {}, hb(hs.data(), lo, D), basis, n, T(-0.5));
+        (void)rpp::ops::vector_assign(cpu, {}, hb(ha.data(), 0, D), hcb(rhs.data(), lo, D), basis, n);
+        (void)rpp::ops::vector_add(cpu, {}, hb(hadd.data(), 0, D), hcb(lhs.data(), lo, D), hcb(rhs.data(), 0, lo),
+                                   basis, n, T(0.5), T(2));
+        (void)rpp::ops::vector_inplace_add(cpu, {}, hb(hi.data(), 0, D), hcb(rhs.data(), lo, D), basis, n, T(-1));
+        (void)rpp::ops::vector_set_constant(cpu, {}, hb(hc.data(), lo, D), basis, n, T(0.25));
+        (void)rpp::ops::tensor_set_identity(cpu, {}, hb(hid.data(), 0, D), basis, n, T(3));
+        (void)rpp::ops::tensor_add_identity(cpu, {}, hb(hai.data(), 0, D), basis, n, T(-1));
+        Dev<T> s(lhs), a(nanv), ad(nanv), ip(lhs), c(lhs), id(lhs), ai(lhs), l(lhs), r(rhs);
+        (void)rpp::ops::vector_scalar_multiply(gpu, {}, hb(s.p, lo, D), basis, n, T(-0.5));
+        (void)rpp::ops::vector_assign(gpu, {}, hb(a.p, 0, D), hcb(r.p, lo, D), basis, n);
+        (void)rpp::ops::vector_add(gpu, {}, hb(ad.p, 0, D), hcb(l.p, lo, D), hcb(r.p, 0, lo), basis, n, T(0.5),
+                                   T(2));
+        (void)rpp::ops::vector_inplace_add(gpu, {}, hb(ip.p, 0, D), hcb(r.p, lo, D), basis, n, T(-1));
+        (void)rpp::ops::vector_set_constant(gpu, {}, hb(c.p, lo, D), basis, n, T(0.25));
+        (void)rpp::ops::tensor_set_identity(gpu, {}, hb(id.p, 0, D), basis, n, T(3));
+        (void)rpp::ops::tensor_add_identity(gpu, {}, hb(ai.p, 0, D), basis, n, T(-1));
+        report(tag("vector_scalar_multiply"), s.host(), hs);
+        report(tag("vector_assign"), a.host(), ha);
+        report(tag("vector_add"), ad.host(), hadd);
+        report(tag("vector_inplace_add"), ip.host(), hi);
+        report(tag("vector_set_constant"), c.host(), hc);
+        report(tag("tensor_set_identity"), id.host(), hid);
+        report(tag("tensor_add_identity"), ai.host(), hai);
+    }
+    if (width >= 2) {  // linear operations on a Lie grading (the Hall-set LieBasis)
+        rpp::basis::HallBasis<Arch> hall(static_cast<Degree>(width), D);
+        auto lb = hall.to_lie_basis();
+        const Index ls = lb.true_size();
+        std::vector<T> x(static_cast<size_t>(n * ls)), y(static_cast<size_t>(n * ls));
+        std::uniform_real_distribution<double> u(-1.0, 1.0);
+        for (auto& v : x) v = static_cast<T>(u(rng));
+        for (auto& v : y) v = static_cast<T>(u(rng));
+        auto lie_b = [&](T* p, Degree mn) { return rpp::make_lie_batch(p, ls, lb, mn, D); };
+        auto lie_cb = [&](const T* p, Degree mn) { return rpp::make_lie_batch(p, ls, lb, mn, D); };
+        std::vector<T> hadd(static_cast<size_t>(n * ls), T(7)), hasg = hadd;
+        (void)rpp::ops::vector_add(cpu, {}, lie_b(hadd.data(), 1), lie_cb(x.data(), 1), lie_cb(y.data(), 2), lb, n,
+                                   T(1), T(-3));
+        (void)rpp::ops::vector_assign(cpu, {}, lie_b(hasg.data(), 1), lie_cb(y.data(), 2), lb, n);
+        Dev<T> gadd(std::vector<T>(hadd.size(), T(7))), gasg(std::vector<T>(hadd.size(), T(7))), gx(x), gy(y);
+        (void)rpp::ops::vector_add(gpu, {}, lie_b(gadd.p, 1), lie_cb(gx.p, 1), lie_cb(gy.p, 2), lb, n, T(1), T(-3));
+        (void)rpp::ops::vector_assign(gpu, {}, lie_b(gasg.p, 1), lie_cb(gy.p, 2), lb, n);
+        report(tag("vector_add (Lie)"), gadd.host(), hadd);
+        report(tag("vector_assign (Lie)"), gasg.host(), hasg);
+    }
     if (width >= 2) {  // tensor_to_lie / lie_to_tensor with the reference-built matrices
         rpp::basis::HallBasis<Arch> hall(static_cast<Degree>(width), D);
         auto lb = hall.to_lie_basis();

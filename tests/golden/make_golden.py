@@ -104,6 +104,28 @@ def main():
     lie = O.ref_lie_convert(True, True, 3, 8, lg)
     back = O.ref_lie_convert(False, True, 3, 8, lie)
     out["logsig_w3d8"] = dict(increments=inc, signature=sig, log=lg, lie=lie, back=back)
+    # linear operations on partial windows, tensor (3,4) and Lie (3,4)
+    # gradings: 3 objects each
+    rng = np.random.default_rng(0x11)
+    for kind, start in (("tensor", O.degree_begin(3, 4)), ("lie", O.lie_degree_begin(3, 4))):
+        dim = start[-1]
+        x, y, z = (rng.uniform(-1, 1, (3, dim)) for _ in range(3))
+        assign = np.full((3, dim), 7.0)
+        R.vector_assign(start, 4, assign, y, ow=(1, 3), aw=(2, 4))
+        add = np.full((3, dim), 7.0)
+        R.vector_add(start, 4, add, x, y, ow=(0, 3), lw=(1, 4), rw=(0, 2), alpha=0.75, beta=-1.5)
+        inplace = z.copy()
+        R.vector_inplace_add(start, 4, inplace, x, lw=(1, 4), rw=(0, 2), alpha=-0.25)
+        scaled = z.copy()
+        R.vector_scalar_multiply(start, 4, scaled, 1.25, vw=(2, 4))
+        const = z.copy()
+        R.vector_set_constant(start, 4, const, -0.5, vw=(0, 1))
+        ident = z.copy()
+        R.tensor_set_identity(start, 4, ident, 2.0, vw=(0, 2))
+        addid = z.copy()
+        R.tensor_add_identity(start, 4, addid, 0.5)
+        out[f"linear_{kind}_w3d4"] = dict(x=x, y=y, z=z, assign=assign, add=add, inplace=inplace, scaled=scaled,
+                                          const=const, ident=ident, addid=addid)
     for name, arrays in out.items():
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
     print("wrote", len(out), "fixtures to", HERE)

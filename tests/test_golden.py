@@ -81,7 +81,72 @@ def test_oracle_pairing_and_chains():
     assert rel_err(lg, g["log"]) <= TOL
 
 
+LINEAR = [("tensor", 3, 4), ("lie", 3, 4)]
+
+
+def linear_start(kind):
+    return O.degree_begin(3, 4) if kind == "tensor" else O.lie_degree_begin(3, 4)
+
+
+def linear_cases(f):
+    """(name, initial, call) of each op in the linear fixtures; `call(impl,
+    start, arrays)` runs it through an implementation with the oracle's
+    argument order."""
+    return [
+        ("assign", np.full_like(f["y"], 7.0), lambda I, st, a: I.vector_assign(st, 4, a, f["y"], ow=(1, 3), aw=(2, 4))),
+        ("add", np.full_like(f["x"], 7.0),
+         lambda I, st, a: I.vector_add(st, 4, a, f["x"], f["y"], ow=(0, 3), lw=(1, 4), rw=(0, 2), alpha=0.75,
+                                       beta=-1.5)),
+        ("inplace", f["z"].copy(),
+         lambda I, st, a: I.vector_inplace_add(st, 4, a, f["x"], lw=(1, 4), rw=(0, 2), alpha=-0.25)),
+        ("scaled", f["z"].copy(), lambda I, st, a: I.vector_scalar_multiply(st, 4, a, 1.25, vw=(2, 4))),
+        ("const", f["z"].copy(), lambda I, st, a: I.vector_set_constant(st, 4, a, -0.5, vw=(0, 1))),
+        ("ident", f["z"].copy(), lambda I, st, a: I.tensor_set_identity(st, 4, a, 2.0, vw=(0, 2))),
+        ("addid", f["z"].copy(), lambda I, st, a: I.tensor_add_identity(st, 4, a, 0.5)),
+    ]
+
+
+@pytest.mark.parametrize("kind", ["tensor", "lie"])
+def test_oracle_linear_operations(kind):
+    f = fixture(f"linear_{kind}_w3d4")
+    start = linear_start(kind)
+    for name, init, call in linear_cases(f):
+        call(O.orc, start, init)
+        assert np.array_equal(init, f[name]), name
+
+
 # ---------------------------------------------------------------------- GPU
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["tensor", "lie"])
+def test_gpu_linear_operations_against_golden(dev, kind):
+    import torch
+
+    import rpp_hip
+
+    f = fixture(f"linear_{kind}_w3d4")
+    B = rpp_hip.TensorBasis(3, 4) if kind == "tensor" else rpp_hip.LieBasis(3, 4)
+
+    def t(x):
+        return torch.tensor(np.ascontiguousarray(x), device=dev)
+
+    def run(name, init, fn):
+        buf = t(init)
+        fn(buf)
+        torch.cuda.synchronize()
+        assert np.array_equal(buf.cpu().numpy(), f[name]), name
+
+    Bt = rpp_hip.Batch
+    run("assign", np.full_like(f["y"], 7.0), lambda o: rpp_hip.vector_assign(Bt(o, 1, 3), Bt(t(f["y"]), 2, 4), B))
+    run("add", np.full_like(f["x"], 7.0),
+        lambda o: rpp_hip.vector_add(Bt(o, 0, 3), Bt(t(f["x"]), 1, 4), Bt(t(f["y"]), 0, 2), B, alpha=0.75,
+                                     beta=-1.5))
+    run("inplace", f["z"], lambda o: rpp_hip.vector_inplace_add(Bt(o, 1, 4), Bt(t(f["x"]), 0, 2), B, alpha=-0.25))
+    run("scaled", f["z"], lambda o: rpp_hip.vector_scalar_multiply(Bt(o, 2, 4), B, scalar=1.25))
+    run("const", f["z"], lambda o: rpp_hip.vector_set_constant(Bt(o, 0, 1), B, value=-0.5))
+    run("ident", f["z"], lambda o: rpp_hip.tensor_set_identity(Bt(o, 0, 2), B, scalar=2.0))
+    run("addid", f["z"], lambda o: rpp_hip.tensor_add_identity(o, B, scalar=0.5))
+
 
 @pytest.mark.gpu
 def test_gpu_against_golden(dev):
