@@ -359,6 +359,27 @@ int rpp_hip_lie_basis_copy(int32_t width, int32_t depth, int64_t* pairs,
                            double* t2l_values, int64_t* t2l_indices,
                            int64_t* t2l_offsets);
 
+/* The Hall basis metadata on disk.  No counterpart in the reference, whose
+ * builders (basis/hall_basis.hpp, sparse/l2t_matrix.hpp, sparse/t2l_matrix.hpp)
+ * run again in every process; for callers that build large shapes once and
+ * reuse them across processes.
+ *   rpp_hip_lie_basis_save: writes the metadata of (width, depth), building it
+ *     if this process has not, to `path` (a little-endian binary file: format
+ *     tag, shape, sizes, the arrays of rpp_hip_lie_basis_copy in double and
+ *     int64, an FNV-1a checksum), atomically (temporary file, then rename).
+ *   rpp_hip_lie_basis_load: reads such a file, checks the checksum, every
+ *     size and every index range, and makes it this process's metadata for its
+ *     (width, depth), returned through width/depth (either may be NULL).  A
+ *     file that cannot be opened is RPP_HIP_INVALID_ARGUMENT; a damaged or
+ *     inconsistent one RPP_HIP_CONTRACT_VIOLATION, as is a file that differs
+ *     from the metadata this process already uses for that shape.
+ * With the environment variable RPP_HIP_LIE_CACHE_DIR set to a directory, the
+ * library does the same by itself: the first use of a shape reads
+ * rpp_lie_w<width>_d<depth>.bin there, or builds the metadata and writes that
+ * file. */
+int rpp_hip_lie_basis_save(int32_t width, int32_t depth, const char* path);
+int rpp_hip_lie_basis_load(const char* path, int32_t* width, int32_t* depth);
+
 /* ------------------------------------------------- batched Chen recursion */
 
 /* Signatures of many piecewise-linear paths of uneven length in ONE

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <string>
 
@@ -527,6 +528,34 @@ int rpp_hip_lie_basis_copy(int32_t width, int32_t depth, int64_t* pairs, int64_t
     cp(t2l_values, md->t2l_csc_values);
     cp(t2l_indices, md->t2l_csc_indices);
     cp(t2l_offsets, md->t2l_csc_offsets);
+    return 0;
+}
+
+int rpp_hip_lie_basis_save(int32_t width, int32_t depth, const char* path) {
+    const rpp_hip_basis b{width, depth};
+    if (int rc = check_basis(b)) return rc;
+    if (!path || !*path) return fail(RPP_HIP_INVALID_ARGUMENT, "null or empty path");
+    auto md = lie_metadata(width, depth);
+    std::string err;
+    if (!save_lie_metadata(*md, path, &err)) return fail(RPP_HIP_INVALID_ARGUMENT, err);
+    return 0;
+}
+
+int rpp_hip_lie_basis_load(const char* path, int32_t* width, int32_t* depth) {
+    if (!path || !*path) return fail(RPP_HIP_INVALID_ARGUMENT, "null or empty path");
+    if (std::FILE* f = std::fopen(path, "rb")) {
+        std::fclose(f);
+    } else {
+        return fail(RPP_HIP_INVALID_ARGUMENT, std::string("cannot open ") + path);
+    }
+    std::string err;
+    auto md = load_lie_metadata(path, &err);
+    if (!md) return fail(RPP_HIP_CONTRACT_VIOLATION, err);
+    const rpp_hip_basis b{md->width, md->depth};
+    if (int rc = check_basis(b)) return rc;
+    if (int rc = install_lie_metadata(md, &err)) return fail(rc, err);
+    if (width) *width = md->width;
+    if (depth) *depth = md->depth;
     return 0;
 }
 

@@ -15,9 +15,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include <unistd.h>
+
 #include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
+#include <string>
 #include <memory>
 #include <mutex>
 #include <unordered_map>
@@ -245,16 +251,224 @@ std::shared_ptr<const LieMetadata> build_lie_metadata(int width, int depth) {
     return md;
 }
 
-std::shared_ptr<const LieMetadata> lie_metadata(int width, int depth) {
-    static std::mutex mu;
-    static std::map<std::pair<int, int>, std::shared_ptr<const LieMetadata>> cache;
-    std::lock_guard<std::mutex> lock(mu);
-    auto key = std::make_pair(width, depth);
-    auto it = cache.find(key);
-    if (it != cache.end()) return it->second;
-    auto md = build_lie_metadata(width, depth);
-    cache.emplace(key, md);
+// ------------------------------------------------------------ on-disk form
+//
+// One little-endian binary file per (width, depth):
+//   char[8]  "RPPLIE01"
+//   int32    width, depth
+//   int64    true_size, tensor_dim, l2t_nnz, t2l_nnz
+//   int64    pairs[2 * true_size], degree_begin[depth + 2]
+//   L2T CSC  double values[l2t_nnz], int64 rows[l2t_nnz], int64 offsets[true_size]
+//   T2L CSC  double values[t2l_nnz], int64 rows[t2l_nnz], int64 offsets[tensor_dim + 1]
+//   uint64   FNV-1a of every byte before it
+// The values are stored in double, which holds them exactly (integers and
+// integers / degree); the per-dtype device copies are derived from them.
+
+namespace {
+
+constexpr char kLieMagic[8] = {'R', 'P', 'P', 'L', 'I', 'E', '0', '1'};
+
+struct Fnv {
+    uint64_t h = 1469598103934665603ull;
+    void add(const void* p, size_t n) {
+        const auto* b = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    }
+};
+
+struct Writer {
+    std::FILE* f;
+    Fnv sum;
+    bool ok = true;
+    void put(const void* p, size_t n) {
+        sum.add(p, n);
+        ok = ok && std::fwrite(p, 1, n, f) == n;
+    }
+    template <typename V>
+    void vec(const std::vector<V>& v) {
+        if (!v.empty()) put(v.data(), v.size() * sizeof(V));
+    }
+};
+
+struct Reader {
+    std::vector<unsigned char> bytes;
+    size_t pos = 0;
+    bool get(void* p, size_t n) {
+        if (n > bytes.size() - pos) return false;
+        std::memcpy(p, bytes.data() + pos, n);
+        pos += n;
+        return true;
+    }
+    template <typename V>
+    bool vec(std::vector<V>& v, int64_t count) {
+        if (count < 0 || static_cast<uint64_t>(count) > (bytes.size() - pos) / sizeof(V)) return false;
+        v.resize(static_cast<size_t>(count));
+        return count == 0 || get(v.data(), static_cast<size_t>(count) * sizeof(V));
+    }
+};
+
+// A compressed matrix is well formed: offsets start at 0, never decrease and
+// end at nnz, and every inner index lies in [0, inner).
+bool valid_compressed(const std::vector<int64_t>& offsets, const std::vector<int64_t>& indices, int64_t inner) {
+    if (offsets.empty() || offsets.front() != 0 || offsets.back() != static_cast<int64_t>(indices.size()))
+        return false;
+    for (size_t i = 1; i < offsets.size(); ++i)
+        if (offsets[i] < offsets[i - 1]) return false;
+    for (int64_t r : indices)
+        if (r < 0 || r >= inner) return false;
+    return true;
+}
+
+}  // namespace
+
+bool save_lie_metadata(const LieMetadata& md, const std::string& path, std::string* err) {
+    static std::atomic<unsigned> serial{0};
+    const std::string tmp = path + ".tmp." + std::to_string(static_cast<long long>(getpid())) + "." +
+                            std::to_string(serial.fetch_add(1));
+    std::FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) {
+        if (err) *err = "cannot open " + tmp + " for writing";
+        return false;
+    }
+    Writer w{f};
+    w.put(kLieMagic, sizeof(kLieMagic));
+    const int32_t wd[2] = {md.width, md.depth};
+    w.put(wd, sizeof(wd));
+    const int64_t sizes[4] = {md.true_size, md.tensor_dim, static_cast<int64_t>(md.l2t_csc_values.size()),
+                              static_cast<int64_t>(md.t2l_csc_values.size())};
+    w.put(sizes, sizeof(sizes));
+    w.vec(md.pairs);
+    w.vec(md.degree_begin);
+    w.vec(md.l2t_csc_values);
+    w.vec(md.l2t_csc_indices);
+    w.vec(md.l2t_csc_offsets);
+    w.vec(md.t2l_csc_values);
+    w.vec(md.t2l_csc_indices);
+    w.vec(md.t2l_csc_offsets);
+    const uint64_t check = w.sum.h;
+    w.ok = w.ok && std::fwrite(&check, 1, sizeof(check), f) == sizeof(check);
+    const bool closed = std::fclose(f) == 0;
+    // rename is atomic, so concurrent writers and readers see a whole file
+    if (!w.ok || !closed || std::rename(tmp.c_str(), path.c_str()) != 0) {
+        std::remove(tmp.c_str());
+        if (err) *err = "cannot write " + path;
+        return false;
+    }
+    return true;
+}
+
+std::shared_ptr<const LieMetadata> load_lie_metadata(const std::string& path, std::string* err) {
+    auto bad = [&](const std::string& why) -> std::shared_ptr<const LieMetadata> {
+        if (err) *err = path + ": " + why;
+        return nullptr;
+    };
+    std::FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return bad("cannot open");
+    Reader r;
+    unsigned char buf[1 << 16];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) r.bytes.insert(r.bytes.end(), buf, buf + got);
+    std::fclose(f);
+    if (r.bytes.size() < sizeof(kLieMagic) + sizeof(uint64_t)) return bad("truncated");
+    uint64_t stored;
+    std::memcpy(&stored, r.bytes.data() + r.bytes.size() - sizeof(stored), sizeof(stored));
+    r.bytes.resize(r.bytes.size() - sizeof(stored));
+    Fnv sum;
+    sum.add(r.bytes.data(), r.bytes.size());
+    if (sum.h != stored) return bad("checksum mismatch");
+    char magic[8];
+    if (!r.get(magic, sizeof(magic)) || std::memcmp(magic, kLieMagic, sizeof(magic)) != 0)
+        return bad("not a Lie metadata file of this format version");
+    int32_t wd[2];
+    int64_t sizes[4];
+    if (!r.get(wd, sizeof(wd)) || !r.get(sizes, sizeof(sizes))) return bad("truncated header");
+    auto md = std::make_shared<LieMetadata>();
+    md->width = wd[0];
+    md->depth = wd[1];
+    md->true_size = sizes[0];
+    md->tensor_dim = sizes[1];
+    if (md->width < 1 || md->depth < 0 || md->depth > kMaxDepth || md->true_size < 1) return bad("bad shape");
+    int64_t dim = 0, p = 1;
+    for (int k = 0; k <= md->depth; ++k, p *= md->width) {
+        if (p > (int64_t{1} << 40)) return bad("bad shape");
+        dim += p;
+    }
+    if (md->tensor_dim != dim) return bad("tensor dimension does not match (width, depth)");
+    const int64_t ts = md->true_size;
+    bool ok = r.vec(md->pairs, 2 * ts) && r.vec(md->degree_begin, md->depth + 2) &&
+              r.vec(md->l2t_csc_values, sizes[2]) && r.vec(md->l2t_csc_indices, sizes[2]) &&
+              r.vec(md->l2t_csc_offsets, ts) && r.vec(md->t2l_csc_values, sizes[3]) &&
+              r.vec(md->t2l_csc_indices, sizes[3]) && r.vec(md->t2l_csc_offsets, dim + 1);
+    if (!ok || r.pos != r.bytes.size()) return bad("section sizes do not match the header");
+    if (md->degree_begin.front() != 0 || md->degree_begin.back() != ts) return bad("degree offsets inconsistent");
+    for (size_t k = 1; k < md->degree_begin.size(); ++k)
+        if (md->degree_begin[k] < md->degree_begin[k - 1]) return bad("degree offsets decrease");
+    for (int64_t v : md->pairs)
+        if (v < 0 || v >= ts) return bad("Hall pair out of range");
+    if (!valid_compressed(md->l2t_csc_offsets, md->l2t_csc_indices, dim) ||
+        !valid_compressed(md->t2l_csc_offsets, md->t2l_csc_indices, std::max<int64_t>(ts - 1, 0)))
+        return bad("matrix structure invalid");
+    Csc l2t{md->l2t_csc_values, md->l2t_csc_indices, md->l2t_csc_offsets};
+    Csc t2l{md->t2l_csc_values, md->t2l_csc_indices, md->t2l_csc_offsets};
+    md->l2t = to_csr(l2t, dim);
+    md->t2l = to_csr(t2l, std::max<int64_t>(ts - 1, 0));
     return md;
+}
+
+namespace {
+
+std::mutex g_lie_mu;
+std::map<std::pair<int, int>, std::shared_ptr<const LieMetadata>> g_lie_cache;
+
+bool same_metadata(const LieMetadata& a, const LieMetadata& b) {
+    return a.width == b.width && a.depth == b.depth && a.pairs == b.pairs && a.degree_begin == b.degree_begin &&
+           a.l2t_csc_values == b.l2t_csc_values && a.l2t_csc_indices == b.l2t_csc_indices &&
+           a.l2t_csc_offsets == b.l2t_csc_offsets && a.t2l_csc_values == b.t2l_csc_values &&
+           a.t2l_csc_indices == b.t2l_csc_indices && a.t2l_csc_offsets == b.t2l_csc_offsets;
+}
+
+// The cache file of (width, depth) under RPP_HIP_LIE_CACHE_DIR, or "" when
+// the variable is unset.
+std::string cache_file(int width, int depth) {
+    const char* dir = std::getenv("RPP_HIP_LIE_CACHE_DIR");
+    if (!dir || !*dir) return {};
+    return std::string(dir) + "/rpp_lie_w" + std::to_string(width) + "_d" + std::to_string(depth) + ".bin";
+}
+
+}  // namespace
+
+std::shared_ptr<const LieMetadata> lie_metadata(int width, int depth) {
+    std::lock_guard<std::mutex> lock(g_lie_mu);
+    auto key = std::make_pair(width, depth);
+    auto it = g_lie_cache.find(key);
+    if (it != g_lie_cache.end()) return it->second;
+    std::shared_ptr<const LieMetadata> md;
+    const std::string file = cache_file(width, depth);
+    if (!file.empty()) {
+        md = load_lie_metadata(file, nullptr);
+        if (md && (md->width != width || md->depth != depth)) md = nullptr;
+    }
+    if (!md) {
+        md = build_lie_metadata(width, depth);
+        // a missing, stale or damaged cache file is rewritten; failure to
+        // write only loses the cache
+        if (!file.empty()) save_lie_metadata(*md, file, nullptr);
+    }
+    g_lie_cache.emplace(key, md);
+    return md;
+}
+
+int install_lie_metadata(std::shared_ptr<const LieMetadata> md, std::string* err) {
+    std::lock_guard<std::mutex> lock(g_lie_mu);
+    auto key = std::make_pair(md->width, md->depth);
+    auto it = g_lie_cache.find(key);
+    if (it == g_lie_cache.end()) {
+        g_lie_cache.emplace(key, std::move(md));
+        return 0;
+    }
+    if (same_metadata(*it->second, *md)) return 0;
+    if (err) *err = "different Lie metadata for this (width, depth) is already in use in this process";
+    return RPP_HIP_CONTRACT_VIOLATION;
 }
 
 namespace {

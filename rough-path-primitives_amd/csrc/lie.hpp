@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "rpp_hip.h"
@@ -35,7 +36,20 @@ struct LieMetadata {
 };
 
 std::shared_ptr<const LieMetadata> build_lie_metadata(int width, int depth);
+// The process-wide metadata of (width, depth): built on first use, or read
+// from RPP_HIP_LIE_CACHE_DIR when that names a directory holding a valid file
+// for it (a missing or invalid file is rebuilt and written there).
 std::shared_ptr<const LieMetadata> lie_metadata(int width, int depth);
+
+// The on-disk form (lie.cpp): save writes atomically (temporary file, then
+// rename); load validates the checksum, the sizes and every index range and
+// rebuilds the CSR forms, returning nullptr with a message on failure.
+bool save_lie_metadata(const LieMetadata& md, const std::string& path, std::string* err);
+std::shared_ptr<const LieMetadata> load_lie_metadata(const std::string& path, std::string* err);
+// Makes md the process-wide metadata of its (width, depth); an identical one
+// already in use is accepted, a different one is refused (the device copies
+// may already be built from it).
+int install_lie_metadata(std::shared_ptr<const LieMetadata> md, std::string* err);
 
 struct DeviceCsr {
     void* values = nullptr;

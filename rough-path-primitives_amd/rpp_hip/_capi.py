@@ -170,7 +170,8 @@ ENTRY_POINTS = {
     "rpp_hip_signature_backward": SignatureBackwardDesc,
 }
 
-OTHER_SYMBOLS = ["rpp_hip_lie_basis_sizes", "rpp_hip_lie_basis_copy", "rpp_hip_last_error",
+OTHER_SYMBOLS = ["rpp_hip_lie_basis_sizes", "rpp_hip_lie_basis_copy", "rpp_hip_lie_basis_save",
+                 "rpp_hip_lie_basis_load", "rpp_hip_last_error",
                  "rpp_hip_version", "rpp_hip_signature_backward_scratch_bytes", "rpp_hip_signature_tiles",
                  "rpp_hip_fmexp_tile"]
 
@@ -202,6 +203,8 @@ def lib() -> C.CDLL:
         lib_.rpp_hip_version.restype = C.c_char_p
         lib_.rpp_hip_lie_basis_sizes.argtypes = [_i32, _i32, _vp]
         lib_.rpp_hip_lie_basis_copy.argtypes = [_i32, _i32] + [_vp] * 8
+        lib_.rpp_hip_lie_basis_save.argtypes = [_i32, _i32, C.c_char_p]
+        lib_.rpp_hip_lie_basis_load.argtypes = [C.c_char_p, _vp, _vp]
         lib_.rpp_hip_signature_backward_scratch_bytes.argtypes = [C.POINTER(SignatureBackwardDesc), _i32]
         lib_.rpp_hip_signature_backward_scratch_bytes.restype = _i64
         lib_.rpp_hip_signature_tiles.argtypes = [_i32, _vp, _i32]

@@ -460,6 +460,26 @@ def lie_basis(width: int, depth: int) -> dict:
                 t2l=dict(data=t2l[0], indices=t2l[1], offsets=t2l[2]))
 
 
+def lie_basis_save(width: int, depth: int, path) -> None:
+    """Write the Hall basis metadata of (width, depth) to ``path`` (the
+    library's checksummed binary format)."""
+    lib = A.lib()
+    rc = lib.rpp_hip_lie_basis_save(width, depth, str(path).encode())
+    if rc != 0:
+        raise RppHipError(rc, lib.rpp_hip_last_error().decode())
+
+
+def lie_basis_load(path) -> tuple:
+    """Read a file written by :func:`lie_basis_save` and make it this process's
+    metadata for its (width, depth); returns (width, depth)."""
+    lib = A.lib()
+    w, d = C.c_int32(0), C.c_int32(0)
+    rc = lib.rpp_hip_lie_basis_load(str(path).encode(), C.byref(w), C.byref(d))
+    if rc != 0:
+        raise RppHipError(rc, lib.rpp_hip_last_error().decode())
+    return w.value, d.value
+
+
 # -------------------------------------------------- batched Chen recursion
 
 def fmexp_tile(width: int, depth: int, dtype):
