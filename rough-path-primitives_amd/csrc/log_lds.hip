@@ -1,14 +1,19 @@
-// ft_log with the object resident in LDS (operations/intermediate/ft_log.hpp:66-85).
+// ft_log with the object's lower degrees resident in LDS
+// (operations/intermediate/ft_log.hpp:66-85).
 //
-// The reference's Horner scheme: out = 0; for d = N..1: out += c_d 1, then
-// out <- out (x) Y on degrees [0, N-d+1], c_d = (-1)^(d+1) / d, Y = arg on
-// degrees >= 1.  Here one work-group holds Y and out for one object in LDS
-// (2 dim values) and each step computes every new coefficient it owns into
-// registers from the old out and Y, then writes them back: two barriers per
-// step instead of one per degree, and HBM sees one read of arg and one write
-// of out per object.  The shape is fixed at compile time, so every degree
-// offset and division by width^j is a constant; the split order (ascending j)
-// is that of the product kernels.
+// The reference's Horner scheme: P = 0; for d = N..1: P += c_d 1, then
+// P <- P (x) Y on degrees [0, N-d+1], c_d = (-1)^(d+1) / d, Y = arg on
+// degrees >= 1; out = P.  Only the last step (d = 1) reaches degree N, and
+// there the top degrees of both factors enter element-wise: out_N[w] =
+// sum_{j<N} P_{N-j}[w / d^j] Y_j[w mod d^j] + P_0 Y_N[w], and nothing reads
+// out_N afterwards.  So one work-group keeps only degrees 0..N-1 of Y and of
+// P in LDS (2 start_of_degree(N) values: about 2/width of the object), reads
+// Y_N from HBM at the last step and writes every degree of that step to HBM
+// directly.  Each earlier step computes its new coefficients into registers
+// from the old P and Y, then writes them back: two barriers per step.  HBM
+// sees one read of arg and one write of out per object.  The shape is fixed
+// at compile time, so every degree offset and division by width^j is a
+// constant; the split order (ascending j) is that of the product kernels.
 #include "common.hpp"
 #include "log_lds.hpp"
 
@@ -33,6 +38,7 @@ struct LogGeo {
         return s;
     }
     static constexpr int dim = start(N + 1);
+    static constexpr int low = start(N);  // degrees 0..N-1, the LDS-resident part
     static constexpr int slots(int k, int nt) { return (words(k) + nt - 1) / nt; }
     static constexpr int reg_offset(int k, int nt) {
         int o = 0;
@@ -41,12 +47,23 @@ struct LogGeo {
     }
 };
 
-// 1024 threads unless one object's coefficients would then need more than 8
-// registers per thread (the 128-register budget of a 1024-thread group
-// would spill); 512 threads get 256 registers each.
+// LDS of one work-group: Y and P on degrees 0..N-1.
+template <typename T, int W, int N>
+constexpr size_t log_lds_bytes() {
+    return 2 * static_cast<size_t>(LogGeo<W, N>::low) * sizeof(T);
+}
+
+// At most 80 KiB, so that two work-groups share a compute unit and one's
+// barrier-separated steps overlap the other's.
+constexpr size_t kLogLdsCap = 80 * 1024;
+
+// 1024 threads for objects of more than 4096 coefficients, 512 below.  At
+// width 3, depth 8 in FP64 (16384 objects) 1024 threads measured 1.20 ms with
+// 97 registers and no spills; 512 threads needed 141 registers, so one
+// work-group per compute unit (1.77 ms), or spilled when held to two.
 template <typename T, int W, int N>
 constexpr int log_threads() {
-    return LogGeo<W, N>::dim * sizeof(T) > 8 * 1024 * 8 ? 512 : 1024;
+    return LogGeo<W, N>::dim > 4096 ? 1024 : 512;
 }
 
 template <typename T, int W, int N, int NT>
@@ -55,29 +72,30 @@ k_log_lds(View<T> out, View<const T> arg, int64_t n) {
     using G = LogGeo<W, N>;
     extern __shared__ __align__(16) unsigned char log_raw[];
     T* ys = reinterpret_cast<T*>(log_raw);
-    T* os = ys + G::dim;
+    T* ps = ys + G::low;
     const int t = threadIdx.x;
     const int ylo = max(arg.mn, 1), yhi = min(arg.mx, N);
-    T reg[G::reg_offset(N + 1, NT) > 0 ? G::reg_offset(N + 1, NT) : 1];
+    T reg[G::reg_offset(N, NT) > 0 ? G::reg_offset(N, NT) : 1];
     for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
         const T* a = arg.obj(b);
         __syncthreads();  // the previous object's readers are done
-        for (int i = t; i < G::dim; i += NT) {
+        for (int i = t; i < G::low; i += NT) {
             int k = 0;
 #pragma unroll
-            for (int j = 1; j <= N; ++j)
+            for (int j = 1; j < N; ++j)
                 if (i >= G::start(j)) k = j;
             ys[i] = (k >= ylo && k <= yhi) ? a[i] : T(0);
-            os[i] = T(0);
+            ps[i] = T(0);
         }
-        if (t == 0) os[0] = (N % 2 == 0 ? T(-1) : T(1)) / T(N);
+        if (t == 0) ps[0] = (N % 2 == 0 ? T(-1) : T(1)) / T(N);
         __syncthreads();
+        // steps d = N..2 reach degree K = N - d + 1 <= N - 1
 #pragma unroll 1
-        for (int d = N; d >= 1; --d) {
+        for (int d = N; d >= 2; --d) {
             const int K = N - d + 1;
-            // new out_k = sum_{j=1..k} P_{k-j} (x) Y_j for 1 <= k <= K, P = out
+            // new P_k = sum_{j=1..k} P_{k-j} (x) Y_j for 1 <= k <= K
 #pragma unroll
-            for (int k = 1; k <= N; ++k) {
+            for (int k = 1; k < N; ++k) {
                 if (k > K) continue;
 #pragma unroll
                 for (int s = 0; s < G::slots(k, NT); ++s) {
@@ -88,7 +106,7 @@ k_log_lds(View<T> out, View<const T> arg, int64_t n) {
                         for (int j = 1; j <= k; ++j) {
                             if (j >= ylo && j <= yhi) {
                                 const int p = w / G::words(j);
-                                acc += os[G::start(k - j) + p] * ys[G::start(j) + (w - p * G::words(j))];
+                                acc += ps[G::start(k - j) + p] * ys[G::start(j) + (w - p * G::words(j))];
                             }
                         }
                         reg[G::reg_offset(k, NT) + s] = acc;
@@ -100,26 +118,56 @@ k_log_lds(View<T> out, View<const T> arg, int64_t n) {
             }
             __syncthreads();
 #pragma unroll
-            for (int k = 1; k <= N; ++k) {
+            for (int k = 1; k < N; ++k) {
                 if (k > K) continue;
 #pragma unroll
                 for (int s = 0; s < G::slots(k, NT); ++s) {
                     const int w = t + s * NT;
-                    if (G::words(k) % NT == 0 || w < G::words(k)) os[G::start(k) + w] = reg[G::reg_offset(k, NT) + s];
+                    if (G::words(k) % NT == 0 || w < G::words(k)) ps[G::start(k) + w] = reg[G::reg_offset(k, NT) + s];
                 }
             }
             // the next step's identity term (degree 0 of the product is 0)
-            if (t == 0) os[0] = d > 1 ? ((d - 1) % 2 == 0 ? T(-1) : T(1)) / T(d - 1) : T(0);
+            if (t == 0) ps[0] = ((d - 1) % 2 == 0 ? T(-1) : T(1)) / T(d - 1);
             __syncthreads();
         }
+        // the last step (d = 1, P_0 = c_1 = 1) straight to HBM; degree 0 of
+        // the product is 0
         T* o = out.obj(b);
-        for (int i = t; i < G::dim; i += NT) o[i] = os[i];
+        if (t == 0) o[0] = T(0);
+#pragma unroll
+        for (int k = 1; k < N; ++k) {
+            for (int w = t; w < G::words(k); w += NT) {
+                T acc = T(0);
+#pragma unroll
+                for (int j = 1; j <= k; ++j) {
+                    if (j >= ylo && j <= yhi) {
+                        const int p = w / G::words(j);
+                        acc += ps[G::start(k - j) + p] * ys[G::start(j) + (w - p * G::words(j))];
+                    }
+                }
+                o[G::start(k) + w] = acc;
+            }
+        }
+        const T p0 = ps[0];
+        const bool top = N >= ylo && N <= yhi;
+        for (int w = t; w < G::words(N); w += NT) {
+            T acc = T(0);
+#pragma unroll
+            for (int j = 1; j < N; ++j) {
+                if (j >= ylo && j <= yhi) {
+                    const int p = w / G::words(j);
+                    acc += ps[G::start(N - j) + p] * ys[G::start(j) + (w - p * G::words(j))];
+                }
+            }
+            if (top) acc += p0 * a[G::start(N) + w];
+            o[G::start(N) + w] = acc;
+        }
     }
 }
 
 template <typename T, int W, int N>
 int log_launch(int64_t n, const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& arg, hipStream_t stream) {
-    constexpr size_t lds = 2 * static_cast<size_t>(LogGeo<W, N>::dim) * sizeof(T);
+    constexpr size_t lds = log_lds_bytes<T, W, N>();
     constexpr int NT = log_threads<T, W, N>();
     static const bool raised = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_log_lds<T, W, N, NT>),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -136,14 +184,11 @@ using LogFn = int (*)(int64_t, const rpp_hip_tensor_batch&, const rpp_hip_tensor
 
 template <typename T>
 LogFn<T> log_kernel(int width, int depth) {
-// Only shapes whose object fits in 64 KiB of LDS: with two or more work-groups
-// per compute unit the barrier-separated steps of one object overlap another's;
-// one work-group per unit (width 3, depth 8 in FP64: 157 KiB) measured slower
-// than the per-degree kernel.
 #define RPP_LOG(W, N) \
-    if (width == W && depth == N && 2 * LogGeo<W, N>::dim * sizeof(T) <= 64 * 1024) return &log_launch<T, W, N>;
-    RPP_LOG(2, 8) RPP_LOG(2, 10) RPP_LOG(3, 6) RPP_LOG(3, 7) RPP_LOG(3, 8) RPP_LOG(4, 5) RPP_LOG(4, 6)
-    RPP_LOG(5, 4) RPP_LOG(5, 5) RPP_LOG(6, 4) RPP_LOG(8, 4)
+    if (width == W && depth == N && log_lds_bytes<T, W, N>() <= kLogLdsCap) return &log_launch<T, W, N>;
+    RPP_LOG(2, 8) RPP_LOG(2, 10) RPP_LOG(2, 12) RPP_LOG(3, 6) RPP_LOG(3, 7) RPP_LOG(3, 8) RPP_LOG(4, 5)
+    RPP_LOG(4, 6) RPP_LOG(4, 7) RPP_LOG(5, 4) RPP_LOG(5, 5) RPP_LOG(5, 6) RPP_LOG(6, 4) RPP_LOG(6, 5)
+    RPP_LOG(8, 4) RPP_LOG(8, 5)
 #undef RPP_LOG
     return nullptr;
 }

@@ -311,8 +311,8 @@ def test_antipode_reflect(dev, op, shape, windows, dtype):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
-@pytest.mark.parametrize("shape", [(2, 8), (2, 10), (3, 6), (3, 7), (3, 8), (4, 5), (4, 6), (5, 4), (5, 5), (6, 4),
-                                   (8, 4)])
+@pytest.mark.parametrize("shape", [(2, 8), (2, 10), (2, 12), (3, 6), (3, 7), (3, 8), (4, 5), (4, 6), (4, 7), (5, 4),
+                                   (5, 5), (5, 6), (6, 4), (6, 5), (8, 4), (8, 5)])
 @pytest.mark.parametrize("aw", [(0, 99), (1, 99), (2, 99), (1, 3)], ids=["a0N", "a1N", "a2N", "a13"])
 def test_ft_log_lds_shapes(dev, shape, aw, dtype):
     """ft_log at the shapes with an LDS-resident kernel (full out window),
