@@ -31,7 +31,14 @@
 #include <rpp/operations/basic/ft_inplace_mul.hpp>
 #include <rpp/operations/basic/ft_mul.hpp>
 #include <rpp/basis/lie_basis.hpp>
+#include <rpp/operations/basic/left_hs_adj_lmul.hpp>
+#include <rpp/operations/basic/left_hs_adj_rmul.hpp>
+#include <rpp/operations/basic/left_hs_mul.hpp>
 #include <rpp/operations/basic/lie_to_tensor.hpp>
+#include <rpp/operations/basic/st_adj_mul.hpp>
+#include <rpp/operations/basic/st_fma.hpp>
+#include <rpp/operations/basic/st_inplace_fma.hpp>
+#include <rpp/operations/basic/st_mul.hpp>
 #include <rpp/operations/basic/tensor_add_identity.hpp>
 #include <rpp/operations/basic/tensor_set_identity.hpp>
 #include <rpp/operations/basic/tensor_antipode.hpp>
@@ -424,6 +431,96 @@ public:
         return SpMV::hip_launch(cfg, b, bases, n, matrix);
     }
 };
+
+// The shuffle family (operations/basic/st_*.hpp, left_hs_*.hpp).
+template <typename Accum_, typename Arch_>
+class STMul<hip::strategies::BlockStrategy<Accum_, Arch_>>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases, int64_t n,
+                                         Accum beta = Accum{1}) noexcept {
+        using namespace hip::detail;
+        rpp_hip_ft_mul_desc d{tensor_basis_of(bases), n, tensor_of<Accum>(std::get<0>(b)),
+                              tensor_of<Accum>(std::get<1>(b)), tensor_of<Accum>(std::get<2>(b)),
+                              static_cast<double>(beta)};
+        return to_error(RPP_HIP_CALL(rpp_hip_st_mul, d, cfg.stream));
+    }
+};
+
+template <typename Accum_, typename Arch_>
+class STFma<hip::strategies::BlockStrategy<Accum_, Arch_>>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases, int64_t n,
+                                         Accum alpha = Accum{1}, Accum beta = Accum{1}) noexcept {
+        using namespace hip::detail;
+        rpp_hip_ft_fma_desc d{tensor_basis_of(bases), n, tensor_of<Accum>(std::get<0>(b)),
+                              tensor_of<Accum>(std::get<1>(b)), tensor_of<Accum>(std::get<2>(b)),
+                              tensor_of<Accum>(std::get<3>(b)), static_cast<double>(alpha),
+                              static_cast<double>(beta)};
+        return to_error(RPP_HIP_CALL(rpp_hip_st_fma, d, cfg.stream));
+    }
+};
+
+template <typename Accum_, typename Arch_>
+class STInplaceFma<hip::strategies::BlockStrategy<Accum_, Arch_>>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases, int64_t n,
+                                         Accum alpha = Accum{1}, Accum beta = Accum{1}) noexcept {
+        using namespace hip::detail;
+        rpp_hip_st_inplace_fma_desc d{tensor_basis_of(bases), n, tensor_of<Accum>(std::get<0>(b)),
+                                      tensor_of<Accum>(std::get<1>(b)), tensor_of<Accum>(std::get<2>(b)),
+                                      static_cast<double>(alpha), static_cast<double>(beta)};
+        return to_error(RPP_HIP_CALL(rpp_hip_st_inplace_fma, d, cfg.stream));
+    }
+};
+
+template <typename Accum_, typename Arch_>
+class STAdjMul<hip::strategies::BlockStrategy<Accum_, Arch_>>
+    : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
+public:
+    RPP_HIP_OP_COMMON;
+    template <typename Batches, typename Bases>
+    static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases,
+                                         int64_t n) noexcept {
+        using namespace hip::detail;
+        rpp_hip_ft_adjoint_desc d{tensor_basis_of(bases), n, tensor_of<Accum>(std::get<0>(b)),
+                                  tensor_of<Accum>(std::get<1>(b)), tensor_of<Accum>(std::get<2>(b))};
+        return to_error(RPP_HIP_CALL(rpp_hip_st_adj_mul, d, cfg.stream));
+    }
+};
+
+// left_hs_mul, left_hs_adj_lmul, left_hs_adj_rmul: (out, integrator,
+// integrand, beta); the adjoints scale by beta as the reference's GPU
+// operators do.
+#define RPP_HIP_HALF_SHUFFLE_OP(CLASS, ENTRY)                                                               \
+    template <typename Accum_, typename Arch_>                                                              \
+    class CLASS<hip::strategies::BlockStrategy<Accum_, Arch_>>                                              \
+        : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {                             \
+    public:                                                                                                 \
+        RPP_HIP_OP_COMMON;                                                                                  \
+        template <typename Batches, typename Bases>                                                         \
+        static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases,        \
+                                             int64_t n, Accum beta = Accum{1}) noexcept {                   \
+            using namespace hip::detail;                                                                    \
+            rpp_hip_half_shuffle_desc d{tensor_basis_of(bases), n, tensor_of<Accum>(std::get<0>(b)),        \
+                                        tensor_of<Accum>(std::get<1>(b)), tensor_of<Accum>(std::get<2>(b)), \
+                                        static_cast<double>(beta)};                                         \
+            return to_error(RPP_HIP_CALL(ENTRY, d, cfg.stream));                                            \
+        }                                                                                                   \
+    };
+
+RPP_HIP_HALF_SHUFFLE_OP(LeftHSMul, rpp_hip_left_hs_mul)
+RPP_HIP_HALF_SHUFFLE_OP(LeftHSAdjLMul, rpp_hip_left_hs_adj_lmul)
+RPP_HIP_HALF_SHUFFLE_OP(LeftHSAdjRMul, rpp_hip_left_hs_adj_rmul)
+#undef RPP_HIP_HALF_SHUFFLE_OP
 
 namespace hip_detail {
 

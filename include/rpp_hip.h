@@ -289,6 +289,67 @@ int rpp_hip_tensor_set_identity_f64(const rpp_hip_vector_scalar_desc* desc, void
 int rpp_hip_tensor_add_identity_f32(const rpp_hip_vector_scalar_desc* desc, void* stream);
 int rpp_hip_tensor_add_identity_f64(const rpp_hip_vector_scalar_desc* desc, void* stream);
 
+/* ---------------------------------------------------- the shuffle family */
+
+/* Shuffle products on the dual (functional) side.  For a word w of degree k,
+ * (b . c)[w] = sum over the subsets S of w's positions of b[w|S] c[w|S^c],
+ * each factor read only on its window; the left half-shuffle (x < y)[a w] =
+ * sum over S of x[a w|S] y[w|S^c] fixes the first letter on x's side.
+ * Depth at most 16 (2^k subsets per coefficient); outputs must not share
+ * memory with inputs.
+ *   rpp_hip_st_mul: rpp::ops::st_mul (operations/basic/st_mul.hpp:48-83,
+ *     cpu/.../basic/st_mul.hpp:37-66): out = beta lhs . rhs.
+ *   rpp_hip_st_fma: rpp::ops::st_fma (operations/basic/st_fma.hpp:50-89,
+ *     cpu/.../st_fma.hpp:41-79): out = alpha a + beta b . c, a read only on
+ *     its window; out may be a itself (same data and strides). */
+int rpp_hip_st_mul_f32(const rpp_hip_ft_mul_desc* desc, void* stream);
+int rpp_hip_st_mul_f64(const rpp_hip_ft_mul_desc* desc, void* stream);
+int rpp_hip_st_fma_f32(const rpp_hip_ft_fma_desc* desc, void* stream);
+int rpp_hip_st_fma_f64(const rpp_hip_ft_fma_desc* desc, void* stream);
+
+/* rpp::ops::st_inplace_fma (operations/basic/st_inplace_fma.hpp:45-85,
+ * cpu/.../st_inplace_fma.hpp:38-68): a <- alpha a + beta b . c on a's
+ * window; b and c must not share memory with a. */
+typedef struct rpp_hip_st_inplace_fma_desc {
+    rpp_hip_basis basis;
+    int64_t num_batches;
+    rpp_hip_tensor_batch a, b, c;
+    double alpha, beta;
+} rpp_hip_st_inplace_fma_desc;
+int rpp_hip_st_inplace_fma_f32(const rpp_hip_st_inplace_fma_desc* desc, void* stream);
+int rpp_hip_st_inplace_fma_f64(const rpp_hip_st_inplace_fma_desc* desc, void* stream);
+
+/* rpp::ops::st_adj_mul (operations/basic/st_adj_mul.hpp:40-73,
+ * cpu/.../st_adj_mul.hpp:43-82): the adjoint of shuffle multiplication by
+ * op, <out, x> = <arg, op . x>, on out's window. */
+int rpp_hip_st_adj_mul_f32(const rpp_hip_ft_adjoint_desc* desc, void* stream);
+int rpp_hip_st_adj_mul_f64(const rpp_hip_ft_adjoint_desc* desc, void* stream);
+
+/* The left half-shuffle and its adjoints:
+ *   rpp_hip_left_hs_mul: rpp::ops::left_hs_mul (operations/basic/
+ *     left_hs_mul.hpp:53-93, cpu/.../left_hs_mul.hpp:37-84):
+ *     out = beta integrator < integrand; degree 0 of out is 0.
+ *   rpp_hip_left_hs_adj_lmul: rpp::ops::left_hs_adj_lmul (operations/basic/
+ *     left_hs_adj_lmul.hpp:33-61, gpu/block/operations/basic/
+ *     left_hs_adj_lmul.hpp:114-177): <out, z> = beta <integrand, integrator < z>.
+ *   rpp_hip_left_hs_adj_rmul: rpp::ops::left_hs_adj_rmul (left_hs_adj_rmul.hpp
+ *     :33-61, gpu/.../left_hs_adj_rmul.hpp:77-143): <out, z> =
+ *     beta <integrator, z < integrand>; degree 0 of out is 0.
+ * beta scales the adjoints as the reference's GPU operators do (its CPU
+ * operators compute beta = 1). */
+typedef struct rpp_hip_half_shuffle_desc {
+    rpp_hip_basis basis;
+    int64_t num_batches;
+    rpp_hip_tensor_batch out, integrator, integrand;
+    double beta;
+} rpp_hip_half_shuffle_desc;
+int rpp_hip_left_hs_mul_f32(const rpp_hip_half_shuffle_desc* desc, void* stream);
+int rpp_hip_left_hs_mul_f64(const rpp_hip_half_shuffle_desc* desc, void* stream);
+int rpp_hip_left_hs_adj_lmul_f32(const rpp_hip_half_shuffle_desc* desc, void* stream);
+int rpp_hip_left_hs_adj_lmul_f64(const rpp_hip_half_shuffle_desc* desc, void* stream);
+int rpp_hip_left_hs_adj_rmul_f32(const rpp_hip_half_shuffle_desc* desc, void* stream);
+int rpp_hip_left_hs_adj_rmul_f64(const rpp_hip_half_shuffle_desc* desc, void* stream);
+
 /* --------------------------------------------------------- sparse and Lie */
 
 enum rpp_hip_matrix_format { RPP_HIP_CSR = 0, RPP_HIP_CSC = 1 };

@@ -233,6 +233,52 @@ class _Impl:
     def ft_adj_rmul(self, width, depth, out, op, arg, ow=None, pw=None, gw=None, n=None):
         return self._adj("ft_adj_rmul", width, depth, out, op, arg, ow, pw, gw, n)
 
+    # Shuffle family.  kind 0 st_mul(out, b, c), 1 st_fma(out, a, b, c),
+    # 2 st_inplace_fma(a, b, c) with out == a.
+    def _st(self, kind, width, depth, out, a, b, c, ow, aw, bw, cw, alpha, beta, n):
+        sfx, ct = _sfx(out.dtype)
+        n = out.shape[0] if n is None else n
+        f = _full(depth)
+        a = out if a is None else a
+        _call(self._lib(), self._name("st_fma", sfx), _i(kind), _i(width), _i(depth), _ll(n),
+              *_arg(out), *_win(ow or f), *_arg(a), *_win(aw or f), *_arg(b), *_win(bw or f),
+              *_arg(c), *_win(cw or f), ct(alpha), ct(beta))
+        return out
+
+    def st_mul(self, width, depth, out, lhs, rhs, ow=None, lw=None, rw=None, beta=1.0, n=None):
+        return self._st(0, width, depth, out, None, lhs, rhs, ow, None, lw, rw, 0.0, beta, n)
+
+    def st_fma(self, width, depth, out, a, b, c, ow=None, aw=None, bw=None, cw=None, alpha=1.0, beta=1.0, n=None):
+        return self._st(1, width, depth, out, a, b, c, ow, aw, bw, cw, alpha, beta, n)
+
+    def st_inplace_fma(self, width, depth, a, b, c, aw=None, bw=None, cw=None, alpha=1.0, beta=1.0, n=None):
+        return self._st(2, width, depth, a, None, b, c, aw, aw, bw, cw, alpha, beta, n)
+
+    # opcode 0 st_adj_mul(out, op, arg), 1 left_hs_mul(out, integrator,
+    # integrand, beta), 2 left_hs_adj_lmul(out, integrator, integrand),
+    # 3 left_hs_adj_rmul(out, integrator, integrand)
+    def _sh2(self, opcode, width, depth, out, p, q, ow, pw, qw, beta, n):
+        sfx, ct = _sfx(out.dtype)
+        n = out.shape[0] if n is None else n
+        f = _full(depth)
+        _call(self._lib(), self._name("shuffle2", sfx), _i(opcode), _i(width), _i(depth), _ll(n),
+              *_arg(out), *_win(ow or f), *_arg(p), *_win(pw or f), *_arg(q), *_win(qw or f), ct(beta))
+        return out
+
+    def st_adj_mul(self, width, depth, out, op, arg, ow=None, pw=None, gw=None, n=None):
+        return self._sh2(0, width, depth, out, op, arg, ow, pw, gw, 1.0, n)
+
+    def left_hs_mul(self, width, depth, out, integrator, integrand, ow=None, xw=None, yw=None, beta=1.0, n=None):
+        return self._sh2(1, width, depth, out, integrator, integrand, ow, xw, yw, beta, n)
+
+    def left_hs_adj_lmul(self, width, depth, out, integrator, integrand, ow=None, xw=None, yw=None, beta=1.0,
+                         n=None):
+        return self._sh2(2, width, depth, out, integrator, integrand, ow, xw, yw, beta, n)
+
+    def left_hs_adj_rmul(self, width, depth, out, integrator, integrand, ow=None, xw=None, yw=None, beta=1.0,
+                         n=None):
+        return self._sh2(3, width, depth, out, integrator, integrand, ow, xw, yw, beta, n)
+
     # Linear operations over any grading: `start` = degree offsets
     # start_of_degree(k), k = 0..depth+1 (degree_begin or lie_degree_begin).
 

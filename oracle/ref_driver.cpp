@@ -274,6 +274,66 @@ int vector_scalar(int op, const long long* start, int N, long long n, T* vec, lo
     }
 }
 
+// Shuffle family, through the reference's free functions.  kind as the
+// oracle's orc_batch_st_fma: 0 st_mul, 1 st_fma, 2 st_inplace_fma (out == a).
+template <typename T>
+int st_fma(int kind, int w, int N, long long n, T* out, long long os, int omin, int omax, const T* a, long long as,
+           int amin, int amax, const T* b, long long bs, int bmin, int bmax, const T* c, long long cs, int cmin,
+           int cmax, T alpha, T beta) {
+    BasisHolder bh(w, N);
+    auto O = batch<T>(out, os, bh.basis, omin, omax);
+    auto Bb = batch<T>(b, bs, bh.basis, bmin, bmax);
+    auto Cb = batch<T>(c, cs, bh.basis, cmin, cmax);
+    const auto count = static_cast<Index>(n);
+    switch (kind) {
+        case 0: return code_of(rpp::ops::st_mul(Strat<T>{}, {}, O, Bb, Cb, bh.basis, count, beta));
+        case 1:
+            return code_of(rpp::ops::st_fma(Strat<T>{}, {}, O, batch<T>(a, as, bh.basis, amin, amax), Bb, Cb, bh.basis,
+                                            count, alpha, beta));
+        case 2: return code_of(rpp::ops::st_inplace_fma(Strat<T>{}, {}, O, Bb, Cb, bh.basis, count, alpha, beta));
+        default: return 3;
+    }
+}
+
+// opcode as the oracle's orc_batch_shuffle2: 0 st_adj_mul(out, op=p, arg=q),
+// 1 left_hs_mul(out, integrator=p, integrand=q, beta), 2 left_hs_adj_lmul,
+// 3 left_hs_adj_rmul (out, integrator=p, integrand=q).  The two half-shuffle
+// adjoints are applied through their operation objects, one object at a time:
+// their free functions forward a beta that the cpu::single_thread operators
+// do not take.
+template <typename T>
+int shuffle2(int opcode, int w, int N, long long n, T* out, long long os, int omin, int omax, const T* p,
+             long long ps, int pmin, int pmax, const T* q, long long qs, int qmin, int qmax, T beta) {
+    BasisHolder bh(w, N);
+    const auto count = static_cast<Index>(n);
+    auto O = batch<T>(out, os, bh.basis, omin, omax);
+    auto P = batch<T>(p, ps, bh.basis, pmin, pmax);
+    auto Q = batch<T>(q, qs, bh.basis, qmin, qmax);
+    switch (opcode) {
+        case 0: return code_of(rpp::ops::st_adj_mul(Strat<T>{}, {}, O, P, Q, bh.basis, count));
+        case 1: return code_of(rpp::ops::left_hs_mul(Strat<T>{}, {}, O, P, Q, bh.basis, count, beta));
+        case 2:
+        case 3: {
+            typename Strat<T>::Context ctx{};
+            auto pack = rpp::basis::make_basis_pack(bh.basis);
+            for (long long i = 0; i < n; ++i) {
+                rpp::DenseTensorView<T*, TBasis> ov(out + i * os, bh.basis, static_cast<Degree>(omin),
+                                                    static_cast<Degree>(omax));
+                rpp::DenseTensorView<const T*, TBasis> pv(p + i * ps, bh.basis, static_cast<Degree>(pmin),
+                                                          static_cast<Degree>(pmax));
+                rpp::DenseTensorView<const T*, TBasis> qv(q + i * qs, bh.basis, static_cast<Degree>(qmin),
+                                                          static_cast<Degree>(qmax));
+                if (opcode == 2)
+                    rpp::ops::LeftHSAdjLMul<Strat<T>>{ctx, pack}(ctx, ov, pv, qv);
+                else
+                    rpp::ops::LeftHSAdjRMul<Strat<T>>{ctx, pack}(ctx, ov, pv, qv);
+            }
+            return 0;
+        }
+        default: return 3;
+    }
+}
+
 // Chen recursion as a reference user writes it: one ft_fmexp per increment
 // over the whole batch, ping-pong buffers, exponent view (1,1) over a
 // width+1 buffer whose slot 0 is unused.  Paths share one length L;
@@ -399,6 +459,22 @@ extern "C" {
     int ref_signature_##SFX(int w, int N, long long n, long long L,            \
                             const T* incr, T* out, int threads) {              \
         return signature<T>(w, N, n, L, incr, out, threads);                   \
+    }                                                                          \
+    int ref_st_fma_##SFX(int kind, int w, int N, long long n, T* out,          \
+                         long long os, int omin, int omax, const T* a,         \
+                         long long as, int amin, int amax, const T* b,         \
+                         long long bs, int bmin, int bmax, const T* c,         \
+                         long long cs, int cmin, int cmax, T alpha, T beta) {  \
+        return st_fma<T>(kind, w, N, n, out, os, omin, omax, a, as, amin,      \
+                         amax, b, bs, bmin, bmax, c, cs, cmin, cmax, alpha,    \
+                         beta);                                                \
+    }                                                                          \
+    int ref_shuffle2_##SFX(int opcode, int w, int N, long long n, T* out,      \
+                           long long os, int omin, int omax, const T* p,       \
+                           long long ps, int pmin, int pmax, const T* q,       \
+                           long long qs, int qmin, int qmax, T beta) {         \
+        return shuffle2<T>(opcode, w, N, n, out, os, omin, omax, p, ps, pmin,  \
+                           pmax, q, qs, qmin, qmax, beta);                     \
     }                                                                          \
     int ref_vector_assign_##SFX(const long long* start, int N, long long n,    \
                                 T* out, long long os, int omin, int omax,      \

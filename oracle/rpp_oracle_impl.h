@@ -545,6 +545,250 @@ int FN(orc_batch_vector_scalar)(int op, const long long* start, int depth, long 
     return 0;
 }
 
+/* --- shuffle and half-shuffle products -------------------------------------
+ * A word of degree k is held as its letters lt[0..k-1], first letter first.
+ * A subset S of its positions is a mask whose bit (k-1-p) marks position p,
+ * and masks run in ascending order, as in
+ * operations/implementation/word_shuffle_product.hpp:49-93 with
+ * basis/tensor_basis.hpp:58-91.  `pre` >= 0 prepends that letter to the
+ * first factor's word (the left half-shuffle,
+ * word_half_shuffle_product.hpp:20-66). */
+
+static void FN(letters_of)(const orc_basis* B, int k, long long idx, int* lt)
+{
+    for (int p = k - 1; p >= 0; --p) {
+        lt[p] = (int)(idx % B->width);
+        idx /= B->width;
+    }
+}
+
+/* sum over S of l[(pre) w|S] r[w|S^c], each factor only on its window */
+static T FN(shuffle_word)(const orc_basis* B, const int* lt, int k, int pre, const T* l, int lmin, int lmax,
+                          const T* r, int rmin, int rmax)
+{
+    T acc = (T)0;
+    const int extra = pre >= 0 ? 1 : 0;
+    for (unsigned long long mask = 0; mask < (1ULL << k); ++mask) {
+        int ld = 0, rd = 0;
+        long long li = 0, ri = 0;
+        for (int p = 0; p < k; ++p) {
+            if ((mask >> (k - 1 - p)) & 1ULL) {
+                ++ld;
+                li = li * B->width + lt[p];
+            } else {
+                ++rd;
+                ri = ri * B->width + lt[p];
+            }
+        }
+        if (ld + extra < lmin || ld + extra > lmax || rd < rmin || rd > rmax) continue;
+        const long long lidx = pre >= 0 ? pre * B->pow[ld] + li : li;
+        acc = acc + l[B->start[ld + extra] + lidx] * r[B->start[rd] + ri];
+    }
+    return acc;
+}
+
+/* The adjoint sum of out word u (letters ul, degree m): over op degrees
+ * j in [jlo, jhi] and op words v of degree j, the sum over the masks of weight
+ * j on m+j positions (bit set: the next letter of v, else the next letter of
+ * u) of op[(pre) v] * arg[(pre) merge], each (j, v) summed on its own first
+ * (operations/implementation/word_shuffle_adjoint_multiply.hpp:22-95,
+ * shuffle_adjoint_op_loop.hpp:20-60). */
+static T FN(adjoint_word)(const orc_basis* B, const int* ul, int m, int pre, const T* op, const T* arg, int jlo,
+                          int jhi)
+{
+    T acc = (T)0;
+    int vl[ORC_MAX_DEPTH + 1];
+    const int extra = pre >= 0 ? 1 : 0;
+    for (int j = jlo; j <= jhi; ++j) {
+        const int L = m + j;
+        for (long long v = 0; v < B->pow[j]; ++v) {
+            FN(letters_of)(B, j, v, vl);
+            const T opv = op[B->start[j + extra] + (pre >= 0 ? pre * B->pow[j] : 0) + v];
+            T inner = (T)0;
+            for (unsigned long long mask = 0; mask < (1ULL << L); ++mask) {
+                if (__builtin_popcountll(mask) != j) continue;
+                int iu = 0, iv = 0;
+                long long idx = 0;
+                for (int p = 0; p < L; ++p) {
+                    const int letter = ((mask >> (L - 1 - p)) & 1ULL) ? vl[iv++] : ul[iu++];
+                    idx = idx * B->width + letter;
+                }
+                inner = inner + opv * arg[B->start[L + extra] + (pre >= 0 ? pre * B->pow[L] : 0) + idx];
+            }
+            acc = acc + inner;
+        }
+    }
+    return acc;
+}
+
+/* STMul / STFma / STInplaceFma (cpu/.../basic/st_mul.hpp:37-66,
+ * st_fma.hpp:41-79, st_inplace_fma.hpp:38-68).  kind 0: out = beta b.c;
+ * 1: out = alpha a + beta b.c; 2: a = alpha a + beta b.c (out == a). */
+static void FN(st_fma_obj)(const orc_basis* B, int kind, T* out, int omin, int omax, const T* a, int amin, int amax,
+                           const T* b, int bmin, int bmax, const T* c, int cmin, int cmax, T alpha, T beta)
+{
+    int lt[ORC_MAX_DEPTH + 1];
+    const int b0 = bmin <= 0 && 0 <= bmax, c0 = cmin <= 0 && 0 <= cmax, a0 = amin <= 0 && 0 <= amax;
+    if (omin == 0 && omax >= 0) {
+        T v = (T)0;
+        if (kind == 0) {
+            if (b0 && c0) v = beta * b[0] * c[0];
+        } else if (kind == 1) {
+            if (a0) v = v + alpha * a[0];
+            if (b0 && c0) v = v + beta * b[0] * c[0];
+        } else {
+            v = alpha * out[0];
+            if (b0 && c0) v = v + beta * b[0] * c[0];
+        }
+        out[0] = v;
+    }
+    for (int k = omin > 1 ? omin : 1; k <= omax; ++k) {
+        for (long long i = 0; i < B->pow[k]; ++i) {
+            FN(letters_of)(B, k, i, lt);
+            const T ws = FN(shuffle_word)(B, lt, k, -1, b, bmin, bmax, c, cmin, cmax);
+            T* o = out + B->start[k] + i;
+            if (kind == 0) {
+                *o = beta * ws;
+            } else if (kind == 1) {
+                T v = (T)0;
+                if (amin <= k && k <= amax) v = v + alpha * a[B->start[k] + i];
+                *o = v + beta * ws;
+            } else {
+                *o = alpha * *o + beta * ws;
+            }
+        }
+    }
+}
+
+/* STAdjMul (cpu/.../basic/st_adj_mul.hpp:43-82): <out, x> = <arg, op . x>. */
+static void FN(st_adj_mul_obj)(const orc_basis* B, T* out, int omin, int omax, const T* op, int pmin, int pmax,
+                               const T* arg, int gmin, int gmax)
+{
+    int ul[ORC_MAX_DEPTH + 1];
+    for (int m = omin; m <= omax; ++m) {
+        const int jlo = pmin > gmin - m ? pmin : gmin - m;
+        const int jhi = pmax < gmax - m ? pmax : gmax - m;
+        for (long long i = 0; i < B->pow[m]; ++i) {
+            FN(letters_of)(B, m, i, ul);
+            out[B->start[m] + i] = FN(adjoint_word)(B, ul, m, -1, op, arg, jlo, jhi);
+        }
+    }
+}
+
+/* LeftHSMul (cpu/.../basic/left_hs_mul.hpp:37-84): out[a w] = beta sum over
+ * S of x[a w|S] y[w|S^c]; degree 0 of out is 0. */
+static void FN(left_hs_mul_obj)(const orc_basis* B, T* out, int omin, int omax, const T* x, int xmin, int xmax,
+                                const T* y, int ymin, int ymax, T beta)
+{
+    int lt[ORC_MAX_DEPTH + 1];
+    if (omin == 0 && omax >= 0) out[0] = (T)0;
+    for (int k = omin > 1 ? omin : 1; k <= omax; ++k) {
+        for (long long i = 0; i < B->pow[k]; ++i) {
+            FN(letters_of)(B, k, i, lt);
+            out[B->start[k] + i] = beta * FN(shuffle_word)(B, lt + 1, k - 1, lt[0], x, xmin, xmax, y, ymin, ymax);
+        }
+    }
+}
+
+/* LeftHSAdjLMul (cpu/.../basic/left_hs_adj_lmul.hpp:43-118):
+ * <out, z> = <y, x < z>.  out[u] = sum over prefixes a of the adjoint sum
+ * with op x[a v] and arg y[a merge], scaled by beta as the reference's GPU
+ * operator does (gpu/block/operations/basic/left_hs_adj_lmul.hpp:114-177;
+ * the CPU operator has no beta, i.e. beta = 1). */
+static void FN(left_hs_adj_lmul_obj)(const orc_basis* B, T* out, int omin, int omax, const T* x, int xmin, int xmax,
+                                     const T* y, int ymin, int ymax, T beta)
+{
+    int ul[ORC_MAX_DEPTH + 1];
+    for (int m = omin; m <= omax; ++m) {
+        int jlo = xmin - 1 > ymin - 1 - m ? xmin - 1 : ymin - 1 - m;
+        if (jlo < 0) jlo = 0;
+        const int jhi = xmax - 1 < ymax - 1 - m ? xmax - 1 : ymax - 1 - m;
+        for (long long i = 0; i < B->pow[m]; ++i) {
+            FN(letters_of)(B, m, i, ul);
+            T acc = (T)0;
+            for (int a = 0; a < B->width; ++a) acc = acc + FN(adjoint_word)(B, ul, m, a, x, y, jlo, jhi);
+            out[B->start[m] + i] = beta * acc;
+        }
+    }
+}
+
+/* LeftHSAdjRMul (cpu/.../basic/left_hs_adj_rmul.hpp:37-103):
+ * <out, z> = <x, z < y>.  out[a u] = beta times the adjoint sum of u with op
+ * y[v] and arg x[a merge] (beta as for the left adjoint); degree 0 of out is
+ * 0. */
+static void FN(left_hs_adj_rmul_obj)(const orc_basis* B, T* out, int omin, int omax, const T* x, int xmin, int xmax,
+                                     const T* y, int ymin, int ymax, T beta)
+{
+    int ul[ORC_MAX_DEPTH + 1];
+    if (omin == 0 && omax >= 0) out[0] = (T)0;
+    for (int m = omin > 1 ? omin : 1; m <= omax; ++m) {
+        const int jlo = ymin > xmin - m ? ymin : xmin - m;
+        const int jhi = ymax < xmax - m ? ymax : xmax - m;
+        for (long long i = 0; i < B->pow[m]; ++i) {
+            FN(letters_of)(B, m, i, ul);
+            /* the prefix letter goes to the arg side; op = y carries none */
+            T acc = (T)0;
+            for (int j = jlo; j <= jhi; ++j) {
+                const int L = m - 1 + j;
+                int vl[ORC_MAX_DEPTH + 1];
+                for (long long v = 0; v < B->pow[j]; ++v) {
+                    FN(letters_of)(B, j, v, vl);
+                    const T opv = y[B->start[j] + v];
+                    T inner = (T)0;
+                    for (unsigned long long mask = 0; mask < (1ULL << L); ++mask) {
+                        if (__builtin_popcountll(mask) != j) continue;
+                        int iu = 1, iv = 0;
+                        long long idx = 0;
+                        for (int p = 0; p < L; ++p) {
+                            const int letter = ((mask >> (L - 1 - p)) & 1ULL) ? vl[iv++] : ul[iu++];
+                            idx = idx * B->width + letter;
+                        }
+                        inner = inner + opv * x[B->start[L + 1] + ul[0] * B->pow[L] + idx];
+                    }
+                    acc = acc + inner;
+                }
+            }
+            out[B->start[m] + i] = beta * acc;
+        }
+    }
+}
+
+/* batched forms; st ops: kind as st_fma_obj, out == a for kind 2 */
+int FN(orc_batch_st_fma)(int kind, int width, int depth, long long n, T* out, long long os, int omin, int omax,
+                         const T* a, long long as, int amin, int amax, const T* b, long long bs, int bmin, int bmax,
+                         const T* c, long long cs, int cmin, int cmax, T alpha, T beta)
+{
+    orc_basis B;
+    if (orc_basis_init(&B, width, depth)) return 3;
+    for (long long i = 0; i < n; ++i)
+        FN(st_fma_obj)(&B, kind, OBJ(out, os, i), omin, omax, a ? OBJ(a, as, i) : 0, amin, amax, OBJ(b, bs, i), bmin,
+                       bmax, OBJ(c, cs, i), cmin, cmax, alpha, beta);
+    return 0;
+}
+
+/* op 0: st_adj_mul(out, op=p, arg=q); 1: left_hs_mul(out, x=p, y=q, beta);
+ * 2: left_hs_adj_lmul(out, x=p, y=q, beta); 3: left_hs_adj_rmul(out, x=p, y=q, beta) */
+int FN(orc_batch_shuffle2)(int opcode, int width, int depth, long long n, T* out, long long os, int omin, int omax,
+                           const T* p, long long ps, int pmin, int pmax, const T* q, long long qs, int qmin, int qmax,
+                           T beta)
+{
+    orc_basis B;
+    if (orc_basis_init(&B, width, depth)) return 3;
+    for (long long i = 0; i < n; ++i) {
+        T* o = OBJ(out, os, i);
+        const T* P = OBJ(p, ps, i);
+        const T* Q = OBJ(q, qs, i);
+        switch (opcode) {
+            case 0: FN(st_adj_mul_obj)(&B, o, omin, omax, P, pmin, pmax, Q, qmin, qmax); break;
+            case 1: FN(left_hs_mul_obj)(&B, o, omin, omax, P, pmin, pmax, Q, qmin, qmax, beta); break;
+            case 2: FN(left_hs_adj_lmul_obj)(&B, o, omin, omax, P, pmin, pmax, Q, qmin, qmax, beta); break;
+            case 3: FN(left_hs_adj_rmul_obj)(&B, o, omin, omax, P, pmin, pmax, Q, qmin, qmax, beta); break;
+            default: return 3;
+        }
+    }
+    return 0;
+}
+
 int FN(orc_batch_tensor_pairing)(int width, int depth, long long n, T* out,
                                  long long ostride, const T* f, long long fs,
                                  int fmin, int fmax, const T* x, long long xs,

@@ -17,6 +17,7 @@
 #include "signature.hpp"
 #include "signature_reg.hpp"
 #include "signature_chain.hpp"
+#include "shuffle_ops.hpp"
 #include "vector_ops.hpp"
 
 namespace rpp_hip {
@@ -302,6 +303,76 @@ int lie_convert_impl(bool to_lie, const rpp_hip_lie_convert_desc* d, void* s) {
                   : launch_spmv<T>(d->num_batches, tv, lv, M, 1.0, as_stream(s));
 }
 
+// ------------------------------------------------------- the shuffle family
+
+int check_shuffle_basis(const rpp_hip_basis& b) {
+    RPP_TRY(check_basis(b));
+    if (b.depth > kShuffleMaxDepth)
+        return fail(RPP_HIP_NOT_IMPLEMENTED, "shuffle products are limited to depth 16");
+    return 0;
+}
+
+bool overlaps(const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& in) {
+    return out.data == in.data && out.min_degree <= out.max_degree && in.min_degree <= in.max_degree;
+}
+
+template <typename T>
+int shuffle_impl(int op, const rpp_hip_basis& basis, int64_t n, const rpp_hip_tensor_batch& out,
+                 const rpp_hip_tensor_batch& p, const rpp_hip_tensor_batch& q, const rpp_hip_tensor_batch* r,
+                 double alpha, double beta, void* s) {
+    RPP_TRY(check_shuffle_basis(basis));
+    if (n < 0) return fail(RPP_HIP_INVALID_ARGUMENT, "negative num_batches");
+    RPP_TRY(check_view(out, basis, n, "out", true));
+    RPP_TRY(check_view(p, basis, n, "operand", false));
+    RPP_TRY(check_view(q, basis, n, "operand", false));
+    if (r) RPP_TRY(check_view(*r, basis, n, "addend", false));
+    if (overlaps(out, p) || overlaps(out, q))
+        return fail(RPP_HIP_INVALID_ARGUMENT, "shuffle products: the output must not share memory with the factors");
+    if (r && overlaps(out, *r) &&
+        (out.batch_stride != r->batch_stride || out.coeff_stride != r->coeff_stride))
+        return fail(RPP_HIP_INVALID_ARGUMENT, "st_fma: out may alias the addend only with the same strides");
+    const DevBasis B = make_basis(basis.width, basis.depth);
+    return launch_shuffle<T>(op, B, n, out, p, q, r, alpha, beta, as_stream(s));
+}
+
+template <typename T>
+int st_mul_impl(const rpp_hip_ft_mul_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    return shuffle_impl<T>(kStMul, d->basis, d->num_batches, d->out, d->lhs, d->rhs, nullptr, 0.0, d->beta, s);
+}
+
+template <typename T>
+int st_fma_impl(const rpp_hip_ft_fma_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    return shuffle_impl<T>(kStFma, d->basis, d->num_batches, d->out, d->b, d->c, &d->a, d->alpha, d->beta, s);
+}
+
+template <typename T>
+int st_inplace_fma_impl(const rpp_hip_st_inplace_fma_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    return shuffle_impl<T>(kStInplace, d->basis, d->num_batches, d->a, d->b, d->c, nullptr, d->alpha, d->beta, s);
+}
+
+template <typename T>
+int st_adj_mul_impl(const rpp_hip_ft_adjoint_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    return shuffle_impl<T>(kStAdjMul, d->basis, d->num_batches, d->out, d->op, d->arg, nullptr, 0.0, 1.0, s);
+}
+
+template <typename T, int OP>
+int half_shuffle_impl(const rpp_hip_half_shuffle_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    return shuffle_impl<T>(OP, d->basis, d->num_batches, d->out, d->integrator, d->integrand, nullptr, 0.0, d->beta,
+                           s);
+}
+
+template <typename T>
+int left_hs_mul_impl(const rpp_hip_half_shuffle_desc* d, void* s) { return half_shuffle_impl<T, kHsMul>(d, s); }
+template <typename T>
+int left_hs_adj_lmul_impl(const rpp_hip_half_shuffle_desc* d, void* s) { return half_shuffle_impl<T, kHsAdjL>(d, s); }
+template <typename T>
+int left_hs_adj_rmul_impl(const rpp_hip_half_shuffle_desc* d, void* s) { return half_shuffle_impl<T, kHsAdjR>(d, s); }
+
 // ---------------------------------------------------------- linear operations
 
 // Number of Lie elements of degree k over `width` letters (Witt's formula:
@@ -450,6 +521,13 @@ RPP_C_ENTRY(rpp_hip_ft_inplace_mul, rpp_hip_ft_inplace_mul_desc, ft_inplace_mul_
 RPP_C_ENTRY(rpp_hip_ft_fmexp, rpp_hip_ft_fmexp_desc, fmexp_impl)
 RPP_C_ENTRY(rpp_hip_tensor_pairing, rpp_hip_tensor_pairing_desc, pairing_impl)
 RPP_C_ENTRY(rpp_hip_sparse_matrix_vector_product, rpp_hip_spmv_desc, spmv_impl)
+RPP_C_ENTRY(rpp_hip_st_mul, rpp_hip_ft_mul_desc, st_mul_impl)
+RPP_C_ENTRY(rpp_hip_st_fma, rpp_hip_ft_fma_desc, st_fma_impl)
+RPP_C_ENTRY(rpp_hip_st_inplace_fma, rpp_hip_st_inplace_fma_desc, st_inplace_fma_impl)
+RPP_C_ENTRY(rpp_hip_st_adj_mul, rpp_hip_ft_adjoint_desc, st_adj_mul_impl)
+RPP_C_ENTRY(rpp_hip_left_hs_mul, rpp_hip_half_shuffle_desc, left_hs_mul_impl)
+RPP_C_ENTRY(rpp_hip_left_hs_adj_lmul, rpp_hip_half_shuffle_desc, left_hs_adj_lmul_impl)
+RPP_C_ENTRY(rpp_hip_left_hs_adj_rmul, rpp_hip_half_shuffle_desc, left_hs_adj_rmul_impl)
 RPP_C_ENTRY(rpp_hip_vector_assign, rpp_hip_vector_assign_desc, vector_assign_impl)
 RPP_C_ENTRY(rpp_hip_vector_add, rpp_hip_vector_add_desc, vector_add_impl)
 RPP_C_ENTRY(rpp_hip_vector_inplace_add, rpp_hip_vector_inplace_add_desc, vector_inplace_add_impl)

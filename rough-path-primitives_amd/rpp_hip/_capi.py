@@ -99,6 +99,16 @@ class LieConvertDesc(C.Structure):
     _fields_ = [("basis", Basis), ("num_batches", _i64), ("tensor", TensorBatch), ("lie", TensorBatch)]
 
 
+class StInplaceFmaDesc(C.Structure):
+    _fields_ = [("basis", Basis), ("num_batches", _i64), ("a", TensorBatch), ("b", TensorBatch),
+                ("c", TensorBatch), ("alpha", _f64), ("beta", _f64)]
+
+
+class HalfShuffleDesc(C.Structure):
+    _fields_ = [("basis", Basis), ("num_batches", _i64), ("out", TensorBatch), ("integrator", TensorBatch),
+                ("integrand", TensorBatch), ("beta", _f64)]
+
+
 class GradedBasis(C.Structure):
     _fields_ = [("kind", _i32), ("width", _i32), ("depth", _i32)]
 
@@ -159,6 +169,13 @@ ENTRY_POINTS = {
     "rpp_hip_sparse_matrix_vector_product": SpmvDesc,
     "rpp_hip_tensor_to_lie": LieConvertDesc,
     "rpp_hip_lie_to_tensor": LieConvertDesc,
+    "rpp_hip_st_mul": FtMulDesc,
+    "rpp_hip_st_fma": FtFmaDesc,
+    "rpp_hip_st_inplace_fma": StInplaceFmaDesc,
+    "rpp_hip_st_adj_mul": AdjointDesc,
+    "rpp_hip_left_hs_mul": HalfShuffleDesc,
+    "rpp_hip_left_hs_adj_lmul": HalfShuffleDesc,
+    "rpp_hip_left_hs_adj_rmul": HalfShuffleDesc,
     "rpp_hip_vector_assign": VectorAssignDesc,
     "rpp_hip_vector_add": VectorAddDesc,
     "rpp_hip_vector_inplace_add": VectorInplaceAddDesc,

@@ -315,6 +315,66 @@ def ft_adj_rmul(out, op, arg, basis: TensorBasis, num_batches=None, stream=None)
     _call("rpp_hip_ft_adj_rmul" + _sfx(out, op, arg), d, stream)
 
 
+# --------------------------------------------------------- shuffle family
+
+def st_mul(out, lhs, rhs, basis: TensorBasis, num_batches=None, beta: float = 1.0, stream=None):
+    """rpp::ops::st_mul (operations/basic/st_mul.hpp:48-83): out = beta lhs . rhs
+    (shuffle product)."""
+    out, lhs, rhs = (_as_batch(x, basis) for x in (out, lhs, rhs))
+    d = A.FtMulDesc(basis._c(), _n(num_batches, out), out._c(), lhs._c(), rhs._c(), beta)
+    _call("rpp_hip_st_mul" + _sfx(out, lhs, rhs), d, stream)
+
+
+def st_fma(out, a, b, c, basis: TensorBasis, num_batches=None, alpha: float = 1.0, beta: float = 1.0, stream=None):
+    """rpp::ops::st_fma (operations/basic/st_fma.hpp:50-89): out = alpha a + beta b . c."""
+    out, a, b, c = (_as_batch(x, basis) for x in (out, a, b, c))
+    d = A.FtFmaDesc(basis._c(), _n(num_batches, out), out._c(), a._c(), b._c(), c._c(), alpha, beta)
+    _call("rpp_hip_st_fma" + _sfx(out, a, b, c), d, stream)
+
+
+def st_inplace_fma(a, b, c, basis: TensorBasis, num_batches=None, alpha: float = 1.0, beta: float = 1.0,
+                   stream=None):
+    """rpp::ops::st_inplace_fma (operations/basic/st_inplace_fma.hpp:45-85):
+    a <- alpha a + beta b . c."""
+    a, b, c = (_as_batch(x, basis) for x in (a, b, c))
+    d = A.StInplaceFmaDesc(basis._c(), _n(num_batches, a), a._c(), b._c(), c._c(), alpha, beta)
+    _call("rpp_hip_st_inplace_fma" + _sfx(a, b, c), d, stream)
+
+
+def st_adj_mul(out, op, arg, basis: TensorBasis, num_batches=None, stream=None):
+    """rpp::ops::st_adj_mul (operations/basic/st_adj_mul.hpp:40-73):
+    <out, x> = <arg, op . x>."""
+    out, op, arg = (_as_batch(x, basis) for x in (out, op, arg))
+    d = A.AdjointDesc(basis._c(), _n(num_batches, out), out._c(), op._c(), arg._c())
+    _call("rpp_hip_st_adj_mul" + _sfx(out, op, arg), d, stream)
+
+
+def _half(name, out, integrator, integrand, basis, num_batches, beta, stream):
+    out, x, y = (_as_batch(t, basis) for t in (out, integrator, integrand))
+    d = A.HalfShuffleDesc(basis._c(), _n(num_batches, out), out._c(), x._c(), y._c(), beta)
+    _call(name + _sfx(out, x, y), d, stream)
+
+
+def left_hs_mul(out, integrator, integrand, basis: TensorBasis, num_batches=None, beta: float = 1.0, stream=None):
+    """rpp::ops::left_hs_mul (operations/basic/left_hs_mul.hpp:53-93):
+    out = beta integrator < integrand (left half-shuffle)."""
+    _half("rpp_hip_left_hs_mul", out, integrator, integrand, basis, num_batches, beta, stream)
+
+
+def left_hs_adj_lmul(out, integrator, integrand, basis: TensorBasis, num_batches=None, beta: float = 1.0,
+                     stream=None):
+    """rpp::ops::left_hs_adj_lmul (operations/basic/left_hs_adj_lmul.hpp:33-61):
+    <out, z> = beta <integrand, integrator < z>."""
+    _half("rpp_hip_left_hs_adj_lmul", out, integrator, integrand, basis, num_batches, beta, stream)
+
+
+def left_hs_adj_rmul(out, integrator, integrand, basis: TensorBasis, num_batches=None, beta: float = 1.0,
+                     stream=None):
+    """rpp::ops::left_hs_adj_rmul (operations/basic/left_hs_adj_rmul.hpp:33-61):
+    <out, z> = beta <integrator, z < integrand>."""
+    _half("rpp_hip_left_hs_adj_rmul", out, integrator, integrand, basis, num_batches, beta, stream)
+
+
 # -------------------------------------------------------- linear operations
 
 def vector_assign(out, arg, basis, num_batches=None, stream=None):

@@ -246,6 +246,50 @@ void run(int width, int depth, Index n) {
         report(tag("tensor_set_identity"), id.host(), hid);
         report(tag("tensor_add_identity"), ai.host(), hai);
     }
+    if (depth <= 5) {  // the shuffle family; the CPU half-shuffle adjoints through their operation objects
+        const Degree lo = depth >= 2 ? 1 : 0;
+        std::vector<T> hm = nanv, hf = nanv, hi = add, ha = nanv, hh = nanv, hl = nanv, hr = nanv;
+        (void)rpp::ops::st_mul(cpu, {}, hb(hm.data(), 0, D), hcb(lhs.data(), 0, D), hcb(rhs.data(), lo, D), basis, n,
+                               T(0.5));
+        (void)rpp::ops::st_fma(cpu, {}, hb(hf.data(), 0, D), hcb(add.data(), lo, D), hcb(lhs.data(), 0, D),
+                               hcb(rhs.data(), 0, D), basis, n, T(2), T(-1));
+        (void)rpp::ops::st_inplace_fma(cpu, {}, hb(hi.data(), 0, D), hcb(lhs.data(), 0, D), hcb(rhs.data(), 0, D),
+                                       basis, n, T(0.25), T(1));
+        (void)rpp::ops::st_adj_mul(cpu, {}, hb(ha.data(), 0, D), hcb(lhs.data(), 0, D), hcb(rhs.data(), 0, D), basis,
+                                   n);
+        (void)rpp::ops::left_hs_mul(cpu, {}, hb(hh.data(), 0, D), hcb(lhs.data(), 0, D), hcb(rhs.data(), 0, D), basis,
+                                    n, T(1));
+        {
+            using Strategy = rpp::cpu::strategies::SingleThreadStrategy<T>;
+            auto const ctx = Strategy::make_context(nullptr);
+            auto const pack = rpp::basis::make_basis_pack(basis);
+            for (Index i = 0; i < n; ++i) {
+                rpp::DenseTensorView<T*, rpp::basis::TensorBasis<Arch>> lo_v(hl.data() + i * dim, basis, 0, D);
+                rpp::DenseTensorView<T*, rpp::basis::TensorBasis<Arch>> ro_v(hr.data() + i * dim, basis, 0, D);
+                rpp::DenseTensorView<const T*, rpp::basis::TensorBasis<Arch>> x_v(lhs.data() + i * dim, basis, 0, D);
+                rpp::DenseTensorView<const T*, rpp::basis::TensorBasis<Arch>> y_v(rhs.data() + i * dim, basis, 0, D);
+                rpp::ops::LeftHSAdjLMul<Strategy>{ctx, pack}(ctx, lo_v, x_v, y_v);
+                rpp::ops::LeftHSAdjRMul<Strategy>{ctx, pack}(ctx, ro_v, x_v, y_v);
+            }
+        }
+        Dev<T> m(nanv), f(nanv), in(add), a(nanv), h(nanv), l(nanv), r(nanv), x(lhs), y(rhs), z(add);
+        (void)rpp::ops::st_mul(gpu, {}, hb(m.p, 0, D), hcb(x.p, 0, D), hcb(y.p, lo, D), basis, n, T(0.5));
+        (void)rpp::ops::st_fma(gpu, {}, hb(f.p, 0, D), hcb(z.p, lo, D), hcb(x.p, 0, D), hcb(y.p, 0, D), basis, n, T(2),
+                               T(-1));
+        (void)rpp::ops::st_inplace_fma(gpu, {}, hb(in.p, 0, D), hcb(x.p, 0, D), hcb(y.p, 0, D), basis, n, T(0.25),
+                                       T(1));
+        (void)rpp::ops::st_adj_mul(gpu, {}, hb(a.p, 0, D), hcb(x.p, 0, D), hcb(y.p, 0, D), basis, n);
+        (void)rpp::ops::left_hs_mul(gpu, {}, hb(h.p, 0, D), hcb(x.p, 0, D), hcb(y.p, 0, D), basis, n, T(1));
+        (void)rpp::ops::left_hs_adj_lmul(gpu, {}, hb(l.p, 0, D), hcb(x.p, 0, D), hcb(y.p, 0, D), basis, n);
+        (void)rpp::ops::left_hs_adj_rmul(gpu, {}, hb(r.p, 0, D), hcb(x.p, 0, D), hcb(y.p, 0, D), basis, n);
+        report(tag("st_mul"), m.host(), hm);
+        report(tag("st_fma"), f.host(), hf);
+        report(tag("st_inplace_fma"), in.host(), hi);
+        report(tag("st_adj_mul"), a.host(), ha);
+        report(tag("left_hs_mul"), h.host(), hh);
+        report(tag("left_hs_adj_lmul"), l.host(), hl);
+        report(tag("left_hs_adj_rmul"), r.host(), hr);
+    }
     if (width >= 2) {  // linear operations on a Lie grading (the Hall-set LieBasis)
         rpp::basis::HallBasis<Arch> hall(static_cast<Degree>(width), D);
         auto lb = hall.to_lie_basis();

@@ -126,6 +126,32 @@ def main():
         R.tensor_add_identity(start, 4, addid, 0.5)
         out[f"linear_{kind}_w3d4"] = dict(x=x, y=y, z=z, assign=assign, add=add, inplace=inplace, scaled=scaled,
                                           const=const, ident=ident, addid=addid)
+    # the shuffle family at (3,4): 3 objects, damped inputs, partial windows
+    rng = np.random.default_rng(0x5F)
+    w, N = 3, 4
+    dim = dim_of(w, N)
+    x, y, z = (damped(rng, 3, w, N) for _ in range(3))
+    sh = dict(x=x, y=y, z=z)
+    o = np.full((3, dim), 7.0)
+    R.st_mul(w, N, o, x, y, ow=(0, 4), lw=(0, 3), rw=(1, 4), beta=-0.5)
+    sh["st_mul"] = o
+    o = np.full((3, dim), 7.0)
+    R.st_fma(w, N, o, z, x, y, ow=(1, 4), aw=(0, 2), bw=(0, 4), cw=(0, 4), alpha=0.75, beta=1.25)
+    sh["st_fma"] = o
+    o = z.copy()
+    R.st_inplace_fma(w, N, o, x, y, aw=(0, 3), bw=(1, 4), cw=(0, 4), alpha=0.5, beta=2.0)
+    sh["st_inplace_fma"] = o
+    o = np.full((3, dim), 7.0)
+    R.st_adj_mul(w, N, o, x, y, ow=(0, 4), pw=(0, 2), gw=(0, 4))
+    sh["st_adj_mul"] = o
+    o = np.full((3, dim), 7.0)
+    R.left_hs_mul(w, N, o, x, y, ow=(0, 4), xw=(1, 4), yw=(0, 3), beta=1.5)
+    sh["left_hs_mul"] = o
+    for name in ("left_hs_adj_lmul", "left_hs_adj_rmul"):
+        o = np.full((3, dim), 7.0)
+        getattr(R, name)(w, N, o, x, y, ow=(0, 3), xw=(0, 4), yw=(0, 4))
+        sh[name] = o
+    out["shuffle_w3d4"] = sh
     for name, arrays in out.items():
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
     print("wrote", len(out), "fixtures to", HERE)

@@ -66,6 +66,28 @@ FT_INPLACE_CASES = [
 ]
 
 
+# Range cases of the shuffle adjoints at depth 4 (out, op, arg)
+# (tests/rpp/shuffle_adjoint_test_helper.hpp:19-36) and the two extra cases of
+# the half-shuffle adjoints (half_shuffle_adjoint_test_helper.hpp:86-108).
+SHUFFLE_ADJ_CASES = [
+    ("full_views", (0, 4), (0, 4), (0, 4)),
+    ("unit_operator", (0, 4), (0, 0), (1, 2)),
+    ("positive_operator_minimum", (0, 4), (1, 2), (2, 3)),
+    ("single_degree_operands", (0, 4), (1, 1), (3, 3)),
+    ("operator_above_argument", (0, 4), (3, 4), (0, 2)),
+    ("output_below_support", (0, 1), (0, 1), (3, 4)),
+    ("output_above_support", (3, 4), (1, 2), (1, 2)),
+    ("unit_argument", (0, 4), (0, 0), (0, 0)),
+    ("unit_argument_missing_operator", (0, 4), (1, 2), (0, 0)),
+    ("unit_output_present", (0, 0), (2, 2), (2, 2)),
+    ("unit_output_absent", (0, 0), (1, 1), (2, 2)),
+    ("interior_output", (1, 3), (0, 1), (2, 3)),
+    ("single_high_output", (4, 4), (0, 0), (4, 4)),
+    ("mixed_words", (1, 2), (1, 2), (2, 4)),
+    ("unit_cotangent_only", (0, 4), (0, 4), (0, 0)),
+]
+
+
 def rel_err(a, b) -> float:
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)

@@ -115,7 +115,69 @@ def test_oracle_linear_operations(kind):
         assert np.array_equal(init, f[name]), name
 
 
+SHUFFLE_CALLS = [
+    ("st_mul", lambda I, o, f: I.st_mul(3, 4, o, f["x"], f["y"], ow=(0, 4), lw=(0, 3), rw=(1, 4), beta=-0.5)),
+    ("st_fma", lambda I, o, f: I.st_fma(3, 4, o, f["z"], f["x"], f["y"], ow=(1, 4), aw=(0, 2), bw=(0, 4), cw=(0, 4),
+                                        alpha=0.75, beta=1.25)),
+    ("st_inplace_fma", lambda I, o, f: I.st_inplace_fma(3, 4, o, f["x"], f["y"], aw=(0, 3), bw=(1, 4), cw=(0, 4),
+                                                        alpha=0.5, beta=2.0)),
+    ("st_adj_mul", lambda I, o, f: I.st_adj_mul(3, 4, o, f["x"], f["y"], ow=(0, 4), pw=(0, 2), gw=(0, 4))),
+    ("left_hs_mul", lambda I, o, f: I.left_hs_mul(3, 4, o, f["x"], f["y"], ow=(0, 4), xw=(1, 4), yw=(0, 3),
+                                                  beta=1.5)),
+    ("left_hs_adj_lmul", lambda I, o, f: I.left_hs_adj_lmul(3, 4, o, f["x"], f["y"], ow=(0, 3))),
+    ("left_hs_adj_rmul", lambda I, o, f: I.left_hs_adj_rmul(3, 4, o, f["x"], f["y"], ow=(0, 3))),
+]
+
+
+def test_oracle_shuffle_family():
+    f = fixture("shuffle_w3d4")
+    for name, call in SHUFFLE_CALLS:
+        o = f["z"].copy() if name == "st_inplace_fma" else np.full_like(f["x"], 7.0)
+        call(O.orc, o, f)
+        assert np.array_equal(o, f[name]), name
+
+
 # ---------------------------------------------------------------------- GPU
+
+@pytest.mark.gpu
+def test_gpu_shuffle_family_against_golden(dev):
+    import torch
+
+    import rpp_hip
+
+    f = fixture("shuffle_w3d4")
+    B = rpp_hip.TensorBasis(3, 4)
+    Bt = rpp_hip.Batch
+
+    def t(x):
+        return torch.tensor(np.ascontiguousarray(x), device=dev)
+
+    x, y, z = t(f["x"]), t(f["y"]), t(f["z"])
+
+    def check(name, o):
+        torch.cuda.synchronize()
+        assert_close(o.cpu().numpy(), f[name], np.float64, name)
+
+    o = t(np.full_like(f["x"], 7.0))
+    rpp_hip.st_mul(Bt(o, 0, 4), Bt(x, 0, 3), Bt(y, 1, 4), B, beta=-0.5)
+    check("st_mul", o)
+    o = t(np.full_like(f["x"], 7.0))
+    rpp_hip.st_fma(Bt(o, 1, 4), Bt(z, 0, 2), x, y, B, alpha=0.75, beta=1.25)
+    check("st_fma", o)
+    o = t(f["z"])
+    rpp_hip.st_inplace_fma(Bt(o, 0, 3), Bt(x, 1, 4), y, B, alpha=0.5, beta=2.0)
+    check("st_inplace_fma", o)
+    o = t(np.full_like(f["x"], 7.0))
+    rpp_hip.st_adj_mul(o, Bt(x, 0, 2), y, B)
+    check("st_adj_mul", o)
+    o = t(np.full_like(f["x"], 7.0))
+    rpp_hip.left_hs_mul(o, Bt(x, 1, 4), Bt(y, 0, 3), B, beta=1.5)
+    check("left_hs_mul", o)
+    for name in ("left_hs_adj_lmul", "left_hs_adj_rmul"):
+        o = t(np.full_like(f["x"], 7.0))
+        getattr(rpp_hip, name)(Bt(o, 0, 3), x, y, B)
+        check(name, o)
+
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("kind", ["tensor", "lie"])
