@@ -110,6 +110,11 @@ int check_view(const rpp_hip_tensor_batch& t, const rpp_hip_basis& b, int64_t n,
     return 0;
 }
 
+int check_count(int64_t n) {
+    if (n < 0) return fail(RPP_HIP_INVALID_ARGUMENT, "negative num_batches");
+    return 0;
+}
+
 #define RPP_TRY(expr)               \
     do {                            \
         if (int rc_ = (expr)) return rc_; \
@@ -275,32 +280,61 @@ int spmv_impl(const rpp_hip_spmv_desc* d, void* s) {
     return launch_spmv<T>(d->num_batches, d->out, d->arg, M, d->alpha, as_stream(s));
 }
 
+// tensor_to_lie / lie_to_tensor with the library's matrices.  Both are block
+// diagonal by degree (a Lie element of degree k expands into, and is
+// recovered from, tensor words of degree k only), so a window pair converts
+// the degrees [dlo, dhi] that both windows hold, reading the argument only
+// there, and the rest of the output window is 0 (the argument is zero outside
+// its window).  With the reference's windows (tensor from degree 0, Lie from
+// degree 1) this is exactly its view-relative product
+// (cpu/.../linalg/sparse_matrix_vector.hpp:58-59, 148-149).
 template <typename T>
 int lie_convert_impl(bool to_lie, const rpp_hip_lie_convert_desc* d, void* s) {
     if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
     RPP_TRY(check_basis(d->basis));
     if (d->basis.width < 2) return fail(RPP_HIP_NOT_IMPLEMENTED, "Lie conversions need width >= 2");
+    RPP_TRY(check_count(d->num_batches));
     RPP_TRY(check_view(d->tensor, d->basis, d->num_batches, "tensor", !to_lie));
-    if (d->lie.min_degree < 1 || d->lie.max_degree > d->basis.depth || d->lie.coeff_stride < 1 ||
-        d->lie.batch_stride < 0)
+    if (d->lie.min_degree < 1 || d->lie.max_degree > d->basis.depth)
         return fail(RPP_HIP_INVALID_ARGUMENT, "lie: window must lie in [1, depth]");
+    RPP_TRY(check_view(d->lie, d->basis, d->num_batches, "lie", to_lie));
+    const int64_t n = d->num_batches;
     const DeviceCsr* l2t = nullptr;
     const DeviceCsr* t2l = nullptr;
     RPP_TRY(device_lie_matrices<T>(d->basis.width, d->basis.depth, &l2t, &t2l));
     auto md = lie_metadata(d->basis.width, d->basis.depth);
     const DevBasis B = make_basis(d->basis.width, d->basis.depth);
-    rpp_hip_vector_batch tv{d->tensor.data, d->tensor.batch_stride, d->tensor.coeff_stride,
-                            B.start[d->tensor.min_degree], md->tensor_dim - B.start[d->tensor.min_degree]};
-    const int64_t lbeg = md->degree_begin[static_cast<size_t>(d->lie.min_degree)];
-    rpp_hip_vector_batch lv{d->lie.data, d->lie.batch_stride, d->lie.coeff_stride, lbeg, md->true_size - lbeg};
+    const std::vector<int64_t>& lb = md->degree_begin;  // Lie storage index of degree k's first key
+    const rpp_hip_tensor_batch& out = to_lie ? d->lie : d->tensor;
+    const rpp_hip_tensor_batch& arg = to_lie ? d->tensor : d->lie;
+    auto ostart = [&](int k) { return to_lie ? lb[static_cast<size_t>(k)] : B.start[k]; };
+    auto astart = [&](int k) { return to_lie ? B.start[k] : lb[static_cast<size_t>(k)]; };
+    const int dlo = std::max({d->tensor.min_degree, d->lie.min_degree, 1});
+    const int dhi = std::min(d->tensor.max_degree, d->lie.max_degree);
+    const hipStream_t stream = as_stream(s);
+    if (out.min_degree <= out.max_degree && n > 0) {
+        const int64_t w0 = ostart(out.min_degree), w1 = ostart(out.max_degree + 1);
+        const int64_t c0 = dlo <= dhi ? ostart(dlo) : w1, c1 = dlo <= dhi ? ostart(dhi + 1) : w1;
+        if (c0 > w0)
+            RPP_TRY(launch_vector_op<T>(kVecSetConstant, n, out, IndexRange{w0, c0}, nullptr, IndexRange{0, 0},
+                                        nullptr, IndexRange{0, 0}, 0.0, 0.0, stream));
+        if (w1 > c1)
+            RPP_TRY(launch_vector_op<T>(kVecSetConstant, n, out, IndexRange{c1, w1}, nullptr, IndexRange{0, 0},
+                                        nullptr, IndexRange{0, 0}, 0.0, 0.0, stream));
+    }
+    if (dlo > dhi || n == 0) return 0;
+    // matrix rows: T2L rows are Lie keys - 1, L2T rows tensor words; columns
+    // the other way round
+    const int64_t obeg = ostart(dlo), rows = ostart(dhi + 1) - obeg;
+    const int64_t abeg = astart(dlo), ncols = astart(dhi + 1) - abeg;
+    const int64_t row0 = to_lie ? obeg - 1 : obeg;
+    const int64_t col0 = to_lie ? abeg : abeg - 1;
+    const rpp_hip_vector_batch ov{out.data, out.batch_stride, out.coeff_stride, obeg, rows};
+    const rpp_hip_vector_batch av{arg.data, arg.batch_stride, arg.coeff_stride, abeg, ncols};
     const DeviceCsr* m = to_lie ? t2l : l2t;
-    if (lie_spmv_fits<T>(*m, to_lie ? tv : lv))
-        return to_lie ? launch_lie_spmv<T>(d->num_batches, lv, tv, *m, as_stream(s))
-                      : launch_lie_spmv<T>(d->num_batches, tv, lv, *m, as_stream(s));
-    rpp_hip_sparse_matrix M{RPP_HIP_CSR, m->values, m->indices, m->offsets, m->nnz, m->rows, m->cols};
-    // R9: matrix row 0 <-> out.begin(), column 0 <-> arg.begin().
-    return to_lie ? launch_spmv<T>(d->num_batches, lv, tv, M, 1.0, as_stream(s))
-                  : launch_spmv<T>(d->num_batches, tv, lv, M, 1.0, as_stream(s));
+    if (lie_spmv_fits<T>(*m, av)) return launch_lie_spmv<T>(n, ov, av, *m, row0, rows, col0, stream);
+    const rpp_hip_sparse_matrix M{RPP_HIP_CSR, m->values, m->indices, m->offsets + row0, m->nnz, rows, ncols};
+    return launch_spmv<T>(n, ov, av, M, 1.0, stream, col0);
 }
 
 // ------------------------------------------------------- the shuffle family
@@ -435,11 +469,6 @@ int check_same_or_distinct(const rpp_hip_tensor_batch& out, const rpp_hip_tensor
     if (out.data == in.data && nonempty(out) && nonempty(in) &&
         (out.batch_stride != in.batch_stride || out.coeff_stride != in.coeff_stride))
         return fail(RPP_HIP_INVALID_ARGUMENT, std::string(what) + ": out may alias an operand only with the same strides");
-    return 0;
-}
-
-int check_count(int64_t n) {
-    if (n < 0) return fail(RPP_HIP_INVALID_ARGUMENT, "negative num_batches");
     return 0;
 }
 

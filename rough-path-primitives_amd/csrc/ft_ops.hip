@@ -513,7 +513,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_spmv_csr_lds(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t abs_, int64_t abeg,
                int64_t count, const T* values, const int64_t* indices, const int64_t* offsets, int64_t rows,
-               T alpha, int64_t n) {
+               T alpha, int64_t n, int64_t cshift) {
     extern __shared__ __align__(16) unsigned char spmv_raw[];
     T* xs = reinterpret_cast<T*>(spmv_raw);
     const int t = threadIdx.x;
@@ -528,7 +528,7 @@ k_spmv_csr_lds(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int
         for (int64_t row = grp; row < rows; row += groups) {
             const int64_t e0 = offsets[row], e1 = offsets[row + 1];
             T acc = T(0);
-            for (int64_t e = e0 + lane; e < e1; e += kSpmvLanes) acc += values[e] * xs[indices[e]];
+            for (int64_t e = e0 + lane; e < e1; e += kSpmvLanes) acc += values[e] * xs[indices[e] - cshift];
 #pragma unroll
             for (int off = kSpmvLanes / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, kSpmvLanes);
             if (lane == 0) o[(obeg + row) * ocs] = alpha * acc;
@@ -543,14 +543,14 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_spmv_csr(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t abs_,
            int64_t acs, int64_t abeg, const T* values, const int64_t* indices,
-           const int64_t* offsets, int64_t rows, T alpha, int64_t n) {
+           const int64_t* offsets, int64_t rows, T alpha, int64_t n, int64_t cshift) {
     const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (row >= rows) return;
     const int64_t e0 = offsets[row], e1 = offsets[row + 1];
     for (int64_t b = blockIdx.y; b < n; b += gridDim.y) {
         const T* a = arg + b * abs_;
         T acc = T(0);
-        for (int64_t e = e0; e < e1; ++e) acc += values[e] * a[(abeg + indices[e]) * acs];
+        for (int64_t e = e0; e < e1; ++e) acc += values[e] * a[(abeg + indices[e] - cshift) * acs];
         out[b * obs + (obeg + row) * ocs] = alpha * acc;
     }
 }
@@ -740,7 +740,7 @@ int launch_antipode(bool sign_by_degree, const DevBasis& B, int64_t n, const rpp
 
 template <typename T>
 int launch_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg,
-                const rpp_hip_sparse_matrix& M, double alpha, hipStream_t stream) {
+                const rpp_hip_sparse_matrix& M, double alpha, hipStream_t stream, int64_t col_shift) {
     if (n <= 0) return 0;
     const unsigned gy = static_cast<unsigned>(std::min<int64_t>(n, 65535));
     T* o = static_cast<T*>(out.data);
@@ -760,14 +760,14 @@ int launch_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector
                 const unsigned grid = static_cast<unsigned>(std::min<int64_t>(n, device_cu_count() * per_cu));
                 k_spmv_csr_lds<T><<<grid, kBlock, lds, stream>>>(o, out.batch_stride, out.coeff_stride, out.begin, a,
                                                                arg.batch_stride, arg.begin, count, v, M.indices,
-                                                               M.offsets, M.rows, T(alpha), n);
+                                                               M.offsets, M.rows, T(alpha), n, col_shift);
                 return check_hip(hipGetLastError(), "k_spmv_csr_lds launch");
             }
         }
         dim3 grid(static_cast<unsigned>((M.rows + kBlock - 1) / kBlock), gy);
         k_spmv_csr<T><<<grid, kBlock, 0, stream>>>(o, out.batch_stride, out.coeff_stride, out.begin, a,
                                                    arg.batch_stride, arg.coeff_stride, arg.begin, v,
-                                                   M.indices, M.offsets, M.rows, T(alpha), n);
+                                                   M.indices, M.offsets, M.rows, T(alpha), n, col_shift);
         return check_hip(hipGetLastError(), "k_spmv_csr launch");
     }
     if (out.size > 0) {
@@ -800,7 +800,7 @@ int launch_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector
     template int launch_antipode<T>(bool, const DevBasis&, int64_t, const rpp_hip_tensor_batch&,  \
                                     const rpp_hip_tensor_batch&, hipStream_t);                     \
     template int launch_spmv<T>(int64_t, const rpp_hip_vector_batch&, const rpp_hip_vector_batch&, \
-                                const rpp_hip_sparse_matrix&, double, hipStream_t);
+                                const rpp_hip_sparse_matrix&, double, hipStream_t, int64_t);
 
 RPP_INSTANTIATE(float)
 RPP_INSTANTIATE(double)

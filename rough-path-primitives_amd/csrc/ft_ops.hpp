@@ -38,8 +38,12 @@ template <typename T>
 int launch_log_adjoint(const DevBasis& B, int64_t n, const rpp_hip_tensor_batch& grad_arg,
                        const rpp_hip_tensor_batch& arg, const rpp_hip_tensor_batch& cotangent, hipStream_t stream);
 
+// out = alpha M arg with view-relative indexing (row 0 <-> out.begin, column
+// c <-> arg.begin + c - col_shift).  A CSR matrix whose entries only name
+// columns [col_shift, col_shift + cols) can so be applied to an argument
+// segment that starts at that column; the public operation uses col_shift 0.
 template <typename T>
 int launch_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg,
-                const rpp_hip_sparse_matrix& M, double alpha, hipStream_t stream);
+                const rpp_hip_sparse_matrix& M, double alpha, hipStream_t stream, int64_t col_shift = 0);
 
 }  // namespace rpp_hip

@@ -408,6 +408,22 @@ std::shared_ptr<const LieMetadata> load_lie_metadata(const std::string& path, st
     if (!valid_compressed(md->l2t_csc_offsets, md->l2t_csc_indices, dim) ||
         !valid_compressed(md->t2l_csc_offsets, md->t2l_csc_indices, std::max<int64_t>(ts - 1, 0)))
         return bad("matrix structure invalid");
+    // both matrices are block diagonal by degree, which the conversions rely on
+    std::vector<int64_t> tstart(static_cast<size_t>(md->depth) + 2, 0);
+    for (int k = 1, pk = 1; k <= md->depth + 1; ++k, pk *= md->width) tstart[k] = tstart[k - 1] + pk;
+    auto degree_in = [](const std::vector<int64_t>& begin, int64_t i) {
+        return static_cast<int>(std::upper_bound(begin.begin(), begin.end(), i) - begin.begin()) - 1;
+    };
+    for (int64_t key = 1; key < ts; ++key) {
+        const int k = degree_in(md->degree_begin, key);
+        for (int64_t e = md->l2t_csc_offsets[key - 1]; e < md->l2t_csc_offsets[key]; ++e)
+            if (degree_in(tstart, md->l2t_csc_indices[e]) != k) return bad("L2T entry outside its degree");
+    }
+    for (int64_t w = 0; w < dim; ++w) {
+        const int k = degree_in(tstart, w);
+        for (int64_t e = md->t2l_csc_offsets[w]; e < md->t2l_csc_offsets[w + 1]; ++e)
+            if (degree_in(md->degree_begin, md->t2l_csc_indices[e] + 1) != k) return bad("T2L entry outside its degree");
+    }
     Csc l2t{md->l2t_csc_values, md->l2t_csc_indices, md->l2t_csc_offsets};
     Csc t2l{md->t2l_csc_values, md->t2l_csc_indices, md->t2l_csc_offsets};
     md->l2t = to_csr(l2t, dim);

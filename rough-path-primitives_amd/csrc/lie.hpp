@@ -71,12 +71,14 @@ struct DeviceCsr {
 template <typename T>
 int device_lie_matrices(int width, int depth, const DeviceCsr** l2t, const DeviceCsr** t2l);
 
-// out = M arg over a batch with the packed form of M (lie_spmv.hip); matrix
-// row 0 is out.begin, column 0 is arg.begin.  lie_spmv_fits tells whether the
-// packed kernel applies (packed form present, one argument segment fits in LDS).
+// out = M[row0 : row0 + rows] arg over a batch with the packed form of M
+// (lie_spmv.hip): the slice's row r is out.begin + r, and its entries, which
+// must name only columns [col0, col0 + arg.size), read arg.begin + c - col0.
+// lie_spmv_fits tells whether the packed kernel applies (packed form present,
+// one argument segment fits in LDS).
 template <typename T>
 int launch_lie_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg, const DeviceCsr& m,
-                    hipStream_t stream);
+                    int64_t row0, int64_t rows, int64_t col0, hipStream_t stream);
 template <typename T>
 bool lie_spmv_fits(const DeviceCsr& m, const rpp_hip_vector_batch& arg);
 

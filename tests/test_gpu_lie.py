@@ -94,3 +94,72 @@ def test_log_signature_pipeline_matches_reference(dev):
     lie = torch.zeros((n, lie_ref.shape[1]), dtype=torch.float64, device=dev)
     rpp_hip.tensor_to_lie(lie, lg, basis)
     assert_close(lie.cpu().numpy()[:, 1:], lie_ref[:, 1:], np.float64, "C3 pipeline")
+
+
+def _lie_expected(lie, x_zero, to_lie, ts, dim, dtype):
+    if to_lie:
+        m = lie["t2l"]
+        exp = np.zeros((x_zero.shape[0], ts), dtype=dtype)
+        O.spmv(m["data"], m["indices"], m["offsets"], dim, exp, x_zero, csc=True, out_offset=1, out_size=ts - 1)
+    else:
+        m = lie["l2t"]
+        exp = np.zeros((x_zero.shape[0], dim), dtype=dtype)
+        O.spmv(m["data"], m["indices"], m["offsets"], ts - 1, exp, x_zero, csc=True, out_offset=0, out_size=dim,
+               arg_offset=1)
+    return exp
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", [(3, 4), (2, 6), (3, 8), (3, 9)])
+@pytest.mark.parametrize("to_lie", [True, False], ids=["t2l", "l2t"])
+def test_lie_conversions_on_windows(dev, shape, to_lie, dtype):
+    """Both conversions on window pairs: only the output window is written,
+    the argument is read only on its window (NaN elsewhere), degrees that the
+    argument's window lacks come out 0.  The matrices are block diagonal by
+    degree, so the expectation is the full product of the zero-extended
+    argument restricted to the output window.  (3,9) in FP64 exercises the
+    generic CSR path (its argument does not fit in LDS)."""
+    w, N = shape
+    if shape == (3, 9) and dtype == np.float32:
+        pytest.skip("the packed path is covered at the other shapes")
+    basis = rpp_hip.TensorBasis(w, N)
+    lie = rpp_hip.lie_basis(w, N)
+    ts, dim = lie["true_size"], lie["tensor_dim"]
+    tdb = O.degree_begin(w, N)
+    ldb = [int(v) for v in lie["degree_begin"]]
+    rng = np.random.default_rng(w * 17 + N)
+    n = 3
+    twins = [(0, N), (1, N), (2, N - 1), (0, 1), (N, N)]
+    lwins = [(1, N), (2, N), (1, 2), (N, N), (2, 3)]
+    pairs = [(twins[i], lwins[j]) for i in range(len(twins)) for j in range(len(lwins)) if (i + j) % 2 == 0]
+    if shape == (3, 9):
+        pairs = [((0, N), (1, N)), ((2, N - 1), (2, 3))]
+    for tw, lw in pairs:
+        if to_lie:
+            x = damped_batch(rng, n, w, N, dtype)
+            xz = x.copy()
+            xz[:, :tdb[tw[0]]] = 0
+            xz[:, tdb[tw[1] + 1]:] = 0
+            xin = x.copy()
+            xin[:, :tdb[tw[0]]] = np.nan
+            xin[:, tdb[tw[1] + 1]:] = np.nan
+            exp = _lie_expected(lie, xz, True, ts, dim, dtype)
+            lo, hi = ldb[lw[0]], ldb[lw[1] + 1]
+            out = to_dev(np.full((n, ts), 7.0, dtype=dtype), dev)
+            rpp_hip.tensor_to_lie(rpp_hip.Batch(out, *lw), rpp_hip.Batch(to_dev(xin, dev), *tw), basis)
+        else:
+            y = rng.uniform(-1, 1, size=(n, ts)).astype(dtype)
+            y[:, 0] = 0
+            yz = y.copy()
+            yz[:, :ldb[lw[0]]] = 0
+            yz[:, ldb[lw[1] + 1]:] = 0
+            yin = y.copy()
+            yin[:, :ldb[lw[0]]] = np.nan
+            yin[:, ldb[lw[1] + 1]:] = np.nan
+            exp = _lie_expected(lie, yz, False, ts, dim, dtype)
+            lo, hi = tdb[tw[0]], tdb[tw[1] + 1]
+            out = to_dev(np.full((n, dim), 7.0, dtype=dtype), dev)
+            rpp_hip.lie_to_tensor(rpp_hip.Batch(out, *tw), rpp_hip.Batch(to_dev(yin, dev), *lw), basis)
+        got = out.cpu().numpy()
+        assert np.all(got[:, :lo] == 7.0) and np.all(got[:, hi:] == 7.0), (tw, lw)
+        assert_close(got[:, lo:hi], exp[:, lo:hi], dtype, f"{'t2l' if to_lie else 'l2t'} {shape} {tw} {lw}")

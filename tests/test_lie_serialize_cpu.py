@@ -122,6 +122,10 @@ def test_damaged_or_inconsistent_files_are_refused(tmp_path):
     struct.pack_into("<q", body, 16 + 8, lo["dim"] + 1)  # tensor dimension of another shape
     cases["wrong dimension"] = reseal(bytes(body))
     cases["trailing bytes"] = reseal(data[:-8] + b"\0" * 8)
+    body = bytearray(data[:-8])
+    # the last T2L entry belongs to a top-degree word: point it at a degree-1 key
+    struct.pack_into("<q", body, lo["t2l_indices"] + 8 * (lo["t2l_nnz"] - 1), 0)
+    cases["entry across degrees"] = reseal(bytes(body))
     for name, blob in cases.items():
         p = tmp_path / (name.replace(" ", "_") + ".bin")
         p.write_bytes(blob)
