@@ -538,12 +538,14 @@ k_spmv_csr_lds(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int
 }
 
 // CSR rows (cpu/.../linalg/sparse_matrix_vector.hpp:45-73): one thread per
-// row, grid.y over objects.
+// row, grid.y over objects.  The row sum runs in entry order with contraction
+// off, as the reference's does, so the results agree bit for bit.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_spmv_csr(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t abs_,
            int64_t acs, int64_t abeg, const T* values, const int64_t* indices,
            const int64_t* offsets, int64_t rows, T alpha, int64_t n, int64_t cshift) {
+#pragma clang fp contract(off)
     const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (row >= rows) return;
     const int64_t e0 = offsets[row], e1 = offsets[row + 1];

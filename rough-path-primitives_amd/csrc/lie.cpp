@@ -493,8 +493,9 @@ struct DeviceMatrices {
     DeviceCsr l2t, t2l;
 };
 
+// row_blocks: the first row of each degree and, last, the row count.
 template <typename T>
-int upload(const Csr& h, DeviceCsr& d) {
+int upload(const Csr& h, const std::vector<int64_t>& row_blocks, DeviceCsr& d) {
     d.rows = h.rows;
     d.cols = h.cols;
     d.nnz = static_cast<int64_t>(h.values.size());
@@ -514,13 +515,15 @@ int upload(const Csr& h, DeviceCsr& d) {
     if (hipMemcpy(d.offsets, h.offsets.data(), h.offsets.size() * sizeof(int64_t), hipMemcpyHostToDevice) !=
         hipSuccess)
         return 9;
-    // packed form: value table + (table index, column) words
+    // packed form: value table (its last slot the padding value -0.0) and
+    // (table index, column) words, each row padded to whole uint4 groups
     std::vector<T> table;
     std::map<T, uint32_t> slot;
-    bool packable = h.cols < (int64_t{1} << 24) && d.nnz < (int64_t{1} << 31) && !vals.empty();
+    bool packable = h.cols < (int64_t{1} << 24) - 1 && h.rows < (int64_t{1} << 31) && d.nnz < (int64_t{1} << 31) &&
+                    !vals.empty();
     for (size_t e = 0; packable && e < vals.size(); ++e) {
         if (slot.count(vals[e])) continue;
-        if (table.size() == 256) {
+        if (table.size() == 255) {
             packable = false;
             break;
         }
@@ -528,18 +531,42 @@ int upload(const Csr& h, DeviceCsr& d) {
         table.push_back(vals[e]);
     }
     if (packable) {
-        std::vector<uint32_t> packed(vals.size());
-        for (size_t e = 0; e < vals.size(); ++e)
-            packed[e] = (slot[vals[e]] << 24) | static_cast<uint32_t>(h.indices[e]);
-        std::vector<int32_t> off32(h.offsets.begin(), h.offsets.end());
-        if (hipMalloc(reinterpret_cast<void**>(&d.packed), packed.size() * sizeof(uint32_t)) != hipSuccess) return 5;
-        if (hipMalloc(reinterpret_cast<void**>(&d.offsets32), off32.size() * sizeof(int32_t)) != hipSuccess) return 5;
-        if (hipMalloc(&d.table, table.size() * sizeof(T)) != hipSuccess) return 5;
-        if (hipMemcpy(d.packed, packed.data(), packed.size() * sizeof(uint32_t), hipMemcpyHostToDevice) !=
+        const uint32_t pad = (static_cast<uint32_t>(table.size()) << 24) | 0xffffffu;
+        table.push_back(-T(0));
+        std::vector<int64_t> group(static_cast<size_t>(h.rows) + 1, 0);  // first group of each row
+        for (int64_t r = 0; r < h.rows; ++r) {
+            const int64_t len = h.offsets[static_cast<size_t>(r) + 1] - h.offsets[static_cast<size_t>(r)];
+            group[static_cast<size_t>(r) + 1] = group[static_cast<size_t>(r)] + (len + 3) / 4;
+        }
+        if (group.back() >= (int64_t{1} << 31)) return 0;  // no packed form
+        std::vector<uint32_t> words(static_cast<size_t>(group.back()) * 4, pad);
+        for (int64_t r = 0; r < h.rows; ++r) {
+            size_t at = static_cast<size_t>(group[static_cast<size_t>(r)]) * 4;
+            for (int64_t e = h.offsets[static_cast<size_t>(r)]; e < h.offsets[static_cast<size_t>(r) + 1]; ++e)
+                words[at++] = (slot[vals[static_cast<size_t>(e)]] << 24) | static_cast<uint32_t>(h.indices[static_cast<size_t>(e)]);
+        }
+        std::vector<int32_t> order(static_cast<size_t>(h.rows));
+        for (int64_t r = 0; r < h.rows; ++r) order[static_cast<size_t>(r)] = static_cast<int32_t>(r);
+        auto length = [&](int32_t r) { return h.offsets[static_cast<size_t>(r) + 1] - h.offsets[static_cast<size_t>(r)]; };
+        for (size_t k = 0; k + 1 < row_blocks.size(); ++k)
+            std::stable_sort(order.begin() + row_blocks[k], order.begin() + row_blocks[k + 1],
+                             [&](int32_t a, int32_t b) { return length(a) > length(b); });
+        std::vector<int4> desc(order.size());
+        for (size_t s = 0; s < order.size(); ++s) {
+            const size_t r = static_cast<size_t>(order[s]);
+            desc[s] = int4{order[s], static_cast<int32_t>(group[r]), static_cast<int32_t>(group[r + 1]), 0};
+        }
+        if (hipMalloc(reinterpret_cast<void**>(&d.packed4), std::max<size_t>(words.size(), 4) * sizeof(uint32_t)) !=
                 hipSuccess ||
-            hipMemcpy(d.offsets32, off32.data(), off32.size() * sizeof(int32_t), hipMemcpyHostToDevice) !=
-                hipSuccess ||
-            hipMemcpy(d.table, table.data(), table.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            hipMalloc(&d.table, table.size() * sizeof(T)) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void**>(&d.rowdesc), std::max<size_t>(desc.size(), 1) * sizeof(int4)) !=
+                hipSuccess)
+            return 5;
+        if ((!words.empty() &&
+             hipMemcpy(d.packed4, words.data(), words.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) ||
+            hipMemcpy(d.table, table.data(), table.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess ||
+            (!desc.empty() &&
+             hipMemcpy(d.rowdesc, desc.data(), desc.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess))
             return 9;
         d.table_size = static_cast<int32_t>(table.size());
     }
@@ -560,8 +587,19 @@ int device_lie_matrices(int width, int depth, const DeviceCsr** l2t, const Devic
     auto it = cache.find(key);
     if (it == cache.end()) {
         auto dm = std::make_unique<DeviceMatrices>();
-        if (int rc = upload<T>(md->l2t, dm->l2t)) return fail(rc, "uploading the L2T matrix failed");
-        if (int rc = upload<T>(md->t2l, dm->t2l)) return fail(rc, "uploading the T2L matrix failed");
+        // degree blocks: L2T rows are tensor words, T2L rows Hall keys - 1
+        std::vector<int64_t> tensor_blocks, lie_blocks;
+        int64_t start = 0, level = 1;
+        for (int k = 0; k <= depth; ++k) {
+            tensor_blocks.push_back(start);
+            start += level;
+            level *= width;
+        }
+        tensor_blocks.push_back(md->l2t.rows);
+        for (int k = 1; k <= depth; ++k) lie_blocks.push_back(md->degree_begin[static_cast<size_t>(k)] - 1);
+        lie_blocks.push_back(md->t2l.rows);
+        if (int rc = upload<T>(md->l2t, tensor_blocks, dm->l2t)) return fail(rc, "uploading the L2T matrix failed");
+        if (int rc = upload<T>(md->t2l, lie_blocks, dm->t2l)) return fail(rc, "uploading the T2L matrix failed");
         it = cache.emplace(key, std::move(dm)).first;
     }
     *l2t = &it->second->l2t;

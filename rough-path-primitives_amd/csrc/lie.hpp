@@ -56,14 +56,21 @@ struct DeviceCsr {
     int64_t* indices = nullptr;
     int64_t* offsets = nullptr;
     int64_t rows = 0, cols = 0, nnz = 0;
-    // Packed form, present when the matrix has at most 256 distinct values,
-    // fewer than 2^24 columns and fewer than 2^31 entries: each entry is
-    // (index into `table` << 24) | column, rows delimited by 32-bit offsets.
-    // A quarter of the bytes of the CSR arrays.
-    uint32_t* packed = nullptr;
-    int32_t* offsets32 = nullptr;
+    // Packed form, present when the matrix has at most 255 distinct values,
+    // fewer than 2^24 - 1 columns and fewer than 2^31 rows and entries: each
+    // entry is one word (index into `table` << 24) | column, a quarter of the
+    // bytes of the CSR arrays.  Every row is padded to a multiple of four
+    // entries with the word (table_size - 1) << 24 | 0xffffff, whose table
+    // value is -0.0 (a term that leaves any sum unchanged), and read as
+    // uint4 groups.
+    uint4* packed4 = nullptr;
     void* table = nullptr;
     int32_t table_size = 0;  // 0: no packed form
+    // With the packed form: {row, first group, end group, 0} of the rows in
+    // the order of their entry counts, longest first, within each degree
+    // block (so a slice of whole degrees is a permutation of itself); the
+    // packed kernel assigns rows to threads in this order.
+    int4* rowdesc = nullptr;
 };
 
 // Device copies for the current device and dtype, uploaded on first use and
@@ -74,6 +81,7 @@ int device_lie_matrices(int width, int depth, const DeviceCsr** l2t, const Devic
 // out = M[row0 : row0 + rows] arg over a batch with the packed form of M
 // (lie_spmv.hip): the slice's row r is out.begin + r, and its entries, which
 // must name only columns [col0, col0 + arg.size), read arg.begin + c - col0.
+// The slice must consist of whole degrees (DeviceCsr::rowdesc).
 // lie_spmv_fits tells whether the packed kernel applies (packed form present,
 // one argument segment fits in LDS).
 template <typename T>

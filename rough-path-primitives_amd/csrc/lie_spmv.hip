@@ -1,103 +1,165 @@
 // tensor_to_lie / lie_to_tensor over the packed Lie matrices (lie.hpp).
 //
 // The conversions multiply every object of a batch by the same sparse matrix
-// (T2L or L2T).  A work-group stages the argument segments of G objects and
-// the matrix's value table in LDS, then streams the matrix once for all G
-// objects: groups of 8 lanes take a row each, read its packed entries
-// consecutively, gather from LDS and reduce in a fixed order.  Matrix traffic
-// from L2 is therefore 4 bytes per entry per G objects.  A call may take a
-// slice of the rows (the matrix's offsets from row0) whose entries name only
-// the columns [col0, col0 + arg.size), the argument segment staged.
+// (T2L or L2T, block diagonal by degree).  A work-group stages the argument
+// segments of OB objects in LDS, laid out [column][object], and each thread
+// computes one (row, object) output as a sequential sum over the row's
+// entries in ascending column order, acc = acc + v * x with contraction off.
+// That is the reference's sum term by term, for a CSR matrix
+// (cpu/.../linalg/sparse_matrix_vector.hpp:60-71) and for the CSC scatter of
+// the builders' matrices (:146-160, which adds into each row in column
+// order), so results agree bit for bit.
+//
+// Thread t serves object t % OB of row slot t / OB: the OB threads of a slot
+// read the same entries (one address), and their argument reads are OB
+// consecutive words.  Each row's entries are padded to a multiple of four
+// (DeviceCsr::packed4) with entries that add (-0) * 0 = -0, which leaves
+// every sum unchanged (x + -0 = x for every x, signed zeros, infinities and
+// NaN included), so a row is read in 16-byte groups with no per-entry test.  Slots take the rows in the matrix's length order
+// (longest first within each degree, DeviceCsr::rowdesc), so the rows of one
+// wavefront have similar entry counts and the longest rows start first.  The
+// sums are chains of dependent steps, so latency is hidden by many of them:
+// 1024-thread work-groups holding at most 80 KiB of LDS where they can, two
+// to a compute unit (one stages its objects while the other sums), and
+// sixteen entries in flight per thread while sixteen are summed.
 #include "common.hpp"
 #include "lie.hpp"
 
 #include <algorithm>
 
+#pragma clang fp contract(off)
+
 namespace rpp_hip {
 
 namespace {
 
-constexpr int kLieBlock = 1024;
-constexpr int kLieLanes = 8;
-constexpr int kLieTable = 256;
+constexpr int kLieThreads = 1024;
 constexpr size_t kLieLdsMax = 160 * 1024;
+constexpr size_t kLieLdsShared = 80 * 1024;
 
-template <typename T, int G>
-__global__ void __launch_bounds__(kLieBlock)
-k_lie_spmv(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t abs_, int64_t acs, int64_t abeg,
-           int64_t cols, int64_t col0, const uint32_t* packed, const int32_t* offsets, const T* table,
-           int table_size, int64_t rows, int64_t n) {
+template <typename T>
+constexpr size_t lie_table_bytes(int table_size) {
+    return (static_cast<size_t>(table_size) * sizeof(T) + 15) / 16 * 16;
+}
+
+// One packed entry: acc + table[w >> 24] * x[column].  The column is taken
+// relative to the staged segment and clamped to its extra zero slot, where
+// the padding entries (column 0xffffff) land.
+template <typename T, int OB>
+__device__ __forceinline__ T lie_term(T acc, uint32_t w, const T* tab, const T* xs, uint32_t col0, uint32_t cols,
+                                      uint32_t o) {
+    const uint32_t rel = min((w & 0xffffffu) - col0, cols);
+    const T p = tab[w >> 24] * xs[rel * OB + o];
+    return acc + p;
+}
+
+// Two work-groups of kLieThreads per compute unit: 8 waves per SIMD.
+template <typename T, int OB>
+__global__ void __launch_bounds__(kLieThreads) __attribute__((amdgpu_waves_per_eu(8)))
+k_lie_rows(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t abs_, int64_t acs, int64_t abeg,
+           uint32_t cols, uint32_t col0, const uint4* packed4, const int4* rowdesc, const T* table, int table_size,
+           int64_t row0, int rows, int64_t n) {
     extern __shared__ __align__(16) unsigned char lie_raw[];
     T* tab = reinterpret_cast<T*>(lie_raw);
-    T* xs = tab + kLieTable;
-    const int t = threadIdx.x;
-    const int lane = t % kLieLanes;
-    const int grp = t / kLieLanes;
-    constexpr int groups = kLieBlock / kLieLanes;
-    for (int i = t; i < table_size; i += kLieBlock) tab[i] = table[i];
-    for (int64_t b0 = static_cast<int64_t>(blockIdx.x) * G; b0 < n; b0 += static_cast<int64_t>(gridDim.x) * G) {
-        const int gn = static_cast<int>(min<int64_t>(G, n - b0));
-        __syncthreads();
-        for (int g = 0; g < gn; ++g) {
-            const T* a = arg + (b0 + g) * abs_;
-            for (int64_t i = t; i < cols; i += kLieBlock) xs[g * cols + i] = a[(abeg + i) * acs];
+    T* xs = reinterpret_cast<T*>(lie_raw + lie_table_bytes<T>(table_size));
+    constexpr int kSlots = kLieThreads / OB;
+    const uint32_t t = threadIdx.x;
+    const uint32_t o = t % OB;
+    const int slot = static_cast<int>(t / OB);
+    for (int i = static_cast<int>(t); i < table_size; i += kLieThreads) tab[i] = table[i];
+    for (int64_t b0 = static_cast<int64_t>(blockIdx.x) * OB; b0 < n; b0 += static_cast<int64_t>(gridDim.x) * OB) {
+        const uint32_t gn = static_cast<uint32_t>(min<int64_t>(OB, n - b0));
+        __syncthreads();  // the previous group's sums have read xs
+        // staged word c * OB + o (thread t stages object o = t % OB, which
+        // 1024 / OB steps keep fixed): consecutive threads store consecutive
+        // LDS words, and U loads are in flight before the first store (with
+        // two work-groups per compute unit, 64 KiB in flight per CU)
+        constexpr int U = 4;
+        const T* src = arg + (b0 + static_cast<int64_t>(o)) * abs_ + abeg * acs;
+        const bool live = o < gn;
+        for (uint32_t c0 = t / OB; c0 <= cols; c0 += U * kSlots) {
+            T v[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const uint32_t c = c0 + j * kSlots;
+                v[j] = (live && c < cols) ? src[static_cast<int64_t>(c) * acs] : T(0);
+            }
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const uint32_t c = c0 + j * kSlots;
+                if (c <= cols) xs[c * OB + o] = v[j];
+            }
         }
         __syncthreads();
-        for (int64_t row = grp; row < rows; row += groups) {
-            const int32_t e0 = offsets[row], e1 = offsets[row + 1];
-            T acc[G];
+        for (int s = slot; s < rows; s += kSlots) {
+            // {row, first group, end group} of the slot's row: one load, no
+            // chain of dependent index reads.  Each group holds four entries;
+            // D groups are summed while the next D are in flight, so the
+            // longest rows (the critical path of a work-group) wait on one
+            // load round trip per 4 * D entries.
+            const int4 d = rowdesc[row0 + s];
+            T acc = T(0);
+            constexpr int D = 4;
+            uint4 u[D];
 #pragma unroll
-            for (int g = 0; g < G; ++g) acc[g] = T(0);
-            // four entries per lane in flight: the loads of one step are all
-            // issued before the first gather waits on them
-            constexpr int U = 4;
-            for (int32_t e = e0 + lane; e < e1; e += U * kLieLanes) {
-                uint32_t u[U];
+            for (int j = 0; j < D; ++j) u[j] = d.y + j < d.z ? packed4[d.y + j] : make_uint4(0, 0, 0, 0);
+            for (int g = d.y; g < d.z; g += D) {
+                uint4 nx[D];
 #pragma unroll
-                for (int j = 0; j < U; ++j) {
-                    const int32_t ej = e + j * kLieLanes;
-                    u[j] = ej < e1 ? packed[ej] : 0u;
-                }
+                for (int j = 0; j < D; ++j) nx[j] = g + D + j < d.z ? packed4[g + D + j] : make_uint4(0, 0, 0, 0);
 #pragma unroll
-                for (int j = 0; j < U; ++j) {
-                    if (e + j * kLieLanes < e1) {
-                        const T v = tab[u[j] >> 24];
-                        const uint32_t c = (u[j] & 0xffffffu) - static_cast<uint32_t>(col0);
-#pragma unroll
-                        for (int g = 0; g < G; ++g)
-                            if (g < gn) acc[g] += v * xs[g * cols + c];
+                for (int j = 0; j < D; ++j) {
+                    if (g + j < d.z) {
+                        acc = lie_term<T, OB>(acc, u[j].x, tab, xs, col0, cols, o);
+                        acc = lie_term<T, OB>(acc, u[j].y, tab, xs, col0, cols, o);
+                        acc = lie_term<T, OB>(acc, u[j].z, tab, xs, col0, cols, o);
+                        acc = lie_term<T, OB>(acc, u[j].w, tab, xs, col0, cols, o);
                     }
                 }
-            }
 #pragma unroll
-            for (int g = 0; g < G; ++g) {
-#pragma unroll
-                for (int off = kLieLanes / 2; off > 0; off >>= 1) acc[g] += __shfl_down(acc[g], off, kLieLanes);
+                for (int j = 0; j < D; ++j) u[j] = nx[j];
             }
-            if (lane == 0) {
-#pragma unroll
-                for (int g = 0; g < G; ++g)
-                    if (g < gn) out[(b0 + g) * obs + (obeg + row) * ocs] = acc[g];
-            }
+            if (o < gn) out[(b0 + o) * obs + (obeg + (d.x - row0)) * ocs] = acc;
         }
     }
 }
 
-template <typename T, int G>
+template <typename T>
+size_t lie_lds_bytes(const DeviceCsr& m, int64_t cols, int ob) {
+    return lie_table_bytes<T>(m.table_size) + static_cast<size_t>(cols + 1) * ob * sizeof(T);
+}
+
+// Objects per work-group: the most (up to 8) whose staging fits in 80 KiB,
+// so that at least two work-groups share a compute unit; else one object if
+// it fits in the whole LDS; 0 if not even that.
+template <typename T>
+int lie_objects_per_group(const DeviceCsr& m, int64_t cols) {
+    for (int ob : {8, 4, 2, 1})
+        if (lie_lds_bytes<T>(m, cols, ob) <= kLieLdsShared) return ob;
+    return lie_lds_bytes<T>(m, cols, 1) <= kLieLdsMax ? 1 : 0;
+}
+
+template <typename T, int OB>
 int lie_launch(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg, const DeviceCsr& m,
-               int64_t row0, int64_t rows, int64_t col0, size_t lds, hipStream_t stream) {
-    static const bool raised = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_lie_spmv<T, G>),
+               int64_t row0, int64_t rows, int64_t col0, hipStream_t stream) {
+    static const bool raised = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_lie_rows<T, OB>),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                                    static_cast<int>(kLieLdsMax)) == hipSuccess;
+    const size_t lds = lie_lds_bytes<T>(m, arg.size, OB);
     if (!raised && lds > 64 * 1024) return fail(RPP_HIP_OUT_OF_RESOURCES, "cannot raise the LDS limit");
-    const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(4, static_cast<int64_t>(kLieLdsMax / lds)));
-    const int64_t groups = (n + G - 1) / G;
+    // object groups are dealt to work-groups in turn, so the grid is what the
+    // compute units hold at once (threads, registers and this LDS size)
+    int resident = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_lie_rows<T, OB>, kLieThreads, lds) != hipSuccess)
+        resident = 1;
+    const int64_t per_cu = std::max(1, resident);
+    const int64_t groups = (n + OB - 1) / OB;
     const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min(groups, device_cu_count() * per_cu)));
-    k_lie_spmv<T, G><<<grid, kLieBlock, lds, stream>>>(
+    k_lie_rows<T, OB><<<grid, kLieThreads, lds, stream>>>(
         static_cast<T*>(out.data), out.batch_stride, out.coeff_stride, out.begin, static_cast<const T*>(arg.data),
-        arg.batch_stride, arg.coeff_stride, arg.begin, arg.size, col0, m.packed, m.offsets32 + row0,
-        static_cast<const T*>(m.table), m.table_size, rows, n);
-    return check_hip(hipGetLastError(), "k_lie_spmv launch");
+        arg.batch_stride, arg.coeff_stride, arg.begin, static_cast<uint32_t>(arg.size), static_cast<uint32_t>(col0),
+        m.packed4, m.rowdesc, static_cast<const T*>(m.table), m.table_size, row0, static_cast<int>(rows), n);
+    return check_hip(hipGetLastError(), "k_lie_rows launch");
 }
 
 }  // namespace
@@ -106,26 +168,20 @@ template <typename T>
 int launch_lie_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg, const DeviceCsr& m,
                     int64_t row0, int64_t rows, int64_t col0, hipStream_t stream) {
     if (n <= 0 || rows <= 0) return 0;
-    if (m.table_size == 0 || row0 < 0 || row0 + rows > m.rows || col0 < 0 || col0 + arg.size > m.cols)
+    if (m.table_size == 0 || !m.rowdesc || !m.packed4 || row0 < 0 || row0 + rows > m.rows || col0 < 0 || col0 + arg.size > m.cols)
         return fail(RPP_HIP_NOT_IMPLEMENTED, "no packed Lie matrix for this row and column range");
-    const size_t per_object = static_cast<size_t>(std::max<int64_t>(arg.size, 1)) * sizeof(T);
-    const size_t fixed = kLieTable * sizeof(T);
-    const int64_t fit = static_cast<int64_t>((kLieLdsMax - fixed) / per_object);
-    if (fit < 1) return fail(RPP_HIP_NOT_IMPLEMENTED, "Lie argument does not fit in LDS");
-    const int G = static_cast<int>(std::min<int64_t>({fit, 4, n}));
-    const size_t lds = fixed + static_cast<size_t>(G) * per_object;
-    switch (G) {
-        case 1: return lie_launch<T, 1>(n, out, arg, m, row0, rows, col0, lds, stream);
-        case 2: return lie_launch<T, 2>(n, out, arg, m, row0, rows, col0, lds, stream);
-        case 3: return lie_launch<T, 3>(n, out, arg, m, row0, rows, col0, lds, stream);
-        default: return lie_launch<T, 4>(n, out, arg, m, row0, rows, col0, lds, stream);
+    switch (lie_objects_per_group<T>(m, arg.size)) {
+        case 8: return lie_launch<T, 8>(n, out, arg, m, row0, rows, col0, stream);
+        case 4: return lie_launch<T, 4>(n, out, arg, m, row0, rows, col0, stream);
+        case 2: return lie_launch<T, 2>(n, out, arg, m, row0, rows, col0, stream);
+        case 1: return lie_launch<T, 1>(n, out, arg, m, row0, rows, col0, stream);
+        default: return fail(RPP_HIP_NOT_IMPLEMENTED, "Lie argument does not fit in LDS");
     }
 }
 
 template <typename T>
 bool lie_spmv_fits(const DeviceCsr& m, const rpp_hip_vector_batch& arg) {
-    const size_t per_object = static_cast<size_t>(std::max<int64_t>(arg.size, 1)) * sizeof(T);
-    return m.table_size > 0 && kLieTable * sizeof(T) + per_object <= kLieLdsMax;
+    return m.table_size > 0 && m.rowdesc && m.packed4 && lie_objects_per_group<T>(m, arg.size) > 0;
 }
 
 template int launch_lie_spmv<float>(int64_t, const rpp_hip_vector_batch&, const rpp_hip_vector_batch&,

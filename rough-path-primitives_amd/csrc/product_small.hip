@@ -71,8 +71,8 @@ k_product_small(View<T> out, View<const T> lhs, View<const T> rhs, View<const T>
         for (int s = 0; s < (G::dim + NT - 1) / NT; ++s) {
             const int i = t + s * NT;
             if (G::dim % NT == 0 || i < G::dim) {
-                if (FULL || (lb <= i && i < le)) sl[i] = L[i];
-                if (FULL || (rb <= i && i < re)) sr[i] = R[i];
+                if (FULL || (lb <= i && i < le)) sl[i] = stream_load(&L[i]);
+                if (FULL || (rb <= i && i < re)) sr[i] = stream_load(&R[i]);
             }
         }
         __syncthreads();
@@ -99,7 +99,7 @@ k_product_small(View<T> out, View<const T> lhs, View<const T> rhs, View<const T>
                     }
                     T v = beta * acc;
                     if (use_add) v = alpha * A[G::start(k) + w] + v;
-                    O[G::start(k) + w] = v;
+                    stream_store(v, &O[G::start(k) + w]);
                 }
             }
         }

@@ -81,6 +81,27 @@ def test_oracle_pairing_and_chains():
     assert rel_err(lg, g["log"]) <= TOL
 
 
+def test_oracle_lie_conversions_exact():
+    """tensor_to_lie and lie_to_tensor at (3,8): the oracle's CSC product with
+    the library's Hall matrices (byte-identical to the reference builders')
+    reproduces the reference's outputs bit for bit: both sum each row in
+    ascending column order, one rounding per product and per addition."""
+    import rpp_hip
+
+    g = fixture("logsig_w3d8")
+    lie = rpp_hip.lie_basis(3, 8)
+    ts, dim = lie["true_size"], lie["tensor_dim"]
+    m = lie["t2l"]
+    got = np.zeros_like(g["lie"])
+    O.spmv(m["data"], m["indices"], m["offsets"], dim, got, g["log"], csc=True, out_offset=1, out_size=ts - 1)
+    assert np.array_equal(got[:, 1:], g["lie"][:, 1:])
+    m = lie["l2t"]
+    back = np.zeros_like(g["back"])
+    O.spmv(m["data"], m["indices"], m["offsets"], ts - 1, back, g["lie"], csc=True, out_offset=0, out_size=dim,
+           arg_offset=1)
+    assert np.array_equal(back, g["back"])
+
+
 LINEAR = [("tensor", 3, 4), ("lie", 3, 4)]
 
 
@@ -283,6 +304,10 @@ def test_gpu_against_golden(dev):
     lie = t(np.zeros(g["lie"].shape))
     rpp_hip.tensor_to_lie(lie, lg, b38)
     assert_close(lie.cpu().numpy()[:, 1:], g["lie"][:, 1:], np.float64, "tensor_to_lie (3,8)")
+    # on the reference's own log-signatures the conversions agree bit for bit
+    rpp_hip.tensor_to_lie(lie, t(g["log"]), b38)
+    assert np.array_equal(lie.cpu().numpy()[:, 1:], g["lie"][:, 1:]), "tensor_to_lie (3,8) not bit-identical"
     back = torch.empty_like(lg)
     rpp_hip.lie_to_tensor(back, t(g["lie"]), b38)
     assert_close(back.cpu().numpy(), g["back"], np.float64, "lie_to_tensor (3,8)")
+    assert np.array_equal(back.cpu().numpy(), g["back"]), "lie_to_tensor (3,8) not bit-identical"

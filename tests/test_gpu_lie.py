@@ -65,6 +65,8 @@ def test_tensor_to_lie_and_back(dev, shape, dtype):
     rpp_hip.tensor_to_lie(lie_out, to_dev(x, dev), basis)
     got_lie = lie_out.cpu().numpy()
     assert_close(got_lie, exp_lie, dtype, f"t2l {shape}")
+    # each row is summed in the reference's order: bit for bit
+    assert np.array_equal(got_lie, exp_lie), f"t2l {shape} not bit-identical"
     l2t = lie["l2t"]
     exp_t = np.zeros((n, dim), dtype=dtype)
     O.spmv(l2t["data"], l2t["indices"], l2t["offsets"], ts - 1, exp_t, got_lie, csc=True, out_offset=0,
@@ -72,6 +74,7 @@ def test_tensor_to_lie_and_back(dev, shape, dtype):
     t_out = to_dev(np.full((n, dim), np.nan, dtype=dtype), dev)
     rpp_hip.lie_to_tensor(t_out, lie_out, basis)
     assert_close(t_out.cpu().numpy(), exp_t, dtype, f"l2t {shape}")
+    assert np.array_equal(t_out.cpu().numpy(), exp_t), f"l2t {shape} not bit-identical"
 
 
 def test_log_signature_pipeline_matches_reference(dev):
@@ -94,6 +97,9 @@ def test_log_signature_pipeline_matches_reference(dev):
     lie = torch.zeros((n, lie_ref.shape[1]), dtype=torch.float64, device=dev)
     rpp_hip.tensor_to_lie(lie, lg, basis)
     assert_close(lie.cpu().numpy()[:, 1:], lie_ref[:, 1:], np.float64, "C3 pipeline")
+    # from the reference's own log-signatures the conversion is bit-identical
+    rpp_hip.tensor_to_lie(lie, torch.tensor(log_ref, device=dev), basis)
+    assert np.array_equal(lie.cpu().numpy()[:, 1:], lie_ref[:, 1:])
 
 
 def _lie_expected(lie, x_zero, to_lie, ts, dim, dtype):
@@ -162,4 +168,31 @@ def test_lie_conversions_on_windows(dev, shape, to_lie, dtype):
             rpp_hip.lie_to_tensor(rpp_hip.Batch(out, *tw), rpp_hip.Batch(to_dev(yin, dev), *lw), basis)
         got = out.cpu().numpy()
         assert np.all(got[:, :lo] == 7.0) and np.all(got[:, hi:] == 7.0), (tw, lw)
-        assert_close(got[:, lo:hi], exp[:, lo:hi], dtype, f"{'t2l' if to_lie else 'l2t'} {shape} {tw} {lw}")
+        what = f"{'t2l' if to_lie else 'l2t'} {shape} {tw} {lw}"
+        assert_close(got[:, lo:hi], exp[:, lo:hi], dtype, what)
+        assert np.array_equal(got[:, lo:hi], exp[:, lo:hi]), what + " not bit-identical"
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_lie_conversions_large_batch(dev, dtype):
+    """(3,8) over 4099 objects: more object groups than resident work-groups
+    (each work-group takes several in turn) and a last group that is only
+    partly filled; both conversions bit-identical to the oracle."""
+    w, N, n = 3, 8, 4099
+    basis = rpp_hip.TensorBasis(w, N)
+    lie = rpp_hip.lie_basis(w, N)
+    ts, dim = lie["true_size"], lie["tensor_dim"]
+    rng = np.random.default_rng(0x38)
+    x = rng.uniform(-1, 1, size=(n, dim)).astype(dtype)
+    exp_lie = _lie_expected(lie, x, True, ts, dim, dtype)
+    lie_out = to_dev(np.full((n, ts), 7.0, dtype=dtype), dev)
+    rpp_hip.tensor_to_lie(lie_out, to_dev(x, dev), basis)
+    got = lie_out.cpu().numpy()
+    assert np.all(got[:, 0] == 7.0)
+    assert np.array_equal(got[:, 1:], exp_lie[:, 1:])
+    y = rng.uniform(-1, 1, size=(n, ts)).astype(dtype)
+    y[:, 0] = np.nan  # slot 0 is never read
+    exp_t = _lie_expected(lie, np.nan_to_num(y, nan=0.0), False, ts, dim, dtype)
+    t_out = to_dev(np.full((n, dim), np.nan, dtype=dtype), dev)
+    rpp_hip.lie_to_tensor(t_out, to_dev(y, dev), basis)
+    assert np.array_equal(t_out.cpu().numpy(), exp_t)
