@@ -548,9 +548,15 @@ int upload(const Csr& h, const std::vector<int64_t>& row_blocks, DeviceCsr& d) {
         std::vector<int32_t> order(static_cast<size_t>(h.rows));
         for (int64_t r = 0; r < h.rows; ++r) order[static_cast<size_t>(r)] = static_cast<int32_t>(r);
         auto length = [&](int32_t r) { return h.offsets[static_cast<size_t>(r) + 1] - h.offsets[static_cast<size_t>(r)]; };
-        for (size_t k = 0; k + 1 < row_blocks.size(); ++k)
-            std::stable_sort(order.begin() + row_blocks[k], order.begin() + row_blocks[k + 1],
-                             [&](int32_t a, int32_t b) { return length(a) > length(b); });
+        // Long rows are taken longest first, so that the critical path
+        // starts early and a wavefront's rows end together.  Short rows stay
+        // in order: then a wavefront's stores cover consecutive words, which
+        // matters when the output (a tensor) is much larger than the argument.
+        d.long_rows = h.rows > 0 && group.back() >= 4 * h.rows ? 1 : 0;
+        if (d.long_rows)
+            for (size_t k = 0; k + 1 < row_blocks.size(); ++k)
+                std::stable_sort(order.begin() + row_blocks[k], order.begin() + row_blocks[k + 1],
+                                 [&](int32_t a, int32_t b) { return length(a) > length(b); });
         std::vector<int4> desc(order.size());
         for (size_t s = 0; s < order.size(); ++s) {
             const size_t r = static_cast<size_t>(order[s]);

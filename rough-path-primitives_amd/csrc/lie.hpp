@@ -67,10 +67,12 @@ struct DeviceCsr {
     void* table = nullptr;
     int32_t table_size = 0;  // 0: no packed form
     // With the packed form: {row, first group, end group, 0} of the rows in
-    // the order of their entry counts, longest first, within each degree
-    // block (so a slice of whole degrees is a permutation of itself); the
-    // packed kernel assigns rows to threads in this order.
+    // the order the packed kernel assigns them to threads: with long rows
+    // (an average of at least four groups, 16 entries), by entry count,
+    // longest first, within each degree block (so a slice of whole degrees
+    // is a permutation of itself); otherwise in row order.
     int4* rowdesc = nullptr;
+    int32_t long_rows = 0;
 };
 
 // Device copies for the current device and dtype, uploaded on first use and

@@ -54,7 +54,12 @@ __device__ __forceinline__ T lie_term(T acc, uint32_t w, const T* tab, const T* 
 }
 
 // Two work-groups of kLieThreads per compute unit: 8 waves per SIMD.
-template <typename T, int OB>
+// A thread sums R rows at a time (their loads overlap) and keeps D groups of
+// each in flight while D are summed: (R, D) = (1, 4) for matrices with long
+// rows, whose longest rows are a work-group's critical path, and (3 or 4, 1)
+// for short rows, where a row's two dependent loads (descriptor, entries)
+// would otherwise dominate.
+template <typename T, int OB, int R, int D>
 __global__ void __launch_bounds__(kLieThreads) __attribute__((amdgpu_waves_per_eu(8)))
 k_lie_rows(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t abs_, int64_t acs, int64_t abeg,
            uint32_t cols, uint32_t col0, const uint4* packed4, const int4* rowdesc, const T* table, int table_size,
@@ -91,35 +96,53 @@ k_lie_rows(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t
             }
         }
         __syncthreads();
-        for (int s = slot; s < rows; s += kSlots) {
-            // {row, first group, end group} of the slot's row: one load, no
-            // chain of dependent index reads.  Each group holds four entries;
-            // D groups are summed while the next D are in flight, so the
-            // longest rows (the critical path of a work-group) wait on one
-            // load round trip per 4 * D entries.
-            const int4 d = rowdesc[row0 + s];
-            T acc = T(0);
-            constexpr int D = 4;
-            uint4 u[D];
+        for (int s0 = slot; s0 < rows; s0 += R * kSlots) {
+            // {row, first group, end group} of each row: one load, no chain
+            // of dependent index reads; each group holds four entries
+            int4 d[R];
+            int len[R];
+            T acc[R];
+            int most = 0;
 #pragma unroll
-            for (int j = 0; j < D; ++j) u[j] = d.y + j < d.z ? packed4[d.y + j] : make_uint4(0, 0, 0, 0);
-            for (int g = d.y; g < d.z; g += D) {
-                uint4 nx[D];
+            for (int r = 0; r < R; ++r) {
+                const int s = s0 + r * kSlots;
+                d[r] = s < rows ? rowdesc[row0 + s] : make_int4(0, 0, 0, 0);
+                len[r] = d[r].z - d[r].y;
+                most = max(most, len[r]);
+                acc[r] = T(0);
+            }
+            uint4 u[R][D];
 #pragma unroll
-                for (int j = 0; j < D; ++j) nx[j] = g + D + j < d.z ? packed4[g + D + j] : make_uint4(0, 0, 0, 0);
+            for (int r = 0; r < R; ++r)
 #pragma unroll
-                for (int j = 0; j < D; ++j) {
-                    if (g + j < d.z) {
-                        acc = lie_term<T, OB>(acc, u[j].x, tab, xs, col0, cols, o);
-                        acc = lie_term<T, OB>(acc, u[j].y, tab, xs, col0, cols, o);
-                        acc = lie_term<T, OB>(acc, u[j].z, tab, xs, col0, cols, o);
-                        acc = lie_term<T, OB>(acc, u[j].w, tab, xs, col0, cols, o);
+                for (int j = 0; j < D; ++j) u[r][j] = j < len[r] ? packed4[d[r].y + j] : make_uint4(0, 0, 0, 0);
+            for (int k = 0; k < most; k += D) {
+                uint4 nx[R][D];
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int j = 0; j < D; ++j)
+                        nx[r][j] = k + D + j < len[r] ? packed4[d[r].y + k + D + j] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+#pragma unroll
+                    for (int j = 0; j < D; ++j) {
+                        if (k + j < len[r]) {
+                            acc[r] = lie_term<T, OB>(acc[r], u[r][j].x, tab, xs, col0, cols, o);
+                            acc[r] = lie_term<T, OB>(acc[r], u[r][j].y, tab, xs, col0, cols, o);
+                            acc[r] = lie_term<T, OB>(acc[r], u[r][j].z, tab, xs, col0, cols, o);
+                            acc[r] = lie_term<T, OB>(acc[r], u[r][j].w, tab, xs, col0, cols, o);
+                        }
                     }
                 }
 #pragma unroll
-                for (int j = 0; j < D; ++j) u[j] = nx[j];
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int j = 0; j < D; ++j) u[r][j] = nx[r][j];
             }
-            if (o < gn) out[(b0 + o) * obs + (obeg + (d.x - row0)) * ocs] = acc;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (s0 + r * kSlots < rows && o < gn) out[(b0 + o) * obs + (obeg + (d[r].x - row0)) * ocs] = acc[r];
         }
     }
 }
@@ -130,19 +153,23 @@ size_t lie_lds_bytes(const DeviceCsr& m, int64_t cols, int ob) {
 }
 
 // Objects per work-group: the most (up to 8) whose staging fits in 80 KiB,
-// so that at least two work-groups share a compute unit; else one object if
-// it fits in the whole LDS; 0 if not even that.
+// so that two work-groups share a compute unit; else two objects, or one,
+// in the whole LDS (one work-group per compute unit: each table read and
+// each step of a long row then serves two objects, which measured faster
+// than two one-object work-groups); 0 if not even one fits.
 template <typename T>
 int lie_objects_per_group(const DeviceCsr& m, int64_t cols) {
-    for (int ob : {8, 4, 2, 1})
+    for (int ob : {8, 4, 2})
         if (lie_lds_bytes<T>(m, cols, ob) <= kLieLdsShared) return ob;
-    return lie_lds_bytes<T>(m, cols, 1) <= kLieLdsMax ? 1 : 0;
+    for (int ob : {2, 1})
+        if (lie_lds_bytes<T>(m, cols, ob) <= kLieLdsMax) return ob;
+    return 0;
 }
 
-template <typename T, int OB>
+template <typename T, int OB, int R, int D>
 int lie_launch(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg, const DeviceCsr& m,
                int64_t row0, int64_t rows, int64_t col0, hipStream_t stream) {
-    static const bool raised = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_lie_rows<T, OB>),
+    static const bool raised = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_lie_rows<T, OB, R, D>),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                                    static_cast<int>(kLieLdsMax)) == hipSuccess;
     const size_t lds = lie_lds_bytes<T>(m, arg.size, OB);
@@ -150,12 +177,12 @@ int lie_launch(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_
     // object groups are dealt to work-groups in turn, so the grid is what the
     // compute units hold at once (threads, registers and this LDS size)
     int resident = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_lie_rows<T, OB>, kLieThreads, lds) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_lie_rows<T, OB, R, D>, kLieThreads, lds) != hipSuccess)
         resident = 1;
     const int64_t per_cu = std::max(1, resident);
     const int64_t groups = (n + OB - 1) / OB;
     const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min(groups, device_cu_count() * per_cu)));
-    k_lie_rows<T, OB><<<grid, kLieThreads, lds, stream>>>(
+    k_lie_rows<T, OB, R, D><<<grid, kLieThreads, lds, stream>>>(
         static_cast<T*>(out.data), out.batch_stride, out.coeff_stride, out.begin, static_cast<const T*>(arg.data),
         arg.batch_stride, arg.coeff_stride, arg.begin, static_cast<uint32_t>(arg.size), static_cast<uint32_t>(col0),
         m.packed4, m.rowdesc, static_cast<const T*>(m.table), m.table_size, row0, static_cast<int>(rows), n);
@@ -170,11 +197,18 @@ int launch_lie_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_ve
     if (n <= 0 || rows <= 0) return 0;
     if (m.table_size == 0 || !m.rowdesc || !m.packed4 || row0 < 0 || row0 + rows > m.rows || col0 < 0 || col0 + arg.size > m.cols)
         return fail(RPP_HIP_NOT_IMPLEMENTED, "no packed Lie matrix for this row and column range");
-    switch (lie_objects_per_group<T>(m, arg.size)) {
-        case 8: return lie_launch<T, 8>(n, out, arg, m, row0, rows, col0, stream);
-        case 4: return lie_launch<T, 4>(n, out, arg, m, row0, rows, col0, stream);
-        case 2: return lie_launch<T, 2>(n, out, arg, m, row0, rows, col0, stream);
-        case 1: return lie_launch<T, 1>(n, out, arg, m, row0, rows, col0, stream);
+    const bool long_rows = m.long_rows != 0;
+    // joint rows of short-row matrices: as many as fit in 64 registers
+    constexpr int RS = sizeof(T) == 8 ? 3 : 4;
+    switch (lie_objects_per_group<T>(m, arg.size) * (long_rows ? -1 : 1)) {
+        case 8: return lie_launch<T, 8, RS, 1>(n, out, arg, m, row0, rows, col0, stream);
+        case 4: return lie_launch<T, 4, RS, 1>(n, out, arg, m, row0, rows, col0, stream);
+        case 2: return lie_launch<T, 2, RS, 1>(n, out, arg, m, row0, rows, col0, stream);
+        case 1: return lie_launch<T, 1, RS, 1>(n, out, arg, m, row0, rows, col0, stream);
+        case -8: return lie_launch<T, 8, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
+        case -4: return lie_launch<T, 4, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
+        case -2: return lie_launch<T, 2, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
+        case -1: return lie_launch<T, 1, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
         default: return fail(RPP_HIP_NOT_IMPLEMENTED, "Lie argument does not fit in LDS");
     }
 }
