@@ -47,6 +47,13 @@ constexpr int ipow(int b, int e) {
     return r;
 }
 
+// 1/m.  Every use has m fixed after unrolling, so the division folds to a
+// literal, correctly rounded like a runtime division: no table, no load.
+template <typename T>
+__device__ __forceinline__ T recip(int m) {
+    return T(1) / T(m);
+}
+
 template <int W, int N, int NT, int M, int RL>
 struct Geo {
     // parents of level i (> M) inside one subtree: words of level i-1 under q
@@ -216,13 +223,13 @@ struct Tile {
 // Chain values A^(k)_M along the prefix q for every target k > M (uniform):
 // ach[k] = Horner of S_0 .. S_M with x[q_i] / (k - i + 1).
 template <typename T, int W, int N, int NT, int M, int RL>
-__device__ __forceinline__ void chain_values(const Tile<T, W, N, NT, M, RL>& S, const T* xs, const T* inv, int q,
+__device__ __forceinline__ void chain_values(const Tile<T, W, N, NT, M, RL>& S, const T* xs, int q,
                                              T (&ach)[N + 1]) {
 #pragma unroll
     for (int k = M + 1; k <= N; ++k) {
         T a = S.s0;
 #pragma unroll
-        for (int i = 1; i <= M; ++i) a = a * (xs[S.letter(q, i)] * inv[k - i + 1]) + S.chain(i);
+        for (int i = 1; i <= M; ++i) a = (a * recip<T>(k - i + 1)) * xs[S.letter(q, i)] + S.chain(i);
         if (M == 0) a = S.root;
         ach[k] = a;
     }
@@ -230,16 +237,16 @@ __device__ __forceinline__ void chain_values(const Tile<T, W, N, NT, M, RL>& S, 
 
 // One in-place Chen step S <- S (x) exp(x) on the tile, level-synchronous
 // (see the file comment).  xs: W increments (uniform, unchanged until the
-// step returns), inv[m] = 1/m, lds: two buffers of G::lds_elems; phase i
+// step returns), lds: two buffers of G::lds_elems; phase i
 // reads A^(k)_{i-1} at prev[(k - i) * parents(i) + p] and writes A^(k)_i at
 // cur[(k - i - 1) * words(i) + p*W + b].
 template <typename T, int W, int N, int NT, int M, int RL>
-__device__ __forceinline__ void chen_step_reg(Tile<T, W, N, NT, M, RL>& S, const T* xs, const T* inv, T* lds0,
+__device__ __forceinline__ void chen_step_reg(Tile<T, W, N, NT, M, RL>& S, const T* xs, T* lds0,
                                               T* lds1, int q) {
     using G = Geo<W, N, NT, M, RL>;
     const int t = threadIdx.x;
     T ach[N + 1];
-    chain_values<T, W, N, NT, M, RL>(S, xs, inv, q, ach);
+    chain_values<T, W, N, NT, M, RL>(S, xs, q, ach);
     T xr[W];
 #pragma unroll
     for (int b = 0; b < W; ++b) xr[b] = xs[b];
@@ -253,12 +260,13 @@ __device__ __forceinline__ void chen_step_reg(Tile<T, W, N, NT, M, RL>& S, const
             if (p < G::parents(i)) {
 #pragma unroll
                 for (int k = N; k >= i; --k) {
-                    const T c = inv[k - i + 1];
+                    // the parent value scaled once, then W fused multiply-adds
                     const T ap = (i == M + 1) ? ach[k] : prev[(k - i) * G::parents(i) + p];
+                    const T apc = ap * recip<T>(k - i + 1);
 #pragma unroll
                     for (int b = 0; b < W; ++b) {
                         T& sv = S.word(i, s, b);
-                        const T v = ap * (xr[b] * c) + sv;
+                        const T v = apc * xr[b] + sv;
                         if (k == i) {
                             sv = v;
                         } else {
@@ -276,7 +284,7 @@ __device__ __forceinline__ void chen_step_reg(Tile<T, W, N, NT, M, RL>& S, const
     for (int k = M; k >= 1; --k) {
         T a = S.s0;
 #pragma unroll
-        for (int i = 1; i <= k; ++i) a = a * (xs[S.letter(q, i)] * inv[k - i + 1]) + S.chain(i);
+        for (int i = 1; i <= k; ++i) a = (a * recip<T>(k - i + 1)) * xs[S.letter(q, i)] + S.chain(i);
         S.set_chain(k, a);
     }
 }
@@ -292,9 +300,7 @@ k_sig_fwd_reg(DevBasis B, int64_t n_items, const int64_t* order, const T* incr, 
     // increments, double-buffered by parity: a step still reading increment t
     // (its chain update follows the last barrier) never sees t + 1 arrive
     __shared__ T xs[2][W];
-    __shared__ T inv[N + 2];
     __shared__ unsigned slot;
-    if (threadIdx.x <= N + 1) inv[threadIdx.x] = threadIdx.x == 0 ? T(1) : T(1) / T(threadIdx.x);
     const int64_t dim = B.start[N + 1];
     for (;;) {
         __syncthreads();
@@ -317,7 +323,7 @@ k_sig_fwd_reg(DevBasis B, int64_t n_items, const int64_t* order, const T* incr, 
                 if (t + 1 < t1) xnext = incr[(t + 1) * W + threadIdx.x];
             }
             __syncthreads();
-            chen_step_reg<T, W, N, NT, M, RL>(S, xt, inv, lds0, lds1, q);
+            chen_step_reg<T, W, N, NT, M, RL>(S, xt, lds0, lds1, q);
             if (ck_every > 0 && ((t - t0 + 1) % ck_every) == 0) {
                 T* dst = ckpts + (ck_offs[path] + (t - t0 + 1) / ck_every - 1) * dim;
                 S.template move<true>(B, dst, 1, q, q == 0);
@@ -339,8 +345,6 @@ k_fmexp_tile(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int6
     __shared__ T lds1[G::lds_elems];
     __shared__ T sstate[G::state_elems];
     __shared__ T xs[W];
-    __shared__ T inv[N + 2];
-    if (threadIdx.x <= N + 1) inv[threadIdx.x] = threadIdx.x == 0 ? T(1) : T(1) / T(threadIdx.x);
     for (int64_t item = blockIdx.x; item < n * G::P; item += gridDim.x) {
         const int64_t b = item / G::P;
         const int q = static_cast<int>(item - b * G::P);
@@ -349,7 +353,7 @@ k_fmexp_tile(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int6
         __syncthreads();
         if (threadIdx.x < W) xs[threadIdx.x] = ex.obj(b)[1 + threadIdx.x];
         __syncthreads();
-        chen_step_reg<T, W, N, NT, M, RL>(S, xs, inv, lds0, lds1, q);
+        chen_step_reg<T, W, N, NT, M, RL>(S, xs, lds0, lds1, q);
         S.template move<true>(B, out.obj(b), 1, q, q == 0);
     }
 }
@@ -379,12 +383,12 @@ int fmexp_launch(const DevBasis& B, const rpp_hip_tensor_batch& out, const rpp_h
 //   lds0/lds1: Abar^(K)_{i-1}, K in [i, N], at (K - i) * parents(i) + p.
 template <typename T, int W, int N, int NT, int M, int RL>
 __device__ __forceinline__ void bwd_step(const Tile<T, W, N, NT, M, RL>& S, Tile<T, W, N, NT, M, RL>& Gt, const T* xs,
-                                         const T* inv, T* astore, T* lds0, T* lds1, int q, T (&xbar)[W], T& xq,
+                                         T* astore, T* lds0, T* lds1, int q, T (&xbar)[W], T& xq,
                                          T& s0bar) {
     using G = Geo<W, N, NT, M, RL>;
     const int t = threadIdx.x;
     T ach[N + 1];
-    chain_values<T, W, N, NT, M, RL>(S, xs, inv, q, ach);
+    chain_values<T, W, N, NT, M, RL>(S, xs, q, ach);
     T xr[W];
 #pragma unroll
     for (int b = 0; b < W; ++b) xr[b] = xs[b];
@@ -397,13 +401,13 @@ __device__ __forceinline__ void bwd_step(const Tile<T, W, N, NT, M, RL>& S, Tile
             if (p < G::parents(i)) {
 #pragma unroll
                 for (int K = N; K > i; --K) {
-                    const T c = inv[K - i + 1];
                     const T ap =
                         (i == M + 1) ? ach[K] : astore[G::astore_offset(i - 1) + (K - i) * G::parents(i) + p];
+                    const T apc = ap * recip<T>(K - i + 1);
 #pragma unroll
                     for (int b = 0; b < W; ++b)
                         astore[G::astore_offset(i) + (K - i - 1) * G::words(i) + p * W + b] =
-                            ap * (xr[b] * c) + S.word(i, s, b);
+                            apc * xr[b] + S.word(i, s, b);
                 }
             }
         }
@@ -423,7 +427,7 @@ __device__ __forceinline__ void bwd_step(const Tile<T, W, N, NT, M, RL>& S, Tile
                 for (int b = 0; b < W; ++b) gsum[b] = T(0);
 #pragma unroll
                 for (int K = N; K >= i; --K) {
-                    const T c = inv[K - i + 1];
+                    const T c = recip<T>(K - i + 1);
                     const T apar =
                         (i == M + 1) ? ach[K] : astore[G::astore_offset(i - 1) + (K - i) * G::parents(i) + p];
                     const T apc = apar * c;
@@ -446,8 +450,8 @@ __device__ __forceinline__ void bwd_step(const Tile<T, W, N, NT, M, RL>& S, Tile
                     } else if (M == 1) {
                         // Abar^(K)_1 of the root word q: A_1 = S_0 x[q] / K + S_1[q]
                         Gt.root += acc;
-                        xq += acc * S.s0 * inv[K];
-                        s0bar += acc * xs[q] * inv[K];
+                        xq += acc * S.s0 * recip<T>(K);
+                        s0bar += acc * xs[q] * recip<T>(K);
                     } else {
                         // M = 0: Abar^(K)_0, the cotangent of S_0
                         s0bar += acc;
@@ -475,10 +479,8 @@ k_sig_bwd_reg(DevBasis B, int64_t n_items, const int64_t* order, const T* incr, 
     __shared__ T gstate[G::state_elems];
     __shared__ T xs[W];
     __shared__ T xneg[W];
-    __shared__ T inv[N + 2];
     __shared__ T red[NT / 64][W];
     __shared__ unsigned slot;
-    if (threadIdx.x <= N + 1) inv[threadIdx.x] = threadIdx.x == 0 ? T(1) : T(1) / T(threadIdx.x);
     const int t = threadIdx.x;
     for (;;) {
         __syncthreads();
@@ -508,7 +510,7 @@ k_sig_bwd_reg(DevBasis B, int64_t n_items, const int64_t* order, const T* incr, 
                 const T* src = ckpts + (ck_offs[path] + tl / ck_every - 1) * B.start[N + 1];
                 S.template move<false>(B, const_cast<T*>(src), 1, q, true);
             } else {
-                chen_step_reg<T, W, N, NT, M, RL>(S, xneg, inv, lds0, lds1, q);
+                chen_step_reg<T, W, N, NT, M, RL>(S, xneg, lds0, lds1, q);
             }
             T xbar[W];
 #pragma unroll
@@ -518,7 +520,7 @@ k_sig_bwd_reg(DevBasis B, int64_t n_items, const int64_t* order, const T* incr, 
             // before the higher targets add into it.
             T xq = (M == 1 && t == 0) ? Gt.root * S.s0 : T(0);
             T s0bar = T(0);  // unused: S_0 is the constant 1 along a path
-            bwd_step<T, W, N, NT, M, RL>(S, Gt, xs, inv, astore, lds0, lds1, q, xbar, xq, s0bar);
+            bwd_step<T, W, N, NT, M, RL>(S, Gt, xs, astore, lds0, lds1, q, xbar, xq, s0bar);
             // target level M = 1: root A_1 = S_0 x[q] + S_1[q], seeded by G_1[q]
             if (M == 1 && t == 0) {
 #pragma unroll
@@ -580,10 +582,8 @@ k_fmexp_adj_tile(DevBasis B, View<const T> mult, View<const T> ex, View<const T>
     __shared__ T sstate[G::state_elems];
     __shared__ T gstate[G::state_elems];
     __shared__ T xs[W];
-    __shared__ T inv[N + 2];
     __shared__ T red[NT / 64][W];
     const int t = threadIdx.x;
-    if (t <= N + 1) inv[t] = t == 0 ? T(1) : T(1) / T(t);
     for (int64_t item = blockIdx.x; item < n * G::P; item += gridDim.x) {
         const int64_t b = item / G::P;
         const int q = static_cast<int>(item - b * G::P);
@@ -599,7 +599,7 @@ k_fmexp_adj_tile(DevBasis B, View<const T> mult, View<const T> ex, View<const T>
         // target level 1 when M = 1: new S_1[q] = S_0 x[q] + S_1[q]
         T xq = (M == 1 && t == 0) ? Gt.root * S.s0 : T(0);
         T s0bar = (M == 1 && t == 0) ? Gt.root * xs[q] : T(0);
-        bwd_step<T, W, N, NT, M, RL>(S, Gt, xs, inv, astore, lds0, lds1, q, xbar, xq, s0bar);
+        bwd_step<T, W, N, NT, M, RL>(S, Gt, xs, astore, lds0, lds1, q, xbar, xq, s0bar);
         if (M == 1 && t == 0) {
 #pragma unroll
             for (int a = 0; a < W; ++a)
