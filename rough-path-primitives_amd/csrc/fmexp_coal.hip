@@ -27,6 +27,13 @@ constexpr int cpow(int b, int e) {
     return r;
 }
 
+// 1/m; m is fixed after unrolling at every use, so this folds to a literal
+// (correctly rounded, like a runtime division).
+template <typename T>
+__device__ __forceinline__ T recip(int m) {
+    return T(1) / T(m);
+}
+
 template <int W, int N, int NT, int M>
 struct CoalGeo {
     static constexpr int words(int i) { return cpow(W, i - M); }  // subtree words of level i > M
@@ -49,9 +56,7 @@ k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int6
     __shared__ T lds0[G::lds_elems];
     __shared__ T lds1[G::lds_elems];
     __shared__ T xs[W];
-    __shared__ T inv[N + 2];
     const int t = threadIdx.x;
-    if (t <= N + 1) inv[t] = t == 0 ? T(1) : T(1) / T(t);
     for (int64_t item = blockIdx.x; item < n * G::P; item += gridDim.x) {
         const int64_t b = item / G::P;
         const int q = static_cast<int>(item - b * G::P);
@@ -80,10 +85,10 @@ k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int6
         for (int k = N; k >= M + 1; --k) {
             T a = chain[0];
 #pragma unroll
-            for (int i = 1; i <= M; ++i) a = a * (xs[(q / cpow(W, M - i)) % W] * inv[k - i + 1]) + chain[i];
+            for (int i = 1; i <= M; ++i) a = (a * recip<T>(k - i + 1)) * xs[(q / cpow(W, M - i)) % W] + chain[i];
 #pragma unroll
             for (int i = M + 1; i <= k; ++i) {
-                const T c = inv[k - i + 1];
+                const T c = recip<T>(k - i + 1);
                 const T* prev = ((i - M) & 1) ? lds1 : lds0;
                 T* cur = ((i - M) & 1) ? lds0 : lds1;
 #pragma unroll
@@ -93,7 +98,7 @@ k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int6
                         const int p = r / W;
                         const int letter = r - p * W;
                         const T ap = (i == M + 1) ? a : prev[p];
-                        const T v = ap * (xs[letter] * c) + reg[G::offset(i) + s];
+                        const T v = (ap * c) * xs[letter] + reg[G::offset(i) + s];
                         if (i == k) {
                             reg[G::offset(i) + s] = v;
                         } else {
@@ -110,7 +115,7 @@ k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int6
         for (int k = M; k >= 1; --k) {
             T a = chain[0];
 #pragma unroll
-            for (int i = 1; i <= k; ++i) a = a * (xs[(q / cpow(W, M - i)) % W] * inv[k - i + 1]) + chain[i];
+            for (int i = 1; i <= k; ++i) a = (a * recip<T>(k - i + 1)) * xs[(q / cpow(W, M - i)) % W] + chain[i];
             chain[k] = a;
         }
         T* dst = out.obj(b);
