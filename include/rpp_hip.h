@@ -473,7 +473,8 @@ typedef struct rpp_hip_signature_desc {
     const int64_t* ckpt_offsets;
     int32_t tile_threads;
     int32_t tile_split;
-    int32_t tile_kind;    /* 0: Horner register tile, 1: two-barrier chain tile */
+    int32_t tile_kind;    /* 0: Horner register tile, 1: two-barrier chain tile,
+                             2: thread-owned subtree tile */
     int32_t tile_expand;  /* chain tiles: levels expanded per job; register
                              tiles: top levels held in registers (0: any) */
     const int64_t* order;
@@ -520,7 +521,10 @@ int rpp_hip_signature_backward_f64(const rpp_hip_signature_backward_desc* desc, 
  * bytes) and returns how many exist.  kind 0 tiles (level-synchronous Horner,
  * the top `expand` subtree levels in registers and the rest in LDS) serve
  * forward and backward (split <= 1); kind 1 tiles (two barriers per
- * increment, `expand` levels per job) serve the forward pass.  A descriptor selects one with
+ * increment, `expand` levels per job) serve the forward pass, and so do kind 2
+ * tiles (one thread per (path, prefix of `split` letters), each keeping its
+ * subtree and a private copy of its prefix chain in registers: no LDS, no
+ * barriers; `threads` is the work-group size).  A descriptor selects one with
  * tile_threads/tile_split/tile_kind/tile_expand; tile_threads = 0 takes the
  * library default for (width, depth), tile_threads < 0 the storage-generic
  * kernel.  No reference counterpart: the reference has no batched signature

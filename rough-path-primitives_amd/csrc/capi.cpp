@@ -17,6 +17,7 @@
 #include "signature.hpp"
 #include "signature_reg.hpp"
 #include "signature_chain.hpp"
+#include "signature_thread.hpp"
 #include "shuffle_ops.hpp"
 #include "vector_ops.hpp"
 
@@ -727,6 +728,14 @@ int signature_impl(const rpp_hip_signature_desc* d, void* s) {
         }
         return fail(RPP_HIP_NOT_IMPLEMENTED, "no compiled chain tile matches");
     }
+    if (d->tile_kind == 2) {
+        for (const auto& c : thread_tiles<T>()) {
+            if (c.width != d->basis.width || c.depth != d->basis.depth) continue;
+            if (d->tile_threads > 0 && (c.threads != d->tile_threads || c.split != d->tile_split)) continue;
+            return c.forward(B, *d, order, stream);
+        }
+        return fail(RPP_HIP_NOT_IMPLEMENTED, "no compiled thread-subtree tile matches");
+    }
     if (const RegTile<T>* tile = pick_tile<T>(d->basis, d->tile_threads, d->tile_split, d->tile_expand, false)) {
         void* cnt = nullptr;
         RPP_TRY(workspace(kSlotCounter, 256, &cnt));
@@ -871,12 +880,15 @@ extern "C" int rpp_hip_signature_tiles(int32_t dtype_bytes, int32_t* out, int32_
         }
         ++n;
     };
-    auto emit = [&](const auto& reg, const auto& chain) {
+    auto emit = [&](const auto& reg, const auto& chain, const auto& thread) {
         for (const auto& t : chain) put(t.width, t.depth, t.threads, t.split, 1, t.expand);
         for (const auto& t : reg) put(t.width, t.depth, t.threads, t.split, 0, t.reg_levels);
+        for (const auto& t : thread) put(t.width, t.depth, t.threads, t.split, 2, 0);
     };
-    if (dtype_bytes == 4) emit(rpp_hip::reg_tiles<float>(), rpp_hip::chain_tiles<float>());
-    else if (dtype_bytes == 8) emit(rpp_hip::reg_tiles<double>(), rpp_hip::chain_tiles<double>());
+    if (dtype_bytes == 4)
+        emit(rpp_hip::reg_tiles<float>(), rpp_hip::chain_tiles<float>(), rpp_hip::thread_tiles<float>());
+    else if (dtype_bytes == 8)
+        emit(rpp_hip::reg_tiles<double>(), rpp_hip::chain_tiles<double>(), rpp_hip::thread_tiles<double>());
     else return rpp_hip::fail(RPP_HIP_INVALID_ARGUMENT, "dtype_bytes must be 4 or 8");
     return n;
 }

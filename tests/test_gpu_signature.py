@@ -214,10 +214,10 @@ def test_autograd_fmexp_and_log_gradcheck(dev, shape):
 
 
 def test_chain_tiles_checkpoints_match(dev):
-    """Checkpoints written by the two-barrier tiles equal the signatures of
-    the path prefixes."""
+    """Checkpoints written by the two-barrier tiles and the thread-subtree
+    tiles equal the signatures of the path prefixes."""
     for (w, N, nt, split, kind, expand) in _tiles(torch.float64):
-        if kind != 1:
+        if kind not in (1, 2):
             continue
         rng = np.random.default_rng(nt + split)
         inc, off = paths(rng, 3, w, 4, 7)
