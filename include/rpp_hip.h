@@ -491,8 +491,10 @@ int rpp_hip_signature_f64(const rpp_hip_signature_desc* desc, void* stream);
  * the reconstruction is re-anchored at each of them (sqrt(L) checkpointing).
  * scratch: device workspace of rpp_hip_signature_backward_scratch_bytes() for
  * the storage-generic kernel (NULL: library workspace).  tile_threads /
- * tile_split select a register tile as for the forward pass (split <= 1);
- * order / num_increments as for the forward pass. */
+ * tile_split select a register tile as for the forward pass (split <= 1), or
+ * with tile_kind 2 a thread-owned subtree tile that has a backward (its
+ * work-group is one (path, first letter) with one thread per prefix of
+ * tile_split letters); order / num_increments as for the forward pass. */
 typedef struct rpp_hip_signature_backward_desc {
     rpp_hip_basis basis;
     int64_t num_paths;
@@ -510,6 +512,7 @@ typedef struct rpp_hip_signature_backward_desc {
     const int64_t* order;
     int64_t num_increments;
     int32_t tile_expand;  /* register levels of the tile (0: any) */
+    int32_t tile_kind;    /* 0: Horner register tile, 2: thread-owned subtree tile */
 } rpp_hip_signature_backward_desc;
 int64_t rpp_hip_signature_backward_scratch_bytes(const rpp_hip_signature_backward_desc* desc,
                                                  int32_t dtype_bytes);
@@ -524,7 +527,8 @@ int rpp_hip_signature_backward_f64(const rpp_hip_signature_backward_desc* desc, 
  * increment, `expand` levels per job) serve the forward pass, and so do kind 2
  * tiles (one thread per (path, prefix of `split` letters), each keeping its
  * subtree and a private copy of its prefix chain in registers: no LDS, no
- * barriers; `threads` is the work-group size).  A descriptor selects one with
+ * barriers; `threads` is the work-group size; `expand` is 1 when the tile
+ * also has a backward).  A descriptor selects one with
  * tile_threads/tile_split/tile_kind/tile_expand; tile_threads = 0 takes the
  * library default for (width, depth), tile_threads < 0 the storage-generic
  * kernel.  No reference counterpart: the reference has no batched signature

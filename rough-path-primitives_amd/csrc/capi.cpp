@@ -811,6 +811,14 @@ int signature_backward_impl(const rpp_hip_signature_backward_desc* d, void* s) {
     } else if (total < 0) {
         return fail(RPP_HIP_INVALID_ARGUMENT, "num_increments must be offsets[num_paths] when order is given");
     }
+    if (d->scratch == nullptr && d->tile_threads >= 0 && d->tile_kind == 2) {
+        for (const auto& c : thread_tiles<T>()) {
+            if (c.width != d->basis.width || c.depth != d->basis.depth || c.backward == nullptr) continue;
+            if (d->tile_threads > 0 && (c.threads != d->tile_threads || c.split != d->tile_split)) continue;
+            return c.backward(B, *d, order, total, stream);
+        }
+        return fail(RPP_HIP_NOT_IMPLEMENTED, "no compiled thread-subtree backward tile matches");
+    }
     if (d->scratch == nullptr && d->tile_threads >= 0) {
         if (const RegTile<T>* tile =
                 pick_tile<T>(d->basis, d->tile_threads, d->tile_split, d->tile_expand, true)) {
@@ -883,7 +891,7 @@ extern "C" int rpp_hip_signature_tiles(int32_t dtype_bytes, int32_t* out, int32_
     auto emit = [&](const auto& reg, const auto& chain, const auto& thread) {
         for (const auto& t : chain) put(t.width, t.depth, t.threads, t.split, 1, t.expand);
         for (const auto& t : reg) put(t.width, t.depth, t.threads, t.split, 0, t.reg_levels);
-        for (const auto& t : thread) put(t.width, t.depth, t.threads, t.split, 2, 0);
+        for (const auto& t : thread) put(t.width, t.depth, t.threads, t.split, 2, t.backward ? 1 : 0);
     };
     if (dtype_bytes == 4)
         emit(rpp_hip::reg_tiles<float>(), rpp_hip::chain_tiles<float>(), rpp_hip::thread_tiles<float>());

@@ -15,6 +15,8 @@ struct ThreadTile {
     int split;    // prefix letters per thread
     int threads;  // work-group size
     int (*forward)(const DevBasis&, const rpp_hip_signature_desc&, const int64_t*, hipStream_t);
+    // the backward with the same prefix length (nullptr unless split <= depth - 2)
+    int (*backward)(const DevBasis&, const rpp_hip_signature_backward_desc&, const int64_t*, int64_t, hipStream_t);
 };
 
 template <typename T>

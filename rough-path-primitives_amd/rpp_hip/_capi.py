@@ -147,7 +147,7 @@ class SignatureBackwardDesc(C.Structure):
                 ("signatures", TensorBatch), ("cotangent", TensorBatch), ("grad_increments", _vp),
                 ("checkpoint_every", _i32), ("checkpoints", _vp), ("ckpt_offsets", _vp),
                 ("scratch", _vp), ("tile_threads", _i32), ("tile_split", _i32), ("order", _vp),
-                ("num_increments", _i64), ("tile_expand", _i32)]
+                ("num_increments", _i64), ("tile_expand", _i32), ("tile_kind", _i32)]
 
 
 # entry point -> descriptor type; every one also takes a stream pointer.

@@ -65,6 +65,7 @@ def tune(width: int, depth: int, dtype=torch.float64, paths: int = 1024, length:
     off = torch.arange(paths + 1, dtype=torch.int64, device=dev) * length
     cands = [(t, s, k, e) for (w, n, t, s, k, e) in ops.signature_tiles(dtype) if (w, n) == (width, depth)]
     cands.append((-1, 0, 0, 0))  # the storage-generic kernel
+    thread_bwd = ops.thread_backward_tiles(width, depth, dtype)
     result = {"forward": {}, "backward": {}}
     sig = ops.signature(inc, off, basis)
     ell = torch.randn(basis.size(), generator=g, dtype=torch.float64).to(dev, dtype)
@@ -73,10 +74,10 @@ def tune(width: int, depth: int, dtype=torch.float64, paths: int = 1024, length:
         result["forward"][key] = _time(
             lambda: ops.signature(inc, off, basis, out=sig, tile_threads=t, tile_split=s, tile_kind=k,
                                   tile_expand=e), reps)
-        if backward and k == 0 and s <= 1:
+        if backward and ((k == 0 and s <= 1) or (k == 2 and (t, s) in thread_bwd)):
             result["backward"][key] = _time(
                 lambda: ops.signature_backward(inc, off, basis, sig, ell, tile_threads=t, tile_split=s,
-                                               tile_expand=e), reps)
+                                               tile_expand=e, tile_kind=k), reps)
     entry = {}
     for kind in ("forward", "backward"):
         if result[kind]:

@@ -557,7 +557,8 @@ def fmexp_tile(width: int, depth: int, dtype):
 def signature_tiles(dtype) -> list:
     """Compiled tiles (width, depth, threads, split, kind, expand) for a dtype:
     kind 1 = two-barrier forward tile, kind 0 = Horner register tile (forward
-    and, for split <= 1, backward)."""
+    and, for split <= 1, backward), kind 2 = thread-owned subtree tile
+    (forward, and backward where expand = 1)."""
     import numpy as np
     nbytes = torch.tensor([], dtype=dtype).element_size()
     lib = A.lib()
@@ -565,6 +566,12 @@ def signature_tiles(dtype) -> list:
     buf = np.zeros(6 * max(n, 1), dtype=np.int32)
     lib.rpp_hip_signature_tiles(nbytes, buf.ctypes.data, n)
     return [tuple(int(v) for v in buf[6 * i: 6 * i + 6]) for i in range(n)]
+
+
+def thread_backward_tiles(width: int, depth: int, dtype) -> list:
+    """(threads, split) of the thread-owned subtree tiles of (width, depth)
+    that have a backward."""
+    return [(t, s) for (w, d, t, s, k, e) in signature_tiles(dtype) if (w, d) == (width, depth) and k == 2 and e == 1]
 
 
 def _offsets_of(lengths, device) -> torch.Tensor:
@@ -617,15 +624,16 @@ def signature_backward(increments: torch.Tensor, offsets: torch.Tensor, basis: T
                        checkpoints: Optional[torch.Tensor] = None,
                        ckpt_offsets: Optional[torch.Tensor] = None, tile_threads: int = 0,
                        tile_split: int = 0, order: Optional[torch.Tensor] = None, tile_expand: int = 0,
-                       stream=None) -> torch.Tensor:
+                       stream=None, tile_kind: int = 0) -> torch.Tensor:
     """d loss / d increments given d loss / d signatures (cotangent may be 1-D,
-    one functional for every path)."""
+    one functional for every path).  tile_kind 0 selects a Horner register
+    tile, 2 a thread-owned subtree tile (rpp_hip_signature_tiles)."""
     n = offsets.numel() - 1
     inc = increments.contiguous()
     if tile_threads == 0:
         from . import autotune
         best = autotune.best(basis.width, basis.depth, increments.dtype, "backward") or (0, 0, 0, 0)
-        tile_threads, tile_split, tile_expand = best[0], best[1], best[3]
+        tile_threads, tile_split, tile_kind, tile_expand = best[0], best[1], best[2], best[3]
     if grad is None:
         grad = torch.empty_like(inc)
     d = A.SignatureBackwardDesc(basis._c(), n, inc.data_ptr(), offsets.data_ptr(),
@@ -634,6 +642,6 @@ def signature_backward(increments: torch.Tensor, offsets: torch.Tensor, basis: T
                                 checkpoints.data_ptr() if checkpoints is not None else None,
                                 ckpt_offsets.data_ptr() if ckpt_offsets is not None else None, None,
                                 tile_threads, tile_split, order.data_ptr() if order is not None else None,
-                                inc.shape[0], tile_expand)
+                                inc.shape[0], tile_expand, tile_kind)
     _call("rpp_hip_signature_backward" + _sfx(inc, signatures, cotangent), d, stream)
     return grad
