@@ -162,6 +162,35 @@ def test_every_register_tile_backward_fp64(dev, ckpt):
         assert err < 1e-10, f"tile {(w, N, nt, split, kind, expand)} ckpt={ckpt}: {err:.3e}"
 
 
+def test_thread_backward_per_path_cotangent(dev):
+    """The thread-owned subtree backward with a cotangent batch (one row per
+    path, not a broadcast functional) on uneven paths, an empty one included,
+    against the oracle's gradient path by path."""
+    for (w, N, nt, split, kind, expand) in _tiles(torch.float64):
+        if kind != 2 or expand != 1:
+            continue
+        rng = np.random.default_rng(w * 31 + N + split)
+        lengths = np.array([3, 0, 6, 1, 5])
+        n = lengths.size
+        off = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+        inc = rng.normal(0.0, 0.5, size=(int(off[-1]), w))
+        basis = rpp_hip.TensorBasis(w, N)
+        dim = basis.size()
+        cot = rng.normal(size=(n, dim))
+        expected = np.zeros_like(inc)
+        for p in range(n):
+            a, b = int(off[p]), int(off[p + 1])
+            if b > a:
+                expected[a:b] = O.signature_vjp(w, N, inc[a:b], np.array([0, b - a]), cot[p])
+        inc_d = torch.tensor(inc, device=dev)
+        off_d = torch.tensor(off, device=dev)
+        sig = rpp_hip.signature(inc_d, off_d, basis)
+        grad = rpp_hip.signature_backward(inc_d, off_d, basis, sig, torch.tensor(cot, device=dev),
+                                          tile_threads=nt, tile_split=split, tile_kind=2)
+        err = rel_err(grad.cpu().numpy(), expected)
+        assert err < 1e-10, f"thread tile {(w, N, nt, split)}: {err:.3e}"
+
+
 def test_autograd_signature_and_ft_mul(dev):
     from rpp_hip import autograd as AG
     w, N, n = 3, 4, 5
