@@ -157,29 +157,37 @@ k_sig_fwd_thread(DevBasis B, int n_paths, const int64_t* order, const T* incr, c
 #pragma unroll
     for (int r = 0; r < G::total; ++r) sub[r] = T(0);
     const int64_t dim = B.start[N + 1];
-    // increments one step ahead: the whole row (uniform) and the prefix letters'
-    T xn[W], xqn[L > 0 ? L : 1];
-    if (t0 < t1) {
+    // Increments one step ahead, in two register sets used in turn (the loop
+    // is unrolled by two, so no copies): the row, uniform across the
+    // wavefront, and the prefix letters' entries, loaded from the row pointer
+    // with 32-bit lane offsets.
+    T xa[W], xqa[L > 0 ? L : 1], xb[W], xqb[L > 0 ? L : 1];
+    auto load = [&](T (&x)[W], T (&xq)[L > 0 ? L : 1], int64_t t) {
+        const T* row = incr + t * W;
 #pragma unroll
-        for (int b = 0; b < W; ++b) xn[b] = incr[t0 * W + b];
+        for (int b = 0; b < W; ++b) x[b] = row[b];
 #pragma unroll
-        for (int i = 0; i < L; ++i) xqn[i] = incr[t0 * W + ql[i]];
-    }
-    for (int64_t t = t0; t < t1; ++t) {
-        T x[W], xq[L > 0 ? L : 1];
-#pragma unroll
-        for (int b = 0; b < W; ++b) x[b] = xn[b];
-#pragma unroll
-        for (int i = 0; i < L; ++i) xq[i] = xqn[i];
-        if (t + 1 < t1) {
-#pragma unroll
-            for (int b = 0; b < W; ++b) xn[b] = incr[(t + 1) * W + b];
-#pragma unroll
-            for (int i = 0; i < L; ++i) xqn[i] = incr[(t + 1) * W + ql[i]];
+        for (int i = 0; i < L; ++i) xq[i] = row[ql[i]];
+    };
+    // checkpoints: a countdown instead of a division per increment
+    int ck_left = ck_every;
+    int64_t ck_index = ck_every > 0 ? ck_offs[path] : 0;
+    auto checkpoint = [&]() {
+        if (ck_every > 0 && --ck_left == 0) {
+            ck_left = ck_every;
+            if (live) thread_store<T, W, N, L>(B, ckpts + ck_index * dim, 1, q, chain, sub);
+            ++ck_index;
         }
-        thread_step<T, W, N, L>(chain, sub, x, xq);
-        if (ck_every > 0 && ((t - t0 + 1) % ck_every) == 0 && live)
-            thread_store<T, W, N, L>(B, ckpts + (ck_offs[path] + (t - t0 + 1) / ck_every - 1) * dim, 1, q, chain, sub);
+    };
+    if (t0 < t1) load(xa, xqa, t0);
+    for (int64_t t = t0; t < t1; t += 2) {
+        if (t + 1 < t1) load(xb, xqb, t + 1);
+        thread_step<T, W, N, L>(chain, sub, xa, xqa);
+        checkpoint();
+        if (t + 1 >= t1) break;
+        if (t + 2 < t1) load(xa, xqa, t + 2);
+        thread_step<T, W, N, L>(chain, sub, xb, xqb);
+        checkpoint();
     }
     if (live) thread_store<T, W, N, L>(B, out.obj(path), out.cs, q, chain, sub);
 }
