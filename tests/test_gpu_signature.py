@@ -162,11 +162,12 @@ def test_every_register_tile_backward_fp64(dev, ckpt):
         assert err < 1e-10, f"tile {(w, N, nt, split, kind, expand)} ckpt={ckpt}: {err:.3e}"
 
 
-def test_thread_backward_per_path_cotangent(dev):
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
+def test_thread_backward_per_path_cotangent(dev, dtype):
     """The thread-owned subtree backward with a cotangent batch (one row per
     path, not a broadcast functional) on uneven paths, an empty one included,
-    against the oracle's gradient path by path."""
-    for (w, N, nt, split, kind, expand) in _tiles(torch.float64):
+    against the FP64 oracle's gradient path by path (FP32 within 1e-4)."""
+    for (w, N, nt, split, kind, expand) in _tiles(dtype):
         if kind != 2 or expand != 1:
             continue
         rng = np.random.default_rng(w * 31 + N + split)
@@ -182,13 +183,14 @@ def test_thread_backward_per_path_cotangent(dev):
             a, b = int(off[p]), int(off[p + 1])
             if b > a:
                 expected[a:b] = O.signature_vjp(w, N, inc[a:b], np.array([0, b - a]), cot[p])
-        inc_d = torch.tensor(inc, device=dev)
+        inc_d = torch.tensor(inc, device=dev, dtype=dtype)
         off_d = torch.tensor(off, device=dev)
         sig = rpp_hip.signature(inc_d, off_d, basis)
-        grad = rpp_hip.signature_backward(inc_d, off_d, basis, sig, torch.tensor(cot, device=dev),
+        grad = rpp_hip.signature_backward(inc_d, off_d, basis, sig, torch.tensor(cot, device=dev, dtype=dtype),
                                           tile_threads=nt, tile_split=split, tile_kind=2)
-        err = rel_err(grad.cpu().numpy(), expected)
-        assert err < 1e-10, f"thread tile {(w, N, nt, split)}: {err:.3e}"
+        err = rel_err(grad.double().cpu().numpy(), expected)
+        tol = 1e-10 if dtype == torch.float64 else 1e-4
+        assert err < tol, f"thread tile {(w, N, nt, split)} {dtype}: {err:.3e}"
 
 
 def test_autograd_signature_and_ft_mul(dev):
