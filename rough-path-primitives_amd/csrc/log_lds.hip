@@ -66,8 +66,16 @@ constexpr int log_threads() {
     return LogGeo<W, N>::dim > 4096 ? 1024 : 512;
 }
 
+// FP32 work-groups are held to 64 registers, so two share a compute unit
+// (width 3 depth 8 over 16384 objects: 0.98 -> 0.89 ms); in FP64 that limit
+// spills and one work-group per compute unit is faster.
+template <typename T>
+constexpr int log_waves() {
+    return sizeof(T) == 4 ? 8 : 1;
+}
+
 template <typename T, int W, int N, int NT>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(log_waves<T>())))
 k_log_lds(View<T> out, View<const T> arg, int64_t n) {
     using G = LogGeo<W, N>;
     extern __shared__ __align__(16) unsigned char log_raw[];
@@ -79,13 +87,26 @@ k_log_lds(View<T> out, View<const T> arg, int64_t n) {
     for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
         const T* a = arg.obj(b);
         __syncthreads();  // the previous object's readers are done
-        for (int i = t; i < G::low; i += NT) {
+        // every staging load of the thread is issued before the first LDS
+        // store waits on one
+        constexpr int SL = (G::low + NT - 1) / NT;
+        T yv[SL];
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+            const int i = t + s * NT;
             int k = 0;
 #pragma unroll
             for (int j = 1; j < N; ++j)
                 if (i >= G::start(j)) k = j;
-            ys[i] = (k >= ylo && k <= yhi) ? a[i] : T(0);
-            ps[i] = T(0);
+            yv[s] = (i < G::low && k >= ylo && k <= yhi) ? a[i] : T(0);
+        }
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+            const int i = t + s * NT;
+            if (i < G::low) {
+                ys[i] = yv[s];
+                ps[i] = T(0);
+            }
         }
         if (t == 0) ps[0] = (N % 2 == 0 ? T(-1) : T(1)) / T(N);
         __syncthreads();
