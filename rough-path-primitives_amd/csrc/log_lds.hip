@@ -53,14 +53,16 @@ constexpr size_t log_lds_bytes() {
     return 2 * static_cast<size_t>(LogGeo<W, N>::low) * sizeof(T);
 }
 
-// At most 80 KiB, so that two work-groups share a compute unit and one's
-// barrier-separated steps overlap the other's.
+// At most 80 KiB, so that LDS leaves room for two work-groups per compute
+// unit, whose barrier-separated steps then overlap; registers decide whether
+// two are resident (below).
 constexpr size_t kLogLdsCap = 80 * 1024;
 
 // 1024 threads for objects of more than 4096 coefficients, 512 below.  At
-// width 3, depth 8 in FP64 (16384 objects) 1024 threads measured 1.20 ms with
-// 97 registers and no spills; 512 threads needed 141 registers, so one
-// work-group per compute unit (1.77 ms), or spilled when held to two.
+// width 3, depth 8 in FP64 (16384 objects) 1024 threads take about 100
+// registers, so one work-group per compute unit (1.13 ms); held to 64
+// registers for two, they spill and run slower (1.59 ms).  512 threads needed
+// 141 registers (1.77 ms).
 template <typename T, int W, int N>
 constexpr int log_threads() {
     return LogGeo<W, N>::dim > 4096 ? 1024 : 512;
