@@ -324,18 +324,21 @@ int lie_convert_impl(bool to_lie, const rpp_hip_lie_convert_desc* d, void* s) {
                                         nullptr, IndexRange{0, 0}, 0.0, 0.0, stream));
     }
     if (dlo > dhi || n == 0) return 0;
-    // matrix rows: T2L rows are Lie keys - 1, L2T rows tensor words; columns
-    // the other way round
-    const int64_t obeg = ostart(dlo), rows = ostart(dhi + 1) - obeg;
-    const int64_t abeg = astart(dlo), ncols = astart(dhi + 1) - abeg;
-    const int64_t row0 = to_lie ? obeg - 1 : obeg;
-    const int64_t col0 = to_lie ? abeg : abeg - 1;
-    const rpp_hip_vector_batch ov{out.data, out.batch_stride, out.coeff_stride, obeg, rows};
-    const rpp_hip_vector_batch av{arg.data, arg.batch_stride, arg.coeff_stride, abeg, ncols};
     const DeviceCsr* m = to_lie ? t2l : l2t;
-    if (lie_spmv_fits<T>(*m, av)) return launch_lie_spmv<T>(n, ov, av, *m, row0, rows, col0, stream);
-    const rpp_hip_sparse_matrix M{RPP_HIP_CSR, m->values, m->indices, m->offsets + row0, m->nnz, rows, ncols};
-    return launch_spmv<T>(n, ov, av, M, 1.0, stream, col0);
+    // the degrees [lo, hi] in one launch: matrix rows are Lie keys - 1 (T2L)
+    // or tensor words (L2T), columns the other way round
+    auto convert = [&](int lo, int hi) {
+        const int64_t obeg = ostart(lo), rows = ostart(hi + 1) - obeg;
+        const int64_t abeg = astart(lo), ncols = astart(hi + 1) - abeg;
+        const int64_t row0 = to_lie ? obeg - 1 : obeg;
+        const int64_t col0 = to_lie ? abeg : abeg - 1;
+        const rpp_hip_vector_batch ov{out.data, out.batch_stride, out.coeff_stride, obeg, rows};
+        const rpp_hip_vector_batch av{arg.data, arg.batch_stride, arg.coeff_stride, abeg, ncols};
+        if (lie_spmv_fits<T>(*m, av)) return launch_lie_spmv<T>(n, ov, av, *m, row0, rows, col0, stream);
+        const rpp_hip_sparse_matrix M{RPP_HIP_CSR, m->values, m->indices, m->offsets + row0, m->nnz, rows, ncols};
+        return launch_spmv<T>(n, ov, av, M, 1.0, stream, col0);
+    };
+    return convert(dlo, dhi);
 }
 
 // ------------------------------------------------------- the shuffle family

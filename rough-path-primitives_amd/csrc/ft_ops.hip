@@ -505,33 +505,29 @@ k_antipode(DevBasis B, View<T> out, View<const T> arg, int64_t n) {
 
 // CSR rows with the argument staged in LDS: a work-group streams objects; it
 // copies the object's argument segment (the matrix columns) into LDS with
-// coalesced loads, then groups of 8 lanes take one row each, read the row's
-// entries consecutively and gather from LDS, and reduce in a fixed order.
-// The sparse matrix itself is shared by every object and stays in L2.
-constexpr int kSpmvLanes = 8;
+// coalesced loads, then each thread sums whole rows in entry order with
+// contraction off, as the reference's loop does, so the results agree bit
+// for bit.  The sparse matrix itself is shared by every object and stays in
+// L2.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_spmv_csr_lds(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t abs_, int64_t abeg,
                int64_t count, const T* values, const int64_t* indices, const int64_t* offsets, int64_t rows,
                T alpha, int64_t n, int64_t cshift) {
+#pragma clang fp contract(off)
     extern __shared__ __align__(16) unsigned char spmv_raw[];
     T* xs = reinterpret_cast<T*>(spmv_raw);
     const int t = threadIdx.x;
-    const int lane = t % kSpmvLanes;
-    const int grp = t / kSpmvLanes;
-    constexpr int groups = kBlock / kSpmvLanes;
     for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
         const T* a = arg + b * abs_ + abeg;
         for (int64_t i = t; i < count; i += kBlock) xs[i] = a[i];
         __syncthreads();
         T* o = out + b * obs;
-        for (int64_t row = grp; row < rows; row += groups) {
+        for (int64_t row = t; row < rows; row += kBlock) {
             const int64_t e0 = offsets[row], e1 = offsets[row + 1];
             T acc = T(0);
-            for (int64_t e = e0 + lane; e < e1; e += kSpmvLanes) acc += values[e] * xs[indices[e] - cshift];
-#pragma unroll
-            for (int off = kSpmvLanes / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, kSpmvLanes);
-            if (lane == 0) o[(obeg + row) * ocs] = alpha * acc;
+            for (int64_t e = e0; e < e1; ++e) acc += values[e] * xs[indices[e] - cshift];
+            o[(obeg + row) * ocs] = alpha * acc;
         }
         __syncthreads();
     }

@@ -44,6 +44,30 @@ def test_spmv_view_relative(dev, fmt, dtype):
     got = out.cpu().numpy()
     assert np.all(got[:, :out_begin] == 7) and np.all(got[:, out_begin + rows:] == 7)
     assert_close(got, expected, dtype, f"spmv {fmt}")
+    if fmt == "csr":
+        # rows summed in entry order, contraction off: bit for bit
+        assert np.array_equal(got, expected)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_spmv_csr_staged_argument_exact(dev, dtype):
+    """CSR product with enough rows for the LDS-staged kernel (argument in
+    LDS, one thread per row): equal to the oracle's row loop bit for bit."""
+    rng = np.random.default_rng(11)
+    rows, cols, n = 300, 500, 7
+    dense = rng.uniform(-1, 1, size=(rows, cols)) * (rng.uniform(size=(rows, cols)) < 0.05)
+    offs = np.concatenate([[0], np.cumsum((dense != 0).sum(axis=1))])
+    idx = np.concatenate([np.nonzero(dense[r])[0] for r in range(rows)])
+    vals = np.concatenate([dense[r][dense[r] != 0] for r in range(rows)])
+    arg = rng.uniform(-1, 1, size=(n, cols)).astype(dtype)
+    expected = np.zeros((n, rows), dtype=dtype)
+    O.spmv(vals, idx, offs, rows, expected, arg, csc=False, out_offset=0, out_size=rows, alpha=0.75)
+    M = rpp_hip.SparseMatrix("csr", to_dev(vals.astype(dtype), dev), to_dev(idx.astype(np.int64), dev),
+                             to_dev(offs.astype(np.int64), dev), rows, cols)
+    out = to_dev(np.zeros((n, rows), dtype=dtype), dev)
+    rpp_hip.sparse_matrix_vector_product(out, to_dev(arg, dev), M, out_begin=0, arg_begin=0, out_size=rows,
+                                         alpha=0.75)
+    assert np.array_equal(out.cpu().numpy(), expected)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
