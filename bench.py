@@ -337,6 +337,23 @@ def ft_mul_section(rpp, torch, O, dev):
         entry["hbm_frac"] = entry["achieved_gbs"] / HBM_PEAK_GBS
         if n == 1 << 17:
             entry["traffic_bytes_per_call"] = load_traffic("k_product_small_w4d4_131072pairs_fp64_traffic.json")
+        if n == 1024:
+            # the same calls replayed from one captured graph: the kernel and
+            # its launch without the Python wrapper's per-call work
+            calls = 50
+            graph = torch.cuda.CUDAGraph()
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                rpp.ft_mul(O_, L, R, basis)
+            torch.cuda.current_stream().wait_stream(side)
+            torch.cuda.synchronize()
+            with torch.cuda.graph(graph):
+                for _ in range(calls):
+                    rpp.ft_mul(O_, L, R, basis)
+            gms, _, _ = timed(torch, graph.replay, 20, 3)
+            entry["graph_us_per_call"] = gms * 1e3 / calls
+            entry["graph_achieved_gbs"] = 3 * n * dim * 8 / (gms / calls * 1e-3) / 1e9
         if n == 1024 and O.has_ref():
             out_cpu = np.zeros((n, dim))
             t0 = time.perf_counter()
