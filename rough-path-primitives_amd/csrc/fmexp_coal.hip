@@ -153,27 +153,31 @@ int coal_launch(const DevBasis& B, const rpp_hip_tensor_batch& out, const rpp_hi
 }  // namespace
 
 #define RPP_COAL(T, W, N, NT, M, BPC) \
-    {sizeof(T), W, N, NT, M, BPC, &coal_launch<T, W, N, NT, M, BPC>}
+    FmexpTile<T> { sizeof(T), W, N, NT, M, BPC, &coal_launch<T, W, N, NT, M, BPC> }
 
+// One tile per (width, depth) serves plain ft_fmexp, the fastest of the
+// candidates timed on MI355X over 32 chained calls (threads, split: FP64 / FP32
+// microseconds per call):
+//   width 5 depth 6, 4096 objects:  (128,2) 249 / 190, (64,2) 250 / 166, (256,1) 275 / 126
+//   width 6 depth 6, 2048 objects:  (128,2) 338 / 179, (256,2) 341 / 181, (256,1) 414 / 283
+//   width 8 depth 5, 2048 objects:  (256,1) 236 / 118, (256,2) 236 / 187, (128,2) 242 / 124
+//   width 3 depth 8, 8192 objects:  (256,1) 262 / 146, (128,1) 268 / 174, (256,2) 315 / 279
+//   width 4 depth 4, 2^17 objects:  (64,0) 143 / 75, (64,1) 213 / 189
+// An FP32 item moves half the bytes of an FP64 one in the same number of
+// dependent steps, so FP32 prefers fewer, larger items (one prefix letter).
 template <typename T>
 const std::vector<FmexpTile<T>>& fmexp_coal_tiles() {
+    constexpr bool f32 = sizeof(T) == 4;
     static const std::vector<FmexpTile<T>> tiles = {
-        // width 5, depth 6 (19531 coefficients)
-        RPP_COAL(T, 5, 6, 128, 2, 0), RPP_COAL(T, 5, 6, 64, 2, 0), RPP_COAL(T, 5, 6, 256, 2, 0),
-        RPP_COAL(T, 5, 6, 256, 1, 0), RPP_COAL(T, 5, 6, 128, 2, 16), RPP_COAL(T, 5, 6, 64, 3, 0),
-        // width 6, depth 6 (55987)
-        RPP_COAL(T, 6, 6, 256, 2, 0), RPP_COAL(T, 6, 6, 128, 2, 0), RPP_COAL(T, 6, 6, 256, 1, 0),
-        // width 8, depth 5 (37449)
-        RPP_COAL(T, 8, 5, 256, 2, 0), RPP_COAL(T, 8, 5, 128, 2, 0), RPP_COAL(T, 8, 5, 256, 1, 0),
-        // width 3, depth 8 (9841)
-        RPP_COAL(T, 3, 8, 256, 2, 0), RPP_COAL(T, 3, 8, 128, 3, 0), RPP_COAL(T, 3, 8, 256, 3, 0),
-        // width 4, depth 4 (341)
-        RPP_COAL(T, 4, 4, 64, 0, 0), RPP_COAL(T, 4, 4, 64, 1, 0),
+        f32 ? RPP_COAL(T, 5, 6, 256, 1, 0) : RPP_COAL(T, 5, 6, 128, 2, 0),
+        RPP_COAL(T, 6, 6, 128, 2, 0),
+        RPP_COAL(T, 8, 5, 256, 1, 0),
+        RPP_COAL(T, 3, 8, 256, 1, 0),
+        RPP_COAL(T, 4, 4, 64, 0, 0),
     };
     return tiles;
 }
 
-// The first listed tile of (width, depth) serves plain ft_fmexp.
 template <typename T>
 const FmexpTile<T>* fmexp_coal_tile(int width, int depth) {
     for (const auto& t : fmexp_coal_tiles<T>())

@@ -161,11 +161,14 @@ def test_adjoint_multiplications(dev, side, windows, dtype):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
-def test_chained_fmexp_c2_shape(dev, dtype):
-    """The headline shape (width 5, depth 6), 64 paths x 16 increments of the
-    plain chain, against the oracle chain."""
-    w, N, n, L = 5, 6, 64, 16
-    rng = np.random.default_rng(0xC2)
+@pytest.mark.parametrize("shape", [(5, 6, 64, 16), (6, 6, 24, 6), (8, 5, 24, 6), (3, 8, 64, 8), (4, 4, 300, 8)],
+                         ids=["w5d6", "w6d6", "w8d5", "w3d8", "w4d4"])
+def test_chained_fmexp_c2_shape(dev, shape, dtype):
+    """The plain chain at every shape with a coalesced-ownership tile (width 5,
+    depth 6 is the headline shape): paths x increments of chained ft_fmexp
+    against the oracle chain."""
+    w, N, n, L = shape
+    rng = np.random.default_rng(0xC2 + w * 10 + N)
     basis = rpp_hip.TensorBasis(w, N)
     dim = basis.size()
     inc = rng.normal(0.0, np.sqrt(1.0 / 128), size=(n, L, w))
