@@ -45,12 +45,19 @@ constexpr size_t lie_table_bytes(int table_size) {
 // One packed entry: acc + table[w >> 24] * x[column].  The column is taken
 // relative to the staged segment and clamped to its extra zero slot, where
 // the padding entries (column 0xffffff) land.
-template <typename T, int OB>
-__device__ __forceinline__ T lie_term(T acc, uint32_t w, const T* tab, const T* xs, uint32_t col0, uint32_t cols,
-                                      uint32_t o) {
+// A thread serves J objects og * J .. og * J + J - 1 of its row: the entry is
+// decoded and its table value read once for all of them.
+template <typename T, int OB, int J>
+__device__ __forceinline__ void lie_term(T (&acc)[J], uint32_t w, const T* tab, const T* xs, uint32_t col0,
+                                         uint32_t cols, uint32_t og) {
     const uint32_t rel = min((w & 0xffffffu) - col0, cols);
-    const T p = tab[w >> 24] * xs[rel * OB + o];
-    return acc + p;
+    const T v = tab[w >> 24];
+    const T* x = xs + rel * OB + og * J;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const T p = v * x[j];
+        acc[j] = acc[j] + p;
+    }
 }
 
 // Two work-groups of kLieThreads per compute unit: 8 waves per SIMD.
@@ -58,8 +65,11 @@ __device__ __forceinline__ T lie_term(T acc, uint32_t w, const T* tab, const T* 
 // each in flight while D are summed: (R, D) = (1, 4) for matrices with long
 // rows, whose longest rows are a work-group's critical path, and (3 or 4, 1)
 // for short rows, where a row's two dependent loads (descriptor, entries)
-// would otherwise dominate.
-template <typename T, int OB, int R, int D>
+// would otherwise dominate.  Each thread serves J of the OB objects of its
+// row (J = OB for long rows when there are enough rows to occupy the
+// threads: the entry decoding, table read and LDS instruction then serve
+// every object), so OB / J threads share a row slot.
+template <typename T, int OB, int R, int D, int J>
 __global__ void __launch_bounds__(kLieThreads) __attribute__((amdgpu_waves_per_eu(8)))
 k_lie_rows(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t abs_, int64_t acs, int64_t abeg,
            uint32_t cols, uint32_t col0, const uint4* packed4, const int4* rowdesc, const T* table, int table_size,
@@ -67,10 +77,13 @@ k_lie_rows(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t
     extern __shared__ __align__(16) unsigned char lie_raw[];
     T* tab = reinterpret_cast<T*>(lie_raw);
     T* xs = reinterpret_cast<T*>(lie_raw + lie_table_bytes<T>(table_size));
-    constexpr int kSlots = kLieThreads / OB;
+    static_assert(OB % J == 0, "a row slot serves whole objects");
+    constexpr int kStage = kLieThreads / OB;        // staging steps per pass
+    constexpr int kSlots = kLieThreads / (OB / J);  // row slots
     const uint32_t t = threadIdx.x;
-    const uint32_t o = t % OB;
-    const int slot = static_cast<int>(t / OB);
+    const uint32_t o = t % OB;  // staged object
+    const uint32_t og = t % (OB / J);
+    const int slot = static_cast<int>(t / (OB / J));
     for (int i = static_cast<int>(t); i < table_size; i += kLieThreads) tab[i] = table[i];
     for (int64_t b0 = static_cast<int64_t>(blockIdx.x) * OB; b0 < n; b0 += static_cast<int64_t>(gridDim.x) * OB) {
         const uint32_t gn = static_cast<uint32_t>(min<int64_t>(OB, n - b0));
@@ -82,16 +95,16 @@ k_lie_rows(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t
         constexpr int U = 4;
         const T* src = arg + (b0 + static_cast<int64_t>(o)) * abs_ + abeg * acs;
         const bool live = o < gn;
-        for (uint32_t c0 = t / OB; c0 <= cols; c0 += U * kSlots) {
+        for (uint32_t c0 = t / OB; c0 <= cols; c0 += U * kStage) {
             T v[U];
 #pragma unroll
             for (int j = 0; j < U; ++j) {
-                const uint32_t c = c0 + j * kSlots;
+                const uint32_t c = c0 + j * kStage;
                 v[j] = (live && c < cols) ? src[static_cast<int64_t>(c) * acs] : T(0);
             }
 #pragma unroll
             for (int j = 0; j < U; ++j) {
-                const uint32_t c = c0 + j * kSlots;
+                const uint32_t c = c0 + j * kStage;
                 if (c <= cols) xs[c * OB + o] = v[j];
             }
         }
@@ -101,7 +114,7 @@ k_lie_rows(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t
             // of dependent index reads; each group holds four entries
             int4 d[R];
             int len[R];
-            T acc[R];
+            T acc[R][J];
             int most = 0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -109,7 +122,8 @@ k_lie_rows(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t
                 d[r] = s < rows ? rowdesc[row0 + s] : make_int4(0, 0, 0, 0);
                 len[r] = d[r].z - d[r].y;
                 most = max(most, len[r]);
-                acc[r] = T(0);
+#pragma unroll
+                for (int j = 0; j < J; ++j) acc[r][j] = T(0);
             }
             uint4 u[R][D];
 #pragma unroll
@@ -128,10 +142,10 @@ k_lie_rows(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t
 #pragma unroll
                     for (int j = 0; j < D; ++j) {
                         if (k + j < len[r]) {
-                            acc[r] = lie_term<T, OB>(acc[r], u[r][j].x, tab, xs, col0, cols, o);
-                            acc[r] = lie_term<T, OB>(acc[r], u[r][j].y, tab, xs, col0, cols, o);
-                            acc[r] = lie_term<T, OB>(acc[r], u[r][j].z, tab, xs, col0, cols, o);
-                            acc[r] = lie_term<T, OB>(acc[r], u[r][j].w, tab, xs, col0, cols, o);
+                            lie_term<T, OB, J>(acc[r], u[r][j].x, tab, xs, col0, cols, og);
+                            lie_term<T, OB, J>(acc[r], u[r][j].y, tab, xs, col0, cols, og);
+                            lie_term<T, OB, J>(acc[r], u[r][j].z, tab, xs, col0, cols, og);
+                            lie_term<T, OB, J>(acc[r], u[r][j].w, tab, xs, col0, cols, og);
                         }
                     }
                 }
@@ -142,7 +156,11 @@ k_lie_rows(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t
             }
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                if (s0 + r * kSlots < rows && o < gn) out[(b0 + o) * obs + (obeg + (d[r].x - row0)) * ocs] = acc[r];
+#pragma unroll
+                for (int j = 0; j < J; ++j) {
+                    const uint32_t ob = og * J + j;
+                    if (s0 + r * kSlots < rows && ob < gn) out[(b0 + ob) * obs + (obeg + (d[r].x - row0)) * ocs] = acc[r][j];
+                }
         }
     }
 }
@@ -166,10 +184,10 @@ int lie_objects_per_group(const DeviceCsr& m, int64_t cols) {
     return 0;
 }
 
-template <typename T, int OB, int R, int D>
+template <typename T, int OB, int R, int D, int J = 1>
 int lie_launch(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg, const DeviceCsr& m,
                int64_t row0, int64_t rows, int64_t col0, hipStream_t stream) {
-    static const bool raised = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_lie_rows<T, OB, R, D>),
+    static const bool raised = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_lie_rows<T, OB, R, D, J>),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                                    static_cast<int>(kLieLdsMax)) == hipSuccess;
     const size_t lds = lie_lds_bytes<T>(m, arg.size, OB);
@@ -177,12 +195,12 @@ int lie_launch(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_
     // object groups are dealt to work-groups in turn, so the grid is what the
     // compute units hold at once (threads, registers and this LDS size)
     int resident = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_lie_rows<T, OB, R, D>, kLieThreads, lds) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_lie_rows<T, OB, R, D, J>, kLieThreads, lds) != hipSuccess)
         resident = 1;
     const int64_t per_cu = std::max(1, resident);
     const int64_t groups = (n + OB - 1) / OB;
     const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min(groups, device_cu_count() * per_cu)));
-    k_lie_rows<T, OB, R, D><<<grid, kLieThreads, lds, stream>>>(
+    k_lie_rows<T, OB, R, D, J><<<grid, kLieThreads, lds, stream>>>(
         static_cast<T*>(out.data), out.batch_stride, out.coeff_stride, out.begin, static_cast<const T*>(arg.data),
         arg.batch_stride, arg.coeff_stride, arg.begin, static_cast<uint32_t>(arg.size), static_cast<uint32_t>(col0),
         m.packed4, m.rowdesc, static_cast<const T*>(m.table), m.table_size, row0, static_cast<int>(rows), n);
@@ -198,6 +216,9 @@ int launch_lie_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_ve
     if (m.table_size == 0 || !m.rowdesc || !m.packed4 || row0 < 0 || row0 + rows > m.rows || col0 < 0 || col0 + arg.size > m.cols)
         return fail(RPP_HIP_NOT_IMPLEMENTED, "no packed Lie matrix for this row and column range");
     const bool long_rows = m.long_rows != 0;
+    // long rows: one thread per row for all objects of a group when the rows
+    // occupy at least half of a work-group's threads that way
+    const bool joint = rows >= kLieThreads / 2;
     // joint rows of short-row matrices: as many as fit in 64 registers
     constexpr int RS = sizeof(T) == 8 ? 3 : 4;
     switch (lie_objects_per_group<T>(m, arg.size) * (long_rows ? -1 : 1)) {
@@ -205,9 +226,15 @@ int launch_lie_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_ve
         case 4: return lie_launch<T, 4, RS, 1>(n, out, arg, m, row0, rows, col0, stream);
         case 2: return lie_launch<T, 2, RS, 1>(n, out, arg, m, row0, rows, col0, stream);
         case 1: return lie_launch<T, 1, RS, 1>(n, out, arg, m, row0, rows, col0, stream);
-        case -8: return lie_launch<T, 8, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
-        case -4: return lie_launch<T, 4, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
-        case -2: return lie_launch<T, 2, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
+        case -8:
+            return joint ? lie_launch<T, 8, 1, 4, 8>(n, out, arg, m, row0, rows, col0, stream)
+                         : lie_launch<T, 8, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
+        case -4:
+            return joint ? lie_launch<T, 4, 1, 4, 4>(n, out, arg, m, row0, rows, col0, stream)
+                         : lie_launch<T, 4, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
+        case -2:
+            return joint ? lie_launch<T, 2, 1, 4, 2>(n, out, arg, m, row0, rows, col0, stream)
+                         : lie_launch<T, 2, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
         case -1: return lie_launch<T, 1, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
         default: return fail(RPP_HIP_NOT_IMPLEMENTED, "Lie argument does not fit in LDS");
     }
