@@ -61,6 +61,9 @@ k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int6
         const int64_t b = item / G::P;
         const int q = static_cast<int>(item - b * G::P);
         const T* src = mult.obj(b);
+        // the exponent row is read with the multiplier, so the two latencies
+        // overlap instead of adding up across the barrier below
+        const T xv = t < W ? ex.obj(b)[(1 + t) * ex.cs] : T(0);
         T reg[G::total > 0 ? G::total : 1];
         T chain[M + 1];
 #pragma unroll
@@ -79,7 +82,7 @@ k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int6
             }
         }
         __syncthreads();  // the previous item's readers of xs and LDS are done
-        if (t < W) xs[t] = ex.obj(b)[(1 + t) * ex.cs];
+        if (t < W) xs[t] = xv;
         __syncthreads();
 #pragma unroll
         for (int k = N; k >= M + 1; --k) {
