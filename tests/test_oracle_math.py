@@ -135,7 +135,7 @@ def test_adjoint_pairing_identity(side):
         O.orc.ft_mul(w, N, prod, a, t)
     else:
         O.orc.ft_mul(w, N, prod, t, a)
-    assert abs(float(adj @ t.T) - float(g @ prod.T)) < 1e-12
+    assert abs((adj @ t.T).item() - (g @ prod.T).item()) < 1e-12
 
 
 def test_chen_identity_and_single_segment():
