@@ -71,6 +71,71 @@ long long orc_tensor_dimension(int width, int depth)
     return B.start[depth + 1];
 }
 
+/* Order of the terms of one output level in cpu::single_thread's product.
+ *
+ * The reference multiplies in tiles (cpu/single_thread/operations/basic/
+ * detail/ft_multiply.hpp) with tile power 5: 32-wide subtiles, 1024-entry
+ * tiles (:25-26).  Its Operation constructor (:114-145) fixes, per width,
+ * the tile letters T and the first tiled output degree F; multiply_oneside
+ * (:1554-1598) then forms output degree k
+ *   - for k < F by contiguous_terms / low_degree_terms
+ *     (contiguous_level_terms, :985-1027): rhs degree j ascending;
+ *   - for k >= F by right_tiled_terms (:1405-1502) with right blocks of
+ *     R = (T == 0 ? 1 : 2T) letters: j = k (lhs[0] * rhs_k, skipped when
+ *     lhs[0] == 0), j = 0 (lhs_k * rhs[0], skipped when rhs[0] == 0), then
+ *     right_tiled_normal_terms j = R .. k-1 ascending (:1297-1336), then
+ *     right_tiled_small_terms j = R-1 .. 1 descending (:1338-1403);
+ *   - at width 1 by width_one_terms (:1126-1173): lhs degree ascending, so
+ *     j descending.
+ * Every term is rounded on its own (x86-64 without FMA: a product, then a
+ * sum), so the order fixes the result bit for bit. */
+static void orc_tile_geometry(int width, int* first_tiled, int* right_letters)
+{
+    const long long tile_width = 32, tile_extent = 1024;
+    int letters = width > tile_width ? 0 : 1;
+    long long size = letters == 0 ? 1 : width;
+    int first = 2 + (width <= tile_width ? 1 : 0);
+    if (width > 1 && width < tile_width) {
+        for (; size * width <= tile_width; ++letters, size *= width) {
+        }
+        first = 2 * letters + 1;
+        long long level = size * size * width;
+        for (; level <= tile_extent; ++first, level *= width) {
+        }
+    }
+    *first_tiled = first;
+    *right_letters = letters == 0 ? 1 : 2 * letters;
+}
+
+/* The rhs degrees j of output degree k in summation order, restricted to
+ * [jlo, jhi] (the degrees both operand windows allow); lhs0_zero / rhs0_zero
+ * say whether lhs[0] / rhs[0] is zero (the tiled path skips those scalar
+ * terms).  Returns the count written to seq (at most k + 1). */
+static int orc_term_order(int width, int k, int jlo, int jhi, int lhs0_zero, int rhs0_zero, int* seq)
+{
+    int n = 0;
+    if (jlo > jhi) return 0;
+    if (width == 1) {
+        for (int j = jhi; j >= jlo; --j) seq[n++] = j;
+        return n;
+    }
+    int first, R;
+    orc_tile_geometry(width, &first, &R);
+    if (k < first) {
+        for (int j = jlo; j <= jhi; ++j) seq[n++] = j;
+        return n;
+    }
+#define ORC_IN(j) ((j) >= jlo && (j) <= jhi)
+    if (ORC_IN(k) && !lhs0_zero) seq[n++] = k;
+    if (k > 0 && ORC_IN(0) && !rhs0_zero) seq[n++] = 0;
+    for (int j = R; j <= k - 1; ++j)
+        if (ORC_IN(j)) seq[n++] = j;
+    for (int j = R - 1; j >= 1; --j)
+        if (ORC_IN(j)) seq[n++] = j;
+#undef ORC_IN
+    return n;
+}
+
 /* Writer kinds of the reference (ft_multiply.hpp:176-435):
  *   W_SCALED:  out = beta * tile                       (ScaledTileWriter)
  *   W_ADD:     out = alpha * add + beta * tile, where the

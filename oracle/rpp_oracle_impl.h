@@ -21,9 +21,11 @@
 /* ---- product core ---------------------------------------------------------
  * Level k of lhs (x) rhs restricted to the operand windows, accumulated in T:
  *   tile[w] = sum_{j} lhs_{k-j}[w / d^j] * rhs_j[w % d^j]
- * over j in [max(rhs.min, k - lhs.max), min(rhs.max, k - lhs.min)], j
- * ascending, which is the rhs-degree order of contiguous_level_terms
- * (cpu/single_thread/operations/basic/detail/ft_multiply.hpp:989-1027).
+ * over j in [max(rhs.min, k - lhs.max), min(rhs.max, k - lhs.min)], in the
+ * order of the reference's tiled product (orc_term_order: ascending below the
+ * first tiled degree, cpu/single_thread/operations/basic/detail/
+ * ft_multiply.hpp:985-1027; the right-tiled order at and above it,
+ * :1297-1502), each product and each sum rounded on its own.
  */
 static void FN(level_product)(const orc_basis* B, T* tile, int k,
                               const T* lhs, int lmin, int lmax,
@@ -35,12 +37,18 @@ static void FN(level_product)(const orc_basis* B, T* tile, int k,
     int jhi = rmax < k - lmin ? rmax : k - lmin;
     if (jlo < 0) jlo = 0;
     if (jhi > k) jhi = k;
-    for (int j = jlo; j <= jhi; ++j) {
+    int seq[ORC_MAX_DEPTH + 2];
+    const int lhs0_zero = lmin == 0 && lhs[0] == (T)0;
+    const int rhs0_zero = rmin == 0 && rhs[0] == (T)0;
+    const int count = orc_term_order(B->width, k, jlo, jhi, lhs0_zero, rhs0_zero, seq);
+    for (int t = 0; t < count; ++t) {
+        const int j = seq[t];
         const long long pj = B->pow[j];
         const T* L = lhs + B->start[k - j];
         const T* R = rhs + B->start[j];
         for (long long w = 0; w < n; ++w) {
-            tile[w] += L[w / pj] * R[w % pj];
+            const T prod = L[w / pj] * R[w % pj];
+            tile[w] = tile[w] + prod;
         }
     }
 }
@@ -252,33 +260,30 @@ void FN(orc_ft_adj_lmul)(const orc_basis* B, T* out, int omin, int omax,
     }
 }
 
-/* FTAdjRMul (cpu/.../basic/ft_adj_rmul.hpp:102-134), which is
- * antipode o adj_lmul(antipode op, antipode arg); the signs cancel and it
- * equals out[w] = sum_u arg[wu] op[u] over the same degree windows. */
+static void FN(antipode)(const orc_basis* B, T* out, int omin, int omax, const T* arg, int amin, int amax,
+                         int sign_by_degree);
+
+/* FTAdjRMul (cpu/.../basic/ft_adj_rmul.hpp:102-134), by the reference's own
+ * route: zeroed scratch, antipode of op and of arg, adj_lmul of those into the
+ * scratch, antipode of the result into out, each over the windows of out, op
+ * and arg.  The antipodes only flip signs and reverse words, which rounding
+ * commutes with, so the sums are those of adj_lmul on the reversed words:
+ * out[w] = sum_u arg[wu] op[u], term for term as the reference forms it. */
 void FN(orc_ft_adj_rmul)(const orc_basis* B, T* out, int omin, int omax,
                          const T* op, int pmin, int pmax, const T* arg,
                          int gmin, int gmax)
 {
-    FN(set_constant)(B, out, omin, omax, (T)0);
-    int gmx = gmax < omax + pmax ? gmax : omax + pmax;
-    int gmn = gmin > omin + pmin ? gmin : omin + pmin;
-    if (gmn > gmx) return;
-    for (int ad = gmx; ad >= gmn; --ad) {
-        int olo = ad - pmax > omin ? ad - pmax : omin;
-        int ohi = ad - pmin < omax ? ad - pmin : omax;
-        const T* A = arg + B->start[ad];
-        for (int od = ohi; od >= olo; --od) {
-            const int pd = ad - od;
-            const T* P = op + B->start[pd];
-            T* O = out + B->start[od];
-            const long long osz = B->pow[od], psz = B->pow[pd];
-            for (long long i = 0; i < osz; ++i) {
-                T acc = O[i];
-                for (long long u = 0; u < psz; ++u) acc = acc + A[i * psz + u] * P[u];
-                O[i] = acc;
-            }
-        }
-    }
+    const long long dim = B->start[B->depth + 1];
+    T* ws = (T*)calloc((size_t)(3 * dim), sizeof(T));
+    if (!ws) return;
+    T* ow = ws;
+    T* pw = ws + dim;
+    T* aw = ws + 2 * dim;
+    FN(antipode)(B, pw, pmin, pmax, op, pmin, pmax, 1);
+    FN(antipode)(B, aw, gmin, gmax, arg, gmin, gmax, 1);
+    FN(orc_ft_adj_lmul)(B, ow, omin, omax, pw, pmin, pmax, aw, gmin, gmax);
+    FN(antipode)(B, out, omin, omax, ow, omin, omax, 1);
+    free(ws);
 }
 
 /* SparseMatrixVectorProduct, CSR rows (cpu/.../linalg/sparse_matrix_vector.hpp

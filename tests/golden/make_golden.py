@@ -7,8 +7,9 @@ oracle/_ref/librpp_ref.so on seeded inputs.  The fixtures (inputs and outputs,
 FP64, small batches at the shapes of the benchmark workloads) are committed,
 so the parity tests can use them where the reference is absent.
 
-    python3 tests/golden/make_golden.py
+    python3 tests/golden/make_golden.py [fixture names: only those]
 """
+import hashlib
 import os
 import sys
 
@@ -104,6 +105,24 @@ def main():
     lie = O.ref_lie_convert(True, True, 3, 8, lg)
     back = O.ref_lie_convert(False, True, 3, 8, lie)
     out["logsig_w3d8"] = dict(increments=inc, signature=sig, log=lg, lie=lie, back=back)
+    # ft_log on the worst-conditioned log-signatures of the width 3, depth 8
+    # workload (16384 paths x 64 increments N(0, 1/64), seed 0xC3): the 32
+    # paths whose log-signature has the largest coefficient (up to ~30, where
+    # the Horner sums cancel most).  Kept: the increments, the reference's
+    # ft_log of the reference's signatures, and a SHA-256 of those signatures
+    # (the oracle's chain rebuilds them bit for bit, which the test checks)
+    n_all, L = 16384, 64
+    rng = np.random.default_rng(0xC3)
+    inc_all = rng.normal(0.0, np.sqrt(1.0 / L), size=(n_all, L, 3))
+    sig_all = O.ref_signature(3, 8, inc_all, threads=os.cpu_count() or 1)
+    lg_all = np.zeros_like(sig_all)
+    R.ft_log(3, 8, lg_all, sig_all)
+    worst = np.sort(np.argsort(-np.abs(lg_all).max(axis=1), kind="stable")[:32])
+    sig_w = np.ascontiguousarray(sig_all[worst])
+    out["log_c3_worst_w3d8"] = dict(paths=worst.astype(np.int64), increments=inc_all[worst],
+                                    log=lg_all[worst],
+                                    signature_sha256=np.frombuffer(hashlib.sha256(sig_w.tobytes()).digest(),
+                                                                   dtype=np.uint8))
     # linear operations on partial windows, tensor (3,4) and Lie (3,4)
     # gradings: 3 objects each
     rng = np.random.default_rng(0x11)
@@ -152,9 +171,12 @@ def main():
         getattr(R, name)(w, N, o, x, y, ow=(0, 3), xw=(0, 4), yw=(0, 4))
         sh[name] = o
     out["shuffle_w3d4"] = sh
+    only = set(sys.argv[1:])
     for name, arrays in out.items():
+        if only and name not in only:
+            continue
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
-    print("wrote", len(out), "fixtures to", HERE)
+    print("wrote", len(only) if only else len(out), "fixtures to", HERE)
 
 
 if __name__ == "__main__":

@@ -15,8 +15,11 @@ TOL = {np.float64: 1e-12, np.float32: 1e-5}
 
 
 def check(a, b, dtype):
-    # the oracle restates the reference's accumulation order; allow rounding
+    # the oracle restates the reference's summation order term for term
+    # (its tiled product, ft_multiply.hpp), each product and sum rounded on
+    # its own, so the results agree bit for bit
     assert rel_err(a, b) <= TOL[dtype]
+    assert np.array_equal(a, b), f"not bit-identical: max rel {rel_err(a, b):.3e}"
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
@@ -120,6 +123,44 @@ def test_intermediates(shape, win, dtype):
     O.refimpl.ft_fmexp(w, N, b, mult, arg, ow=ow, mw=mw, ew=aw)
     assert np.array_equal(np.isnan(a), np.isnan(b))
     check(np.nan_to_num(a), np.nan_to_num(b), dtype)
+
+
+# Shapes whose top output degrees the reference forms by its right-tiled
+# product (first tiled degree F: 7 at width 3, 5 at width 5, 4 at widths 6
+# and 8, 11 at width 2, 3 at width 32, 2 above 32), where the summation order
+# differs from ascending rhs degree.
+TILED_SHAPES = [(3, 8), (5, 6), (6, 5), (8, 4), (2, 12), (32, 3), (33, 2)]
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("shape", TILED_SHAPES)
+def test_right_tiled_degrees_bit_identical(shape, dtype):
+    """Products, exp, log and fmexp at shapes that reach the reference's
+    right-tiled product order, bit for bit; with lhs[0] = 0 in one object
+    (the tiled path skips that scalar term)."""
+    w, N = shape
+    rng = np.random.default_rng(w * 7 + N)
+    dim = O.tensor_dimension(w, N)
+    lhs, rhs = damped_batch(rng, 2, w, N, dtype), damped_batch(rng, 2, w, N, dtype)
+    lhs[1, 0] = 0
+    for ow, lw, rw in (((0, N), (0, N), (0, N)), ((1, N), (0, N - 1), (1, N))):
+        a = np.full((2, dim), np.nan, dtype=dtype)
+        b = a.copy()
+        O.orc.ft_mul(w, N, a, lhs, rhs, ow=ow, lw=lw, rw=rw)
+        O.refimpl.ft_mul(w, N, b, lhs, rhs, ow=ow, lw=lw, rw=rw)
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        check(np.nan_to_num(a), np.nan_to_num(b), dtype)
+    for op in ("ft_exp", "ft_log"):
+        a = np.zeros((2, dim), dtype=dtype)
+        b = a.copy()
+        getattr(O.orc, op)(w, N, a, rhs)
+        getattr(O.refimpl, op)(w, N, b, rhs)
+        check(a, b, dtype)
+    a = np.zeros((2, dim), dtype=dtype)
+    b = a.copy()
+    O.orc.ft_fmexp(w, N, a, lhs, rhs)
+    O.refimpl.ft_fmexp(w, N, b, lhs, rhs)
+    check(a, b, dtype)
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
