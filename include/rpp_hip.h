@@ -457,7 +457,8 @@ int rpp_hip_lie_basis_load(const char* path, int32_t* width, int32_t* depth);
  *               offsets back (one host synchronisation) and queues paths by
  *               decreasing length.  Otherwise order is a device permutation
  *               of 0..n-1 (the queue order, longest paths first for balance)
- *               and num_increments = offsets[n]: the call then issues kernel
+ *               and num_increments >= offsets[n] (the row count of
+ *               increments, for instance): the call then issues kernel
  *               launches only, so it can be captured in a hipGraph once the
  *               library workspace has been sized by an uncaptured call.
  * tile_threads > 0 selects a compiled tile (rpp_hip_signature_tiles),
@@ -486,7 +487,9 @@ int rpp_hip_signature_f64(const rpp_hip_signature_desc* desc, void* stream);
 /* Backward of rpp_hip_signature_*: given the final signatures S_L (out of the
  * forward pass) and the cotangent G_L = d loss / d S_L (a tensor batch, or one
  * functional broadcast with batch_stride 0), write d loss / d increments
- * (device [total, width]).  Intermediate states are reconstructed backward
+ * (device [total, width]): rows offsets[0] .. offsets[n]-1 are written, rows
+ * outside that range (which belong to no path) are left untouched; offsets[0]
+ * may be above 0.  Intermediate states are reconstructed backward
  * with S_{t-1} = S_t (x) exp(-dX_t); when the forward pass stored checkpoints
  * the reconstruction is re-anchored at each of them (sqrt(L) checkpointing).
  * scratch: device workspace of rpp_hip_signature_backward_scratch_bytes() for

@@ -807,12 +807,12 @@ int signature_backward_impl(const rpp_hip_signature_backward_desc* d, void* s) {
     if (order == nullptr) {
         std::vector<int64_t> lengths;
         int64_t* host_order = nullptr;
-        RPP_TRY(path_order(d->offsets, d->num_paths, lengths, &host_order, stream));
+        // the tiled kernels index their partial sums by absolute row, so the
+        // buffer spans rows [0, offsets[n]), not only the sum of the lengths
+        RPP_TRY(path_order(d->offsets, d->num_paths, lengths, &host_order, stream, &total));
         order = host_order;
-        total = 0;
-        for (int64_t L : lengths) total += L;
     } else if (total < 0) {
-        return fail(RPP_HIP_INVALID_ARGUMENT, "num_increments must be offsets[num_paths] when order is given");
+        return fail(RPP_HIP_INVALID_ARGUMENT, "num_increments must be >= offsets[num_paths] when order is given");
     }
     if (d->scratch == nullptr && d->tile_threads >= 0 && d->tile_kind == 2) {
         for (const auto& c : thread_tiles<T>()) {

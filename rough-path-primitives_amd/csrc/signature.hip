@@ -301,13 +301,15 @@ k_fmexp_adj_generic(DevBasis B, int64_t n, View<const T> mult, View<const T> ex,
 
 // Queue order: paths by decreasing length (longest first balances the tail).
 int path_order(const int64_t* d_offsets, int64_t n, std::vector<int64_t>& lengths_host,
-               int64_t** d_order, hipStream_t stream) {
+               int64_t** d_order, hipStream_t stream, int64_t* last_offset) {
     std::vector<int64_t> offs(static_cast<size_t>(n) + 1);
     if (int rc = check_hip(hipMemcpyAsync(offs.data(), d_offsets, offs.size() * sizeof(int64_t),
                                           hipMemcpyDeviceToHost, stream),
                            "copying path offsets"))
         return rc;
     if (int rc = check_hip(hipStreamSynchronize(stream), "synchronising for path offsets")) return rc;
+    if (offs[0] < 0) return fail(RPP_HIP_INVALID_ARGUMENT, "offsets[0] must be >= 0");
+    if (last_offset) *last_offset = offs[static_cast<size_t>(n)];
     lengths_host.resize(static_cast<size_t>(n));
     for (int64_t i = 0; i < n; ++i) {
         lengths_host[i] = offs[i + 1] - offs[i];

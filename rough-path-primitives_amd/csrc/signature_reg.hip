@@ -662,11 +662,16 @@ int resident_per_cu(K kernel, int threads) {
     return blocks;
 }
 
+// Partial rows are indexed by absolute increment row; rows outside
+// [offsets[0], offsets[num_paths]) belong to no path, hold no partial sums and
+// are left untouched in grad.
 template <typename T, int W>
-__global__ void k_sum_partials(const T* partial, T* grad, int64_t total, int P) {
+__global__ void k_sum_partials(const T* partial, T* grad, int64_t total, int P, const int64_t* offsets,
+                               int64_t num_paths) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= total * W) return;
     const int64_t t = i / W;
+    if (t < offsets[0] || t >= offsets[num_paths]) return;
     const int b = static_cast<int>(i - t * W);
     T acc = T(0);
     for (int q = 0; q < P; ++q) acc += partial[(t * P + q) * W + b];
@@ -708,7 +713,8 @@ int bwd_launch(const DevBasis& B, const rpp_hip_signature_backward_desc& d, cons
     if (total_incr > 0) {
         const int64_t n = total_incr * W;
         k_sum_partials<T, W><<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(
-            static_cast<const T*>(part), static_cast<T*>(d.grad_increments), total_incr, G::P);
+            static_cast<const T*>(part), static_cast<T*>(d.grad_increments), total_incr, G::P, d.offsets,
+            d.num_paths);
     }
     return check_hip(hipGetLastError(), "k_sum_partials launch");
 }

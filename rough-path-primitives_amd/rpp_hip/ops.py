@@ -635,7 +635,9 @@ def signature_backward(increments: torch.Tensor, offsets: torch.Tensor, basis: T
         best = autotune.best(basis.width, basis.depth, increments.dtype, "backward") or (0, 0, 0, 0)
         tile_threads, tile_split, tile_kind, tile_expand = best[0], best[1], best[2], best[3]
     if grad is None:
-        grad = torch.empty_like(inc)
+        # rows outside [offsets[0], offsets[n]) belong to no path: the library
+        # leaves them untouched, so they start (and stay) at zero
+        grad = torch.zeros_like(inc)
     d = A.SignatureBackwardDesc(basis._c(), n, inc.data_ptr(), offsets.data_ptr(),
                                 Batch(signatures, 0, basis.depth)._c(),
                                 Batch(cotangent, 0, basis.depth)._c(), grad.data_ptr(), checkpoint_every,

@@ -32,17 +32,24 @@ def test_signature_uneven(dev, shape, dtype):
                  dtype, f"signature {shape}")
 
 
-def test_signature_matches_reference_chain_fp64(dev):
-    """Same inputs as the reference's own chain of ft_fmexp (oracle/_ref when built)."""
-    if not O.has_ref():
-        pytest.skip("reference driver not built on this machine")
-    w, N, n, L = 4, 4, 8, 12
-    rng = np.random.default_rng(1)
-    inc = rng.normal(0, 0.3, size=(n, L, w))
-    expected = O.ref_signature(w, N, inc)
-    got = rpp_hip.signature(torch.tensor(inc.reshape(n * L, w), device=dev),
-                            torch.arange(n + 1, device=dev, dtype=torch.int64) * L, rpp_hip.TensorBasis(w, N))
-    assert_close(got.cpu().numpy(), expected, np.float64, "vs reference chain")
+def test_signature_tiles_match_reference_chain_fp64(dev):
+    """Every compiled forward tile at width 5, depth 6 (and the storage-generic
+    kernel) against the reference's own chain of ft_fmexp, from the golden
+    fixture it wrote (2 paths x 16 increments)."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "signature_w5d6.npz")
+    with np.load(path) as z:
+        inc, expected = z["increments"], z["out"]
+    n, L, w = inc.shape
+    basis = rpp_hip.TensorBasis(w, 6)
+    inc_d = torch.tensor(inc.reshape(n * L, w), device=dev)
+    off_d = torch.arange(n + 1, device=dev, dtype=torch.int64) * L
+    tiles = [t for t in _tiles(torch.float64) if (t[0], t[1]) == (5, 6)] + [(5, 6, -1, 0, 0, 0)]
+    assert len(tiles) > 2
+    for (_, _, nt, split, kind, expand) in tiles:
+        got = rpp_hip.signature(inc_d, off_d, basis, tile_threads=nt, tile_split=split, tile_kind=kind,
+                                tile_expand=expand)
+        assert_close(got.cpu().numpy(), expected, np.float64, f"tile {(nt, split, kind, expand)} vs reference")
 
 
 @pytest.mark.parametrize("shape", [(2, 5), (3, 4), (4, 3), (5, 3), (6, 2)])
@@ -160,6 +167,49 @@ def test_every_register_tile_backward_fp64(dev, ckpt):
                                           **kw)
         err = rel_err(grad.cpu().numpy(), expected)
         assert err < 1e-10, f"tile {(w, N, nt, split, kind, expand)} ckpt={ckpt}: {err:.3e}"
+
+
+@pytest.mark.parametrize("device_order", [False, True], ids=["host_order", "device_order"])
+def test_backward_paths_not_starting_at_row_zero(dev, device_order):
+    """offsets[0] > 0 and rows after offsets[n]: every backward tile and the
+    storage-generic kernel write the gradient rows of the paths (against the
+    oracle's gradient) and leave the rows outside them untouched (a sentinel
+    survives).  The tiled kernels index their partial sums by absolute row."""
+    head, tail = 3, 2
+    tiles = [(w, N, nt, split, kind, expand) for (w, N, nt, split, kind, expand) in _tiles(torch.float64)
+             if (kind == 0 and split <= 1) or (kind == 2 and expand == 1)]
+    tiles += [(w, N, -1, 0, 0, 0) for (w, N) in sorted({(t[0], t[1]) for t in tiles})]
+    for (w, N, nt, split, kind, expand) in tiles:
+        rng = np.random.default_rng(w * 13 + N + max(nt, 0) + split)
+        inc, off = paths(rng, 4, w, 1, 5)
+        total = int(off[-1])
+        full = rng.normal(0, 0.5, size=(head + total + tail, w))
+        full[head:head + total] = inc
+        off_shift = off + head
+        basis = rpp_hip.TensorBasis(w, N)
+        ell = rng.normal(size=basis.size())
+        expected = O.signature_vjp(w, N, inc, off, ell)
+        inc_d = torch.tensor(full, device=dev)
+        off_d = torch.tensor(off_shift, device=dev)
+        order = rpp_hip.path_order(off_d) if device_order else None
+        sig = rpp_hip.signature(inc_d, off_d, basis, order=order)
+        assert_close(sig.cpu().numpy(), O.signature(w, N, inc, off), np.float64, f"forward {(w, N)}")
+        sentinel = 12345.0
+        grad = torch.full_like(inc_d, sentinel)
+        rpp_hip.signature_backward(inc_d, off_d, basis, sig, torch.tensor(ell, device=dev), grad=grad,
+                                   tile_threads=nt, tile_split=split, tile_expand=expand, tile_kind=kind,
+                                   order=order)
+        g = grad.cpu().numpy()
+        tag = f"tile {(w, N, nt, split, kind, expand)}"
+        assert np.all(g[:head] == sentinel) and np.all(g[head + total:] == sentinel), f"{tag}: rows outside written"
+        err = rel_err(g[head:head + total], expected)
+        assert err < 1e-10, f"{tag}: {err:.3e}"
+        # the Python wrapper's own gradient buffer is zero outside the paths
+        g2 = rpp_hip.signature_backward(inc_d, off_d, basis, sig, torch.tensor(ell, device=dev),
+                                        tile_threads=nt, tile_split=split, tile_expand=expand, tile_kind=kind,
+                                        order=order).cpu().numpy()
+        assert np.all(g2[:head] == 0) and np.all(g2[head + total:] == 0), f"{tag}: wrapper rows outside"
+        assert np.array_equal(g2[head:head + total], g[head:head + total]), f"{tag}: repeat differs"
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
