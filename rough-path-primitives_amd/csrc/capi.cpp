@@ -116,11 +116,6 @@ int check_count(int64_t n) {
     return 0;
 }
 
-#define RPP_TRY(expr)               \
-    do {                            \
-        if (int rc_ = (expr)) return rc_; \
-    } while (0)
-
 hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
 template <typename T>

@@ -45,15 +45,23 @@ __device__ __forceinline__ int64_t at(int64_t idx, int64_t cs) {
 // degrees descending, each level finished by a barrier.  `add` is consulted
 // only on levels inside its window (ScaledTileAddWriter semantics,
 // ft_multiply.hpp:260-395); without it the level is beta * product
-// (ScaledTileWriter, :176-258).
+// (ScaledTileWriter, :176-258).  Each word's terms are summed in the
+// reference's CPU order and every product and sum is rounded on its own
+// (cpu_order_level_sum, contraction off), so the pass agrees with the
+// reference bit for bit.  lhs[0] and rhs[0] are read before any level is
+// written (degree 0 is the last one), so out may alias them.
 template <typename T, int W, bool CS1>
 __device__ void product_pass(const DevBasis& B, T* o, int64_t ocs, int omn, int omx,
                              const T* l, int64_t lcs, int lmn, int lmx,
                              const T* r, int64_t rcs, int rmn, int rmx,
                              const T* a, int64_t acs, int amn, int amx, bool with_add,
                              T alpha, T beta) {
-    using L = Letters<W>;
-    const uint32_t width = L::width(B);
+#pragma clang fp contract(off)
+    const bool lhs0_zero = lmn == 0 && lmx >= 0 && l[0] == T(0);
+    const bool rhs0_zero = rmn == 0 && rmx >= 0 && r[0] == T(0);
+    auto lval = [&](int d, uint32_t i) { return l[at<CS1>(B.start[d] + i, lcs)]; };
+    auto rval = [&](int d, uint32_t i) { return r[at<CS1>(B.start[d] + i, rcs)]; };
+    const int F = CpuTiling<W>::first(B), R = CpuTiling<W>::right(B);
     for (int k = omx; k >= omn; --k) {
         int jlo = max(rmn, k - lmx);
         int jhi = min(rmx, k - lmn);
@@ -63,23 +71,7 @@ __device__ void product_pass(const DevBasis& B, T* o, int64_t ocs, int omn, int 
         const uint32_t nk = B.pow[k];
         const int64_t ok = B.start[k];
         for (uint32_t w = threadIdx.x; w < nk; w += blockDim.x) {
-            T acc = T(0);
-            if (jlo <= jhi) {
-                uint32_t p = w;
-                uint32_t s = 0;
-                for (int j = 0; j < jlo; ++j) {
-                    const uint32_t q = L::div(B, p);
-                    s += (p - q * width) * B.pow[j];
-                    p = q;
-                }
-                for (int j = jlo;; ++j) {
-                    acc += l[at<CS1>(B.start[k - j] + p, lcs)] * r[at<CS1>(B.start[j] + s, rcs)];
-                    if (j == jhi) break;
-                    const uint32_t q = L::div(B, p);
-                    s += (p - q * width) * B.pow[j];
-                    p = q;
-                }
-            }
+            const T acc = cpu_order_level_sum<W, T>(B, F, R, k, w, jlo, jhi, lhs0_zero, rhs0_zero, lval, rval);
             T v = beta * acc;
             if (use_add) v = alpha * a[at<CS1>(ok + w, acs)] + v;
             o[at<CS1>(ok + w, ocs)] = v;
@@ -135,9 +127,9 @@ template <typename T, int W>
 __global__ void __launch_bounds__(kBlock)
 k_product_lds(DevBasis B, View<T> out, View<const T> lhs, View<const T> rhs, View<const T> add,
               int with_add, T alpha, T beta, int64_t n, int G) {
+#pragma clang fp contract(off)
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    using L = Letters<W>;
-    const uint32_t width = L::width(B);
+    const int F = CpuTiling<W>::first(B), R = CpuTiling<W>::right(B);
     T* sl = reinterpret_cast<T*>(smem_raw);
     const int64_t dim = B.start[B.depth + 1];
     T* sr = sl + G * dim;
@@ -173,22 +165,12 @@ k_product_lds(DevBasis B, View<T> out, View<const T> lhs, View<const T> rhs, Vie
             const int jhi = min(min(rhs.mx, k - lhs.mn), k);
             const T* Lg = sl + g * dim;
             const T* Rg = sr + g * dim;
-            T acc = T(0);
-            if (jlo <= jhi) {
-                uint32_t p = w, sfx = 0;
-                for (int j = 0; j < jlo; ++j) {
-                    const uint32_t q = L::div(B, p);
-                    sfx += (p - q * width) * B.pow[j];
-                    p = q;
-                }
-                for (int j = jlo;; ++j) {
-                    acc += Lg[B.start[k - j] + p] * Rg[B.start[j] + sfx];
-                    if (j == jhi) break;
-                    const uint32_t q = L::div(B, p);
-                    sfx += (p - q * width) * B.pow[j];
-                    p = q;
-                }
-            }
+            const bool lhs0_zero = lhs.mn == 0 && lw && Lg[0] == T(0);
+            const bool rhs0_zero = rhs.mn == 0 && rw && Rg[0] == T(0);
+            const T acc = cpu_order_level_sum<W, T>(
+                B, F, R, k, w, jlo, jhi, lhs0_zero, rhs0_zero,
+                [&](int d, uint32_t x) { return Lg[B.start[d] + x]; },
+                [&](int d, uint32_t x) { return Rg[B.start[d] + x]; });
             T v = beta * acc;
             if (with_add && add.mn <= k && k <= add.mx) v = alpha * add.obj(g0 + g)[i] + v;
             out.obj(g0 + g)[i] = v;

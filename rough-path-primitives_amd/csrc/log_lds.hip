@@ -18,11 +18,17 @@
 // 3, depth 8, 16384 objects: FP64 1.14 -> 0.96 ms, FP32 0.91 -> 0.81 ms;
 // results unchanged bit for bit, as the arithmetic is the same).  The shape is fixed
 // at compile time, so every degree offset and division by width^j is a
-// constant; the split order (ascending j) is that of the product kernels.
+// constant.  Each sum adds its terms in the order of the reference's CPU
+// product (cpu_term_no_rhs0: ascending rhs degree below the first right-tiled
+// degree, the right-tiled order at and above it), every product and sum
+// rounded on its own (contraction off), so the logarithm agrees with the
+// reference's cpu::single_thread ft_log bit for bit.
 #include "common.hpp"
 #include "log_lds.hpp"
 
 #include <algorithm>
+
+#pragma clang fp contract(off)
 
 namespace rpp_hip {
 
@@ -81,6 +87,30 @@ constexpr int log_waves() {
     return sizeof(T) == 4 ? 8 : 1;
 }
 
+// sum_u P_{K-j}[w / width^j] * Y_j[w mod width^j] over the terms u = 0 ..
+// K-1 of output degree K in the reference's product order (j its u-th rhs
+// degree), Y restricted to [ylo, yhi]; every index is a compile-time
+// constant.  TOP: the term j = K = N reads Y_N from y (from HBM) with P_0 = p0.
+template <typename T, int W, int N, int K, bool TOP>
+__device__ __forceinline__ T log_level_sum(const T* ps, const T* ys, int w, int ylo, int yhi, T p0, T y, bool top) {
+    using G = LogGeo<W, N>;
+    constexpr int kF = cpu_first_tiled_degree(W), kR = cpu_right_letters(W);
+    T acc = T(0);
+    static_for<K>([&](auto uc) {
+        constexpr int j = cpu_term_no_rhs0_fr(kF, kR, W == 1, K, decltype(uc)::value);
+        if constexpr (TOP && j == K) {
+            if (top) acc += p0 * y;
+        } else {
+            constexpr int wj = G::words(j), sk = G::start(K - j), sj = G::start(j);
+            if (j >= ylo && j <= yhi) {
+                const int p = w / wj;
+                acc += ps[sk + p] * ys[sj + (w - p * wj)];
+            }
+        }
+    });
+    return acc;
+}
+
 template <typename T, int W, int N, int NT>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(log_waves<T>())))
 k_log_lds(View<T> out, View<const T> arg, int64_t n) {
@@ -129,28 +159,23 @@ k_log_lds(View<T> out, View<const T> arg, int64_t n) {
         for (int d = N; d >= 2; --d) {
             const int K = N - d + 1;
             // new P_k = sum_{j=1..k} P_{k-j} (x) Y_j for 1 <= k <= K
+            static_for<N>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                if constexpr (k >= 1) {
+                    if (k <= K) {
 #pragma unroll
-            for (int k = 1; k < N; ++k) {
-                if (k > K) continue;
-#pragma unroll
-                for (int s = 0; s < G::slots(k, NT); ++s) {
-                    const int w = t + s * NT;
-                    if (G::words(k) % NT == 0 || w < G::words(k)) {
-                        T acc = T(0);
-#pragma unroll
-                        for (int j = 1; j <= k; ++j) {
-                            if (j >= ylo && j <= yhi) {
-                                const int p = w / G::words(j);
-                                acc += ps[G::start(k - j) + p] * ys[G::start(j) + (w - p * G::words(j))];
-                            }
+                        for (int s = 0; s < G::slots(k, NT); ++s) {
+                            const int w = t + s * NT;
+                            if (G::words(k) % NT == 0 || w < G::words(k))
+                                reg[G::reg_offset(k, NT) + s] =
+                                    log_level_sum<T, W, N, k, false>(ps, ys, w, ylo, yhi, T(0), T(0), false);
+                            // keep the scheduler from hoisting every slot's LDS reads at
+                            // once: the in-flight values would not fit in registers
+                            __builtin_amdgcn_sched_barrier(0);
                         }
-                        reg[G::reg_offset(k, NT) + s] = acc;
                     }
-                    // keep the scheduler from hoisting every slot's LDS reads at
-                    // once: the in-flight values would not fit in registers
-                    __builtin_amdgcn_sched_barrier(0);
                 }
-            }
+            });
             __syncthreads();
 #pragma unroll
             for (int k = 1; k < N; ++k) {
@@ -169,34 +194,18 @@ k_log_lds(View<T> out, View<const T> arg, int64_t n) {
         // the product is 0
         T* o = out.obj(b);
         if (t == 0) o[0] = T(0);
-#pragma unroll
-        for (int k = 1; k < N; ++k) {
-            for (int w = t; w < G::words(k); w += NT) {
-                T acc = T(0);
-#pragma unroll
-                for (int j = 1; j <= k; ++j) {
-                    if (j >= ylo && j <= yhi) {
-                        const int p = w / G::words(j);
-                        acc += ps[G::start(k - j) + p] * ys[G::start(j) + (w - p * G::words(j))];
-                    }
-                }
-                o[G::start(k) + w] = acc;
+        static_for<N>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            if constexpr (k >= 1) {
+                for (int w = t; w < G::words(k); w += NT)
+                    o[G::start(k) + w] = log_level_sum<T, W, N, k, false>(ps, ys, w, ylo, yhi, T(0), T(0), false);
             }
-        }
+        });
         const T p0 = ps[0];
         for (int w = t; w < G::words(N); w += NT) {
             // issued before the LDS reads of the sum, whose latency it overlaps
             const T y = top ? a[G::start(N) + w] : T(0);
-            T acc = T(0);
-#pragma unroll
-            for (int j = 1; j < N; ++j) {
-                if (j >= ylo && j <= yhi) {
-                    const int p = w / G::words(j);
-                    acc += ps[G::start(N - j) + p] * ys[G::start(j) + (w - p * G::words(j))];
-                }
-            }
-            if (top) acc += p0 * y;
-            o[G::start(N) + w] = acc;
+            o[G::start(N) + w] = log_level_sum<T, W, N, N, true>(ps, ys, w, ylo, yhi, p0, y, top);
         }
     }
 }

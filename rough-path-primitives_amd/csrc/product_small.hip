@@ -5,9 +5,11 @@
 //                (+ alpha * add_k[w] on add's window)
 //
 // over the window-restricted j range, as product_pass in ft_ops.hip computes
-// it (same terms, same ascending-j order).  With width and depth known to the
-// compiler every degree offset, word count and division by width^j is a
-// constant, so the inner loops unroll into LDS reads and fused multiply-adds.
+// it: the same terms in the reference's CPU order (cpu_term), every product
+// and sum rounded on its own (contraction off), so the results agree with the
+// reference bit for bit.  With width and depth known to the compiler every
+// degree offset, word count, division by width^j and the term order are
+// constants, so the inner loops unroll into LDS reads, multiplies and adds.
 // A work-group takes one object (a grid-stride loop covers batches beyond the
 // grid limit): it stages the object's lhs and rhs windows in LDS with
 // coalesced loads, then writes every output word once with coalesced stores.  All operand reads of an object precede its first write,
@@ -16,6 +18,8 @@
 #include "product_small.hpp"
 
 #include <algorithm>
+
+#pragma clang fp contract(off)
 
 namespace rpp_hip {
 
@@ -78,9 +82,14 @@ k_product_small(View<T> out, View<const T> lhs, View<const T> rhs, View<const T>
         __syncthreads();
         T* O = out.obj(b);
         const T* A = with_add ? add.obj(b) : nullptr;
-#pragma unroll
-        for (int k = N; k >= 0; --k) {
-            if (!FULL && (k < out.mn || k > out.mx)) continue;
+        // the tiled order skips the scalar terms of a zero lhs[0] / rhs[0]
+        const bool lhs0_zero = (FULL || lhs.mn == 0) && sl[0] == T(0);
+        const bool rhs0_zero = (FULL || rhs.mn == 0) && sr[0] == T(0);
+        // output degrees descending; k, each term's j and every index below
+        // are compile-time constants
+        static_for<N + 1>([&](auto ic) {
+            constexpr int k = N - decltype(ic)::value;
+            if (!FULL && (k < out.mn || k > out.mx)) return;
             const int jlo = FULL ? 0 : max(max(rhs.mn, k - lhs.mx), 0);
             const int jhi = FULL ? k : min(min(rhs.mx, k - lhs.mn), k);
             const bool use_add = !FULL && with_add && add.mn <= k && k <= add.mx;
@@ -88,21 +97,25 @@ k_product_small(View<T> out, View<const T> lhs, View<const T> rhs, View<const T>
             for (int s = 0; s < (G::words(k) + NT - 1) / NT; ++s) {
                 const int w = t + s * NT;
                 if (G::words(k) % NT == 0 || w < G::words(k)) {
+                    constexpr int kF = cpu_first_tiled_degree(W), kR = cpu_right_letters(W);
+                    constexpr bool tiled = W > 1 && k >= kF;
                     T acc = T(0);
-#pragma unroll
-                    for (int j = 0; j <= k; ++j) {
+                    static_for<k + 1>([&](auto uc) {
+                        constexpr int j = cpu_term_fr(kF, kR, W == 1, k, decltype(uc)::value);
+                        constexpr int wj = G::words(j), sk = G::start(k - j), sj = G::start(j);
+                        if (tiled && j == k && lhs0_zero) return;
+                        if (tiled && j == 0 && rhs0_zero) return;
                         if (FULL || (j >= jlo && j <= jhi)) {
-                            const int p = w / G::words(j);
-                            const int sfx = w - p * G::words(j);
-                            acc += sl[G::start(k - j) + p] * sr[G::start(j) + sfx];
+                            const int p = w / wj;
+                            acc += sl[sk + p] * sr[sj + (w - p * wj)];
                         }
-                    }
+                    });
                     T v = beta * acc;
                     if (use_add) v = alpha * A[G::start(k) + w] + v;
                     stream_store(v, &O[G::start(k) + w]);
                 }
             }
-        }
+        });
         __syncthreads();
     }
 }

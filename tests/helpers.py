@@ -104,6 +104,16 @@ def assert_close(actual, expected, dtype, what=""):
     assert err <= tol, f"{what}: relative error {err:.3e} > {tol:.0e}"
 
 
+def assert_exact(actual, expected, dtype, what=""):
+    """Within the reference's tolerance, and equal bit for bit (NaN where the
+    expected value is NaN)."""
+    a, e = np.asarray(actual), np.asarray(expected)
+    fa, fe = np.nan_to_num(a, nan=0.0), np.nan_to_num(e, nan=0.0)
+    assert_close(fa, fe, dtype, what)
+    assert np.array_equal(np.isnan(a), np.isnan(e)), f"{what}: NaN positions differ"
+    assert np.array_equal(fa, fe), f"{what}: not bit-identical (max rel {rel_err(fa, fe):.3e})"
+
+
 def degree_begin(width, depth):
     out = [0]
     p = 1

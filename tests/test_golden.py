@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import assert_close, rel_err
+from helpers import assert_close, assert_exact, rel_err
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TOL = 1e-12
@@ -231,6 +231,48 @@ def test_gpu_linear_operations_against_golden(dev, kind):
     run("addid", f["z"], lambda o: rpp_hip.tensor_add_identity(o, B, scalar=0.5))
 
 
+def c3_worst_signatures(f):
+    """The reference's signatures of the fixture's paths, rebuilt by the
+    oracle's chain (bit for bit: their SHA-256 is the fixture's)."""
+    import hashlib
+    inc = f["increments"]
+    n, L, w = inc.shape
+    sig = O.signature(w, 8, inc.reshape(-1, w), np.arange(n + 1, dtype=np.int64) * L)
+    assert hashlib.sha256(np.ascontiguousarray(sig).tobytes()).digest() == f["signature_sha256"].tobytes(), \
+        "the oracle's chain no longer reproduces the reference's signatures"
+    return sig
+
+
+def test_oracle_log_c3_worst():
+    """The 32 worst-conditioned log-signatures of the width 3, depth 8
+    workload (|log S| up to 30): the oracle's ft_log equals the reference's bit
+    for bit."""
+    f = fixture("log_c3_worst_w3d8")
+    sig = c3_worst_signatures(f)
+    got = np.zeros_like(sig)
+    O.orc.ft_log(3, 8, got, sig)
+    assert np.abs(f["log"]).max() > 20
+    assert np.array_equal(got, f["log"]), f"oracle ft_log differs: {rel_err(got, f['log']):.3e}"
+
+
+@pytest.mark.gpu
+def test_gpu_log_c3_worst(dev):
+    """The HIP ft_log (the LDS-resident kernel at width 3, depth 8) on the 32
+    worst-conditioned reference signatures of the log-signature workload:
+    within the reference's 1e-12 relative tolerance, and bit for bit."""
+    import torch
+
+    import rpp_hip
+    f = fixture("log_c3_worst_w3d8")
+    sig = c3_worst_signatures(f)
+    out = torch.full(sig.shape, float("nan"), dtype=torch.float64, device=dev)
+    rpp_hip.ft_log(out, torch.tensor(sig, device=dev), rpp_hip.TensorBasis(3, 8))
+    got = out.cpu().numpy()
+    err = rel_err(got, f["log"])
+    assert err <= TOL, f"ft_log on the worst C3 paths: {err:.3e} > {TOL:.0e}"
+    assert np.array_equal(got, f["log"]), f"ft_log on the worst C3 paths not bit-identical ({err:.3e})"
+
+
 @pytest.mark.gpu
 def test_gpu_against_golden(dev):
     import torch
@@ -250,23 +292,23 @@ def test_gpu_against_golden(dev):
                            rpp_hip.Batch(t(f["rhs"]), *kw["rw"]), basis, beta=kw["beta"])
         else:
             rpp_hip.ft_mul(o, t(f["lhs"]), t(f["rhs"]), basis)
-        assert_close(o.cpu().numpy(), f["out"], np.float64, name)
+        assert_exact(o.cpu().numpy(), f["out"], np.float64, name)
     b34 = rpp_hip.TensorBasis(3, 4)
     f = fixture("fma_w3d4")
     o = t(np.zeros_like(f["out"]))
     rpp_hip.ft_fma(o, t(f["a"]), t(f["b"]), t(f["c"]), b34, alpha=0.75, beta=1.25)
-    assert_close(o.cpu().numpy(), f["out"], np.float64, "fma")
+    assert_exact(o.cpu().numpy(), f["out"], np.float64, "fma")
     for variant in (231, 123, 213):
         g = fixture(f"inplace_fma{variant}_w3d4")
         x = t(g["a"])
         rpp_hip.ft_inplace_fma(variant, x, t(g["b"]), t(g["c"]), b34, alpha=0.5, beta=2.0)
-        assert_close(x.cpu().numpy(), g["out"], np.float64, f"inplace_fma {variant}")
+        assert_exact(x.cpu().numpy(), g["out"], np.float64, f"inplace_fma {variant}")
     for name, op in (("exp", rpp_hip.ft_exp), ("log", rpp_hip.ft_log), ("antipode", rpp_hip.tensor_antipode),
                      ("reflect", rpp_hip.tensor_reflect)):
         g = fixture(f"{name}_w3d4")
         o = t(np.zeros_like(g["out"]))
         op(o, t(g["arg"]), b34)
-        assert_close(o.cpu().numpy(), g["out"], np.float64, name)
+        assert_exact(o.cpu().numpy(), g["out"], np.float64, name)
     for side in ("lmul", "rmul"):
         g = fixture(f"adj_{side}_w3d4")
         o = t(np.zeros_like(g["out"]))

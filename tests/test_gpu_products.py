@@ -6,7 +6,7 @@ import torch
 
 import oracle as O
 import rpp_hip
-from helpers import (FT_FMA_CASES, FT_INPLACE_CASES, FT_MUL_CASES, SHAPES, assert_close, damped_batch,
+from helpers import (FT_FMA_CASES, FT_INPLACE_CASES, FT_MUL_CASES, SHAPES, assert_exact, damped_batch,
                      random_batch, window_mask)
 
 pytestmark = pytest.mark.gpu
@@ -32,7 +32,7 @@ def test_ft_mul_shapes(dev, shape, dtype):
     O.orc.ft_mul(w, N, expected, lhs, rhs, beta=0.75)
     out = to_dev(np.full((n, dim), np.nan, dtype=dtype), dev)
     rpp_hip.ft_mul(out, to_dev(lhs, dev), to_dev(rhs, dev), basis, beta=0.75)
-    assert_close(out.cpu().numpy(), expected, dtype, f"ft_mul {shape}")
+    assert_exact(out.cpu().numpy(), expected, dtype, f"ft_mul {shape}")
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
@@ -57,7 +57,7 @@ def test_ft_mul_windows(dev, case, width, dtype):
     got = out.cpu().numpy()
     m = window_mask(width, N, ow)
     assert np.all(np.isnan(got[:, ~m])), "wrote outside out's window"
-    assert_close(got[:, m], expected[:, m], dtype, name)
+    assert_exact(got[:, m], expected[:, m], dtype, name)
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
@@ -80,7 +80,7 @@ def test_ft_fma_windows(dev, case, dtype):
     got = out.cpu().numpy()
     m = window_mask(w, N, ow)
     assert np.all(np.isnan(got[:, ~m]))
-    assert_close(got[:, m], expected[:, m], dtype, name)
+    assert_exact(got[:, m], expected[:, m], dtype, name)
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
@@ -99,7 +99,7 @@ def test_ft_inplace_fma_windows(dev, case, variant, dtype):
     ad = to_dev(a, dev)
     rpp_hip.ft_inplace_fma(variant, rpp_hip.Batch(ad, *aw), rpp_hip.Batch(to_dev(b, dev), *bw),
                            rpp_hip.Batch(to_dev(c, dev), *cw), basis, alpha=1.5, beta=0.5)
-    assert_close(ad.cpu().numpy(), expected, dtype, f"{variant} {name}")
+    assert_exact(ad.cpu().numpy(), expected, dtype, f"{variant} {name}")
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
@@ -116,7 +116,7 @@ def test_ft_inplace_mul_windows(dev, case, dtype):
     O.orc.ft_inplace_mul(w, N, expected, rhs, lw=lw, rw=rw, beta=2.0)
     ld = to_dev(lhs, dev)
     rpp_hip.ft_inplace_mul(rpp_hip.Batch(ld, *lw), rpp_hip.Batch(to_dev(rhs, dev), *rw), basis, beta=2.0)
-    assert_close(ld.cpu().numpy(), expected, dtype, name)
+    assert_exact(ld.cpu().numpy(), expected, dtype, name)
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
@@ -134,7 +134,7 @@ def test_ft_mul_aliasing(dev, which, dtype):
     ld, rd = to_dev(lhs, dev), to_dev(rhs, dev)
     out = ld if which == "lhs" else rd
     rpp_hip.ft_mul(out, ld, rd, basis)
-    assert_close(out.cpu().numpy(), expected, dtype, f"alias {which}")
+    assert_exact(out.cpu().numpy(), expected, dtype, f"alias {which}")
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
@@ -153,7 +153,7 @@ def test_ft_mul_layouts(dev, dtype):
     out = torch.zeros((dim, n), dtype=getattr(torch, np.dtype(dtype).name), device=dev)
     rpp_hip.ft_mul(rpp_hip.interleaved(out, basis), rpp_hip.interleaved(to_dev(lhs.T, dev), basis),
                    to_dev(rhs1, dev), basis)
-    assert_close(out.cpu().numpy().T, expected, dtype, "interleaved + broadcast")
+    assert_exact(out.cpu().numpy().T, expected, dtype, "interleaved + broadcast")
 
 
 def test_c1_shape_ft_mul(dev):
@@ -166,7 +166,7 @@ def test_c1_shape_ft_mul(dev):
     O.orc.ft_mul(4, 4, expected, lhs, rhs)
     out = torch.empty((1024, 341), dtype=torch.float64, device=dev)
     rpp_hip.ft_mul(out, to_dev(lhs, dev), to_dev(rhs, dev), basis)
-    assert_close(out.cpu().numpy(), expected, np.float64, "C1")
+    assert_exact(out.cpu().numpy(), expected, np.float64, "C1")
 
 
 def test_invalid_arguments_raise(dev):
