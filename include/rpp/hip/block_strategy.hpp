@@ -399,36 +399,38 @@ public:
 };
 
 // tensor_to_lie / lie_to_tensor (operations/basic/tensor_to_lie.hpp:65-103,
-// lie_to_tensor.hpp:70-108): the caller's matrix through the SpMV above.
+// lie_to_tensor.hpp:70-108) with the caller's matrix, CSR or CSC: the
+// library recognises the Hall T2L / L2T matrices of the basis (the output of
+// the reference's make_t2l_matrix / make_l2t_matrix) and serves them with its
+// Lie kernel; any other matrix takes the deterministic sparse product.  Both
+// sum every row in the reference's order.
 template <typename Accum_, typename Arch_, T2LImplementationType Impl>
 class TensorToLie<hip::strategies::BlockStrategy<Accum_, Arch_>, Impl>
     : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
-    using SpMV = SparseMatrixVectorProduct<
-        hip::strategies::BlockStrategy<Accum_, Arch_>,
-        Impl == T2LImplementationType::CSRSparseMatrix ? sparse::MatrixFormat::CSR : sparse::MatrixFormat::CSC>;
-
 public:
     RPP_HIP_OP_COMMON;
     template <typename Batches, typename Bases, typename Matrix>
     static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases, int64_t n,
                                          Matrix const& matrix) noexcept {
-        return SpMV::hip_launch(cfg, b, bases, n, matrix);
+        using namespace hip::detail;
+        rpp_hip_lie_matrix_convert_desc d{tensor_basis_of(bases), n, tensor_of<Accum>(std::get<1>(b)),
+                                          tensor_of<Accum>(std::get<0>(b)), hip_detail::matrix_of<Accum>(matrix)};
+        return to_error(RPP_HIP_CALL(rpp_hip_tensor_to_lie_matrix, d, cfg.stream));
     }
 };
 
 template <typename Accum_, typename Arch_, L2TImplementationType Impl>
 class LieToTensor<hip::strategies::BlockStrategy<Accum_, Arch_>, Impl>
     : public BaseOperation<hip::strategies::BlockStrategy<Accum_, Arch_>> {
-    using SpMV = SparseMatrixVectorProduct<
-        hip::strategies::BlockStrategy<Accum_, Arch_>,
-        Impl == L2TImplementationType::CSRSparseMatrix ? sparse::MatrixFormat::CSR : sparse::MatrixFormat::CSC>;
-
 public:
     RPP_HIP_OP_COMMON;
     template <typename Batches, typename Bases, typename Matrix>
     static Error<const char*> hip_launch(LaunchConfig cfg, Batches const& b, Bases const& bases, int64_t n,
                                          Matrix const& matrix) noexcept {
-        return SpMV::hip_launch(cfg, b, bases, n, matrix);
+        using namespace hip::detail;
+        rpp_hip_lie_matrix_convert_desc d{tensor_basis_of(bases), n, tensor_of<Accum>(std::get<0>(b)),
+                                          tensor_of<Accum>(std::get<1>(b)), hip_detail::matrix_of<Accum>(matrix)};
+        return to_error(RPP_HIP_CALL(rpp_hip_lie_to_tensor_matrix, d, cfg.stream));
     }
 };
 

@@ -379,7 +379,21 @@ typedef struct rpp_hip_vector_batch {
 } rpp_hip_vector_batch;
 
 /* rpp::ops::sparse_matrix_vector_product
- * (operations/linalg/sparse_matrix_vector.hpp:57-96): out = alpha * M arg. */
+ * (operations/linalg/sparse_matrix_vector.hpp:57-96): out = alpha * M arg,
+ * every row summed in the order the reference's cpu::single_thread product
+ * sums it, with no atomics, so results are deterministic and bit-identical to
+ * the reference:
+ *   CSR: entry order, out = alpha * sum (cpu/.../linalg/
+ *        sparse_matrix_vector.hpp:45-73);
+ *   CSC: the out view is zeroed and each row receives value * (alpha * arg)
+ *        in column order (:133-163).  A CSC matrix is read back once (one
+ *        host synchronisation on the stream), checked (offsets 0..nnz,
+ *        non-decreasing; row indices in range) and transposed stably into
+ *        device memory the library owns; that form is cached per (device,
+ *        dtype, the three array pointers, shape), so later calls issue kernel
+ *        launches only.  The library assumes the arrays of a cached matrix do
+ *        not change; after rewriting or freeing them call
+ *        rpp_hip_sparse_matrix_forget(values). */
 typedef struct rpp_hip_spmv_desc {
     int64_t num_batches;
     rpp_hip_vector_batch out, arg;
@@ -388,6 +402,13 @@ typedef struct rpp_hip_spmv_desc {
 } rpp_hip_spmv_desc;
 int rpp_hip_sparse_matrix_vector_product_f32(const rpp_hip_spmv_desc* desc, void* stream);
 int rpp_hip_sparse_matrix_vector_product_f64(const rpp_hip_spmv_desc* desc, void* stream);
+
+/* Drops what the library cached for every matrix whose values array is
+ * `values` (the CSR form of a CSC matrix, the Hall-matrix recognition of the
+ * *_matrix Lie conversions below), synchronising the devices that hold such
+ * a form before freeing it.  No reference counterpart: the reference
+ * re-reads its matrices on every call. */
+int rpp_hip_sparse_matrix_forget(const void* values);
 
 /* rpp::ops::tensor_to_lie (operations/basic/tensor_to_lie.hpp:65-103) and
  * rpp::ops::lie_to_tensor (operations/basic/lie_to_tensor.hpp:70-108) with the
@@ -404,6 +425,28 @@ int rpp_hip_tensor_to_lie_f32(const rpp_hip_lie_convert_desc* desc, void* stream
 int rpp_hip_tensor_to_lie_f64(const rpp_hip_lie_convert_desc* desc, void* stream);
 int rpp_hip_lie_to_tensor_f32(const rpp_hip_lie_convert_desc* desc, void* stream);
 int rpp_hip_lie_to_tensor_f64(const rpp_hip_lie_convert_desc* desc, void* stream);
+
+/* rpp::ops::tensor_to_lie (operations/basic/tensor_to_lie.hpp:71-78) and
+ * rpp::ops::lie_to_tensor (operations/basic/lie_to_tensor.hpp:76-83) with the
+ * caller's matrix, as the reference's free functions take it (typically the
+ * CSC output of make_t2l_matrix / make_l2t_matrix, sparse/t2l_matrix.hpp:57,
+ * sparse/l2t_matrix.hpp:58): out = M arg relative to the view begins.  When
+ * the windows are the reference's (tensor [0, depth], Lie [1, depth]) and M
+ * equals the library's Hall T2L (L2T) of the basis (decided once per matrix
+ * from one read-back, then cached), the library's Lie kernel serves the call;
+ * otherwise the sparse product above (the matrix must fit the views).  Both
+ * sum each row in the reference's order, so the result is the same bits. */
+typedef struct rpp_hip_lie_matrix_convert_desc {
+    rpp_hip_basis basis;
+    int64_t num_batches;
+    rpp_hip_tensor_batch tensor;
+    rpp_hip_tensor_batch lie;
+    rpp_hip_sparse_matrix matrix;
+} rpp_hip_lie_matrix_convert_desc;
+int rpp_hip_tensor_to_lie_matrix_f32(const rpp_hip_lie_matrix_convert_desc* desc, void* stream);
+int rpp_hip_tensor_to_lie_matrix_f64(const rpp_hip_lie_matrix_convert_desc* desc, void* stream);
+int rpp_hip_lie_to_tensor_matrix_f32(const rpp_hip_lie_matrix_convert_desc* desc, void* stream);
+int rpp_hip_lie_to_tensor_matrix_f64(const rpp_hip_lie_matrix_convert_desc* desc, void* stream);
 
 /* Host-side Hall basis metadata (basis/hall_basis.hpp:16-88,
  * sparse/l2t_matrix.hpp:58-114, sparse/t2l_matrix.hpp:57-142).  Sizes first:

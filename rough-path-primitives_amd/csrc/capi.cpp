@@ -15,6 +15,7 @@
 #include "ft_ops.hpp"
 #include "lie.hpp"
 #include "signature.hpp"
+#include "sparse.hpp"
 #include "signature_reg.hpp"
 #include "signature_chain.hpp"
 #include "signature_thread.hpp"
@@ -273,7 +274,10 @@ int spmv_impl(const rpp_hip_spmv_desc* d, void* s) {
     if (M.rows < 0 || M.cols < 0 || M.nnz < 0) return fail(RPP_HIP_INVALID_ARGUMENT, "negative matrix shape");
     if (d->num_batches > 0 && (M.offsets == nullptr || (M.nnz > 0 && (!M.values || !M.indices))))
         return fail(RPP_HIP_INVALID_ARGUMENT, "null matrix arrays");
-    return launch_spmv<T>(d->num_batches, d->out, d->arg, M, d->alpha, as_stream(s));
+    if (d->num_batches == 0) return 0;
+    PreparedMatrix P;
+    RPP_TRY(prepare_matrix<T>(M, &P, as_stream(s)));
+    return launch_spmv<T>(d->num_batches, d->out, d->arg, P.csr, d->alpha, as_stream(s), 0, P.csc);
 }
 
 // tensor_to_lie / lie_to_tensor with the library's matrices.  Both are block
@@ -334,6 +338,63 @@ int lie_convert_impl(bool to_lie, const rpp_hip_lie_convert_desc* d, void* s) {
         return launch_spmv<T>(n, ov, av, M, 1.0, stream, col0);
     };
     return convert(dlo, dhi);
+}
+
+// tensor_to_lie / lie_to_tensor with the caller's matrix, as the reference's
+// operations take it: the view-relative product out = M arg
+// (operations/basic/tensor_to_lie.hpp:53-60, lie_to_tensor.hpp:57-65).  With
+// the reference's windows (tensor [0, depth], Lie [1, depth]) and a matrix
+// equal to the library's Hall T2L (L2T) of the basis, row for row in column
+// order, the product is the library's Lie conversion, which sums each row in
+// that order: the same bits.  Any other matrix or window pair takes the
+// deterministic sparse product.
+template <typename T>
+int lie_matrix_convert_impl(bool to_lie, const rpp_hip_lie_matrix_convert_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    RPP_TRY(check_basis(d->basis));
+    if (d->basis.width < 2) return fail(RPP_HIP_NOT_IMPLEMENTED, "Lie conversions need width >= 2");
+    RPP_TRY(check_count(d->num_batches));
+    RPP_TRY(check_view(d->tensor, d->basis, d->num_batches, "tensor", !to_lie));
+    if (d->lie.min_degree < 1 || d->lie.max_degree > d->basis.depth)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "lie: window must lie in [1, depth]");
+    RPP_TRY(check_view(d->lie, d->basis, d->num_batches, "lie", to_lie));
+    const auto& M = d->matrix;
+    if (M.format != RPP_HIP_CSR && M.format != RPP_HIP_CSC)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "matrix format must be CSR or CSC");
+    if (M.rows < 0 || M.cols < 0 || M.nnz < 0) return fail(RPP_HIP_INVALID_ARGUMENT, "negative matrix shape");
+    if (d->num_batches > 0 && (M.offsets == nullptr || (M.nnz > 0 && (!M.values || !M.indices))))
+        return fail(RPP_HIP_INVALID_ARGUMENT, "null matrix arrays");
+    if (d->num_batches == 0) return 0;
+    const hipStream_t stream = as_stream(s);
+    const int depth = d->basis.depth;
+    if (d->tensor.min_degree == 0 && d->tensor.max_degree == depth && d->lie.min_degree == 1 &&
+        d->lie.max_degree == depth) {
+        bool hall = false;
+        RPP_TRY(matrix_is_hall<T>(M, d->basis.width, depth, to_lie, &hall, stream));
+        if (hall) {
+            const rpp_hip_lie_convert_desc c{d->basis, d->num_batches, d->tensor, d->lie};
+            return lie_convert_impl<T>(to_lie, &c, s);
+        }
+    }
+    // view-relative vectors: the tensor and Lie windows as index ranges
+    auto md = lie_metadata(d->basis.width, depth);
+    const DevBasis B = make_basis(d->basis.width, depth);
+    auto tensor_vec = [&](const rpp_hip_tensor_batch& t) {
+        return rpp_hip_vector_batch{t.data, t.batch_stride, t.coeff_stride, B.start[t.min_degree],
+                                    B.start[t.max_degree + 1] - B.start[t.min_degree]};
+    };
+    auto lie_vec = [&](const rpp_hip_tensor_batch& l) {
+        const auto& lb = md->degree_begin;
+        return rpp_hip_vector_batch{l.data, l.batch_stride, l.coeff_stride, lb[static_cast<size_t>(l.min_degree)],
+                                    lb[static_cast<size_t>(l.max_degree) + 1] - lb[static_cast<size_t>(l.min_degree)]};
+    };
+    const rpp_hip_vector_batch out = to_lie ? lie_vec(d->lie) : tensor_vec(d->tensor);
+    const rpp_hip_vector_batch arg = to_lie ? tensor_vec(d->tensor) : lie_vec(d->lie);
+    if (M.rows > out.size || M.cols > arg.size)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "matrix larger than the views it maps between");
+    PreparedMatrix P;
+    RPP_TRY(prepare_matrix<T>(M, &P, stream));
+    return launch_spmv<T>(d->num_batches, out, arg, P.csr, 1.0, stream, 0, P.csc);
 }
 
 // ------------------------------------------------------- the shuffle family
@@ -603,6 +664,22 @@ int rpp_hip_lie_to_tensor_f32(const rpp_hip_lie_convert_desc* d, void* s) {
 }
 int rpp_hip_lie_to_tensor_f64(const rpp_hip_lie_convert_desc* d, void* s) {
     return lie_convert_impl<double>(false, d, s);
+}
+int rpp_hip_tensor_to_lie_matrix_f32(const rpp_hip_lie_matrix_convert_desc* d, void* s) {
+    return lie_matrix_convert_impl<float>(true, d, s);
+}
+int rpp_hip_tensor_to_lie_matrix_f64(const rpp_hip_lie_matrix_convert_desc* d, void* s) {
+    return lie_matrix_convert_impl<double>(true, d, s);
+}
+int rpp_hip_lie_to_tensor_matrix_f32(const rpp_hip_lie_matrix_convert_desc* d, void* s) {
+    return lie_matrix_convert_impl<float>(false, d, s);
+}
+int rpp_hip_lie_to_tensor_matrix_f64(const rpp_hip_lie_matrix_convert_desc* d, void* s) {
+    return lie_matrix_convert_impl<double>(false, d, s);
+}
+int rpp_hip_sparse_matrix_forget(const void* values) {
+    if (!values) return fail(RPP_HIP_INVALID_ARGUMENT, "null values pointer");
+    return forget_matrix(values);
 }
 
 int rpp_hip_lie_basis_sizes(int32_t width, int32_t depth, int64_t* sizes) {

@@ -490,26 +490,29 @@ k_antipode(DevBasis B, View<T> out, View<const T> arg, int64_t n) {
 // coalesced loads, then each thread sums whole rows in entry order with
 // contraction off, as the reference's loop does, so the results agree bit
 // for bit.  The sparse matrix itself is shared by every object and stays in
-// L2.
+// L2.  scale_arg: the reference's CSC product, whose scalar multiplies each
+// argument value (rounded) before the products instead of the row sum
+// (cpu/.../linalg/sparse_matrix_vector.hpp:150-158).
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_spmv_csr_lds(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t abs_, int64_t abeg,
                int64_t count, const T* values, const int64_t* indices, const int64_t* offsets, int64_t rows,
-               T alpha, int64_t n, int64_t cshift) {
+               T alpha, int scale_arg, int64_t n, int64_t cshift) {
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) unsigned char spmv_raw[];
     T* xs = reinterpret_cast<T*>(spmv_raw);
     const int t = threadIdx.x;
+    const T pre = scale_arg ? alpha : T(1);
     for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
         const T* a = arg + b * abs_ + abeg;
-        for (int64_t i = t; i < count; i += kBlock) xs[i] = a[i];
+        for (int64_t i = t; i < count; i += kBlock) xs[i] = scale_arg ? pre * a[i] : a[i];
         __syncthreads();
         T* o = out + b * obs;
         for (int64_t row = t; row < rows; row += kBlock) {
             const int64_t e0 = offsets[row], e1 = offsets[row + 1];
             T acc = T(0);
             for (int64_t e = e0; e < e1; ++e) acc += values[e] * xs[indices[e] - cshift];
-            o[(obeg + row) * ocs] = alpha * acc;
+            o[(obeg + row) * ocs] = scale_arg ? acc : alpha * acc;
         }
         __syncthreads();
     }
@@ -517,12 +520,13 @@ k_spmv_csr_lds(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int
 
 // CSR rows (cpu/.../linalg/sparse_matrix_vector.hpp:45-73): one thread per
 // row, grid.y over objects.  The row sum runs in entry order with contraction
-// off, as the reference's does, so the results agree bit for bit.
+// off, as the reference's does, so the results agree bit for bit; scale_arg
+// as for k_spmv_csr_lds.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_spmv_csr(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t abs_,
            int64_t acs, int64_t abeg, const T* values, const int64_t* indices,
-           const int64_t* offsets, int64_t rows, T alpha, int64_t n, int64_t cshift) {
+           const int64_t* offsets, int64_t rows, T alpha, int scale_arg, int64_t n, int64_t cshift) {
 #pragma clang fp contract(off)
     const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (row >= rows) return;
@@ -530,8 +534,13 @@ k_spmv_csr(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t
     for (int64_t b = blockIdx.y; b < n; b += gridDim.y) {
         const T* a = arg + b * abs_;
         T acc = T(0);
-        for (int64_t e = e0; e < e1; ++e) acc += values[e] * a[(abeg + indices[e] - cshift) * acs];
-        out[b * obs + (obeg + row) * ocs] = alpha * acc;
+        if (scale_arg) {
+            for (int64_t e = e0; e < e1; ++e) acc += values[e] * (alpha * a[(abeg + indices[e] - cshift) * acs]);
+            out[b * obs + (obeg + row) * ocs] = acc;
+        } else {
+            for (int64_t e = e0; e < e1; ++e) acc += values[e] * a[(abeg + indices[e] - cshift) * acs];
+            out[b * obs + (obeg + row) * ocs] = alpha * acc;
+        }
     }
 }
 
@@ -541,23 +550,6 @@ k_vec_zero(T* out, int64_t obs, int64_t ocs, int64_t obeg, int64_t size, int64_t
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= size) return;
     for (int64_t b = blockIdx.y; b < n; b += gridDim.y) out[b * obs + (obeg + i) * ocs] = T(0);
-}
-
-// CSC columns (sparse_matrix_vector.hpp:133-163) as an atomic scatter after
-// the zero fill; the summation order of a row is therefore not fixed.
-template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_spmv_csc(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t abs_,
-           int64_t acs, int64_t abeg, const T* values, const int64_t* indices,
-           const int64_t* offsets, int64_t cols, T alpha, int64_t n) {
-    const int64_t col = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (col >= cols) return;
-    const int64_t e0 = offsets[col], e1 = offsets[col + 1];
-    for (int64_t b = blockIdx.y; b < n; b += gridDim.y) {
-        const T av = alpha * arg[b * abs_ + (abeg + col) * acs];
-        T* o = out + b * obs;
-        for (int64_t e = e0; e < e1; ++e) atomicAdd(&o[(obeg + indices[e]) * ocs], values[e] * av);
-    }
 }
 
 bool all_contig(std::initializer_list<const rpp_hip_tensor_batch*> ts) {
@@ -720,48 +712,43 @@ int launch_antipode(bool sign_by_degree, const DevBasis& B, int64_t n, const rpp
 
 template <typename T>
 int launch_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg,
-                const rpp_hip_sparse_matrix& M, double alpha, hipStream_t stream, int64_t col_shift) {
+                const rpp_hip_sparse_matrix& M, double alpha, hipStream_t stream, int64_t col_shift, bool csc) {
     if (n <= 0) return 0;
+    if (M.format != RPP_HIP_CSR) return fail(RPP_HIP_INTERNAL, "launch_spmv takes CSR matrices");
     const unsigned gy = static_cast<unsigned>(std::min<int64_t>(n, 65535));
     T* o = static_cast<T*>(out.data);
     const T* a = static_cast<const T*>(arg.data);
     const T* v = static_cast<const T*>(M.values);
-    if (M.format == RPP_HIP_CSR) {
-        if (M.rows == 0) return 0;
-        const int64_t count = std::min<int64_t>(M.cols, arg.size);
-        const size_t lds = static_cast<size_t>(std::max<int64_t>(count, 1)) * sizeof(T);
-        constexpr size_t kSpmvLdsMax = 128 * 1024;
-        if (arg.coeff_stride == 1 && M.cols <= arg.size && lds <= kSpmvLdsMax && M.rows >= 64) {
-            static const bool raised = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spmv_csr_lds<T>),
-                                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                           static_cast<int>(kSpmvLdsMax)) == hipSuccess;
-            if (raised || lds <= 64 * 1024) {
-                const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(8, (160 * 1024) / lds));
-                const unsigned grid = static_cast<unsigned>(std::min<int64_t>(n, device_cu_count() * per_cu));
-                k_spmv_csr_lds<T><<<grid, kBlock, lds, stream>>>(o, out.batch_stride, out.coeff_stride, out.begin, a,
-                                                               arg.batch_stride, arg.begin, count, v, M.indices,
-                                                               M.offsets, M.rows, T(alpha), n, col_shift);
-                return check_hip(hipGetLastError(), "k_spmv_csr_lds launch");
-            }
+    const int scale_arg = csc ? 1 : 0;
+    // CSC semantics zero the whole out view first: rows past the matrix's
+    if (csc && out.size > M.rows) {
+        dim3 zg(static_cast<unsigned>((out.size - M.rows + kBlock - 1) / kBlock), gy);
+        k_vec_zero<T><<<zg, kBlock, 0, stream>>>(o, out.batch_stride, out.coeff_stride, out.begin + M.rows,
+                                                 out.size - M.rows, n);
+        if (int rc = check_hip(hipGetLastError(), "k_vec_zero launch")) return rc;
+    }
+    if (M.rows == 0) return 0;
+    const int64_t count = std::min<int64_t>(M.cols, arg.size);
+    const size_t lds = static_cast<size_t>(std::max<int64_t>(count, 1)) * sizeof(T);
+    constexpr size_t kSpmvLdsMax = 128 * 1024;
+    if (arg.coeff_stride == 1 && M.cols <= arg.size && lds <= kSpmvLdsMax && M.rows >= 64) {
+        static const bool raised = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spmv_csr_lds<T>),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       static_cast<int>(kSpmvLdsMax)) == hipSuccess;
+        if (raised || lds <= 64 * 1024) {
+            const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(8, (160 * 1024) / lds));
+            const unsigned grid = static_cast<unsigned>(std::min<int64_t>(n, device_cu_count() * per_cu));
+            k_spmv_csr_lds<T><<<grid, kBlock, lds, stream>>>(o, out.batch_stride, out.coeff_stride, out.begin, a,
+                                                           arg.batch_stride, arg.begin, count, v, M.indices,
+                                                           M.offsets, M.rows, T(alpha), scale_arg, n, col_shift);
+            return check_hip(hipGetLastError(), "k_spmv_csr_lds launch");
         }
-        dim3 grid(static_cast<unsigned>((M.rows + kBlock - 1) / kBlock), gy);
-        k_spmv_csr<T><<<grid, kBlock, 0, stream>>>(o, out.batch_stride, out.coeff_stride, out.begin, a,
-                                                   arg.batch_stride, arg.coeff_stride, arg.begin, v,
-                                                   M.indices, M.offsets, M.rows, T(alpha), n, col_shift);
-        return check_hip(hipGetLastError(), "k_spmv_csr launch");
     }
-    if (out.size > 0) {
-        dim3 zg(static_cast<unsigned>((out.size + kBlock - 1) / kBlock), gy);
-        k_vec_zero<T><<<zg, kBlock, 0, stream>>>(o, out.batch_stride, out.coeff_stride, out.begin,
-                                                 out.size, n);
-    }
-    if (M.cols > 0) {
-        dim3 grid(static_cast<unsigned>((M.cols + kBlock - 1) / kBlock), gy);
-        k_spmv_csc<T><<<grid, kBlock, 0, stream>>>(o, out.batch_stride, out.coeff_stride, out.begin, a,
-                                                   arg.batch_stride, arg.coeff_stride, arg.begin, v,
-                                                   M.indices, M.offsets, M.cols, T(alpha), n);
-    }
-    return check_hip(hipGetLastError(), "k_spmv_csc launch");
+    dim3 grid(static_cast<unsigned>((M.rows + kBlock - 1) / kBlock), gy);
+    k_spmv_csr<T><<<grid, kBlock, 0, stream>>>(o, out.batch_stride, out.coeff_stride, out.begin, a, arg.batch_stride,
+                                               arg.coeff_stride, arg.begin, v, M.indices, M.offsets, M.rows,
+                                               T(alpha), scale_arg, n, col_shift);
+    return check_hip(hipGetLastError(), "k_spmv_csr launch");
 }
 
 #define RPP_INSTANTIATE(T)                                                                      \
@@ -780,7 +767,7 @@ int launch_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector
     template int launch_antipode<T>(bool, const DevBasis&, int64_t, const rpp_hip_tensor_batch&,  \
                                     const rpp_hip_tensor_batch&, hipStream_t);                     \
     template int launch_spmv<T>(int64_t, const rpp_hip_vector_batch&, const rpp_hip_vector_batch&, \
-                                const rpp_hip_sparse_matrix&, double, hipStream_t, int64_t);
+                                const rpp_hip_sparse_matrix&, double, hipStream_t, int64_t, bool);
 
 RPP_INSTANTIATE(float)
 RPP_INSTANTIATE(double)

@@ -38,12 +38,17 @@ template <typename T>
 int launch_log_adjoint(const DevBasis& B, int64_t n, const rpp_hip_tensor_batch& grad_arg,
                        const rpp_hip_tensor_batch& arg, const rpp_hip_tensor_batch& cotangent, hipStream_t stream);
 
-// out = alpha M arg with view-relative indexing (row 0 <-> out.begin, column
-// c <-> arg.begin + c - col_shift).  A CSR matrix whose entries only name
-// columns [col_shift, col_shift + cols) can so be applied to an argument
-// segment that starts at that column; the public operation uses col_shift 0.
+// out = alpha M arg for a CSR matrix with view-relative indexing (row 0 <->
+// out.begin, column c <-> arg.begin + c - col_shift), each row summed in
+// entry order.  A matrix whose entries only name columns [col_shift,
+// col_shift + cols) can so be applied to an argument segment that starts at
+// that column; the public operation uses col_shift 0.  csc: the semantics of
+// the reference's CSC product for a CSC matrix prepared as CSR
+// (PreparedMatrix): alpha scales each argument value, and out rows past the
+// matrix's are zeroed.
 template <typename T>
 int launch_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg,
-                const rpp_hip_sparse_matrix& M, double alpha, hipStream_t stream, int64_t col_shift = 0);
+                const rpp_hip_sparse_matrix& M, double alpha, hipStream_t stream, int64_t col_shift = 0,
+                bool csc = false);
 
 }  // namespace rpp_hip

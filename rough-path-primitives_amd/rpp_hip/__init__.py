@@ -4,7 +4,7 @@ from ._capi import LIB_PATH, exported_symbols, lib
 from .ops import (Batch, LieBasis, RppHipError, SparseMatrix, TensorBasis, fmexp_tile, ft_adj_lmul, ft_adj_rmul,
                   ft_exp, ft_fma, ft_fmexp, ft_fmexp_adjoint, ft_inplace_fma, ft_inplace_mul, ft_log, ft_log_adjoint,
                   ft_mul, interleaved, lie_basis, lie_basis_load, lie_basis_save, lie_to_tensor, make_tensor_batch, path_order, signature,
-                  signature_backward, signature_tiles, sparse_matrix_vector_product, st_adj_mul, st_fma,
+                  signature_backward, signature_tiles, sparse_matrix_forget, sparse_matrix_vector_product, st_adj_mul, st_fma,
                   st_inplace_fma, st_mul, left_hs_adj_lmul, left_hs_adj_rmul, left_hs_mul, tensor_add_identity,
                   tensor_antipode, tensor_dimension, tensor_pairing, tensor_reflect, tensor_set_identity,
                   tensor_to_lie, to_interleaved, to_object_major, vector_add, vector_assign, vector_inplace_add,
@@ -17,7 +17,7 @@ __all__ = [
     "ft_inplace_mul", "ft_log", "ft_log_adjoint", "ft_mul", "interleaved", "lie_basis", "lie_basis_load",
     "lie_basis_save", "lie_to_tensor",
     "make_tensor_batch", "path_order", "signature", "signature_backward", "signature_tiles",
-    "sparse_matrix_vector_product", "st_adj_mul", "st_fma", "st_inplace_fma", "st_mul", "left_hs_adj_lmul",
+    "sparse_matrix_forget", "sparse_matrix_vector_product", "st_adj_mul", "st_fma", "st_inplace_fma", "st_mul", "left_hs_adj_lmul",
     "left_hs_adj_rmul", "left_hs_mul", "tensor_add_identity", "tensor_antipode", "tensor_dimension", "tensor_pairing",
     "tensor_reflect", "tensor_set_identity", "tensor_to_lie", "to_interleaved", "to_object_major", "vector_add",
     "vector_assign", "vector_inplace_add", "vector_scalar_multiply", "vector_set_constant", "LIB_PATH",

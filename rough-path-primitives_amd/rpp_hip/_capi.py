@@ -99,6 +99,11 @@ class LieConvertDesc(C.Structure):
     _fields_ = [("basis", Basis), ("num_batches", _i64), ("tensor", TensorBatch), ("lie", TensorBatch)]
 
 
+class LieMatrixConvertDesc(C.Structure):
+    _fields_ = [("basis", Basis), ("num_batches", _i64), ("tensor", TensorBatch), ("lie", TensorBatch),
+                ("matrix", SparseMatrix)]
+
+
 class StInplaceFmaDesc(C.Structure):
     _fields_ = [("basis", Basis), ("num_batches", _i64), ("a", TensorBatch), ("b", TensorBatch),
                 ("c", TensorBatch), ("alpha", _f64), ("beta", _f64)]
@@ -169,6 +174,8 @@ ENTRY_POINTS = {
     "rpp_hip_sparse_matrix_vector_product": SpmvDesc,
     "rpp_hip_tensor_to_lie": LieConvertDesc,
     "rpp_hip_lie_to_tensor": LieConvertDesc,
+    "rpp_hip_tensor_to_lie_matrix": LieMatrixConvertDesc,
+    "rpp_hip_lie_to_tensor_matrix": LieMatrixConvertDesc,
     "rpp_hip_st_mul": FtMulDesc,
     "rpp_hip_st_fma": FtFmaDesc,
     "rpp_hip_st_inplace_fma": StInplaceFmaDesc,
@@ -190,7 +197,7 @@ ENTRY_POINTS = {
 OTHER_SYMBOLS = ["rpp_hip_lie_basis_sizes", "rpp_hip_lie_basis_copy", "rpp_hip_lie_basis_save",
                  "rpp_hip_lie_basis_load", "rpp_hip_last_error",
                  "rpp_hip_version", "rpp_hip_signature_backward_scratch_bytes", "rpp_hip_signature_tiles",
-                 "rpp_hip_fmexp_tile"]
+                 "rpp_hip_fmexp_tile", "rpp_hip_sparse_matrix_forget"]
 
 
 def exported_symbols() -> list[str]:
@@ -224,6 +231,8 @@ def lib() -> C.CDLL:
         lib_.rpp_hip_lie_basis_load.argtypes = [C.c_char_p, _vp, _vp]
         lib_.rpp_hip_signature_backward_scratch_bytes.argtypes = [C.POINTER(SignatureBackwardDesc), _i32]
         lib_.rpp_hip_signature_backward_scratch_bytes.restype = _i64
+        lib_.rpp_hip_sparse_matrix_forget.argtypes = [_vp]
+        lib_.rpp_hip_sparse_matrix_forget.restype = C.c_int
         lib_.rpp_hip_signature_tiles.argtypes = [_i32, _vp, _i32]
         lib_.rpp_hip_signature_tiles.restype = C.c_int
         _lib = lib_

@@ -17,6 +17,7 @@ Errors raise :class:`RppHipError` carrying the rpp::ErrorCode value
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 from typing import Optional
 
@@ -447,15 +448,28 @@ def to_object_major(x: torch.Tensor, basis, stream=None) -> torch.Tensor:
 
 # ------------------------------------------------------------- sparse, Lie
 
+def _forget_values(ptr: int) -> None:
+    A.lib().rpp_hip_sparse_matrix_forget(ptr)
+
+
 @dataclass
 class SparseMatrix:
-    """CompressedMatrix with device arrays (sparse/detail/compressed_matrix.hpp:38-126)."""
+    """CompressedMatrix with device arrays (sparse/detail/compressed_matrix.hpp:38-126).
+
+    The library caches what it derives from a matrix (the CSR form of a CSC
+    matrix, whether it is a Hall T2L/L2T matrix) by its device arrays, which
+    must not change while this object is in use; the cache entry is dropped
+    when this object is collected, so its memory can be reused safely."""
     format: str  # "csr" or "csc"
     values: torch.Tensor
     indices: torch.Tensor  # int64
     offsets: torch.Tensor  # int64
     rows: int
     cols: int
+
+    def __post_init__(self):
+        fin = weakref.finalize(self, _forget_values, self.values.data_ptr())
+        fin.atexit = False
 
     def _c(self):
         return A.SparseMatrix(0 if self.format == "csr" else 1, self.values.data_ptr(),
@@ -481,21 +495,46 @@ def sparse_matrix_vector_product(out: torch.Tensor, arg: torch.Tensor, matrix: S
     _call("rpp_hip_sparse_matrix_vector_product" + _sfx(out, arg, matrix.values), d, stream)
 
 
-def tensor_to_lie(lie_out, tensor_in, basis: TensorBasis, num_batches=None, stream=None):
-    """rpp::ops::tensor_to_lie (operations/basic/tensor_to_lie.hpp:65-103) with the
-    library's Hall-basis T2L matrix.  ``lie_out`` holds true_size slots."""
+def sparse_matrix_forget(matrix: SparseMatrix) -> None:
+    """Drop what the library cached for this matrix (its CSR form when it is
+    CSC, its Hall-matrix recognition); call before rewriting or freeing its
+    arrays."""
+    rc = A.lib().rpp_hip_sparse_matrix_forget(matrix.values.data_ptr())
+    if rc:
+        raise RppHipError(rc, A.lib().rpp_hip_last_error().decode())
+
+
+def tensor_to_lie(lie_out, tensor_in, basis: TensorBasis, num_batches=None, stream=None,
+                  matrix: Optional[SparseMatrix] = None):
+    """rpp::ops::tensor_to_lie (operations/basic/tensor_to_lie.hpp:65-103).
+    Without ``matrix``: the library's Hall-basis T2L.  With ``matrix`` (CSR or
+    CSC device arrays, e.g. the reference's make_t2l_matrix output): that
+    matrix, view-relative, as the reference's operation takes it.
+    ``lie_out`` holds true_size slots."""
     lie_out = lie_out if isinstance(lie_out, Batch) else Batch(lie_out, 1, basis.depth)
     tensor_in = _as_batch(tensor_in, basis)
-    d = A.LieConvertDesc(basis._c(), _n(num_batches, tensor_in), tensor_in._c(), lie_out._c())
-    _call("rpp_hip_tensor_to_lie" + _sfx(lie_out, tensor_in), d, stream)
+    n = _n(num_batches, tensor_in)
+    if matrix is None:
+        d = A.LieConvertDesc(basis._c(), n, tensor_in._c(), lie_out._c())
+        _call("rpp_hip_tensor_to_lie" + _sfx(lie_out, tensor_in), d, stream)
+    else:
+        d = A.LieMatrixConvertDesc(basis._c(), n, tensor_in._c(), lie_out._c(), matrix._c())
+        _call("rpp_hip_tensor_to_lie_matrix" + _sfx(lie_out, tensor_in, matrix.values), d, stream)
 
 
-def lie_to_tensor(tensor_out, lie_in, basis: TensorBasis, num_batches=None, stream=None):
-    """rpp::ops::lie_to_tensor (operations/basic/lie_to_tensor.hpp:70-108)."""
+def lie_to_tensor(tensor_out, lie_in, basis: TensorBasis, num_batches=None, stream=None,
+                  matrix: Optional[SparseMatrix] = None):
+    """rpp::ops::lie_to_tensor (operations/basic/lie_to_tensor.hpp:70-108);
+    ``matrix`` as for tensor_to_lie (an L2T matrix)."""
     lie_in = lie_in if isinstance(lie_in, Batch) else Batch(lie_in, 1, basis.depth)
     tensor_out = _as_batch(tensor_out, basis)
-    d = A.LieConvertDesc(basis._c(), _n(num_batches, tensor_out), tensor_out._c(), lie_in._c())
-    _call("rpp_hip_lie_to_tensor" + _sfx(tensor_out, lie_in), d, stream)
+    n = _n(num_batches, tensor_out)
+    if matrix is None:
+        d = A.LieConvertDesc(basis._c(), n, tensor_out._c(), lie_in._c())
+        _call("rpp_hip_lie_to_tensor" + _sfx(tensor_out, lie_in), d, stream)
+    else:
+        d = A.LieMatrixConvertDesc(basis._c(), n, tensor_out._c(), lie_in._c(), matrix._c())
+        _call("rpp_hip_lie_to_tensor_matrix" + _sfx(tensor_out, lie_in, matrix.values), d, stream)
 
 
 def lie_basis(width: int, depth: int) -> dict:

@@ -353,6 +353,27 @@ void run(int width, int depth, Index n) {
                                           d_l2t.outer_dim(), d_l2t.inner_dim()));
         report(tag("tensor_to_lie (CSR)"), glie.host(), hlie);
         report(tag("lie_to_tensor (CSC)"), gten.host(), hten);
+        // the CSC T2L exactly as make_t2l_matrix returns it, through
+        // tensor_to_lie and through sparse_matrix_vector_product with alpha
+        auto [cv, ci, co] = upload(t2l);
+        rpp::sparse::GradedMatrixView<rpp::sparse::MatrixFormat::CSC, const T*, const Index*, const Index*> d_t2l_csc(
+            static_cast<const T*>(cv.p), static_cast<const Index*>(ci.p), static_cast<const Index*>(co.p), t2l.nnz(),
+            t2l.outer_dim(), t2l.inner_dim());
+        std::vector<T> hlie2(static_cast<size_t>(n * ls), T(0)), hlie3 = hlie2;
+        (void)rpp::ops::tensor_to_lie(cpu, {}, lie_b(hlie2.data()), hcb(rhs.data(), 0, D), lb, basis, n, t2l.view());
+        (void)rpp::ops::sparse_matrix_vector_product(cpu, {}, lie_b(hlie3.data()), hcb(rhs.data(), 0, D), lb, basis,
+                                                     n, t2l.view(), T(0.5));
+        Dev<T> glie2(std::vector<T>(static_cast<size_t>(n * ls), T(0))), glie3(glie2.host());
+        (void)rpp::ops::tensor_to_lie(gpu, {}, lie_b(glie2.p), hcb(r.p, 0, D), lb, basis, n, d_t2l_csc);
+        (void)rpp::ops::sparse_matrix_vector_product(gpu, {}, lie_b(glie3.p), hcb(r.p, 0, D), lb, basis, n,
+                                                     d_t2l_csc, T(0.5));
+        report(tag("tensor_to_lie (CSC)"), glie2.host(), hlie2);
+        report(tag("sparse_matrix_vector_product (CSC, alpha 0.5)"), glie3.host(), hlie3);
+        // the library caches what it derived from a matrix by its device
+        // arrays: forget them before they are freed
+        for (const void* v : {static_cast<const void*>(tv.p), static_cast<const void*>(lv.p),
+                              static_cast<const void*>(cv.p)})
+            (void)rpp_hip_sparse_matrix_forget(v);
     }
 }
 

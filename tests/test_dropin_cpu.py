@@ -1,5 +1,10 @@
-"""The hip::block strategy header compiles against the reference's headers
-(syntax check only; no GPU needed)."""
+"""The hip::block strategy header against the reference's own headers, on the
+CPU: the drop-in check (tests/dropin/dropin_check.cpp, which calls every
+reference free function with both strategies) compiles with hipcc AND links
+against librpp_hip.so, and every C-ABI symbol the drop-in layer calls is one
+the library exports.  The resulting binary is built from the reference's
+headers, so it is never run where the reference is absent; the C-ABI it
+drives is exercised on the GPU by the ctypes parity tests."""
 import os
 import subprocess
 
@@ -7,12 +12,33 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF = "/root/reference/include"
+LIBDIR = os.path.join(ROOT, "rough-path-primitives_amd", "rpp_hip")
+HIPCC = "/opt/rocm/bin/hipcc"
+
+
+def _symbols(path, defined):
+    flag = "--defined-only" if defined else "--undefined-only"
+    r = subprocess.run(["nm", "-D", flag, path], capture_output=True, text=True, check=True)
+    return {line.split()[-1] for line in r.stdout.splitlines() if line.split()}
 
 
 @pytest.mark.skipif(not os.path.isdir(REF), reason="reference headers not present on this machine")
-def test_strategy_header_compiles_against_reference():
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_dropin_check_compiles_and_links(tmp_path):
+    lib = os.path.join(LIBDIR, "librpp_hip.so")
+    assert os.path.exists(lib), "build librpp_hip.so first (__graft_entry__.build())"
     src = os.path.join(ROOT, "tests", "dropin", "dropin_check.cpp")
-    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I" + REF, "-I" + os.path.join(ROOT, "include"),
-                        "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", src],
-                       capture_output=True, text=True, timeout=300)
+    exe = str(tmp_path / "dropin_check")
+    r = subprocess.run([HIPCC, "-O2", "-std=c++17", "-I" + REF, "-I" + os.path.join(ROOT, "include"), "-o", exe,
+                        src, "-L" + LIBDIR, "-lrpp_hip", "-Wl,--no-undefined"],
+                       capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-4000:]
+    wanted = {s for s in _symbols(exe, defined=False) if s.startswith("rpp_hip_")}
+    exported = _symbols(lib, defined=True)
+    # the hot-path operations of the drop-in, both dtypes
+    for op in ("ft_mul", "ft_fma", "ft_inplace_fma", "ft_inplace_mul", "ft_exp", "ft_fmexp", "ft_log",
+               "tensor_pairing", "sparse_matrix_vector_product", "tensor_to_lie_matrix", "lie_to_tensor_matrix",
+               "ft_adj_lmul", "ft_adj_rmul"):
+        for sfx in ("f32", "f64"):
+            assert f"rpp_hip_{op}_{sfx}" in wanted, f"the drop-in layer does not call rpp_hip_{op}_{sfx}"
+    assert wanted <= exported, f"symbols the drop-in calls but the library lacks: {sorted(wanted - exported)}"

@@ -44,9 +44,106 @@ def test_spmv_view_relative(dev, fmt, dtype):
     got = out.cpu().numpy()
     assert np.all(got[:, :out_begin] == 7) and np.all(got[:, out_begin + rows:] == 7)
     assert_close(got, expected, dtype, f"spmv {fmt}")
-    if fmt == "csr":
-        # rows summed in entry order, contraction off: bit for bit
-        assert np.array_equal(got, expected)
+    # every row summed in the reference's order (CSR: entry order; CSC: column
+    # order, alpha on the argument), contraction off: bit for bit
+    assert np.array_equal(got, expected)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_spmv_csc_ragged_exact(dev, dtype):
+    """A CSC matrix with empty columns, empty rows, a row named twice in one
+    column and an out view longer than the matrix (zeroed past its rows, as the
+    reference's set_constant does): equal to the oracle's column scatter bit
+    for bit, on repeated calls (the cached CSR form) and after the cache is
+    dropped."""
+    rng = np.random.default_rng(23)
+    rows, cols, n = 40, 300, 9
+    offs, idx, vals = [0], [], []
+    for c in range(cols):
+        k = 0 if c % 7 == 3 else int(rng.integers(0, 4))
+        r = list(rng.integers(0, rows - 3, size=k))  # the last three rows stay empty
+        if c % 11 == 5 and r:
+            r.append(r[0])  # the same row twice in one column
+        idx += r
+        vals += list(rng.uniform(-1, 1, size=len(r)))
+        offs.append(len(idx))
+    idx, vals, offs = (np.array(a) for a in (idx, vals, offs))
+    arg = rng.uniform(-1, 1, size=(n, cols)).astype(dtype)
+    out_size = rows + 5
+    expected = np.full((n, out_size + 2), 7.0, dtype=dtype)
+    O.spmv(vals, idx, offs, cols, expected, arg, csc=True, out_offset=1, out_size=out_size, alpha=-1.25)
+    M = rpp_hip.SparseMatrix("csc", to_dev(vals.astype(dtype), dev), to_dev(idx.astype(np.int64), dev),
+                             to_dev(offs.astype(np.int64), dev), rows, cols)
+    arg_d = to_dev(arg, dev)
+    for rep in range(3):
+        if rep == 2:
+            rpp_hip.sparse_matrix_forget(M)
+        out = to_dev(np.full((n, out_size + 2), 7.0, dtype=dtype), dev)
+        rpp_hip.sparse_matrix_vector_product(out, arg_d, M, out_begin=1, out_size=out_size, alpha=-1.25)
+        assert np.array_equal(out.cpu().numpy(), expected), f"call {rep}"
+
+
+def _hall_matrix(lie, which, dtype, dev):
+    m = lie[which]
+    ts, dim = lie["true_size"], lie["tensor_dim"]
+    rows, cols = (ts - 1, dim) if which == "t2l" else (dim, ts - 1)
+    return rpp_hip.SparseMatrix("csc", to_dev(m["data"].astype(dtype), dev), to_dev(m["indices"], dev),
+                                to_dev(m["offsets"], dev), rows, cols)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_reference_csc_matrices_4099_objects(dev, dtype):
+    """The Hall T2L and L2T in the CSC format of the reference's builders
+    (byte for byte make_t2l_matrix / make_l2t_matrix, tests/test_lie_cpu.py)
+    at width 3, depth 8 over 4099 objects: sparse_matrix_vector_product
+    (alpha 1 and 0.5), and tensor_to_lie / lie_to_tensor given those matrices
+    (served by the Lie kernel once recognised), equal the oracle's CSC column
+    scatter bit for bit, and the library's own conversions; a matrix that
+    differs from the Hall one in one value takes the sparse product and still
+    equals the oracle."""
+    w, N, n = 3, 8, 4099
+    lie = rpp_hip.lie_basis(w, N)
+    ts, dim = lie["true_size"], lie["tensor_dim"]
+    basis = rpp_hip.TensorBasis(w, N)
+    rng = np.random.default_rng(4099)
+    x = damped_batch(rng, n, w, N, dtype)
+    x_d = to_dev(x, dev)
+    T2L, L2T = _hall_matrix(lie, "t2l", dtype, dev), _hall_matrix(lie, "l2t", dtype, dev)
+    t2l, l2t = lie["t2l"], lie["l2t"]
+    exp_lie = {}
+    for alpha in (1.0, 0.5):
+        e = np.full((n, ts), 7.0, dtype=dtype)
+        O.spmv(t2l["data"].astype(dtype), t2l["indices"], t2l["offsets"], dim, e, x, csc=True, out_offset=1,
+               out_size=ts - 1, alpha=alpha)
+        out = to_dev(np.full((n, ts), 7.0, dtype=dtype), dev)
+        rpp_hip.sparse_matrix_vector_product(out, x_d, T2L, out_begin=1, out_size=ts - 1, alpha=alpha)
+        assert np.array_equal(out.cpu().numpy(), e), f"spmv T2L alpha {alpha}"
+        exp_lie[alpha] = e
+    lie_d = to_dev(np.full((n, ts), 7.0, dtype=dtype), dev)
+    rpp_hip.tensor_to_lie(lie_d, x_d, basis, matrix=T2L)
+    got = lie_d.cpu().numpy()
+    assert np.all(got[:, 0] == 7)
+    assert np.array_equal(got[:, 1:], exp_lie[1.0][:, 1:]), "tensor_to_lie with the reference CSC T2L"
+    own = to_dev(np.zeros((n, ts), dtype=dtype), dev)
+    rpp_hip.tensor_to_lie(own, x_d, basis)
+    assert np.array_equal(own.cpu().numpy()[:, 1:], got[:, 1:]), "own T2L"
+    e = np.full((n, dim), np.nan, dtype=dtype)
+    O.spmv(l2t["data"].astype(dtype), l2t["indices"], l2t["offsets"], ts - 1, e, got, csc=True, out_offset=0,
+           out_size=dim, arg_offset=1)
+    t_d = to_dev(np.full((n, dim), np.nan, dtype=dtype), dev)
+    rpp_hip.lie_to_tensor(t_d, lie_d, basis, matrix=L2T)
+    assert np.array_equal(t_d.cpu().numpy(), e), "lie_to_tensor with the reference CSC L2T"
+    # not the Hall matrix: one value changed
+    vals = t2l["data"].astype(dtype).copy()
+    vals[len(vals) // 2] *= 2
+    P = rpp_hip.SparseMatrix("csc", to_dev(vals, dev), to_dev(t2l["indices"], dev), to_dev(t2l["offsets"], dev),
+                             ts - 1, dim)
+    e = np.full((n, ts), 7.0, dtype=dtype)
+    O.spmv(vals, t2l["indices"], t2l["offsets"], dim, e, x, csc=True, out_offset=1, out_size=ts - 1)
+    pl = to_dev(np.full((n, ts), 7.0, dtype=dtype), dev)
+    rpp_hip.tensor_to_lie(pl, x_d, basis, matrix=P)
+    assert np.array_equal(pl.cpu().numpy(), e), "tensor_to_lie with a non-Hall matrix"
+    assert not np.array_equal(pl.cpu().numpy(), got)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -99,31 +196,6 @@ def test_tensor_to_lie_and_back(dev, shape, dtype):
     rpp_hip.lie_to_tensor(t_out, lie_out, basis)
     assert_close(t_out.cpu().numpy(), exp_t, dtype, f"l2t {shape}")
     assert np.array_equal(t_out.cpu().numpy(), exp_t), f"l2t {shape} not bit-identical"
-
-
-def test_log_signature_pipeline_matches_reference(dev):
-    """C3 pipeline at (3,8): signature -> ft_log -> tensor_to_lie, against the
-    reference ops (oracle/_ref) where built."""
-    if not O.has_ref():
-        pytest.skip("reference driver not built on this machine")
-    w, N, n, L = 3, 8, 4, 64
-    rng = np.random.default_rng(0xC3)
-    inc = rng.normal(0, np.sqrt(1 / 64), size=(n, L, w))
-    basis = rpp_hip.TensorBasis(w, N)
-    sig_ref = O.ref_signature(w, N, inc)
-    log_ref = np.zeros_like(sig_ref)
-    O.refimpl.ft_log(w, N, log_ref, sig_ref)
-    lie_ref = O.ref_lie_convert(True, True, w, N, log_ref)
-    sig = rpp_hip.signature(torch.tensor(inc.reshape(n * L, w), device=dev),
-                            torch.arange(n + 1, device=dev, dtype=torch.int64) * L, basis)
-    lg = torch.empty_like(sig)
-    rpp_hip.ft_log(lg, sig, basis)
-    lie = torch.zeros((n, lie_ref.shape[1]), dtype=torch.float64, device=dev)
-    rpp_hip.tensor_to_lie(lie, lg, basis)
-    assert_close(lie.cpu().numpy()[:, 1:], lie_ref[:, 1:], np.float64, "C3 pipeline")
-    # from the reference's own log-signatures the conversion is bit-identical
-    rpp_hip.tensor_to_lie(lie, torch.tensor(log_ref, device=dev), basis)
-    assert np.array_equal(lie.cpu().numpy()[:, 1:], lie_ref[:, 1:])
 
 
 def _lie_expected(lie, x_zero, to_lie, ts, dim, dtype):
