@@ -565,6 +565,22 @@ int64_t rpp_hip_signature_backward_scratch_bytes(const rpp_hip_signature_backwar
 int rpp_hip_signature_backward_f32(const rpp_hip_signature_backward_desc* desc, void* stream);
 int rpp_hip_signature_backward_f64(const rpp_hip_signature_backward_desc* desc, void* stream);
 
+/* rpp_hip_signature_backward_* allocating nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid allocation).
+ * Same descriptor and results (the same kernels), restricted to what needs no
+ * library workspace: the thread-owned subtree tiles (tile_kind 2;
+ * tile_threads 0 takes the shape's first such tile), a device queue order
+ * (order != NULL, num_increments >= offsets[n]) and scratch == NULL.  The
+ * per-row partial sums (num_increments x width^2 values) live in a fixed
+ * 64 MiB device array that the library holds on every GPU it runs on until
+ * the process ends; larger calls are refused with RPP_HIP_INVALID_ARGUMENT.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (different streams, threads or processes' graphs replayed together) race
+ * and must be ordered by the caller. */
+int rpp_hip_signature_backward_noalloc_f32(const rpp_hip_signature_backward_desc* desc, void* stream);
+int rpp_hip_signature_backward_noalloc_f64(const rpp_hip_signature_backward_desc* desc, void* stream);
+
 /* The compiled tiles of rpp_hip_signature_*: writes up to `capacity`
  * 6-tuples (width, depth, threads, split, kind, expand) for the dtype (4 or 8
  * bytes) and returns how many exist.  kind 0 tiles (level-synchronous Horner,

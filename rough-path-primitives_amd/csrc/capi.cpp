@@ -890,7 +890,7 @@ int signature_backward_impl(const rpp_hip_signature_backward_desc* d, void* s) {
         for (const auto& c : thread_tiles<T>()) {
             if (c.width != d->basis.width || c.depth != d->basis.depth || c.backward == nullptr) continue;
             if (d->tile_threads > 0 && (c.threads != d->tile_threads || c.split != d->tile_split)) continue;
-            return c.backward(B, *d, order, total, stream);
+            return c.backward(B, *d, order, total, stream, nullptr);
         }
         return fail(RPP_HIP_NOT_IMPLEMENTED, "no compiled thread-subtree backward tile matches");
     }
@@ -906,10 +906,63 @@ int signature_backward_impl(const rpp_hip_signature_backward_desc* d, void* s) {
     return signature_backward_generic<T>(B, *d, order, stream);
 }
 
+// The backward of signature_backward_impl with the thread-owned subtree
+// tiles, allocating nothing: the per-row partial sums live in the fixed
+// device scratch (noalloc_scratch), and the queue order must be the caller's.
+template <typename T>
+int signature_backward_noalloc_impl(const rpp_hip_signature_backward_desc* d, void* s) {
+    if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
+    RPP_TRY(check_basis(d->basis));
+    if (d->num_paths < 0) return fail(RPP_HIP_INVALID_ARGUMENT, "negative num_paths");
+    if (d->num_paths == 0) return 0;
+    RPP_TRY(check_view(d->signatures, d->basis, d->num_paths, "signatures", false));
+    RPP_TRY(check_view(d->cotangent, d->basis, d->num_paths, "cotangent", false));
+    if (d->signatures.min_degree != 0 || d->signatures.max_degree != d->basis.depth ||
+        d->cotangent.min_degree != 0 || d->cotangent.max_degree != d->basis.depth)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "signature backward: full windows [0, depth] required");
+    if (!d->increments || !d->offsets || !d->grad_increments)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "null increments, offsets or gradient");
+    if (d->checkpoint_every < 0) return fail(RPP_HIP_INVALID_ARGUMENT, "negative checkpoint_every");
+    if (d->checkpoint_every > 0 && (!d->checkpoints || !d->ckpt_offsets))
+        return fail(RPP_HIP_INVALID_ARGUMENT, "checkpoints requested without buffers");
+    if (!d->order)
+        return fail(RPP_HIP_INVALID_ARGUMENT,
+                    "signature backward without allocation: order (the device queue order) is required");
+    if (d->num_increments < 0)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "num_increments must be >= offsets[num_paths] when order is given");
+    if (d->scratch != nullptr)
+        return fail(RPP_HIP_INVALID_ARGUMENT, "signature backward without allocation takes no scratch buffer");
+    if (d->tile_threads < 0 || (d->tile_threads > 0 && d->tile_kind != 2))
+        return fail(RPP_HIP_NOT_IMPLEMENTED,
+                    "signature backward without allocation serves the thread-owned subtree tiles (tile_kind 2)");
+    for (const auto& c : thread_tiles<T>()) {
+        if (c.width != d->basis.width || c.depth != d->basis.depth || c.backward == nullptr) continue;
+        if (d->tile_threads > 0 && (c.threads != d->tile_threads || c.split != d->tile_split)) continue;
+        const size_t need = thread_backward_partial_bytes<T>(d->basis.width, d->num_increments);
+        if (need > kNoAllocScratchBytes)
+            return fail(RPP_HIP_INVALID_ARGUMENT,
+                        "signature backward without allocation needs " + std::to_string(need) +
+                            " bytes of partial sums (num_increments x width^2 values), above its fixed " +
+                            std::to_string(kNoAllocScratchBytes) + "-byte scratch");
+        void* ptr = nullptr;
+        RPP_TRY(noalloc_scratch(&ptr));
+        const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+        return c.backward(B, *d, d->order, d->num_increments, as_stream(s), ptr);
+    }
+    return fail(RPP_HIP_NOT_IMPLEMENTED, "no compiled thread-subtree backward tile matches");
+}
+
 }  // namespace
 }  // namespace rpp_hip
 
 extern "C" {
+
+int rpp_hip_signature_backward_noalloc_f32(const rpp_hip_signature_backward_desc* d, void* s) {
+    return signature_backward_noalloc_impl<float>(d, s);
+}
+int rpp_hip_signature_backward_noalloc_f64(const rpp_hip_signature_backward_desc* d, void* s) {
+    return signature_backward_noalloc_impl<double>(d, s);
+}
 
 int rpp_hip_ft_fmexp_adjoint_f32(const rpp_hip_ft_fmexp_adjoint_desc* d, void* s) {
     return fmexp_adjoint_impl<float>(d, s);

@@ -30,6 +30,7 @@
 #include "signature_thread.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace rpp_hip {
 
@@ -507,13 +508,14 @@ __global__ void k_sum_items(const T* partial, T* grad, int64_t total, const int6
 
 template <typename T, int W, int N, int L, int NT>
 int thread_bwd_launch(const DevBasis& B, const rpp_hip_signature_backward_desc& d, const int64_t* order,
-                      int64_t total_incr, hipStream_t stream) {
+                      int64_t total_incr, hipStream_t stream, void* partial) {
     static_assert(BwdGeo<W, N, L>::NT <= 1024, "one work-group per (path, first letter)");
     const int64_t items = d.num_paths * W;
     if (items >= (int64_t{1} << 31)) return fail(RPP_HIP_NOT_IMPLEMENTED, "too many paths for one launch");
-    void* part = nullptr;
-    if (int rc = workspace(kSlotTmp, static_cast<size_t>(std::max<int64_t>(total_incr, 1)) * W * W * sizeof(T), &part))
-        return rc;
+    void* part = partial;
+    if (part == nullptr) {
+        if (int rc = workspace(kSlotTmp, thread_backward_partial_bytes<T>(W, total_incr), &part)) return rc;
+    }
     k_sig_bwd_thread<T, W, N, L><<<static_cast<unsigned>(items), BwdGeo<W, N, L>::NT, 0, stream>>>(
         B, items, order, static_cast<const T*>(d.increments), d.offsets, make_cview<T>(d.signatures),
         make_cview<T>(d.cotangent), static_cast<T*>(part), d.checkpoint_every, static_cast<const T*>(d.checkpoints),
@@ -530,7 +532,7 @@ int thread_bwd_launch(const DevBasis& B, const rpp_hip_signature_backward_desc& 
 }  // namespace
 
 using ThreadBwdFn = int (*)(const DevBasis&, const rpp_hip_signature_backward_desc&, const int64_t*, int64_t,
-                            hipStream_t);
+                            hipStream_t, void*);
 
 // the backward launcher of a tile, instantiated only for tiles that have one
 template <typename T, int W, int N, int L, int NT, bool BWD>
@@ -561,5 +563,32 @@ const std::vector<ThreadTile<T>>& thread_tiles() {
 
 template const std::vector<ThreadTile<float>>& thread_tiles<float>();
 template const std::vector<ThreadTile<double>>& thread_tiles<double>();
+
+// The allocation-free backward's scratch.  In host code `scratch` names a
+// host-side stand-in, not device memory: only the address hipGetSymbolAddress
+// returns for the current device is ever passed to a kernel.  Every call on
+// one device shares that device's copy, which keeps the last call's contents.
+namespace {
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int noalloc_scratch(void** ptr) {
+    int dev = 0;
+    if (int rc = check_hip(hipGetDevice(&dev), "hipGetDevice")) return rc;
+    if (dev < 0 || dev >= kMaxDevices) return fail(RPP_HIP_NOT_IMPLEMENTED, "device ordinal above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        if (int rc = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)"))
+            return rc;
+        if (p == nullptr) return fail(RPP_HIP_INTERNAL, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return 0;
+}
 
 }  // namespace rpp_hip

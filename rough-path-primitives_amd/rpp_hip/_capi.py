@@ -192,6 +192,7 @@ ENTRY_POINTS = {
     "rpp_hip_tensor_add_identity": VectorScalarDesc,
     "rpp_hip_signature": SignatureDesc,
     "rpp_hip_signature_backward": SignatureBackwardDesc,
+    "rpp_hip_signature_backward_noalloc": SignatureBackwardDesc,
 }
 
 OTHER_SYMBOLS = ["rpp_hip_lie_basis_sizes", "rpp_hip_lie_basis_copy", "rpp_hip_lie_basis_save",
