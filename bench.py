@@ -28,8 +28,11 @@ same line carries, each under its own key:
   roofline        the dominant kernel of the headline path (the ft_fmexp
                   register tile, HBM-bound) from HIP events around its
                   launches;
-  cpu_baseline    the reference cpu::single_thread chain (oracle/_ref) timed
-                  on a bounded sample of the headline workload.
+  cpu_baseline    the oracle's bit-identical restatement of the reference
+                  cpu::single_thread chain on the host's cores, on a bounded
+                  sample of the headline workload, with the reference's
+                  throughput ratio to it measured where the reference is
+                  built (profiles/cpu_baseline_w5d6_chain.json).
 """
 from __future__ import annotations
 
@@ -166,30 +169,41 @@ class Fused:
         return self.rpp.signature(self.inc, self.off, self.basis, out=self.out, order=self.order)
 
 
-def cpu_baseline(inc):
-    """The reference cpu::single_thread chain (oracle/_ref), its batch split
-    over std::threads, on the headline workload (all 4096 paths: about 40 s of
-    CPU time, a few seconds of wall time on 16 threads)."""
+def cpu_baseline(inc, runs=7):
+    """The oracle's restatement of the reference cpu::single_thread chain
+    (bit-identical to it), its batch split over threads on the GPU host's
+    cores (at most 16, the host share of one GPU), on a bounded sample of the
+    headline workload: median of `runs` timed runs after one warm-up.  The
+    reference itself never runs here; its throughput relative to the
+    restatement, measured on identical inputs where the reference is built
+    (oracle/time_cpu_baseline.py), is read from the committed profile."""
+    import statistics
+
     import oracle as O
     threads = max(1, min(16, os.cpu_count() or 1))
-    if O.has_ref():
-        sample = inc
-        kind = "reference"
-        run = lambda: O.ref_signature(WIDTH, DEPTH, sample, threads=threads)  # noqa: E731
-    else:  # the plain-C restatement, one thread
-        sample = inc[:4]
-        threads = 1
-        kind = "port"
-        flat = sample.reshape(-1, WIDTH)
-        off = np.arange(sample.shape[0] + 1, dtype=np.int64) * LENGTH
-        run = lambda: O.signature(WIDTH, DEPTH, flat, off)  # noqa: E731
-    t0 = time.perf_counter()
-    run()
-    dt = time.perf_counter() - t0
+    sample = inc[:1024]
+    flat = np.ascontiguousarray(sample.reshape(-1, WIDTH))
+    off = np.arange(sample.shape[0] + 1, dtype=np.int64) * LENGTH
+    O.signature_threads(WIDTH, DEPTH, flat, off, threads)  # warm-up
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        O.signature_threads(WIDTH, DEPTH, flat, off, threads)
+        times.append(time.perf_counter() - t0)
+    dt = statistics.median(times)
     units = sample.shape[0] * sample.shape[1]
-    return {"value": units / dt, "unit": "path-increments/s", "cores": threads, "kind": kind,
-            "sample": f"{sample.shape[0]} paths x {sample.shape[1]} increments of the headline workload "
-                      f"(width 5, depth 6, FP64), {dt:.2f} s"}
+    res = {"value": units / dt, "unit": "path-increments/s", "cores": threads, "kind": "port",
+           "sample": f"{sample.shape[0]} paths x {sample.shape[1]} increments of the headline workload (width 5, "
+                     f"depth 6, FP64); median of {runs} runs after one warm-up, {dt:.3f} s each, "
+                     f"{threads} threads over contiguous path slices",
+           "runs_s": times}
+    prof = os.path.join(ROOT, "profiles", "cpu_baseline_w5d6_chain.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            ref = json.load(f)
+        res["reference_over_port"] = {c: v["reference_over_restatement"] for c, v in ref["cores"].items()}
+        res["reference_over_port_measured_on"] = f"{ref['host']['cpu']} ({ref['host']['logical_cpus']} CPUs)"
+    return res
 
 
 def load_traffic(name):
@@ -271,40 +285,50 @@ def fwd_bwd(rpp, torch, O, dev, steps=2, warmup=1):
             Gc, Gn = Gn, Gc
         return gpad
 
-    sig = torch.empty((n, dim), dtype=torch.float64, device=dev)
-    grad = torch.empty_like(flat)
-
-    def fused(every=0, ck=None, ck_off=None):
-        kw = dict(checkpoint_every=every, checkpoints=ck, ckpt_offsets=ck_off) if every else {}
+    def fused(dtype, every=0):
+        x = flat.to(dtype)
+        f = ell.to(dtype)
+        out_sig = torch.empty((n, dim), dtype=dtype, device=dev)
+        g = torch.empty_like(x)
+        lss = torch.empty(n, dtype=dtype, device=dev)
+        kw = {}
+        if every:
+            kw = dict(checkpoint_every=every,
+                      checkpoints=torch.empty((n * (L // every), dim), dtype=dtype, device=dev),
+                      ckpt_offsets=torch.arange(n + 1, dtype=torch.int64, device=dev) * (L // every))
 
         def step():
-            rpp.signature(flat, off, basis, out=sig, order=order, **kw)
-            rpp.tensor_pairing(loss, rpp.Batch(ell, 0, N), sig, basis, num_batches=n)
-            rpp.signature_backward(flat, off, basis, sig, ell, grad=grad, order=order, **kw)
-            return grad
+            rpp.signature(x, off, basis, out=out_sig, order=order, **kw)
+            rpp.tensor_pairing(lss, rpp.Batch(f, 0, N), out_sig, basis, num_batches=n)
+            rpp.signature_backward(x, off, basis, out_sig, f, grad=g, order=order, **kw)
+            return g
         return step
 
-    every = 16
-    ck = torch.empty((n * (L // every), dim), dtype=torch.float64, device=dev)
-    ck_off = torch.arange(n + 1, dtype=torch.int64, device=dev) * (L // every)
-    # FP64 oracle gradient of one path
-    g_ref = O.signature_vjp(w, N, inc[0], np.array([0, L], dtype=np.int64), ell.cpu().numpy())
+    # FP64 oracle gradients of four paths spread over the batch
+    chk = [0, n // 3, 2 * n // 3, n - 1]
+    ell_np = ell.cpu().numpy()
+    g_ref = np.concatenate([O.signature_vjp(w, N, inc[p], np.array([0, L], dtype=np.int64), ell_np) for p in chk])
+    every32 = rpp.checkpoint_every_for(torch.float32, L)
     out = {"config": {"paths": n, "width": w, "depth": N, "coefficients": dim, "increments_per_path": L,
-                      "loss": "tensor_pairing with a seeded N(0,1) functional", "dtype": "f64", "steps": steps,
-                      "warmup": warmup}}
-    for name, fn in (("plain", plain), ("fused_reconstruct", fused()),
-                     ("fused_ckpt16", fused(every, ck, ck_off))):
+                      "loss": "tensor_pairing with a seeded N(0,1) functional", "dtype": "f64 (fp32_* lines: f32)",
+                      "steps": steps, "warmup": warmup, "grad_err_paths": chk,
+                      "fp32_default_checkpoint_every": every32}}
+    runs = (("plain", plain), ("fused_reconstruct", fused(torch.float64)),
+            ("fused_ckpt16", fused(torch.float64, 16)), ("fp32_fused_reconstruct", fused(torch.float32)),
+            (f"fp32_fused_ckpt{every32}", fused(torch.float32, every32)))
+    for name, fn in runs:
         ms, wall, g = timed(torch, fn, steps, warmup)
         if name == "plain":
-            g0 = g[0, 1:].view(L, w)
+            got = torch.cat([g[p, 1:].view(L, w) for p in chk])
         else:
-            g0 = g[:L]
+            got = torch.cat([g[p * L:(p + 1) * L] for p in chk])
         out[name] = {"ms_per_step": ms, "wall_ms_per_step": wall, "paths_per_s": n / (ms * 1e-3),
-                     "grad_max_rel_err_vs_fp64_oracle_path0": rel_err(g0.cpu().numpy(), g_ref)}
+                     "grad_max_rel_err_vs_fp64_oracle": rel_err(got.double().cpu().numpy(), g_ref)}
+        del fn
         torch.cuda.empty_cache()
     out["fused_speedup"] = out["plain"]["ms_per_step"] / out["fused_reconstruct"]["ms_per_step"]
     out["fused_ckpt_speedup"] = out["plain"]["ms_per_step"] / out["fused_ckpt16"]["ms_per_step"]
-    del S0, S1, G0, G1, ck, padded, gpad
+    del S0, S1, G0, G1, padded, gpad
     torch.cuda.empty_cache()
     return out
 
@@ -354,15 +378,19 @@ def ft_mul_section(rpp, torch, O, dev):
             gms, _, _ = timed(torch, graph.replay, 20, 3)
             entry["graph_us_per_call"] = gms * 1e3 / calls
             entry["graph_achieved_gbs"] = 3 * n * dim * 8 / (gms / calls * 1e-3) / 1e9
-        if n == 1024 and O.has_ref():
+        if n == 1024:
+            # the oracle's restatement of the reference product (the same
+            # summation order, bit for bit) on one core: median of 7 runs
             out_cpu = np.zeros((n, dim))
-            t0 = time.perf_counter()
-            r = 20
-            for _ in range(r):
-                O.refimpl.ft_mul(w, N, out_cpu, lhs, rhs)
-            entry["cpu_reference_us_per_call"] = (time.perf_counter() - t0) / r * 1e6
-            entry["cpu_reference_cores"] = 1
-            entry["cpu_reference_max_rel_err_vs_gpu"] = rel_err(O_.cpu().numpy(), out_cpu)
+            O.orc.ft_mul(w, N, out_cpu, lhs, rhs)
+            runs = []
+            for _ in range(7):
+                t0 = time.perf_counter()
+                O.orc.ft_mul(w, N, out_cpu, lhs, rhs)
+                runs.append(time.perf_counter() - t0)
+            entry["cpu_port_us_per_call"] = float(np.median(runs)) * 1e6
+            entry["cpu_port_cores"] = 1
+            entry["cpu_port_bit_identical_to_gpu"] = bool(np.array_equal(O_.cpu().numpy(), out_cpu))
         res[label] = entry
         del L, R, O_
         torch.cuda.empty_cache()
@@ -488,12 +516,33 @@ def log_signature(rpp, torch, O, dev, steps=3, warmup=1):
            "log_max_rel_err_vs_fp64_oracle": rel_err(lg_all[:k], log_ref),
            "lie_max_rel_err_vs_fp64_oracle": rel_err(got[:k, 1:].cpu().numpy(), lie_ref),
            "round_trip_max_rel_err_all_paths": float(rt[worst]),
-           "round_trip_worst_path": {"path": worst, "gpu_log_rel_err_vs_oracle": rel_err(lg_all[worst], log_w[0]),
+           "round_trip_worst_path": {"path": worst,
+                                     "gpu_pipeline_log_rel_err_vs_oracle_pipeline": rel_err(lg_all[worst], log_w[0]),
                                      "oracle_round_trip_rel_err": rel_err(back_w, log_w),
-                                     "max_abs_log": float(np.abs(log_w).max())}}
+                                     "max_abs_log": float(np.abs(log_w).max())},
+           "ft_log_worst_conditioned": log_worst_conditioned(rpp, torch, O, dev)}
     del sig, lg, lie_out, back
     torch.cuda.empty_cache()
     return res
+
+
+def log_worst_conditioned(rpp, torch, O, dev):
+    """ft_log of the 32 worst-conditioned log-signatures of this workload
+    (|log S| up to 30) on the reference's own signatures (rebuilt bit for bit
+    by the oracle's chain), against the reference's ft_log from the committed
+    fixture: the parity of ft_log itself, on identical inputs."""
+    import hashlib
+    path = os.path.join(ROOT, "tests", "golden", "log_c3_worst_w3d8.npz")
+    with np.load(path) as z:
+        inc, expected, sha = z["increments"], z["log"], z["signature_sha256"]
+    n, L, w = inc.shape
+    sig = O.signature(w, 8, inc.reshape(-1, w), np.arange(n + 1, dtype=np.int64) * L)
+    same_input = hashlib.sha256(np.ascontiguousarray(sig).tobytes()).digest() == sha.tobytes()
+    out = torch.empty(sig.shape, dtype=torch.float64, device=dev)
+    rpp.ft_log(out, torch.tensor(sig, device=dev), rpp.TensorBasis(w, 8))
+    got = out.cpu().numpy()
+    return {"paths": n, "max_abs_log": float(np.abs(expected).max()), "input_is_reference_signature": bool(same_input),
+            "max_rel_err_vs_reference": rel_err(got, expected), "bit_identical": bool(np.array_equal(got, expected))}
 
 
 # ------------------------------------------------------------- uneven (C4)
@@ -508,7 +557,9 @@ def uneven(rpp, torch, O, dev, steps=3, warmup=1):
     basis = rpp.TensorBasis(w, N)
     off = torch.tensor(off_np, device=dev)
     total = int(off_np[-1])
-    chk = np.argsort(lengths)[:2]
+    # the two shortest and the two longest paths against the FP64 oracle
+    srt = np.argsort(lengths, kind="stable")
+    chk = np.concatenate([srt[:2], srt[-2:]])
     ref = np.concatenate([O.signature(w, N, inc[off_np[p]:off_np[p + 1]], np.array([0, lengths[p]]))
                           for p in chk])
     res = {"config": {"paths": n, "width": w, "depth": N, "coefficients": basis.size(),
@@ -519,14 +570,54 @@ def uneven(rpp, torch, O, dev, steps=3, warmup=1):
         g = rpp.SignatureGraph(torch.tensor(inc, dtype=dt, device=dev), off, basis)
         ms, wall, out = timed(torch, g.replay, steps, warmup)
         tflops = total * chen_flops(w, N) / (ms * 1e-3) / 1e12
+        got = out[torch.tensor(chk, device=dev)].double().cpu().numpy()
         res[name] = {"ms_per_step": ms, "wall_ms_per_step": wall,
                      "path_increments_per_s": total / (ms * 1e-3), "paths_per_s": n / (ms * 1e-3),
                      "vector_tflops": tflops, "vector_frac": tflops / peak,
-                     "max_rel_err_vs_fp64_oracle": rel_err(out[torch.tensor(chk, device=dev)].double().cpu().numpy(),
-                                                           ref)}
+                     "max_rel_err_vs_fp64_oracle": rel_err(got, ref),
+                     "checked_path_lengths": [int(lengths[p]) for p in chk],
+                     "rel_err_longest_path": rel_err(got[-1], ref[-1])}
         del g, out
         torch.cuda.empty_cache()
     return res
+
+
+def fp32_drift(rpp, torch, O, dev):
+    """FP32 gradient error of the batched backward against the FP64 oracle,
+    exp(-x) reconstruction against ceil(sqrt(L)) checkpoints, over path
+    lengths 16 .. 1024 at three shapes (4 paths each, 2 at 1024 increments;
+    increments N(0, 1/L), loss <ell, S> with ell ~ N(0, 1)): the measurement
+    behind the library's checkpoint policy (rpp_hip_signature_checkpoint_every)."""
+    import math
+    rows = []
+    for (w, N) in ((6, 6), (5, 6), (3, 8)):
+        for L in (16, 32, 64, 128, 256, 512, 1024):
+            n = 2 if L == 1024 else 4
+            rng = np.random.default_rng(L * 7 + w)
+            inc = rng.normal(0, np.sqrt(1 / L), size=(n * L, w))
+            off = np.arange(n + 1, dtype=np.int64) * L
+            basis = rpp.TensorBasis(w, N)
+            dim = basis.size()
+            ell = rng.normal(size=dim)
+            ref = O.signature_vjp(w, N, inc, off, ell)
+            row = {"width": w, "depth": N, "increments": L, "paths": n}
+            every = int(math.ceil(math.sqrt(L)))
+            for label, k in (("reconstruct", 0), ("checkpoints", every)):
+                x = torch.tensor(inc, dtype=torch.float32, device=dev)
+                o = torch.tensor(off, device=dev)
+                kw = {}
+                if k:
+                    kw = dict(checkpoint_every=k,
+                              checkpoints=torch.empty((n * (L // k), dim), dtype=torch.float32, device=dev),
+                              ckpt_offsets=torch.arange(n + 1, dtype=torch.int64, device=dev) * (L // k))
+                sig = rpp.signature(x, o, basis, **kw)
+                g = rpp.signature_backward(x, o, basis, sig, torch.tensor(ell, dtype=torch.float32, device=dev), **kw)
+                row[label] = rel_err(g.double().cpu().numpy(), ref)
+            row["checkpoint_every"] = every
+            row["ratio"] = row["reconstruct"] / row["checkpoints"]
+            print(json.dumps(row), file=sys.stderr, flush=True)
+            rows.append(row)
+    return rows
 
 
 def main():
@@ -539,7 +630,8 @@ def main():
                     help="comma list of extra sections to run (" + ",".join(SECTIONS) + "), or 'none'")
     ap.add_argument("--only", default=None,
                     help="run one headline variant (plain_fp64, fused_fp64, plain_fp32, fused_fp32), "
-                         "ft_mul_hbm (2^17 width-4 depth-4 products) or layout, and print its timing only")
+                         "ft_mul_hbm (2^17 width-4 depth-4 products), layout, or fp32_drift (the FP32 "
+                         "gradient error sweep behind the checkpoint policy), and print its result only")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -572,6 +664,9 @@ def main():
         ms, _, _ = timed(torch, lambda: rpp.ft_mul(out, L, R, basis), args.steps, args.warmup)
         print(json.dumps({"variant": args.only, "us_per_call": ms * 1e3,
                           "achieved_gbs": 3 * n * basis.size() * 8 / (ms * 1e-3) / 1e9}))
+        return
+    if args.only == "fp32_drift":
+        print(json.dumps({"variant": args.only, "rows": fp32_drift(rpp, torch, O, dev)}))
         return
     if args.only == "layout":
         print(json.dumps({"variant": args.only, **layout_section(rpp, torch, O, dev, reps=args.steps)}))

@@ -565,6 +565,16 @@ int64_t rpp_hip_signature_backward_scratch_bytes(const rpp_hip_signature_backwar
 int rpp_hip_signature_backward_f32(const rpp_hip_signature_backward_desc* desc, void* stream);
 int rpp_hip_signature_backward_f64(const rpp_hip_signature_backward_desc* desc, void* stream);
 
+/* The checkpoint spacing the library recommends for a batched signature
+ * whose backward pass will follow, for paths of at most max_length
+ * increments: 0 (reconstruct every state by exp(-x)) in FP64 and for FP32
+ * paths shorter than 32 increments, else ceil(sqrt(max_length)) (sqrt(L)
+ * checkpointing).  In FP32 the reconstruction drift roughly doubles the
+ * gradient error from 32 increments up; in FP64 it stays below 5e-13.
+ * Returns a negative ErrorCode for a dtype other than 4 or 8 bytes or a
+ * negative length.  No reference counterpart. */
+int32_t rpp_hip_signature_checkpoint_every(int32_t dtype_bytes, int64_t max_length);
+
 /* rpp_hip_signature_backward_* allocating nothing.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under
  * hipGraph capture, tight serving loops, harnesses that forbid allocation).

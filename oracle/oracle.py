@@ -372,6 +372,34 @@ def signature(width, depth, increments, offsets, dtype=np.float64):
     return out
 
 
+def signature_threads(width, depth, increments, offsets, threads, dtype=np.float64):
+    """signature() with the paths split into `threads` contiguous slices, each
+    computed on its own Python thread (the C call releases the GIL): the
+    restated single-thread strategy run on several cores, as the reference
+    strategy is by splitting its batch."""
+    from concurrent.futures import ThreadPoolExecutor
+    inc = np.ascontiguousarray(increments, dtype=dtype)
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    n = off.shape[0] - 1
+    dim = tensor_dimension(width, depth)
+    out = np.zeros((n, dim), dtype=dtype)
+    bounds = np.linspace(0, n, max(1, threads) + 1).astype(np.int64)
+    sfx, _ = _sfx(dtype)
+
+    def work(a, b):
+        if b <= a:
+            return
+        sub = np.ascontiguousarray(off[a:b + 1] - off[a])
+        part = inc[off[a]:off[b]]
+        view = out[a:b]
+        _call(lib(), f"orc_signature_{sfx}", _i(width), _i(depth), _ll(b - a), sub.ctypes.data_as(_p),
+              part.ctypes.data_as(_p), view.ctypes.data_as(_p), _ll(dim))
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        list(ex.map(lambda i: work(int(bounds[i]), int(bounds[i + 1])), range(len(bounds) - 1)))
+    return out
+
+
 def signature_vjp(width, depth, increments, offsets, ell):
     """Gradient of <ell, S_p> wrt increments, composed from the reference
     adjoints (FP64)."""

@@ -42,13 +42,19 @@ static void FN(level_product)(const orc_basis* B, T* tile, int k,
     const int rhs0_zero = rmin == 0 && rhs[0] == (T)0;
     const int count = orc_term_order(B->width, k, jlo, jhi, lhs0_zero, rhs0_zero, seq);
     for (int t = 0; t < count; ++t) {
+        /* word w = p * d^j + s of level k: prefix p of degree k - j, suffix s
+         * of degree j; each word receives one term per j, in seq's order */
         const int j = seq[t];
-        const long long pj = B->pow[j];
+        const long long pj = B->pow[j], np = B->pow[k - j];
         const T* L = lhs + B->start[k - j];
         const T* R = rhs + B->start[j];
-        for (long long w = 0; w < n; ++w) {
-            const T prod = L[w / pj] * R[w % pj];
-            tile[w] = tile[w] + prod;
+        for (long long p = 0; p < np; ++p) {
+            const T lv = L[p];
+            T* row = tile + p * pj;
+            for (long long sfx = 0; sfx < pj; ++sfx) {
+                const T prod = lv * R[sfx];
+                row[sfx] = row[sfx] + prod;
+            }
         }
     }
 }

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -906,6 +907,13 @@ int signature_backward_impl(const rpp_hip_signature_backward_desc* d, void* s) {
     return signature_backward_generic<T>(B, *d, order, stream);
 }
 
+// FP32 paths of at least this many increments get sqrt(L) checkpoints by
+// default: measured on MI355X against the FP64 oracle, the exp(-x)
+// reconstruction multiplies the FP32 gradient error by 1.2-3.6 from 32
+// increments up (widths 3, 5, 6 at depth 6-8, 32-1024 increments), while in
+// FP64 it stays below 5e-13 at 1024 increments.
+constexpr int64_t kFp32CheckpointFrom = 32;
+
 // The backward of signature_backward_impl with the thread-owned subtree
 // tiles, allocating nothing: the per-row partial sums live in the fixed
 // device scratch (noalloc_scratch), and the queue order must be the caller's.
@@ -956,6 +964,16 @@ int signature_backward_noalloc_impl(const rpp_hip_signature_backward_desc* d, vo
 }  // namespace rpp_hip
 
 extern "C" {
+
+int32_t rpp_hip_signature_checkpoint_every(int32_t dtype_bytes, int64_t max_length) {
+    if (dtype_bytes != 4 && dtype_bytes != 8)
+        return -rpp_hip::fail(RPP_HIP_INVALID_ARGUMENT, "dtype_bytes must be 4 or 8");
+    if (max_length < 0) return -rpp_hip::fail(RPP_HIP_INVALID_ARGUMENT, "negative max_length");
+    if (dtype_bytes == 8 || max_length < rpp_hip::kFp32CheckpointFrom) return 0;
+    int32_t k = static_cast<int32_t>(std::sqrt(static_cast<double>(max_length)));
+    while (static_cast<int64_t>(k) * k < max_length) ++k;
+    return k;
+}
 
 int rpp_hip_signature_backward_noalloc_f32(const rpp_hip_signature_backward_desc* d, void* s) {
     return signature_backward_noalloc_impl<float>(d, s);

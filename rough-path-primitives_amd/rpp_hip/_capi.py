@@ -198,7 +198,7 @@ ENTRY_POINTS = {
 OTHER_SYMBOLS = ["rpp_hip_lie_basis_sizes", "rpp_hip_lie_basis_copy", "rpp_hip_lie_basis_save",
                  "rpp_hip_lie_basis_load", "rpp_hip_last_error",
                  "rpp_hip_version", "rpp_hip_signature_backward_scratch_bytes", "rpp_hip_signature_tiles",
-                 "rpp_hip_fmexp_tile", "rpp_hip_sparse_matrix_forget"]
+                 "rpp_hip_fmexp_tile", "rpp_hip_sparse_matrix_forget", "rpp_hip_signature_checkpoint_every"]
 
 
 def exported_symbols() -> list[str]:
@@ -232,6 +232,8 @@ def lib() -> C.CDLL:
         lib_.rpp_hip_lie_basis_load.argtypes = [C.c_char_p, _vp, _vp]
         lib_.rpp_hip_signature_backward_scratch_bytes.argtypes = [C.POINTER(SignatureBackwardDesc), _i32]
         lib_.rpp_hip_signature_backward_scratch_bytes.restype = _i64
+        lib_.rpp_hip_signature_checkpoint_every.argtypes = [_i32, _i64]
+        lib_.rpp_hip_signature_checkpoint_every.restype = _i32
         lib_.rpp_hip_sparse_matrix_forget.argtypes = [_vp]
         lib_.rpp_hip_sparse_matrix_forget.restype = C.c_int
         lib_.rpp_hip_signature_tiles.argtypes = [_i32, _vp, _i32]

@@ -23,15 +23,20 @@ class SignatureGraph:
 
     increments: [total, width] device tensor, copied into ``self.increments``
     (write new data there between replays); cotangent: [n, dim] or one [dim]
-    functional, copied into ``self.cotangent``.
+    functional, copied into ``self.cotangent``.  checkpoint_every: as for
+    autograd.signature; "auto" (the default) stores sqrt(L) checkpoints only
+    when a backward pass follows in FP32 (ops.checkpoint_every_for).
     """
 
     def __init__(self, increments: torch.Tensor, offsets: torch.Tensor, basis: ops.TensorBasis,
-                 backward: bool = False, cotangent: Optional[torch.Tensor] = None, checkpoint_every: int = 0,
+                 backward: bool = False, cotangent: Optional[torch.Tensor] = None, checkpoint_every="auto",
                  tile: Optional[tuple] = None):
         dev = increments.device
         self.basis = basis
         self.offsets = offsets.to(dev, torch.int64).contiguous()
+        if checkpoint_every == "auto":
+            longest = int((self.offsets[1:] - self.offsets[:-1]).max().item()) if self.offsets.numel() > 1 else 0
+            checkpoint_every = ops.checkpoint_every_for(increments.dtype, longest) if backward else 0
         self.increments = increments.detach().clone().contiguous()
         n = self.offsets.numel() - 1
         self.out = torch.empty((n, basis.size()), dtype=increments.dtype, device=dev)

@@ -613,6 +613,17 @@ def thread_backward_tiles(width: int, depth: int, dtype) -> list:
     return [(t, s) for (w, d, t, s, k, e) in signature_tiles(dtype) if (w, d) == (width, depth) and k == 2 and e == 1]
 
 
+def checkpoint_every_for(dtype, max_length: int) -> int:
+    """The library's checkpoint spacing for a signature whose backward pass
+    follows (rpp_hip_signature_checkpoint_every): 0 in FP64 and for FP32
+    paths under 32 increments, else ceil(sqrt(max_length))."""
+    nbytes = torch.tensor([], dtype=dtype).element_size()
+    k = A.lib().rpp_hip_signature_checkpoint_every(nbytes, int(max_length))
+    if k < 0:
+        raise RppHipError(-k, A.lib().rpp_hip_last_error().decode())
+    return k
+
+
 def _offsets_of(lengths, device) -> torch.Tensor:
     lengths = torch.as_tensor(lengths, dtype=torch.int64)
     off = torch.zeros(lengths.numel() + 1, dtype=torch.int64)

@@ -83,3 +83,16 @@ def test_fmexp_adjoint_validation_on_host():
     assert _call("rpp_hip_ft_fmexp_adjoint_f64", A.FtFmexpAdjointDesc(A.Basis(5, 0), 2, full, x, cot, other,
                                                                       gx)) == 3
     assert _call("rpp_hip_ft_fmexp_adjoint_f64", A.FtFmexpAdjointDesc(basis, 0, full, x, cot, other, gx)) == 0
+
+
+def test_checkpoint_policy():
+    """The library's checkpoint spacing for a backward pass: none in FP64,
+    none for FP32 paths under 32 increments, else ceil(sqrt(L)); bad
+    arguments give a negative error code."""
+    import rpp_hip
+    lib = rpp_hip.lib()
+    assert [lib.rpp_hip_signature_checkpoint_every(4, L) for L in (0, 1, 31, 32, 33, 256, 1000, 1024, 1025)] == \
+        [0, 0, 0, 6, 6, 16, 32, 32, 33]
+    assert lib.rpp_hip_signature_checkpoint_every(8, 4096) == 0
+    assert lib.rpp_hip_signature_checkpoint_every(2, 64) == -3
+    assert lib.rpp_hip_signature_checkpoint_every(4, -1) == -3

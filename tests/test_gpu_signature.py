@@ -413,3 +413,83 @@ def test_signature_backward_fd_c5_shape(dev, ckpt):
         fd = ((lp - lm) / (2 * h)).cpu().numpy()
         worst = max(worst, rel_err(grad[rows, a], fd))
     assert worst < 1e-6, f"C5 shape finite differences: {worst:.3e}"
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
+def test_backward_1024_increments_reconstruction_and_checkpoints(dev, dtype):
+    """Width 5, depth 6, two paths of 1024 increments N(0, 1/1024): the
+    gradient by exp(-x) reconstruction and by sqrt(L) = 32 checkpoints against
+    the FP64 oracle (FP32: reconstruction within 5e-4, checkpoints within
+    2e-4; FP64: both within 1e-10), and the
+    autograd default ("auto": checkpoints every 32 in FP32, none in FP64)
+    equal to the explicit choice bit for bit."""
+    from rpp_hip import autograd as AG
+    w, N, n, L = 5, 6, 2, 1024
+    rng = np.random.default_rng(0x1024)
+    inc = rng.normal(0, np.sqrt(1.0 / L), size=(n * L, w))
+    off = np.arange(n + 1, dtype=np.int64) * L
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    ell = rng.normal(size=dim)
+    expected = O.signature_vjp(w, N, inc, off, ell)
+    inc_d = torch.tensor(inc, dtype=dtype, device=dev)
+    off_d = torch.tensor(off, device=dev)
+    ell_d = torch.tensor(ell, dtype=dtype, device=dev)
+    errs, grads = {}, {}
+    for every in (0, 32):
+        kw = {}
+        if every:
+            kw = dict(checkpoint_every=every, checkpoints=torch.empty((n * (L // every), dim), dtype=dtype, device=dev),
+                      ckpt_offsets=torch.arange(n + 1, dtype=torch.int64, device=dev) * (L // every))
+        sig = rpp_hip.signature(inc_d, off_d, basis, **kw)
+        grads[every] = rpp_hip.signature_backward(inc_d, off_d, basis, sig, ell_d, **kw)
+        errs[every] = rel_err(grads[every].double().cpu().numpy(), expected)
+    if dtype == torch.float32:
+        assert errs[0] < 5e-4 and errs[32] < 2e-4, errs
+    else:
+        assert errs[0] < 1e-10 and errs[32] < 1e-10, errs
+    auto_every = rpp_hip.checkpoint_every_for(dtype, L)
+    assert auto_every == (32 if dtype == torch.float32 else 0)
+    x = inc_d.clone().requires_grad_(True)
+    (AG.signature(x, off_d, N) @ ell_d).sum().backward()
+    assert torch.equal(x.grad, grads[auto_every]), "autograd default differs from the explicit checkpoint choice"
+
+
+_C4_CACHE = {}
+
+
+def _c4_paths():
+    """16 paths of the width-8 depth-5 uneven workload's distribution with
+    lengths spanning 16 .. 1024 (both ends included), and their FP64 oracle
+    signatures (computed once per session)."""
+    if not _C4_CACHE:
+        w, N = 8, 5
+        rng = np.random.default_rng(0xC4)
+        lengths = np.concatenate([[16, 1024], rng.integers(16, 1025, size=14)])
+        off = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+        inc = np.concatenate([rng.normal(0.0, np.sqrt(1.0 / Lp), size=(Lp, w)) for Lp in lengths])
+        _C4_CACHE.update(lengths=lengths, off=off, inc=inc, expected=O.signature(w, N, inc, off))
+    return _C4_CACHE
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
+def test_uneven_graph_real_lengths(dev, dtype):
+    """The uneven-batch path as the benchmark runs it (the fused kernel
+    replayed from a hipGraph, autotuned tile) at width 8, depth 5 on paths of
+    16 .. 1024 increments: every path within the reference tolerance of the
+    FP64 oracle (1e-12 FP64, 1e-5 FP32), the 1024-increment one included, on
+    two replays with a rewrite of the increments in between."""
+    c = _c4_paths()
+    basis = rpp_hip.TensorBasis(8, 5)
+    inc_d = torch.tensor(c["inc"], dtype=dtype, device=dev)
+    g = rpp_hip.SignatureGraph(inc_d, torch.tensor(c["off"], device=dev), basis)
+    npd = np.float64 if dtype == torch.float64 else np.float32
+    for rep in range(2):
+        if rep:
+            g.increments.zero_()
+            torch.cuda.synchronize()
+            g.replay()
+            g.increments.copy_(inc_d)
+        out = g.replay().double().cpu().numpy()
+        for p, Lp in enumerate(c["lengths"]):
+            assert_close(out[p], c["expected"][p], npd, f"replay {rep}, path {p} of {Lp} increments")
