@@ -630,8 +630,10 @@ def main():
                     help="comma list of extra sections to run (" + ",".join(SECTIONS) + "), or 'none'")
     ap.add_argument("--only", default=None,
                     help="run one headline variant (plain_fp64, fused_fp64, plain_fp32, fused_fp32), "
-                         "ft_mul_hbm (2^17 width-4 depth-4 products), layout, or fp32_drift (the FP32 "
-                         "gradient error sweep behind the checkpoint policy), and print its result only")
+                         "ft_mul_hbm (2^17 width-4 depth-4 products), layout, fused_fwd_bwd (the width-6 "
+                         "depth-6 fused forward + backward), log_stages (the width-3 depth-8 log-signature "
+                         "stages) or fp32_drift (the FP32 gradient error sweep behind the checkpoint policy), "
+                         "and print its result only")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -664,6 +666,45 @@ def main():
         ms, _, _ = timed(torch, lambda: rpp.ft_mul(out, L, R, basis), args.steps, args.warmup)
         print(json.dumps({"variant": args.only, "us_per_call": ms * 1e3,
                           "achieved_gbs": 3 * n * basis.size() * 8 / (ms * 1e-3) / 1e9}))
+        return
+    if args.only == "fused_fwd_bwd":
+        # the fused forward + backward of the width 6, depth 6 workload (2048
+        # paths x 256 increments, FP64, exp(-x) reconstruction) on its own,
+        # for profiling its two kernels
+        w, N, n, L = 6, 6, 2048, 256
+        basis = rpp.TensorBasis(w, N)
+        flat = torch.tensor(make_increments(0xC5, n, L, w).reshape(n * L, w), device=dev)
+        off = torch.arange(n + 1, dtype=torch.int64, device=dev) * L
+        order = rpp.path_order(off)
+        ell = torch.tensor(np.random.default_rng(0xC5 + 1).normal(size=basis.size()), device=dev)
+        sig = torch.empty((n, basis.size()), dtype=torch.float64, device=dev)
+        grad = torch.empty_like(flat)
+
+        def step():
+            rpp.signature(flat, off, basis, out=sig, order=order)
+            rpp.signature_backward(flat, off, basis, sig, ell, grad=grad, order=order)
+        ms, _, _ = timed(torch, step, args.steps, args.warmup)
+        print(json.dumps({"variant": args.only, "ms_per_step": ms, "paths_per_s": n / (ms * 1e-3)}))
+        return
+    if args.only == "log_stages":
+        # the four GPU stages of the width 3, depth 8 log-signature workload
+        # (16384 paths x 64 increments, FP64) on their own, for profiling
+        w, N, n, L = 3, 8, 16384, 64
+        basis = rpp.TensorBasis(w, N)
+        flat = torch.tensor(make_increments(0xC3, n, L, w).reshape(n * L, w), device=dev)
+        off = torch.arange(n + 1, dtype=torch.int64, device=dev) * L
+        order = rpp.path_order(off)
+        sig = torch.empty((n, basis.size()), dtype=torch.float64, device=dev)
+        lg, back = torch.empty_like(sig), torch.empty_like(sig)
+        lie = torch.empty((n, rpp.lie_basis(w, N)["true_size"]), dtype=torch.float64, device=dev)
+
+        def step():
+            rpp.signature(flat, off, basis, out=sig, order=order)
+            rpp.ft_log(lg, sig, basis)
+            rpp.tensor_to_lie(lie, lg, basis)
+            rpp.lie_to_tensor(back, lie, basis)
+        ms, _, _ = timed(torch, step, args.steps, args.warmup)
+        print(json.dumps({"variant": args.only, "ms_per_step": ms, "paths_per_s": n / (ms * 1e-3)}))
         return
     if args.only == "fp32_drift":
         print(json.dumps({"variant": args.only, "rows": fp32_drift(rpp, torch, O, dev)}))
