@@ -10,13 +10,17 @@
 // and friends unchanged, and every op below turns the batches into the
 // descriptors of include/rpp_hip.h and calls librpp_hip.so.  Batch data are
 // device pointers (hipMalloc'd by the caller); layouts::StrideLayout and
-// layouts::NoStrideLayout batches are accepted; Scalar must equal Accum
-// (float or double).  launch() is asynchronous on LaunchConfig::stream and
-// returns rpp::Error<const char*> like the reference GPU strategy.
+// layouts::NoStrideLayout batches are accepted, and so is the batch-
+// interleaved layout rpp::hip::layouts::InterleavedLayout below (data an
+// rpp::hip::InterleavedIterator); Scalar must equal Accum (float or double).  launch() is
+// asynchronous on LaunchConfig::stream and returns rpp::Error<const char*>
+// like the reference GPU strategy.
 #ifndef RPP_HIP_BLOCK_STRATEGY_HPP
 #define RPP_HIP_BLOCK_STRATEGY_HPP
 
+#include <cstddef>
 #include <cstdint>
+#include <iterator>
 #include <tuple>
 #include <type_traits>
 #include <utility>
@@ -58,10 +62,121 @@
 #include <rpp/sparse/matrix.hpp>
 #include <rpp/support/error.hpp>
 #include <rpp/views/batch.hpp>
+#include <rpp/views/dense_tensor_view.hpp>
 
 #include "rpp_hip.h"
 
 namespace rpp::hip {
+
+// A random-access iterator over one object of a batch-interleaved batch:
+// element i is base[i * stride].  The reference's StridedIterator
+// (support/strided_iterator.hpp:15-41) has this shape but cannot be
+// instantiated as shipped (its second pair of operator< overloads, :200-210,
+// redefines the first), so the interleaved layout uses this one.
+template <typename T>
+class InterleavedIterator {
+    T* base_ = nullptr;
+    std::ptrdiff_t stride_ = 1;
+
+public:
+    using value_type = std::remove_cv_t<T>;
+    using difference_type = std::ptrdiff_t;
+    using pointer = T*;
+    using reference = T&;
+    using iterator_category = std::random_access_iterator_tag;
+
+    constexpr InterleavedIterator() noexcept = default;
+    constexpr InterleavedIterator(T* base, difference_type stride) noexcept : base_(base), stride_(stride) {}
+
+    constexpr T* base() const noexcept { return base_; }
+    constexpr difference_type stride() const noexcept { return stride_; }
+    constexpr reference operator*() const noexcept { return *base_; }
+    constexpr reference operator[](difference_type i) const noexcept { return base_[i * stride_]; }
+    constexpr InterleavedIterator& operator++() noexcept {
+        base_ += stride_;
+        return *this;
+    }
+    constexpr InterleavedIterator operator++(int) noexcept {
+        InterleavedIterator t(*this);
+        base_ += stride_;
+        return t;
+    }
+    constexpr InterleavedIterator& operator--() noexcept {
+        base_ -= stride_;
+        return *this;
+    }
+    constexpr InterleavedIterator operator--(int) noexcept {
+        InterleavedIterator t(*this);
+        base_ -= stride_;
+        return t;
+    }
+    constexpr InterleavedIterator& operator+=(difference_type k) noexcept {
+        base_ += k * stride_;
+        return *this;
+    }
+    constexpr InterleavedIterator& operator-=(difference_type k) noexcept {
+        base_ -= k * stride_;
+        return *this;
+    }
+    friend constexpr InterleavedIterator operator+(InterleavedIterator it, difference_type k) noexcept {
+        return it += k;
+    }
+    friend constexpr InterleavedIterator operator+(difference_type k, InterleavedIterator it) noexcept {
+        return it += k;
+    }
+    friend constexpr InterleavedIterator operator-(InterleavedIterator it, difference_type k) noexcept {
+        return it -= k;
+    }
+    friend constexpr difference_type operator-(InterleavedIterator const& a, InterleavedIterator const& b) noexcept {
+        return (a.base_ - b.base_) / a.stride_;
+    }
+    friend constexpr bool operator==(InterleavedIterator const& a, InterleavedIterator const& b) noexcept {
+        return a.base_ == b.base_;
+    }
+    friend constexpr bool operator!=(InterleavedIterator const& a, InterleavedIterator const& b) noexcept {
+        return a.base_ != b.base_;
+    }
+    friend constexpr bool operator<(InterleavedIterator const& a, InterleavedIterator const& b) noexcept {
+        return a.base_ < b.base_;
+    }
+    friend constexpr bool operator>(InterleavedIterator const& a, InterleavedIterator const& b) noexcept {
+        return a.base_ > b.base_;
+    }
+    friend constexpr bool operator<=(InterleavedIterator const& a, InterleavedIterator const& b) noexcept {
+        return a.base_ <= b.base_;
+    }
+    friend constexpr bool operator>=(InterleavedIterator const& a, InterleavedIterator const& b) noexcept {
+        return a.base_ >= b.base_;
+    }
+};
+
+namespace layouts {
+
+// The batch-interleaved ("[coefficient][object]") layout that every hip::block
+// kernel reads with coalesced loads: coefficient c of object b sits at
+// base[b + c * n] for n objects.  Batch data is an InterleavedIterator over the
+// base pointer with stride n (the coefficient step inside one object); map()
+// moves the base to object b and keeps the stride, so each view walks its
+// object's coefficients n apart (a Layout of views/batch.hpp:27-68).
+struct InterleavedLayout {
+    template <typename T, typename Index>
+    constexpr InterleavedIterator<T> map(InterleavedIterator<T> data, Index index) const noexcept {
+        return InterleavedIterator<T>(data.base() + index, data.stride());
+    }
+};
+
+}  // namespace layouts
+
+// A batch of num_objects tensors of `basis` stored batch-interleaved at the
+// device pointer data (dim * num_objects scalars), on the window
+// [min_degree, max_degree]: the reference's make_tensor_batch
+// (views/dense_tensor_view.hpp:38-66) over InterleavedLayout.
+template <typename T, typename Basis, typename Degree>
+auto make_interleaved_tensor_batch(T* data, std::ptrdiff_t num_objects, Basis const& basis, Degree min_degree,
+                                   Degree max_degree) noexcept {
+    return rpp::make_tensor_batch(InterleavedIterator<T>(data, num_objects), layouts::InterleavedLayout{}, basis,
+                                  min_degree, max_degree);
+}
 
 namespace strategies {
 
@@ -108,25 +223,55 @@ inline Error<const char*> to_error(int rc) {
     return Error<const char*>(static_cast<ErrorCode>(rc), rpp_hip_last_error());
 }
 
+// Step between consecutive objects of a batch, in elements.
 template <typename Layout>
 int64_t batch_stride_of(Layout const& layout) {
-    if constexpr (std::is_same_v<Layout, layouts::NoStrideLayout>) {
+    if constexpr (std::is_same_v<Layout, rpp::layouts::NoStrideLayout> ||
+                  std::is_same_v<Layout, hip::layouts::InterleavedLayout>) {
         return 1;
     } else {
         return static_cast<int64_t>(layout.stride_);
     }
 }
 
+template <typename Data>
+struct is_strided : std::false_type {};
+template <typename T>
+struct is_strided<InterleavedIterator<T>> : std::true_type {};
+
+// The device pointer of a batch's data (a raw pointer, or the base of an
+// InterleavedIterator) and the step between consecutive coefficients of one
+// object (1, or the iterator's stride).
+template <typename Data>
+auto base_pointer_of(Data const& data) {
+    if constexpr (is_strided<Data>::value) {
+        return data.base();
+    } else {
+        return data;
+    }
+}
+template <typename Data>
+int64_t coeff_stride_of(Data const& data) {
+    if constexpr (is_strided<Data>::value) {
+        return static_cast<int64_t>(data.stride());
+    } else {
+        return 1;
+    }
+}
+
 template <typename Accum, typename Batch>
 rpp_hip_tensor_batch tensor_of(Batch const& b) {
     using Data = std::decay_t<decltype(b.data())>;
-    static_assert(std::is_pointer_v<Data>, "hip::block batches hold raw device pointers");
-    static_assert(std::is_same_v<std::remove_cv_t<std::remove_pointer_t<Data>>, Accum>,
+    using Ptr = decltype(base_pointer_of(std::declval<Data>()));
+    static_assert(std::is_pointer_v<Ptr>,
+                  "hip::block batches hold raw device pointers (or InterleavedIterators over them)");
+    static_assert(std::is_same_v<std::remove_cv_t<std::remove_pointer_t<Ptr>>, Accum>,
                   "hip::block needs Scalar == Accum");
     auto const& md = b.metadata();
-    return rpp_hip_tensor_batch{const_cast<void*>(static_cast<const void*>(b.data())),
-                                batch_stride_of(std::decay_t<decltype(b.layout())>(b.layout())), 1,
-                                static_cast<int32_t>(std::get<1>(md)), static_cast<int32_t>(std::get<2>(md))};
+    return rpp_hip_tensor_batch{const_cast<void*>(static_cast<const void*>(base_pointer_of(b.data()))),
+                                batch_stride_of(std::decay_t<decltype(b.layout())>(b.layout())),
+                                coeff_stride_of(b.data()), static_cast<int32_t>(std::get<1>(md)),
+                                static_cast<int32_t>(std::get<2>(md))};
 }
 
 template <typename Bases>
@@ -357,9 +502,9 @@ rpp_hip_vector_batch vector_of(Batch const& b, Basis const& basis) {
     const auto mx = std::get<2>(md);
     const int64_t begin = static_cast<int64_t>(basis.start_of_degree(mn));
     const int64_t end = static_cast<int64_t>(basis.end_of_degree(mx));
-    return rpp_hip_vector_batch{const_cast<void*>(static_cast<const void*>(b.data())),
-                                hip::detail::batch_stride_of(std::decay_t<decltype(b.layout())>(b.layout())), 1,
-                                begin, end - begin};
+    return rpp_hip_vector_batch{const_cast<void*>(static_cast<const void*>(hip::detail::base_pointer_of(b.data()))),
+                                hip::detail::batch_stride_of(std::decay_t<decltype(b.layout())>(b.layout())),
+                                hip::detail::coeff_stride_of(b.data()), begin, end - begin};
 }
 
 template <typename Accum, typename Matrix>

@@ -129,6 +129,24 @@ void run(int width, int depth, Index n) {
         if (!err) std::printf("ft_mul error: %s\n", err.message().data());
         report(tag("ft_mul"), o.host(), h);
     }
+    {  // ft_mul on batch-interleaved batches (rpp::hip::layouts::InterleavedLayout)
+        std::vector<T> h = nanv, li(lhs.size()), ri(rhs.size());
+        for (Index bb = 0; bb < n; ++bb)
+            for (Index i = 0; i < dim; ++i) {
+                li[i * n + bb] = lhs[bb * dim + i];
+                ri[i * n + bb] = rhs[bb * dim + i];
+            }
+        (void)rpp::ops::ft_mul(cpu, {}, hb(h.data(), 0, D), hcb(lhs.data(), 0, D), hcb(rhs.data(), 1, D), basis, n,
+                               T(0.5));
+        Dev<T> o(nanv), l(li), r(ri);
+        auto il = [&](T* p, Degree mn, Degree mx) { return rpp::hip::make_interleaved_tensor_batch(p, n, basis, mn, mx); };
+        auto err = rpp::ops::ft_mul(gpu, {}, il(o.p, 0, D), il(l.p, 0, D), il(r.p, 1, D), basis, n, T(0.5));
+        if (!err) std::printf("ft_mul (interleaved) error: %s\n", err.message().data());
+        std::vector<T> got = o.host(), back(got.size());
+        for (Index bb = 0; bb < n; ++bb)
+            for (Index i = 0; i < dim; ++i) back[bb * dim + i] = got[i * n + bb];
+        report(tag("ft_mul (interleaved)"), back, h);
+    }
     {  // ft_fma
         std::vector<T> h = nanv;
         (void)rpp::ops::ft_fma(cpu, {}, hb(h.data(), 0, D), hcb(add.data(), 0, 1), hcb(lhs.data(), 0, D),

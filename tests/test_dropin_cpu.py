@@ -42,3 +42,22 @@ def test_dropin_check_compiles_and_links(tmp_path):
         for sfx in ("f32", "f64"):
             assert f"rpp_hip_{op}_{sfx}" in wanted, f"the drop-in layer does not call rpp_hip_{op}_{sfx}"
     assert wanted <= exported, f"symbols the drop-in calls but the library lacks: {sorted(wanted - exported)}"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference headers not present on this machine")
+def test_dropin_descriptors(tmp_path):
+    """The drop-in layer's translation, run on the CPU: the reference's own
+    free functions called with hip::block on object-major and batch-
+    interleaved (rpp::hip::layouts::InterleavedLayout) batches hand the C-ABI
+    exactly the descriptors the batches describe: pointers, batch and
+    coefficient strides, degree windows, scalars, matrices, the stream; a
+    failing entry point comes back as the matching rpp::Error.  The C-ABI is
+    replaced by recorders in tests/dropin/descriptor_check.cpp, so nothing is
+    launched."""
+    src = os.path.join(ROOT, "tests", "dropin", "descriptor_check.cpp")
+    exe = str(tmp_path / "descriptor_check")
+    r = subprocess.run(["g++", "-O1", "-std=c++17", "-I" + REF, "-I" + os.path.join(ROOT, "include"), "-o", exe, src],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-4000:]
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "DESCRIPTORS OK" in r.stdout, r.stdout[-4000:]
