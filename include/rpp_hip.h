@@ -60,6 +60,16 @@ typedef struct rpp_hip_tensor_batch {
     int32_t max_degree;
 } rpp_hip_tensor_batch;
 
+/* Aliasing.  Two views "share memory" when some element the one touches
+ * (its window, over all num_batches objects) is an element the other
+ * touches.  Where an entry point below lets an output be an input, it must be
+ * the same batch (same data, batch_stride and coeff_stride); every other
+ * shared element is rejected with RPP_HIP_INVALID_ARGUMENT before any launch.
+ * The test is exact for two object-major or two batch-interleaved views with
+ * equal strides (for example disjoint column slices of one buffer), and
+ * conservative otherwise: views with different strides whose byte ranges
+ * intersect count as sharing memory. */
+
 /* A batch of ScalarView (views/scalar_view.hpp:17-60): object b at
  * data[b * stride]; make_scalar_batch uses stride 1. */
 typedef struct rpp_hip_scalar_batch {
@@ -70,7 +80,8 @@ typedef struct rpp_hip_scalar_batch {
 /* ---------------------------------------------------------------- products */
 
 /* rpp::ops::ft_mul (operations/basic/ft_mul.hpp:53-79):
- *   out = beta * lhs (x) rhs on out's window; out may alias lhs or rhs. */
+ *   out = beta * lhs (x) rhs on out's window; out may be lhs or rhs (the same
+ *   batch) and shares memory with neither otherwise. */
 typedef struct rpp_hip_ft_mul_desc {
     rpp_hip_basis basis;
     int64_t num_batches;
@@ -131,7 +142,7 @@ int rpp_hip_ft_log_f32(const rpp_hip_ft_unary_desc* desc, void* stream);
 int rpp_hip_ft_log_f64(const rpp_hip_ft_unary_desc* desc, void* stream);
 
 /* rpp::ops::ft_fmexp (operations/intermediate/ft_fmexp.hpp:96-121):
- *   out = multiplier (x) exp(exponent_{>=1}); out must not alias multiplier.
+ *   out = multiplier (x) exp(exponent_{>=1}); out shares memory with neither input.
  *   One kernel per call. */
 typedef struct rpp_hip_ft_fmexp_desc {
     rpp_hip_basis basis;
@@ -189,7 +200,7 @@ int rpp_hip_ft_fmexp_adjoint_f64(const rpp_hip_ft_fmexp_adjoint_desc* desc, void
  * full window [0, depth]; G's window is zero-extended), write
  * grad_arg = d loss / d arg on arg's window (degree 0 gets 0: ft_log reads
  * arg from degree 1).  grad_arg's window must equal arg's and it must not
- * alias arg or the cotangent.  Uses a library workspace of about
+ * share memory with arg or the cotangent.  Uses a library workspace of about
  * (3 + 1/(width-1)) * dim coefficients per object.  No reference counterpart
  * (the reference has no backward for ft_log). */
 typedef struct rpp_hip_ft_log_adjoint_desc {
@@ -204,8 +215,8 @@ int rpp_hip_ft_log_adjoint_f64(const rpp_hip_ft_log_adjoint_desc* desc, void* st
  * (operations/basic/tensor_antipode.hpp:15-39, tensor_reflect.hpp:15-39;
  * TensorGeneralisedAntipode, tensor_generalised_antipode.hpp:17-81): on each
  * degree k of out's window, out[reverse(w)] = (-1)^k arg[w] (antipode) or
- * arg[w] (reflect); 0 where arg's window lacks degree k.  out must not alias
- * arg. */
+ * arg[w] (reflect); 0 where arg's window lacks degree k.  out must not share
+ * memory with arg. */
 int rpp_hip_tensor_antipode_f32(const rpp_hip_ft_unary_desc* desc, void* stream);
 int rpp_hip_tensor_antipode_f64(const rpp_hip_ft_unary_desc* desc, void* stream);
 int rpp_hip_tensor_reflect_f32(const rpp_hip_ft_unary_desc* desc, void* stream);

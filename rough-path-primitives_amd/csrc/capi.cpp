@@ -11,6 +11,7 @@
 #include <cstring>
 #include <string>
 
+#include "alias.hpp"
 #include "common.hpp"
 #include "fmexp_coal.hpp"
 #include "ft_ops.hpp"
@@ -120,6 +121,50 @@ int check_count(int64_t n) {
 
 hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
+// ---------------------------------------------------------------- aliasing
+
+// The elements a tensor batch view's window covers (start: the basis's
+// degree offsets).
+ViewSet view_set(const rpp_hip_tensor_batch& t, const int64_t* start) {
+    if (t.min_degree > t.max_degree) return ViewSet{t.data, t.batch_stride, t.coeff_stride, 0, 0};
+    return ViewSet{t.data, t.batch_stride, t.coeff_stride, start[t.min_degree], start[t.max_degree + 1]};
+}
+
+bool same_view(const rpp_hip_tensor_batch& a, const rpp_hip_tensor_batch& b) {
+    return a.data == b.data && a.batch_stride == b.batch_stride && a.coeff_stride == b.coeff_stride;
+}
+
+ViewSet view_set(const rpp_hip_vector_batch& v) {
+    return ViewSet{v.data, v.batch_stride, v.coeff_stride, v.begin, v.begin + v.size};
+}
+
+template <typename T>
+int check_disjoint(const ViewSet& out, const ViewSet& in, int64_t n, const char* what) {
+    if (views_overlap(out, in, n, sizeof(T)))
+        return fail(RPP_HIP_INVALID_ARGUMENT, std::string(what) + " must not share memory");
+    return 0;
+}
+
+// An output that shares no element with an input.
+template <typename T>
+int check_disjoint(const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& in, const int64_t* start, int64_t n,
+                   const char* what) {
+    return check_disjoint<T>(view_set(out, start), view_set(in, start), n, what);
+}
+
+// An output that is either the very same batch as an input (data and
+// strides: the per-object in-place forms the kernels serve) or shares no
+// element with it.
+template <typename T>
+int check_same_or_disjoint(const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& in, const int64_t* start,
+                           int64_t n, const char* what) {
+    if (same_view(out, in)) return 0;
+    if (views_overlap(view_set(out, start), view_set(in, start), n, sizeof(T)))
+        return fail(RPP_HIP_INVALID_ARGUMENT,
+                    std::string(what) + " may share memory only as the same batch (same data and strides)");
+    return 0;
+}
+
 template <typename T>
 int ft_mul_impl(const rpp_hip_ft_mul_desc* d, void* s) {
     if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
@@ -128,6 +173,8 @@ int ft_mul_impl(const rpp_hip_ft_mul_desc* d, void* s) {
     RPP_TRY(check_view(d->lhs, d->basis, d->num_batches, "lhs", false));
     RPP_TRY(check_view(d->rhs, d->basis, d->num_batches, "rhs", false));
     const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    RPP_TRY(check_same_or_disjoint<T>(d->out, d->lhs, B.start, d->num_batches, "ft_mul: out and lhs"));
+    RPP_TRY(check_same_or_disjoint<T>(d->out, d->rhs, B.start, d->num_batches, "ft_mul: out and rhs"));
     return launch_product<T>(B, d->num_batches, d->out, d->lhs, d->rhs, nullptr, 0.0, d->beta, as_stream(s));
 }
 
@@ -140,6 +187,9 @@ int ft_fma_impl(const rpp_hip_ft_fma_desc* d, void* s) {
     RPP_TRY(check_view(d->b, d->basis, d->num_batches, "b", false));
     RPP_TRY(check_view(d->c, d->basis, d->num_batches, "c", false));
     const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    RPP_TRY(check_same_or_disjoint<T>(d->out, d->a, B.start, d->num_batches, "ft_fma: out and a"));
+    RPP_TRY(check_same_or_disjoint<T>(d->out, d->b, B.start, d->num_batches, "ft_fma: out and b"));
+    RPP_TRY(check_same_or_disjoint<T>(d->out, d->c, B.start, d->num_batches, "ft_fma: out and c"));
     return launch_product<T>(B, d->num_batches, d->out, d->b, d->c, &d->a, d->alpha, d->beta, as_stream(s));
 }
 
@@ -151,6 +201,8 @@ int ft_inplace_fma_impl(const rpp_hip_ft_inplace_fma_desc* d, void* s) {
     RPP_TRY(check_view(d->b, d->basis, d->num_batches, "b", false));
     RPP_TRY(check_view(d->c, d->basis, d->num_batches, "c", false));
     const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    RPP_TRY(check_same_or_disjoint<T>(d->a, d->b, B.start, d->num_batches, "ft_inplace_fma: a and b"));
+    RPP_TRY(check_same_or_disjoint<T>(d->a, d->c, B.start, d->num_batches, "ft_inplace_fma: a and c"));
     switch (d->variant) {
         case 231:  // a <- alpha*a + beta*b(x)c
             return launch_product<T>(B, d->num_batches, d->a, d->b, d->c, &d->a, d->alpha, d->beta,
@@ -172,6 +224,7 @@ int ft_inplace_mul_impl(const rpp_hip_ft_inplace_mul_desc* d, void* s) {
     RPP_TRY(check_view(d->lhs, d->basis, d->num_batches, "lhs", true));
     RPP_TRY(check_view(d->rhs, d->basis, d->num_batches, "rhs", false));
     const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    RPP_TRY(check_same_or_disjoint<T>(d->lhs, d->rhs, B.start, d->num_batches, "ft_inplace_mul: lhs and rhs"));
     return launch_product<T>(B, d->num_batches, d->lhs, d->lhs, d->rhs, nullptr, 0.0, d->beta, as_stream(s));
 }
 
@@ -181,10 +234,8 @@ int unary_impl(int op, const rpp_hip_ft_unary_desc* d, void* s) {
     RPP_TRY(check_basis(d->basis));
     RPP_TRY(check_view(d->out, d->basis, d->num_batches, "out", true));
     RPP_TRY(check_view(d->arg, d->basis, d->num_batches, "arg", false));
-    if (d->out.data == d->arg.data && d->out.min_degree <= d->out.max_degree &&
-        d->arg.min_degree <= d->arg.max_degree)
-        return fail(RPP_HIP_INVALID_ARGUMENT, "out must not alias arg");
     const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    RPP_TRY(check_disjoint<T>(d->out, d->arg, B.start, d->num_batches, "out and arg"));
     return launch_horner<T>(op, B, d->num_batches, d->out, d->arg, nullptr, as_stream(s));
 }
 
@@ -194,10 +245,8 @@ int antipode_impl(bool sign_by_degree, const rpp_hip_ft_unary_desc* d, void* s) 
     RPP_TRY(check_basis(d->basis));
     RPP_TRY(check_view(d->out, d->basis, d->num_batches, "out", true));
     RPP_TRY(check_view(d->arg, d->basis, d->num_batches, "arg", false));
-    if (d->out.data == d->arg.data && d->out.min_degree <= d->out.max_degree &&
-        d->arg.min_degree <= d->arg.max_degree)
-        return fail(RPP_HIP_INVALID_ARGUMENT, "out must not alias arg");
     const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    RPP_TRY(check_disjoint<T>(d->out, d->arg, B.start, d->num_batches, "out and arg"));
     return launch_antipode<T>(sign_by_degree, B, d->num_batches, d->out, d->arg, as_stream(s));
 }
 
@@ -208,11 +257,9 @@ int fmexp_impl(const rpp_hip_ft_fmexp_desc* d, void* s) {
     RPP_TRY(check_view(d->out, d->basis, d->num_batches, "out", true));
     RPP_TRY(check_view(d->multiplier, d->basis, d->num_batches, "multiplier", false));
     RPP_TRY(check_view(d->exponent, d->basis, d->num_batches, "exponent", false));
-    if (d->out.data == d->multiplier.data && d->out.min_degree <= d->out.max_degree)
-        return fail(RPP_HIP_INVALID_ARGUMENT, "ft_fmexp: out must not alias the multiplier");
-    if (d->out.data == d->exponent.data && d->out.min_degree <= d->out.max_degree)
-        return fail(RPP_HIP_INVALID_ARGUMENT, "ft_fmexp: out must not alias the exponent");
     const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    RPP_TRY(check_disjoint<T>(d->out, d->multiplier, B.start, d->num_batches, "ft_fmexp: out and the multiplier"));
+    RPP_TRY(check_disjoint<T>(d->out, d->exponent, B.start, d->num_batches, "ft_fmexp: out and the exponent"));
     return launch_horner<T>(kOpFmexp, B, d->num_batches, d->out, d->exponent, &d->multiplier, as_stream(s));
 }
 
@@ -228,9 +275,9 @@ int log_adjoint_impl(const rpp_hip_ft_log_adjoint_desc* d, void* s) {
     RPP_TRY(check_view(d->grad_arg, d->basis, n, "grad_arg", true));
     if (d->grad_arg.min_degree != d->arg.min_degree || d->grad_arg.max_degree != d->arg.max_degree)
         return fail(RPP_HIP_INVALID_ARGUMENT, "ft_log adjoint: grad_arg must have arg's window");
-    if (d->grad_arg.data == d->arg.data || d->grad_arg.data == d->cotangent.data)
-        return fail(RPP_HIP_INVALID_ARGUMENT, "ft_log adjoint: grad_arg must not alias an input");
     const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    RPP_TRY(check_disjoint<T>(d->grad_arg, d->arg, B.start, n, "ft_log adjoint: grad_arg and arg"));
+    RPP_TRY(check_disjoint<T>(d->grad_arg, d->cotangent, B.start, n, "ft_log adjoint: grad_arg and the cotangent"));
     return launch_log_adjoint<T>(B, n, d->grad_arg, d->arg, d->cotangent, as_stream(s));
 }
 
@@ -242,6 +289,9 @@ int pairing_impl(const rpp_hip_tensor_pairing_desc* d, void* s) {
     RPP_TRY(check_view(d->arg, d->basis, d->num_batches, "arg", false));
     if (d->num_batches > 0 && d->out.data == nullptr) return fail(RPP_HIP_INVALID_ARGUMENT, "out: null data");
     const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    const ViewSet out{d->out.data, d->out.stride, 1, 0, 1};
+    RPP_TRY(check_disjoint<T>(out, view_set(d->functional, B.start), d->num_batches, "tensor_pairing: out and the functional"));
+    RPP_TRY(check_disjoint<T>(out, view_set(d->arg, B.start), d->num_batches, "tensor_pairing: out and arg"));
     return launch_pairing<T>(B, d->num_batches, d->out, d->functional, d->arg, as_stream(s));
 }
 
@@ -252,9 +302,9 @@ int adjoint_impl(bool left, const rpp_hip_ft_adjoint_desc* d, void* s) {
     RPP_TRY(check_view(d->out, d->basis, d->num_batches, "out", true));
     RPP_TRY(check_view(d->op, d->basis, d->num_batches, "op", false));
     RPP_TRY(check_view(d->arg, d->basis, d->num_batches, "arg", false));
-    if ((d->out.data == d->op.data || d->out.data == d->arg.data) && d->out.min_degree <= d->out.max_degree)
-        return fail(RPP_HIP_INVALID_ARGUMENT, "adjoint multiplication: out must not alias an operand");
     const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    RPP_TRY(check_disjoint<T>(d->out, d->op, B.start, d->num_batches, "adjoint multiplication: out and op"));
+    RPP_TRY(check_disjoint<T>(d->out, d->arg, B.start, d->num_batches, "adjoint multiplication: out and arg"));
     return launch_adjoint<T>(left, B, d->num_batches, d->out, d->op, d->arg, as_stream(s));
 }
 
@@ -276,6 +326,8 @@ int spmv_impl(const rpp_hip_spmv_desc* d, void* s) {
     if (d->num_batches > 0 && (M.offsets == nullptr || (M.nnz > 0 && (!M.values || !M.indices))))
         return fail(RPP_HIP_INVALID_ARGUMENT, "null matrix arrays");
     if (d->num_batches == 0) return 0;
+    RPP_TRY(check_disjoint<T>(view_set(d->out), view_set(d->arg), d->num_batches,
+                              "sparse_matrix_vector_product: out and arg"));
     PreparedMatrix P;
     RPP_TRY(prepare_matrix<T>(M, &P, as_stream(s)));
     return launch_spmv<T>(d->num_batches, d->out, d->arg, P.csr, d->alpha, as_stream(s), 0, P.csc);
@@ -300,9 +352,6 @@ int lie_convert_impl(bool to_lie, const rpp_hip_lie_convert_desc* d, void* s) {
         return fail(RPP_HIP_INVALID_ARGUMENT, "lie: window must lie in [1, depth]");
     RPP_TRY(check_view(d->lie, d->basis, d->num_batches, "lie", to_lie));
     const int64_t n = d->num_batches;
-    const DeviceCsr* l2t = nullptr;
-    const DeviceCsr* t2l = nullptr;
-    RPP_TRY(device_lie_matrices<T>(d->basis.width, d->basis.depth, &l2t, &t2l));
     auto md = lie_metadata(d->basis.width, d->basis.depth);
     const DevBasis B = make_basis(d->basis.width, d->basis.depth);
     const std::vector<int64_t>& lb = md->degree_begin;  // Lie storage index of degree k's first key
@@ -310,6 +359,18 @@ int lie_convert_impl(bool to_lie, const rpp_hip_lie_convert_desc* d, void* s) {
     const rpp_hip_tensor_batch& arg = to_lie ? d->tensor : d->lie;
     auto ostart = [&](int k) { return to_lie ? lb[static_cast<size_t>(k)] : B.start[k]; };
     auto astart = [&](int k) { return to_lie ? B.start[k] : lb[static_cast<size_t>(k)]; };
+    {
+        auto set = [&](const rpp_hip_tensor_batch& t, bool lie) {
+            if (t.min_degree > t.max_degree) return ViewSet{t.data, t.batch_stride, t.coeff_stride, 0, 0};
+            const int64_t b0 = lie ? lb[static_cast<size_t>(t.min_degree)] : B.start[t.min_degree];
+            const int64_t b1 = lie ? lb[static_cast<size_t>(t.max_degree) + 1] : B.start[t.max_degree + 1];
+            return ViewSet{t.data, t.batch_stride, t.coeff_stride, b0, b1};
+        };
+        RPP_TRY(check_disjoint<T>(set(d->tensor, false), set(d->lie, true), n, "Lie conversion: tensor and lie"));
+    }
+    const DeviceCsr* l2t = nullptr;
+    const DeviceCsr* t2l = nullptr;
+    RPP_TRY(device_lie_matrices<T>(d->basis.width, d->basis.depth, &l2t, &t2l));
     const int dlo = std::max({d->tensor.min_degree, d->lie.min_degree, 1});
     const int dhi = std::min(d->tensor.max_degree, d->lie.max_degree);
     const hipStream_t stream = as_stream(s);
@@ -393,6 +454,7 @@ int lie_matrix_convert_impl(bool to_lie, const rpp_hip_lie_matrix_convert_desc* 
     const rpp_hip_vector_batch arg = to_lie ? tensor_vec(d->tensor) : lie_vec(d->lie);
     if (M.rows > out.size || M.cols > arg.size)
         return fail(RPP_HIP_INVALID_ARGUMENT, "matrix larger than the views it maps between");
+    RPP_TRY(check_disjoint<T>(view_set(out), view_set(arg), d->num_batches, "Lie conversion: tensor and lie"));
     PreparedMatrix P;
     RPP_TRY(prepare_matrix<T>(M, &P, stream));
     return launch_spmv<T>(d->num_batches, out, arg, P.csr, 1.0, stream, 0, P.csc);
@@ -407,10 +469,6 @@ int check_shuffle_basis(const rpp_hip_basis& b) {
     return 0;
 }
 
-bool overlaps(const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& in) {
-    return out.data == in.data && out.min_degree <= out.max_degree && in.min_degree <= in.max_degree;
-}
-
 template <typename T>
 int shuffle_impl(int op, const rpp_hip_basis& basis, int64_t n, const rpp_hip_tensor_batch& out,
                  const rpp_hip_tensor_batch& p, const rpp_hip_tensor_batch& q, const rpp_hip_tensor_batch* r,
@@ -421,12 +479,10 @@ int shuffle_impl(int op, const rpp_hip_basis& basis, int64_t n, const rpp_hip_te
     RPP_TRY(check_view(p, basis, n, "operand", false));
     RPP_TRY(check_view(q, basis, n, "operand", false));
     if (r) RPP_TRY(check_view(*r, basis, n, "addend", false));
-    if (overlaps(out, p) || overlaps(out, q))
-        return fail(RPP_HIP_INVALID_ARGUMENT, "shuffle products: the output must not share memory with the factors");
-    if (r && overlaps(out, *r) &&
-        (out.batch_stride != r->batch_stride || out.coeff_stride != r->coeff_stride))
-        return fail(RPP_HIP_INVALID_ARGUMENT, "st_fma: out may alias the addend only with the same strides");
     const DevBasis B = make_basis(basis.width, basis.depth);
+    RPP_TRY(check_disjoint<T>(out, p, B.start, n, "shuffle products: the output and a factor"));
+    RPP_TRY(check_disjoint<T>(out, q, B.start, n, "shuffle products: the output and a factor"));
+    if (r) RPP_TRY(check_same_or_disjoint<T>(out, *r, B.start, n, "st_fma: out and the addend"));
     return launch_shuffle<T>(op, B, n, out, p, q, r, alpha, beta, as_stream(s));
 }
 
@@ -523,16 +579,6 @@ IndexRange overlap(IndexRange a, IndexRange b) {
 
 bool nonempty(const rpp_hip_tensor_batch& t) { return t.min_degree <= t.max_degree; }
 
-// An output may be an input only as the very same batch (same data and
-// strides), which the element-wise kernels serve; any other overlap of the
-// two base pointers is rejected.
-int check_same_or_distinct(const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& in, const char* what) {
-    if (out.data == in.data && nonempty(out) && nonempty(in) &&
-        (out.batch_stride != in.batch_stride || out.coeff_stride != in.coeff_stride))
-        return fail(RPP_HIP_INVALID_ARGUMENT, std::string(what) + ": out may alias an operand only with the same strides");
-    return 0;
-}
-
 template <typename T>
 int vector_assign_impl(const rpp_hip_vector_assign_desc* d, void* s) {
     if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
@@ -542,7 +588,7 @@ int vector_assign_impl(const rpp_hip_vector_assign_desc* d, void* s) {
     const rpp_hip_basis b{d->basis.width, d->basis.depth};
     RPP_TRY(check_view(d->out, b, d->num_batches, "out", true));
     RPP_TRY(check_view(d->arg, b, d->num_batches, "arg", false));
-    RPP_TRY(check_same_or_distinct(d->out, d->arg, "vector_assign"));
+    RPP_TRY(check_same_or_disjoint<T>(d->out, d->arg, start, d->num_batches, "vector_assign: out and arg"));
     const IndexRange o = range_of(start, d->out);
     return launch_vector_op<T>(kVecAssign, d->num_batches, d->out, o, &d->arg, overlap(o, range_of(start, d->arg)),
                                nullptr, IndexRange{0, 0}, 0.0, 0.0, as_stream(s));
@@ -558,8 +604,8 @@ int vector_add_impl(const rpp_hip_vector_add_desc* d, void* s) {
     RPP_TRY(check_view(d->out, b, d->num_batches, "out", true));
     RPP_TRY(check_view(d->lhs, b, d->num_batches, "lhs", false));
     RPP_TRY(check_view(d->rhs, b, d->num_batches, "rhs", false));
-    RPP_TRY(check_same_or_distinct(d->out, d->lhs, "vector_add"));
-    RPP_TRY(check_same_or_distinct(d->out, d->rhs, "vector_add"));
+    RPP_TRY(check_same_or_disjoint<T>(d->out, d->lhs, start, d->num_batches, "vector_add: out and lhs"));
+    RPP_TRY(check_same_or_disjoint<T>(d->out, d->rhs, start, d->num_batches, "vector_add: out and rhs"));
     return launch_vector_op<T>(kVecAdd, d->num_batches, d->out, range_of(start, d->out), &d->lhs,
                                range_of(start, d->lhs), &d->rhs, range_of(start, d->rhs), d->alpha, d->beta,
                                as_stream(s));
@@ -574,7 +620,7 @@ int vector_inplace_add_impl(const rpp_hip_vector_inplace_add_desc* d, void* s) {
     const rpp_hip_basis b{d->basis.width, d->basis.depth};
     RPP_TRY(check_view(d->lhs, b, d->num_batches, "lhs", true));
     RPP_TRY(check_view(d->rhs, b, d->num_batches, "rhs", false));
-    RPP_TRY(check_same_or_distinct(d->lhs, d->rhs, "vector_inplace_add"));
+    RPP_TRY(check_same_or_disjoint<T>(d->lhs, d->rhs, start, d->num_batches, "vector_inplace_add: lhs and rhs"));
     if (!nonempty(d->lhs) || !nonempty(d->rhs)) return 0;
     const IndexRange o = overlap(range_of(start, d->lhs), range_of(start, d->rhs));
     return launch_vector_op<T>(kVecInplaceAdd, d->num_batches, d->lhs, o, &d->rhs, o, nullptr, IndexRange{0, 0},
@@ -839,14 +885,11 @@ int fmexp_adjoint_impl(const rpp_hip_ft_fmexp_adjoint_desc* d, void* s) {
         return fail(RPP_HIP_INVALID_ARGUMENT, "ft_fmexp adjoint: exponent windows must end at degree 1");
     if (d->grad_multiplier.min_degree != 0 || d->grad_multiplier.max_degree != d->basis.depth)
         return fail(RPP_HIP_INVALID_ARGUMENT, "ft_fmexp adjoint: grad_multiplier must hold [0, depth]");
-    const void* outs[] = {d->grad_multiplier.data, d->grad_exponent.data};
-    const void* ins[] = {d->multiplier.data, d->exponent.data, d->cotangent.data};
-    for (const void* o : outs)
-        for (const void* i : ins)
-            if (o == i) return fail(RPP_HIP_INVALID_ARGUMENT, "ft_fmexp adjoint: gradients must not alias inputs");
-    if (d->grad_multiplier.data == d->grad_exponent.data)
-        return fail(RPP_HIP_INVALID_ARGUMENT, "ft_fmexp adjoint: the two gradients must not alias");
     const DevBasis B = make_basis(d->basis.width, d->basis.depth);
+    for (const rpp_hip_tensor_batch* o : {&d->grad_multiplier, &d->grad_exponent})
+        for (const rpp_hip_tensor_batch* i : {&d->multiplier, &d->exponent, &d->cotangent})
+            RPP_TRY(check_disjoint<T>(*o, *i, B.start, n, "ft_fmexp adjoint: a gradient and an input"));
+    RPP_TRY(check_disjoint<T>(d->grad_multiplier, d->grad_exponent, B.start, n, "ft_fmexp adjoint: the two gradients"));
     hipStream_t stream = as_stream(s);
     const bool contig = d->multiplier.coeff_stride == 1 && d->cotangent.coeff_stride == 1 &&
                         d->grad_multiplier.coeff_stride == 1 && d->exponent.coeff_stride == 1;

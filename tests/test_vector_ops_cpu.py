@@ -188,7 +188,7 @@ def test_vector_entry_points_validate_on_host():
     # transposing in place is not supported
     d = A.VectorAssignDesc(g, 4, om, il)
     assert _call("rpp_hip_vector_assign_f64", d) == 3
-    assert b"alias" in rpp_hip.lib().rpp_hip_last_error()
+    assert b"share memory" in rpp_hip.lib().rpp_hip_last_error()
     # negative count
     d = A.VectorAddDesc(g, -1, om, om, om, 1.0, 1.0)
     assert _call("rpp_hip_vector_add_f64", d) == 3
