@@ -206,7 +206,9 @@ static void orc_vjp_work_init(orc_vjp_work* w, const orc_basis* B)
 {
     const long long dim = B->start[B->depth + 1];
     w->scratch = (double*)malloc(sizeof(double) * (size_t)(B->pow[B->depth] + 1));
-    w->e = (double*)calloc((size_t)dim, sizeof(double));
+    /* degree-one slots 1..width exist even at depth 0 (the exponent view
+     * (1,1) then lies outside the basis and is never read) */
+    w->e = (double*)calloc((size_t)(dim + B->width), sizeof(double));
     w->E = (double*)malloc(sizeof(double) * (size_t)dim);
     w->Eb = (double*)malloc(sizeof(double) * (size_t)dim);
     w->fact = (double*)malloc(sizeof(double) * (size_t)(B->depth + 1));

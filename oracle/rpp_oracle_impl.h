@@ -872,7 +872,7 @@ int FN(orc_signature)(int width, int depth, long long n_paths,
     T* scratch = (T*)malloc(sizeof(T) * (size_t)(B.pow[depth] + 1));
     T* a = (T*)malloc(sizeof(T) * (size_t)dim);
     T* b = (T*)malloc(sizeof(T) * (size_t)dim);
-    T* e = (T*)calloc((size_t)dim, sizeof(T));
+    T* e = (T*)calloc((size_t)(dim + width), sizeof(T)); /* slots 1..width even at depth 0 */
     for (long long p = 0; p < n_paths; ++p) {
         for (long long i = 0; i < dim; ++i) a[i] = (T)0;
         a[0] = (T)1;

@@ -92,11 +92,25 @@ int check_basis(const rpp_hip_basis& b) {
     if (b.width < 1 || b.width > 65535) return fail(RPP_HIP_INVALID_ARGUMENT, "width must be in [1, 65535]");
     if (b.depth < 0 || b.depth > kMaxDepth)
         return fail(RPP_HIP_INVALID_ARGUMENT, "depth must be in [0, 30]");
+    // width^depth, stopped as soon as it reaches 2^31 (so it cannot wrap)
     uint64_t p = 1;
-    for (int k = 0; k < b.depth; ++k) p *= static_cast<uint64_t>(b.width);
+    for (int k = 0; k < b.depth && p < (uint64_t{1} << 31); ++k) p *= static_cast<uint64_t>(b.width);
     if (p >= (uint64_t{1} << 31))
         return fail(RPP_HIP_OUT_OF_BOUNDS, "width^depth must be below 2^31 words per level");
     return 0;
+}
+
+// A batch view's largest element offset, (n - 1) batch_stride +
+// (end - 1) coeff_stride, must stay below 2^59 elements: then every address
+// the kernels form (in bytes of a double) stays below 2^62, and no offset
+// arithmetic on the host or the device can wrap.
+bool extent_fits(int64_t n, int64_t bs, int64_t cs, int64_t end) {
+    if (n <= 0 || end <= 0) return true;
+    int64_t a, b, c;
+    if (__builtin_mul_overflow(n - 1, bs, &a) || __builtin_mul_overflow(end - 1, cs, &b) ||
+        __builtin_add_overflow(a, b, &c))
+        return false;
+    return c >= 0 && c < (int64_t{1} << 59);
 }
 
 int check_view(const rpp_hip_tensor_batch& t, const rpp_hip_basis& b, int64_t n, const char* name,
@@ -111,6 +125,13 @@ int check_view(const rpp_hip_tensor_batch& t, const rpp_hip_basis& b, int64_t n,
         return fail(RPP_HIP_INVALID_ARGUMENT, std::string(name) + ": an output cannot be broadcast");
     if (n > 0 && t.min_degree <= t.max_degree && t.data == nullptr)
         return fail(RPP_HIP_INVALID_ARGUMENT, std::string(name) + ": null data");
+    if (t.min_degree <= t.max_degree) {
+        // one past the window's last coefficient (width^k < 2^31 per level)
+        int64_t end = 0, p = 1;
+        for (int k = 0; k <= t.max_degree; ++k, p *= b.width) end += p;
+        if (!extent_fits(n, t.batch_stride, t.coeff_stride, end))
+            return fail(RPP_HIP_OUT_OF_BOUNDS, std::string(name) + ": the view extends beyond 2^59 elements");
+    }
     return 0;
 }
 
@@ -285,9 +306,14 @@ template <typename T>
 int pairing_impl(const rpp_hip_tensor_pairing_desc* d, void* s) {
     if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
     RPP_TRY(check_basis(d->basis));
+    RPP_TRY(check_count(d->num_batches));
     RPP_TRY(check_view(d->functional, d->basis, d->num_batches, "functional", false));
     RPP_TRY(check_view(d->arg, d->basis, d->num_batches, "arg", false));
     if (d->num_batches > 0 && d->out.data == nullptr) return fail(RPP_HIP_INVALID_ARGUMENT, "out: null data");
+    if (d->out.stride < 0 || (d->num_batches > 1 && d->out.stride == 0))
+        return fail(RPP_HIP_INVALID_ARGUMENT, "out: the stride must be >= 1 (an output cannot be broadcast)");
+    if (!extent_fits(d->num_batches, d->out.stride, 1, 1))
+        return fail(RPP_HIP_OUT_OF_BOUNDS, "out: the batch extends beyond 2^59 elements");
     const DevBasis B = make_basis(d->basis.width, d->basis.depth);
     const ViewSet out{d->out.data, d->out.stride, 1, 0, 1};
     RPP_TRY(check_disjoint<T>(out, view_set(d->functional, B.start), d->num_batches, "tensor_pairing: out and the functional"));
@@ -308,17 +334,21 @@ int adjoint_impl(bool left, const rpp_hip_ft_adjoint_desc* d, void* s) {
     return launch_adjoint<T>(left, B, d->num_batches, d->out, d->op, d->arg, as_stream(s));
 }
 
-int check_vec(const rpp_hip_vector_batch& v, const char* name) {
+int check_vec(const rpp_hip_vector_batch& v, int64_t n, const char* name) {
     if (v.begin < 0 || v.size < 0 || v.coeff_stride < 1 || v.batch_stride < 0)
         return fail(RPP_HIP_INVALID_ARGUMENT, std::string(name) + ": invalid vector batch");
+    int64_t end;
+    if (__builtin_add_overflow(v.begin, v.size, &end) || !extent_fits(n, v.batch_stride, v.coeff_stride, end))
+        return fail(RPP_HIP_OUT_OF_BOUNDS, std::string(name) + ": the view extends beyond 2^59 elements");
     return 0;
 }
 
 template <typename T>
 int spmv_impl(const rpp_hip_spmv_desc* d, void* s) {
     if (!d) return fail(RPP_HIP_INVALID_ARGUMENT, "null descriptor");
-    RPP_TRY(check_vec(d->out, "out"));
-    RPP_TRY(check_vec(d->arg, "arg"));
+    RPP_TRY(check_count(d->num_batches));
+    RPP_TRY(check_vec(d->out, d->num_batches, "out"));
+    RPP_TRY(check_vec(d->arg, d->num_batches, "arg"));
     const auto& M = d->matrix;
     if (M.format != RPP_HIP_CSR && M.format != RPP_HIP_CSC)
         return fail(RPP_HIP_INVALID_ARGUMENT, "matrix format must be CSR or CSC");
@@ -1012,10 +1042,12 @@ int32_t rpp_hip_signature_checkpoint_every(int32_t dtype_bytes, int64_t max_leng
     if (dtype_bytes != 4 && dtype_bytes != 8)
         return -rpp_hip::fail(RPP_HIP_INVALID_ARGUMENT, "dtype_bytes must be 4 or 8");
     if (max_length < 0) return -rpp_hip::fail(RPP_HIP_INVALID_ARGUMENT, "negative max_length");
+    if (max_length > (int64_t{1} << 60)) return -rpp_hip::fail(RPP_HIP_OUT_OF_BOUNDS, "max_length above 2^60");
     if (dtype_bytes == 8 || max_length < rpp_hip::kFp32CheckpointFrom) return 0;
-    int32_t k = static_cast<int32_t>(std::sqrt(static_cast<double>(max_length)));
-    while (static_cast<int64_t>(k) * k < max_length) ++k;
-    return k;
+    int64_t k = static_cast<int64_t>(std::sqrt(static_cast<double>(max_length)));
+    while (k > 1 && (k - 1) * (k - 1) >= max_length) --k;
+    while (k * k < max_length) ++k;
+    return static_cast<int32_t>(k);
 }
 
 int rpp_hip_signature_backward_noalloc_f32(const rpp_hip_signature_backward_desc* d, void* s) {

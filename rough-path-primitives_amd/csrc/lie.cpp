@@ -388,13 +388,20 @@ std::shared_ptr<const LieMetadata> load_lie_metadata(const std::string& path, st
     md->true_size = sizes[0];
     md->tensor_dim = sizes[1];
     if (md->width < 1 || md->depth < 0 || md->depth > kMaxDepth || md->true_size < 1) return bad("bad shape");
-    int64_t dim = 0, p = 1;
-    for (int k = 0; k <= md->depth; ++k, p *= md->width) {
-        if (p > (int64_t{1} << 40)) return bad("bad shape");
+    int64_t dim = 0, p = 1;  // p = width^k, at most 2^40
+    for (int k = 0; k <= md->depth; ++k) {
         dim += p;
+        if (k < md->depth) {
+            if (p > (int64_t{1} << 40) / md->width) return bad("bad shape");
+            p *= md->width;
+        }
     }
     if (md->tensor_dim != dim) return bad("tensor dimension does not match (width, depth)");
     const int64_t ts = md->true_size;
+    // every section holds at least its count of 8-byte entries, so counts
+    // beyond the file are refused before any arithmetic on them
+    const int64_t cap = static_cast<int64_t>(r.bytes.size() / 8);
+    if (ts > cap / 2 || dim >= cap) return bad("section sizes do not match the header");
     bool ok = r.vec(md->pairs, 2 * ts) && r.vec(md->degree_begin, md->depth + 2) &&
               r.vec(md->l2t_csc_values, sizes[2]) && r.vec(md->l2t_csc_indices, sizes[2]) &&
               r.vec(md->l2t_csc_offsets, ts) && r.vec(md->t2l_csc_values, sizes[3]) &&
@@ -410,7 +417,11 @@ std::shared_ptr<const LieMetadata> load_lie_metadata(const std::string& path, st
         return bad("matrix structure invalid");
     // both matrices are block diagonal by degree, which the conversions rely on
     std::vector<int64_t> tstart(static_cast<size_t>(md->depth) + 2, 0);
-    for (int k = 1, pk = 1; k <= md->depth + 1; ++k, pk *= md->width) tstart[k] = tstart[k - 1] + pk;
+    int64_t pk = 1;
+    for (int k = 1; k <= md->depth + 1; ++k) {
+        tstart[k] = tstart[k - 1] + pk;
+        if (k <= md->depth) pk *= md->width;
+    }
     auto degree_in = [](const std::vector<int64_t>& begin, int64_t i) {
         return static_cast<int>(std::upper_bound(begin.begin(), begin.end(), i) - begin.begin()) - 1;
     };

@@ -171,3 +171,31 @@ def test_alias_predicate_matches_brute_force(tmp_path):
     subprocess.run(["g++", "-O2", "-std=c++17", "-I" + inc, "-o", exe, src], check=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "ALIAS OK" in r.stdout, r.stdout + r.stderr
+
+
+def test_oversized_shapes_and_views_refused_on_host():
+    """width^depth is bounded without wrapping (256^8 = 2^64 wraps to 0 in
+    64-bit arithmetic), view extents must fit below 2^59 elements, and a
+    pairing output cannot be broadcast."""
+    tb = A.TensorBatch(0x100000, 13, 1, 0, 2)
+    for w, d in ((256, 8), (65535, 30), (2, 31), (65536, 2)):
+        rc = _call("rpp_hip_ft_mul_f64", A.FtMulDesc(A.Basis(w, d), 1, tb, tb, tb, 1.0))
+        assert rc in (3, 7), (w, d, rc)
+    assert _call("rpp_hip_ft_mul_f64", A.FtMulDesc(A.Basis(256, 8), 1, tb, tb, tb, 1.0)) == 7
+    huge = A.TensorBatch(0x100000, 1 << 40, 1, 0, 2)
+    rc = _call("rpp_hip_ft_log_f64", A.FtUnaryDesc(A.Basis(3, 2), 1 << 20, huge, A.TensorBatch(0x200000, 13, 1, 0, 2)))
+    assert rc == 7 and b"2^59" in rpp_hip.lib().rpp_hip_last_error()
+    v = A.VectorBatch(0x100000, 1 << 62, 1, 0, 4)
+    M = A.SparseMatrix(0, 0x10, 0x20, 0x30, 0, 4, 4)
+    assert _call("rpp_hip_sparse_matrix_vector_product_f64", A.SpmvDesc(4, v, A.VectorBatch(0x900000, 4, 1, 0, 4),
+                                                                         M, 1.0)) == 7
+    d = A.PairingDesc(A.Basis(3, 2), 4, A.ScalarBatch(0x100000, 0), tb, A.TensorBatch(0x200000, 13, 1, 0, 2))
+    assert _call("rpp_hip_tensor_pairing_f64", d) == 3
+
+
+def test_checkpoint_policy_bounds():
+    lib = rpp_hip.lib()
+    assert lib.rpp_hip_signature_checkpoint_every(4, 1 << 60) == 1 << 30
+    assert lib.rpp_hip_signature_checkpoint_every(4, (1 << 60) + 1) < 0
+    assert lib.rpp_hip_signature_checkpoint_every(4, 1025) == 33
+    assert lib.rpp_hip_signature_checkpoint_every(4, 1024) == 32

@@ -126,6 +126,14 @@ def test_damaged_or_inconsistent_files_are_refused(tmp_path):
     # the last T2L entry belongs to a top-degree word: point it at a degree-1 key
     struct.pack_into("<q", body, lo["t2l_indices"] + 8 * (lo["t2l_nnz"] - 1), 0)
     cases["entry across degrees"] = reseal(bytes(body))
+    # counts beyond the file (2 * true_size would overflow 64 bits) and a
+    # width whose powers overflow
+    body = bytearray(data[:-8])
+    struct.pack_into("<q", body, 16, (1 << 62) + 3)
+    cases["huge true_size"] = reseal(bytes(body))
+    body = bytearray(data[:-8])
+    struct.pack_into("<i", body, 8, (1 << 31) - 1)
+    cases["huge width"] = reseal(bytes(body))
     for name, blob in cases.items():
         p = tmp_path / (name.replace(" ", "_") + ".bin")
         p.write_bytes(blob)
