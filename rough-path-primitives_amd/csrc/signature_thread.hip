@@ -215,34 +215,106 @@ int thread_fwd_launch(const DevBasis& B, const rpp_hip_signature_desc& d, const 
 //     Gbar[u] = sum_v G[u v] exp(x)[v]   and   xbar = d <G, S (x) exp(x)> / dx.
 // Reversing the Horner recursion of every target word w (level K): Abar_K =
 // G[w], and for i = K..1, xbar[w_i] += Abar_i A_{i-1} / (K-i+1),
-// Gbar[w_1..w_i] += Abar_i, Abar_{i-1} = Abar_i x[w_i] / (K-i+1).  A thread
-// takes the targets of its subtree and the chain words it owns (q with the
-// remaining letters 0) and walks them down to level 1 on its own data:
-//   - S_{t-1} is rebuilt from S_t by a Chen step with -x (on levels below N
-//     only: level N of S never enters the backward step),
-//   - the Horner values A come from S_{t-1} and x,
-//   - Gbar of its subtree words is complete in the thread (every extension
-//     of a subtree word lies in the same subtree) and replaces G in place,
-//   - Gbar of a chain word (levels 1..L under a) collects one sum from every
-//     thread below it: each thread writes its L sums to LDS and the block
-//     sums over groups of W^(L-i) consecutive threads in L - 1 stages of W
-//     terms, a fixed order; the result is the item's low-level cotangent for
-//     the next step,
-//   - xbar is summed within the work-group in a fixed order and written per
-//     (increment, item); a second kernel adds the W items of each increment.
-// With the S_0 cotangent unused (S_0 is the constant 1 along a path) no value
-// crosses work-groups.  Padding threads (r >= W^(L-1)) contribute nothing.
+// Gbar[w_1..w_i] += Abar_i, Abar_{i-1} = Abar_i x[w_i] / (K-i+1).
+//
+// Levels >= L, thread-owned.  A thread rebuilds S_{t-1} below level N from
+// S_t by a Chen step with -x (level N of S never enters the backward step),
+// forms the Horner values along q and through its subtree, and walks the
+// targets of its subtree and of its own chain word q down to level L:
+//   - Gbar of its subtree words is complete in the thread (every extension of
+//     a subtree word lies in the same subtree) and replaces G in place;
+//   - Gbar of q (level L), the sum of those targets' Abar_L, is complete too;
+//   - its share of xbar (subtree letters and q's last letter) is summed over
+//     the work-group;
+//   - Abar^(K)_{L-1}, one value per target level K, goes on to level L - 1.
+// Levels < L, shared.  A chain word p of level i < L has W children of level
+// i + 1; its walk sums their Abar^(K)_i, adds its own target (K = i, Abar =
+// G[p]), and goes one level down.  Every such word has an executor thread,
+// and level i's walk of a step runs L - i iterations after that step's
+// thread-owned part, so the walks of all levels and the thread-owned work of
+// later steps overlap, with one barrier per iteration handing each level its
+// inputs.  The Horner values a walk needs are saved, when its step is
+// computed, by the thread whose prefix continues p with zeros.  A step's xbar
+// is complete after its level-1 walk and is then written, summed in a fixed
+// order.  With the S_0 cotangent unused (S_0 is the constant 1 along a path)
+// no value crosses work-groups.  Padding threads (r >= W^(L-1)) own no prefix
+// and contribute nothing, but may execute walks.
 template <int W, int N, int L>
 struct BwdGeo {
     static constexpr int R = tpow(W, L - 1);        // threads with a prefix
     static constexpr int NT = (R + 63) / 64 * 64;  // padded to whole wavefronts
-    // offset of level i (1..L) in the low-level block under a
-    static constexpr int low_offset(int i) {
+    static constexpr int NWV = NT / 64;
+    static constexpr int count(int i) { return tpow(W, i - 1); }  // chain words of level i under a
+    // level i's walk executors: threads [exec(i), exec(i) + count(i)); levels
+    // L-1 .. 1 in turn, each from a new wavefront while that fits
+    static constexpr int exec(int i) {
+        int b = 0;
+        for (int j = L - 1; j > i; --j) {
+            const int end = b + count(j);
+            const int up = (end + 63) / 64 * 64;
+            b = (up + count(j - 1) <= NT) ? up : end;
+        }
+        return b;
+    }
+    // inputs of level i's walks: Abar^(K)_i, K = i+1..N, of each level-(i+1) word
+    static constexpr int vin(int i) { return N - i; }
+    static constexpr int voff(int i) {
         int o = 0;
-        for (int j = 1; j < i; ++j) o += tpow(W, j - 1);
+        for (int j = 1; j < i; ++j) o += count(j + 1) * vin(j);
         return o;
     }
-    static constexpr int low_total = low_offset(L + 1);
+    static constexpr int vtotal = voff(L) > 0 ? voff(L) : 1;
+    // saved for level i's walk of a step: A^(K)_{i-1}, K = i..N, and x[p_i]
+    static constexpr int cin(int i) { return N - i + 2; }
+    static constexpr int coff(int i) {
+        int o = 0;
+        for (int j = 1; j < i; ++j) o += count(j) * cin(j);
+        return o;
+    }
+    static constexpr int ctotal = coff(L) > 0 ? coff(L) : 1;
+    // a step's xbar share from each chain word of levels < L
+    static constexpr int xoff(int i) {
+        int o = 0;
+        for (int j = 1; j < i; ++j) o += count(j);
+        return o;
+    }
+    static constexpr int xtotal = xoff(L) > 0 ? xoff(L) : 1;
+};
+
+// Sums W per-lane values over the 64 lanes of a wavefront.  The values sit in
+// P = 2^LP >= W slots; in each of the first LP exchanges the lanes that differ
+// in one lane bit trade the half of their slots the other lane keeps, so each
+// lane goes on with half as many partial sums, and butterflies over the
+// remaining lane bits finish them.  Lane l then holds the total of slot
+// l >> (6 - LP).  A fixed order, so the sums are deterministic.
+template <typename T, int W>
+struct WaveSlots {
+    static constexpr int LP = W <= 1 ? 0 : W <= 2 ? 1 : W <= 4 ? 2 : W <= 8 ? 3 : W <= 16 ? 4 : W <= 32 ? 5 : 6;
+    static constexpr int P = 1 << LP;
+    static_assert(W <= 64, "at most one slot per lane");
+    __device__ static __forceinline__ T sum(const T (&v)[W], int lane) {
+        T s[P];
+#pragma unroll
+        for (int b = 0; b < P; ++b) s[b] = b < W ? v[b] : T(0);
+        static_for<LP>([&](auto H) {
+            constexpr int o = 32 >> decltype(H)::value;
+            constexpr int half = (P >> decltype(H)::value) / 2;
+            const bool up = (lane & o) != 0;
+            static_for<half>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                const T send = up ? s[j] : s[j + half];
+                const T keep = up ? s[j + half] : s[j];
+                s[j] = keep + __shfl_xor(send, o, 64);
+            });
+        });
+        static_for<6 - LP>([&](auto H) {
+            constexpr int o = 32 >> (LP + decltype(H)::value);
+            s[0] += __shfl_xor(s[0], o, 64);
+        });
+        return s[0];
+    }
+    __device__ static __forceinline__ int slot(int lane) { return lane >> (6 - LP); }
+    __device__ static __forceinline__ bool leader(int lane) { return (lane & ((1 << (6 - LP)) - 1)) == 0; }
 };
 
 template <typename T, int W, int N, int L, int NT = BwdGeo<W, N, L>::NT>
@@ -254,23 +326,28 @@ k_sig_bwd_thread(DevBasis B, int64_t n_items, const int64_t* order, const T* inc
     using GS = ThreadGeo<W, N - 1, L>;  // S below level N
     using GG = ThreadGeo<W, N, L>;      // G, and the Horner values
     using BG = BwdGeo<W, N, L>;
+    using WS = WaveSlots<T, W>;
     constexpr int R = BG::R;
-    constexpr int NWV = NT / 64;
-    __shared__ T glow[BG::low_total];
-    __shared__ T cbuf[2][L > 1 ? L - 1 : 1][NT];
-    __shared__ T red[NWV][W];
+    constexpr int NWV = BG::NWV;
+    static_assert(L == 1 || BG::exec(1) + 1 <= NT, "walk executors fit in the work-group");
+    __shared__ T vbuf[2][BG::vtotal];        // walk inputs, by iteration parity
+    __shared__ T cbuf[L][BG::ctotal];        // saved Horner values, by step mod L
+    __shared__ T xsb[L + 1][BG::xtotal];     // walks' xbar shares, by step mod (L + 1)
+    __shared__ T red[L + 1][NWV][W];         // wavefront xbar sums, by step mod (L + 1)
     const int64_t item = blockIdx.x;
     if (item >= n_items) return;
     const int64_t slot = item / W;
     const int a = static_cast<int>(item - slot * W);
     const int64_t path = order ? order[slot] : slot;
     const int t = threadIdx.x;
+    const int lane = t & 63;
     const bool live = t < R;
     const int q = a * R + (live ? t : 0);
     int ql[L];
 #pragma unroll
     for (int i = 0; i < L; ++i) ql[i] = (q / tpow(W, L - 1 - i)) % W;
-    // S_t (below level N) and G_t
+    // S_t (below level N) and G_t: the thread's subtree, its chain copy and
+    // the cotangent of its chain word q
     T chain[L + 1];
     T subS[GS::total];
     T subG[GG::total];
@@ -288,206 +365,238 @@ k_sig_bwd_thread(DevBasis B, int64_t n_items, const int64_t* order, const T* inc
 #pragma unroll
         for (int rr = 0; rr < GG::words(i); ++rr)
             subG[GG::offset(i) + rr] = g0[(B.start[i] + static_cast<int64_t>(q) * GG::words(i) + rr) * cot.cs];
-    for (int j = t; j < BG::low_total; j += NT) {
-        int lvl = 1;
+    T gL = g0[(B.start[L] + q) * cot.cs];
+    // the walk this thread executes, if any: level ex, chain word ej under a,
+    // and that word's cotangent
+    int ex = 0, ej = 0;
 #pragma unroll
-        for (int i = 2; i <= L; ++i)
-            if (j >= BG::low_offset(i)) lvl = i;
-        const int u = j - BG::low_offset(lvl);
-        glow[j] = g0[(B.start[lvl] + static_cast<int64_t>(a) * tpow(W, lvl - 1) + u) * cot.cs];
-    }
-    __syncthreads();
+    for (int i = 1; i < L; ++i)
+        if (t >= BG::exec(i) && t < BG::exec(i) + BG::count(i)) {
+            ex = i;
+            ej = t - BG::exec(i);
+        }
+    T gst = T(0);
+    if (ex > 0) gst = g0[(B.start[ex] + static_cast<int64_t>(a) * tpow(W, ex - 1) + ej) * cot.cs];
     const int64_t t0 = offs[path], t1 = offs[path + 1];
+    const int64_t len = t1 - t0;
     const int64_t dim = B.start[N + 1];
     // increments one step ahead: the row (uniform) and the prefix letters'
     T xn[W], xqn[L];
-    if (t0 < t1) {
+    if (len > 0) {
 #pragma unroll
         for (int b = 0; b < W; ++b) xn[b] = incr[(t1 - 1) * W + b];
+        const T* row = incr + (t1 - 1) * W;
 #pragma unroll
-        for (int i = 0; i < L; ++i) xqn[i] = incr[(t1 - 1) * W + ql[i]];
+        for (int i = 0; i < L; ++i) xqn[i] = row[ql[i]];
     }
-    for (int64_t tt = t1 - 1; tt >= t0; --tt) {
-        T x[W], xq[L], xneg[W], xqneg[L];
+    // iteration r: the thread-owned part of step t1-1-r (r < len), level i's
+    // walks of step t1-1-(r-(L-i)), and the xbar of step t1-1-(r-L)
+    // ring slots of iteration r: r mod 2, r mod L and r mod (L + 1), kept as
+    // counters
+    const int64_t iters = len > 0 ? len + L : 0;
+    int cslot = 0, xslot = 0;
+    for (int64_t r = 0; r < iters; ++r, cslot = (cslot + 1 == L ? 0 : cslot + 1),
+                 xslot = (xslot == L ? 0 : xslot + 1)) {
+        const int vslot = static_cast<int>(r & 1);
+        if (r < len) {
+            const int64_t tt = t1 - 1 - r;
+            T x[W], xq[L], xneg[W], xqneg[L];
 #pragma unroll
-        for (int b = 0; b < W; ++b) {
-            x[b] = xn[b];
-            xneg[b] = -x[b];
-        }
-#pragma unroll
-        for (int i = 0; i < L; ++i) {
-            xq[i] = xqn[i];
-            xqneg[i] = -xq[i];
-        }
-        if (tt > t0) {
-#pragma unroll
-            for (int b = 0; b < W; ++b) xn[b] = incr[(tt - 1) * W + b];
-#pragma unroll
-            for (int i = 0; i < L; ++i) xqn[i] = incr[(tt - 1) * W + ql[i]];
-        }
-        // S_{t-1}: identity before the first increment, a checkpoint where
-        // one was stored, else S_t (x) exp(-x)
-        const int64_t tl = tt - t0;
-        if (tl == 0) {
-            chain[0] = T(1);
-#pragma unroll
-            for (int i = 1; i <= L; ++i) chain[i] = T(0);
-#pragma unroll
-            for (int rr = 0; rr < GS::total; ++rr) subS[rr] = T(0);
-        } else if (ck_every > 0 && (tl % ck_every) == 0) {
-            const T* c0 = ckpts + (ck_offs[path] + tl / ck_every - 1) * dim;
-#pragma unroll
-            for (int i = 0; i <= L; ++i) chain[i] = c0[B.start[i] + q / tpow(W, L - i)];
-#pragma unroll
-            for (int i = L + 1; i < N; ++i)
-#pragma unroll
-                for (int rr = 0; rr < GS::words(i); ++rr)
-                    subS[GS::offset(i) + rr] = c0[B.start[i] + static_cast<int64_t>(q) * GS::words(i) + rr];
-        } else {
-            thread_step<T, W, N - 1, L>(chain, subS, xneg, xqneg);
-        }
-        // Horner values of the step S_{t-1} -> S_t: A^(K)_L along q for the
-        // subtree targets K > L, and A^(K)_i of subtree levels i < N
-        T ach[N + 1];
-#pragma unroll
-        for (int K = L + 1; K <= N; ++K) {
-            T v = chain[0];
-#pragma unroll
-            for (int i = 1; i <= L; ++i) v = (v * trecip<T>(K - i + 1)) * xq[i - 1] + chain[i];
-            ach[K] = v;
-        }
-        T av[GG::atotal];
-#pragma unroll
-        for (int i = L + 1; i < N; ++i) {
-#pragma unroll
-            for (int u = 0; u < GG::words(i - 1); ++u) {
-#pragma unroll
-                for (int K = i + 1; K <= N; ++K) {
-                    const T ap = (i == L + 1) ? ach[K] : av[GG::aoffset(i - 1) + (K - i) * GG::words(i - 1) + u];
-                    const T apc = ap * trecip<T>(K - i + 1);
-#pragma unroll
-                    for (int b = 0; b < W; ++b)
-                        av[GG::aoffset(i) + (K - i - 1) * GG::words(i) + u * W + b] =
-                            apc * x[b] + subS[GS::offset(i) + u * W + b];
-                }
+            for (int b = 0; b < W; ++b) {
+                x[b] = xn[b];
+                xneg[b] = -x[b];
             }
-        }
-        // reverse over the subtree, levels descending: ab holds Abar^(K)_i of
-        // the level below the one processed (targets K > i), abl Abar^(K)_L
-        T ab[GG::atotal];
 #pragma unroll
-        for (int j = 0; j < GG::atotal; ++j) ab[j] = T(0);
-        T abl[N + 1];
-#pragma unroll
-        for (int K = 0; K <= N; ++K) abl[K] = T(0);
-        T xbar[W];
-#pragma unroll
-        for (int b = 0; b < W; ++b) xbar[b] = T(0);
-#pragma unroll
-        for (int i = N; i > L; --i) {
-#pragma unroll
-            for (int u = 0; u < GG::words(i - 1); ++u) {
-#pragma unroll
-                for (int b = 0; b < W; ++b) {
-                    const int rr = u * W + b;
-                    T g = subG[GG::offset(i) + rr];  // Abar^(i)_i = G
-                    T sum = g;
-#pragma unroll
-                    for (int K = i; K <= N; ++K) {
-                        const T abk = (K == i) ? g : ab[GG::aoffset(i) + (K - i - 1) * GG::words(i) + rr];
-                        if (K > i) sum += abk;
-                        const T prev = (i == L + 1) ? ach[K] : av[GG::aoffset(i - 1) + (K - i) * GG::words(i - 1) + u];
-                        const T abc = abk * trecip<T>(K - i + 1);
-                        xbar[b] += abc * prev;
-                        if (i == L + 1) {
-                            abl[K] += abc * x[b];
-                        } else {
-                            ab[GG::aoffset(i - 1) + (K - i) * GG::words(i - 1) + u] += abc * x[b];
-                        }
-                    }
-                    subG[GG::offset(i) + rr] = sum;
-                }
+            for (int i = 0; i < L; ++i) {
+                xq[i] = xqn[i];
+                xqneg[i] = -xq[i];
             }
-        }
-        // the chain: subtree targets from level L, owned chain targets from
-        // their own level; c[i] collects Gbar of the chain word of level i
-        T c[L + 1];
+            if (tt > t0) {
 #pragma unroll
-        for (int i = 0; i <= L; ++i) c[i] = T(0);
-        T xqbar[L];
+                for (int b = 0; b < W; ++b) xn[b] = incr[(tt - 1) * W + b];
+                const T* row = incr + (tt - 1) * W;
 #pragma unroll
-        for (int i = 0; i < L; ++i) xqbar[i] = T(0);
+                for (int i = 0; i < L; ++i) xqn[i] = row[ql[i]];
+            }
+            // S_{t-1}: identity before the first increment, a checkpoint
+            // where one was stored, else S_t (x) exp(-x)
+            const int64_t tl = tt - t0;
+            if (tl == 0) {
+                chain[0] = T(1);
 #pragma unroll
-        for (int K = N; K >= 1; --K) {
-            const int top = K < L ? K : L;
-            T abk;
-            if (K > L) {
-                abk = abl[K];
+                for (int i = 1; i <= L; ++i) chain[i] = T(0);
+#pragma unroll
+                for (int rr = 0; rr < GS::total; ++rr) subS[rr] = T(0);
+            } else if (ck_every > 0 && (tl % ck_every) == 0) {
+                const T* c0 = ckpts + (ck_offs[path] + tl / ck_every - 1) * dim;
+#pragma unroll
+                for (int i = 0; i <= L; ++i) chain[i] = c0[B.start[i] + q / tpow(W, L - i)];
+#pragma unroll
+                for (int i = L + 1; i < N; ++i)
+#pragma unroll
+                    for (int rr = 0; rr < GS::words(i); ++rr)
+                        subS[GS::offset(i) + rr] = c0[B.start[i] + static_cast<int64_t>(q) * GS::words(i) + rr];
             } else {
-                if (!live || q % tpow(W, L - K) != 0) continue;
-                abk = glow[BG::low_offset(K) + q / tpow(W, L - K) - a * tpow(W, K - 1)];
+                thread_step<T, W, N - 1, L>(chain, subS, xneg, xqneg);
             }
-            T ca[L + 1];
-            ca[0] = chain[0];
+            // Horner values of the step S_{t-1} -> S_t along q: hz[K][i] =
+            // A^(K)_i for i < min(K, L), ach[K] = A^(K)_L for K > L
+            T hz[N + 1][L];
+            T ach[N + 1];
 #pragma unroll
-            for (int i = 1; i < top; ++i) ca[i] = (ca[i - 1] * trecip<T>(K - i + 1)) * xq[i - 1] + chain[i];
+            for (int K = 1; K <= N; ++K) {
+                T v = chain[0];
+                hz[K][0] = v;
 #pragma unroll
-            for (int i = top; i >= 1; --i) {
-                c[i] += abk;
-                const T abc = abk * trecip<T>(K - i + 1);
-                xqbar[i - 1] += abc * ca[i - 1];
-                abk = abc * xq[i - 1];
+                for (int i = 1; i < L; ++i)
+                    if (i < K) {
+                        v = (v * trecip<T>(K - i + 1)) * xq[i - 1] + chain[i];
+                        hz[K][i] = v;
+                    }
+                if (K > L) ach[K] = (v * trecip<T>(K - L + 1)) * xq[L - 1] + chain[L];
             }
-        }
+            // the Horner values of the walks of levels < L, saved by the
+            // thread whose prefix continues the chain word with zeros
 #pragma unroll
-        for (int i = 0; i < L; ++i)
+            for (int i = 1; i < L; ++i)
+                if (live && t % tpow(W, L - i) == 0) {
+                    T* c = &cbuf[cslot][BG::coff(i) + (t / tpow(W, L - i)) * BG::cin(i)];
 #pragma unroll
-            for (int b = 0; b < W; ++b)
-                if (ql[i] == b) xbar[b] += xqbar[i];
-        if (!live) {
+                    for (int K = i; K <= N; ++K) c[K - i] = hz[K][i - 1];
+                    c[N - i + 1] = xq[i - 1];
+                }
+            // Horner values of subtree levels i < N
+            T av[GG::atotal];
 #pragma unroll
-            for (int i = 0; i <= L; ++i) c[i] = T(0);
+            for (int i = L + 1; i < N; ++i) {
+#pragma unroll
+                for (int u = 0; u < GG::words(i - 1); ++u) {
+#pragma unroll
+                    for (int K = i + 1; K <= N; ++K) {
+                        const T ap = (i == L + 1) ? ach[K] : av[GG::aoffset(i - 1) + (K - i) * GG::words(i - 1) + u];
+                        const T apc = ap * trecip<T>(K - i + 1);
+#pragma unroll
+                        for (int b = 0; b < W; ++b)
+                            av[GG::aoffset(i) + (K - i - 1) * GG::words(i) + u * W + b] =
+                                apc * x[b] + subS[GS::offset(i) + u * W + b];
+                    }
+                }
+            }
+            // reverse over the subtree, levels descending: ab holds Abar^(K)_i
+            // of the level below the one processed (targets K > i), abl
+            // Abar^(K)_L
+            T ab[GG::atotal];
+#pragma unroll
+            for (int j = 0; j < GG::atotal; ++j) ab[j] = T(0);
+            T abl[N + 1];
+#pragma unroll
+            for (int K = 0; K <= N; ++K) abl[K] = T(0);
+            T xbar[W];
 #pragma unroll
             for (int b = 0; b < W; ++b) xbar[b] = T(0);
-        }
-        // xbar: wavefront sums, then the work-group's in wavefront order
 #pragma unroll
-        for (int b = 0; b < W; ++b) {
-            T v = xbar[b];
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-            if ((t & 63) == 0) red[t >> 6][b] = v;
-        }
-        // chain sums of levels < L to LDS (stage 0 of the block sums)
+            for (int i = N; i > L; --i) {
 #pragma unroll
-        for (int i = 1; i < L; ++i) cbuf[0][i - 1][t] = c[i];
-        __syncthreads();  // every read of the old low-level cotangent is done
-        if (live) glow[BG::low_offset(L) + t] = c[L];
-        if (t < W) {
-            T acc = T(0);
-            for (int w = 0; w < NWV; ++w) acc += red[w][t];
-            partial[(tt * W + a) * W + t] = acc;
-        }
-        // stage s sums groups of W entries of stage s - 1; after it the sums
-        // of level L - s (groups of W^s threads) are complete
+                for (int u = 0; u < GG::words(i - 1); ++u) {
 #pragma unroll
-        for (int st = 1; st < L; ++st) {
-            const int groups = tpow(W, L - 1 - st);
-            if (t < groups) {
+                    for (int b = 0; b < W; ++b) {
+                        const int rr = u * W + b;
+                        T g = subG[GG::offset(i) + rr];  // Abar^(i)_i = G
+                        T sum = g;
 #pragma unroll
-                for (int i = 1; i <= L - st; ++i) {
-                    T acc = T(0);
-#pragma unroll
-                    for (int b = 0; b < W; ++b) acc += cbuf[(st - 1) & 1][i - 1][t * W + b];
-                    if (i == L - st) {
-                        glow[BG::low_offset(i) + t] = acc;
-                    } else {
-                        cbuf[st & 1][i - 1][t] = acc;
+                        for (int K = i; K <= N; ++K) {
+                            const T abk = (K == i) ? g : ab[GG::aoffset(i) + (K - i - 1) * GG::words(i) + rr];
+                            if (K > i) sum += abk;
+                            const T prev =
+                                (i == L + 1) ? ach[K] : av[GG::aoffset(i - 1) + (K - i) * GG::words(i - 1) + u];
+                            const T abc = abk * trecip<T>(K - i + 1);
+                            xbar[b] += abc * prev;
+                            if (i == L + 1) {
+                                abl[K] += abc * x[b];
+                            } else {
+                                ab[GG::aoffset(i - 1) + (K - i) * GG::words(i - 1) + u] += abc * x[b];
+                            }
+                        }
+                        subG[GG::offset(i) + rr] = sum;
                     }
                 }
             }
-            __syncthreads();
+            // level L: the subtree targets and q's own, on to level L - 1
+            T cl = T(0), xl = T(0);
+            T vout[N - L + 1];
+#pragma unroll
+            for (int K = N; K >= L; --K) {
+                const T abk = (K == L) ? gL : abl[K];
+                cl += abk;
+                const T abc = abk * trecip<T>(K - L + 1);
+                xl += abc * hz[K][L - 1];
+                vout[K - L] = abc * xq[L - 1];
+            }
+            gL = cl;
+#pragma unroll
+            for (int b = 0; b < W; ++b)
+                if (ql[L - 1] == b) xbar[b] += xl;
+            if (live && L > 1) {
+#pragma unroll
+                for (int k = 0; k <= N - L; ++k) vbuf[vslot][BG::voff(L - 1) + t * BG::vin(L - 1) + k] = vout[k];
+            }
+            if (!live) {
+#pragma unroll
+                for (int b = 0; b < W; ++b) xbar[b] = T(0);
+            }
+            const T xs = WS::sum(xbar, lane);
+            if (WS::leader(lane) && WS::slot(lane) < W) red[xslot][t >> 6][WS::slot(lane)] = xs;
         }
-        if (L == 1) __syncthreads();
+        // the walks of levels < L
+        static_for<(L > 1 ? L - 1 : 0)>([&](auto I) {
+            constexpr int i = decltype(I)::value + 1;
+            if (ex != i) return;
+            // the step of this walk was computed L - i iterations ago
+            const int64_t rs = r - (L - i);
+            if (rs < 0 || rs >= len) return;
+            const int cs = cslot >= L - i ? cslot - (L - i) : cslot + i;
+            const int xs = xslot >= L - i ? xslot - (L - i) : xslot + i + 1;
+            const T* vi = &vbuf[vslot ^ 1][BG::voff(i) + ej * W * BG::vin(i)];
+            T V[N - i];
+#pragma unroll
+            for (int k = 0; k < N - i; ++k) {
+                T acc = T(0);
+#pragma unroll
+                for (int c = 0; c < W; ++c) acc += vi[c * BG::vin(i) + k];
+                V[k] = acc;
+            }
+            const T* cf = &cbuf[cs][BG::coff(i) + ej * BG::cin(i)];
+            T ci = T(0), xw = T(0);
+#pragma unroll
+            for (int K = N; K >= i; --K) {
+                const T abk = (K == i) ? gst : V[K - i - 1];
+                ci += abk;
+                const T abc = abk * trecip<T>(K - i + 1);
+                xw += abc * cf[K - i];
+                if (i > 1) vbuf[vslot][BG::voff(i - 1) + ej * BG::vin(i - 1) + (K - i)] = abc * cf[N - i + 1];
+            }
+            gst = ci;
+            xsb[xs][BG::xoff(i) + ej] = xw;
+        });
+        // a step whose walks are all done: its xbar, in a fixed order
+        const int64_t rf = r - L;
+        if (rf >= 0 && rf < len && t < W) {
+            const int sl = xslot == L ? 0 : xslot + 1;  // (r - L) mod (L + 1)
+            T acc = T(0);
+#pragma unroll
+            for (int w = 0; w < NWV; ++w) acc += red[sl][w][t];
+            static_for<(L > 1 ? L - 1 : 0)>([&](auto I) {
+                constexpr int i = L - 1 - decltype(I)::value;  // levels L-1 .. 1
+                if constexpr (i == 1) {
+                    if (a == t) acc += xsb[sl][BG::xoff(1)];
+                } else {
+#pragma unroll
+                    for (int j = 0; j < BG::count(i) / W; ++j) acc += xsb[sl][BG::xoff(i) + j * W + t];
+                }
+            });
+            partial[((t1 - 1 - rf) * W + a) * W + t] = acc;
+        }
+        __syncthreads();
     }
 }
 
