@@ -281,36 +281,102 @@ struct BwdGeo {
     static constexpr int xtotal = xoff(L) > 0 ? xoff(L) : 1;
 };
 
-// Sums W per-lane values over the 64 lanes of a wavefront.  The values sit in
-// P = 2^LP >= W slots; in each of the first LP exchanges the lanes that differ
-// in one lane bit trade the half of their slots the other lane keeps, so each
-// lane goes on with half as many partial sums, and butterflies over the
-// remaining lane bits finish them.  Lane l then holds the total of slot
-// l >> (6 - LP).  A fixed order, so the sums are deterministic.
+// Cross-lane moves of a 32- or 64-bit value: the two halves of a 64-bit one
+// move separately.
+template <typename T, typename F>
+__device__ __forceinline__ T lanes_move(T v, F&& f) {
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint32_t lo = f(static_cast<uint32_t>(u)), hi = f(static_cast<uint32_t>(u >> 32));
+        return __builtin_bit_cast(T, (static_cast<uint64_t>(hi) << 32) | lo);
+    } else {
+        return __builtin_bit_cast(T, f(__builtin_bit_cast(uint32_t, v)));
+    }
+}
+
+// x + the value of the lane that the DPP control CTRL names (within a row of
+// 16 lanes).
+template <int CTRL, typename T>
+__device__ __forceinline__ T add_dpp(T x) {
+    return x + lanes_move(x, [](uint32_t u) {
+               return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(u), CTRL, 0xf, 0xf, false));
+           });
+}
+
+// x summed over the lane pairs that differ in lane bit HALF (32 or 16) in
+// the lanes where that bit is clear, y summed over them where it is set.
+// 64-bit values: the two results of v_permlane32_swap / v_permlane16_swap
+// (x of the lower half of the lanes with y of the lower half, x of the upper
+// half with y of the upper half; even and odd rows of 16 for HALF = 16),
+// added.  32-bit values: a select and a lane shuffle, since the compiler here
+// drops the second result of a 32-bit swap that feeds a float add.
+template <int HALF, typename T>
+__device__ __forceinline__ T swap_sum(T x, T y, int lane) {
+    if constexpr (sizeof(T) == 8) {
+        auto sw = [](uint32_t a, uint32_t b) {
+            if constexpr (HALF == 32) {
+                return __builtin_amdgcn_permlane32_swap(a, b, false, false);
+            } else {
+                return __builtin_amdgcn_permlane16_swap(a, b, false, false);
+            }
+        };
+        const uint64_t ux = __builtin_bit_cast(uint64_t, x), uy = __builtin_bit_cast(uint64_t, y);
+        const auto lo = sw(static_cast<uint32_t>(ux), static_cast<uint32_t>(uy));
+        const auto hi = sw(static_cast<uint32_t>(ux >> 32), static_cast<uint32_t>(uy >> 32));
+        const T a = __builtin_bit_cast(T, (static_cast<uint64_t>(hi[0]) << 32) | lo[0]);
+        const T b = __builtin_bit_cast(T, (static_cast<uint64_t>(hi[1]) << 32) | lo[1]);
+        return a + b;
+    } else {
+        const bool up = (lane & HALF) != 0;
+        const T send = up ? x : y;
+        const T keep = up ? y : x;
+        return keep + __shfl_xor(send, HALF, 64);
+    }
+}
+
+// Sums W <= 8 per-lane values over the 64 lanes of a wavefront.  The values
+// sit in P = 2^LP >= W slots; in each of the first LP exchanges the lanes that
+// differ in one lane bit (32, 16, 8) trade the half of their slots the other
+// lane keeps, so each lane goes on with half as many partial sums: the 32-
+// and 16-lane exchanges by lane swaps, the 8-lane one by a DPP rotation.  DPP
+// butterflies over the remaining lane bits finish the sums.  Lane l then
+// holds the total of slot l >> (6 - LP).  A fixed order, so the sums are
+// deterministic.
 template <typename T, int W>
 struct WaveSlots {
-    static constexpr int LP = W <= 1 ? 0 : W <= 2 ? 1 : W <= 4 ? 2 : W <= 8 ? 3 : W <= 16 ? 4 : W <= 32 ? 5 : 6;
+    static_assert(W >= 1 && W <= 8, "at most 8 slots");
+    static constexpr int LP = W <= 1 ? 0 : W <= 2 ? 1 : W <= 4 ? 2 : 3;
     static constexpr int P = 1 << LP;
-    static_assert(W <= 64, "at most one slot per lane");
     __device__ static __forceinline__ T sum(const T (&v)[W], int lane) {
         T s[P];
 #pragma unroll
         for (int b = 0; b < P; ++b) s[b] = b < W ? v[b] : T(0);
-        static_for<LP>([&](auto H) {
-            constexpr int o = 32 >> decltype(H)::value;
-            constexpr int half = (P >> decltype(H)::value) / 2;
-            const bool up = (lane & o) != 0;
-            static_for<half>([&](auto J) {
-                constexpr int j = decltype(J)::value;
-                const T send = up ? s[j] : s[j + half];
-                const T keep = up ? s[j + half] : s[j];
-                s[j] = keep + __shfl_xor(send, o, 64);
-            });
-        });
-        static_for<6 - LP>([&](auto H) {
-            constexpr int o = 32 >> (LP + decltype(H)::value);
-            s[0] += __shfl_xor(s[0], o, 64);
-        });
+        if constexpr (LP >= 1) {
+#pragma unroll
+            for (int j = 0; j < P / 2; ++j) s[j] = swap_sum<32>(s[j], s[j + P / 2], lane);
+        } else {
+            s[0] = swap_sum<32>(s[0], s[0], lane);  // both halves: each lane's pair sum
+        }
+        if constexpr (LP >= 2) {
+#pragma unroll
+            for (int j = 0; j < P / 4; ++j) s[j] = swap_sum<16>(s[j], s[j + P / 4], lane);
+        } else {
+            s[0] = swap_sum<16>(s[0], s[0], lane);
+        }
+        if constexpr (LP >= 3) {
+            const bool up = (lane & 8) != 0;
+            const T send = up ? s[0] : s[1];
+            const T keep = up ? s[1] : s[0];
+            s[0] = keep + lanes_move(send, [](uint32_t u) {
+                       return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(u), 0x128, 0xf, 0xf,
+                                                                                false));  // row_ror:8
+                   });
+        } else {
+            s[0] = add_dpp<0x128>(s[0]);  // row_ror:8
+        }
+        s[0] = add_dpp<0x141>(s[0]);  // row_half_mirror
+        s[0] = add_dpp<0x4E>(s[0]);   // quad_perm [2,3,0,1]
+        s[0] = add_dpp<0xB1>(s[0]);   // quad_perm [1,0,3,2]
         return s[0];
     }
     __device__ static __forceinline__ int slot(int lane) { return lane >> (6 - LP); }
