@@ -217,8 +217,10 @@ def load_traffic(name):
 def headline(rpp, torch, O, dev, steps, warmup):
     inc = make_increments(0xC2, PATHS, LENGTH, WIDTH)
     n_chk = 8
-    ref64 = O.signature(WIDTH, DEPTH, inc[:n_chk].reshape(-1, WIDTH),
-                        np.arange(n_chk + 1, dtype=np.int64) * LENGTH)
+    chk_off = np.arange(n_chk + 1, dtype=np.int64) * LENGTH
+    ref64 = O.signature(WIDTH, DEPTH, inc[:n_chk].reshape(-1, WIDTH), chk_off)
+    # the oracle's FP32 chain, for the bitwise check of the plain FP32 path
+    ref32 = O.signature(WIDTH, DEPTH, inc[:n_chk].reshape(-1, WIDTH), chk_off, dtype=np.float32)
     results = {}
     roof = None
     for name, cls, dt in (("plain_fp64", Plain, torch.float64), ("fused_fp64", Fused, torch.float64),
@@ -233,9 +235,14 @@ def headline(rpp, torch, O, dev, steps, warmup):
             roof = chain_ms / LENGTH
         else:
             ms, wall, out = timed(torch, runner.step, steps, warmup)
+        got = out[:n_chk].cpu().numpy()
         results[name] = {"ms_per_step": ms, "wall_ms_per_step": wall,
                          "path_increments_per_s": PATHS * LENGTH / (ms * 1e-3),
-                         "max_rel_err_vs_fp64_oracle": rel_err(out[:n_chk].double().cpu().numpy(), ref64)}
+                         "max_rel_err_vs_fp64_oracle": rel_err(got.astype(np.float64), ref64)}
+        if name.startswith("plain"):
+            # the plain chain is the reference's own ft_fmexp recursion
+            results[name]["bit_identical_to_oracle_chain"] = bool(
+                np.array_equal(got, ref64 if dt == torch.float64 else ref32))
         del runner
         torch.cuda.empty_cache()
     return inc, results, roof
@@ -508,6 +515,16 @@ def log_signature(rpp, torch, O, dev, steps=3, warmup=1):
     log_w = np.zeros_like(s_w)
     O.orc.ft_log(w, N, log_w, s_w)
     back_w = np.asarray((lie_l2t(lie, dim, ts) @ (T2L @ log_w.T)).T)
+    # the same path through the drop-in operations: the plain chained
+    # ft_fmexp (the reference's recursion), ft_log and tensor_to_lie, each
+    # compared bit for bit with the oracle's (the reference's) results
+    plain = Plain(rpp, torch, inc[worst:worst + 1], torch.float64, dev, depth=N).chain()
+    lg_p = torch.empty_like(plain)
+    rpp.ft_log(lg_p, plain, basis)
+    lie_p = torch.zeros((1, ts), dtype=torch.float64, device=dev)
+    rpp.tensor_to_lie(lie_p, lg_p, basis)
+    lie_w = np.zeros((1, ts))
+    O.spmv(t2l["data"], t2l["indices"], t2l["offsets"], dim, lie_w, log_w, csc=True, out_offset=1, out_size=ts - 1)
     res = {"config": {"paths": n, "width": w, "depth": N, "coefficients": dim, "lie_dimension": ts - 1,
                       "increments_per_path": L, "dtype": "f64", "steps": steps, "warmup": warmup,
                       "pipeline": "signature -> ft_log -> tensor_to_lie -> lie_to_tensor"},
@@ -516,10 +533,18 @@ def log_signature(rpp, torch, O, dev, steps=3, warmup=1):
            "log_max_rel_err_vs_fp64_oracle": rel_err(lg_all[:k], log_ref),
            "lie_max_rel_err_vs_fp64_oracle": rel_err(got[:k, 1:].cpu().numpy(), lie_ref),
            "round_trip_max_rel_err_all_paths": float(rt[worst]),
-           "round_trip_worst_path": {"path": worst,
-                                     "gpu_pipeline_log_rel_err_vs_oracle_pipeline": rel_err(lg_all[worst], log_w[0]),
-                                     "oracle_round_trip_rel_err": rel_err(back_w, log_w),
-                                     "max_abs_log": float(np.abs(log_w).max())},
+           "round_trip_worst_path": {
+               "path": worst,
+               # the fused signature sums in its own order (1e-16 apart from
+               # the reference's chain); the log's conditioning on this path,
+               # seen in the oracle's own round trip, amplifies that
+               "fused_signature_rel_err_vs_oracle": rel_err(sig[worst:worst + 1].cpu().numpy(), s_w),
+               "fused_pipeline_log_rel_err_vs_oracle_pipeline": rel_err(lg_all[worst], log_w[0]),
+               "oracle_round_trip_rel_err": rel_err(back_w, log_w),
+               "max_abs_log": float(np.abs(log_w).max()),
+               "plain_chain_signature_bit_identical": bool(np.array_equal(plain.cpu().numpy(), s_w)),
+               "plain_chain_log_bit_identical": bool(np.array_equal(lg_p.cpu().numpy(), log_w)),
+               "plain_chain_lie_bit_identical": bool(np.array_equal(lie_p.cpu().numpy()[:, 1:], lie_w[:, 1:]))},
            "ft_log_worst_conditioned": log_worst_conditioned(rpp, torch, O, dev)}
     del sig, lg, lie_out, back
     torch.cuda.empty_cache()

@@ -1089,14 +1089,8 @@ extern "C" int rpp_hip_fmexp_tile(int32_t dtype_bytes, int32_t width, int32_t de
         if (split) *split = t->split;
         return kind;
     };
-    if (dtype_bytes == 4) {
-        if (const auto* c = rpp_hip::fmexp_coal_tile<float>(width, depth)) return report(c, 2);
-        return report(rpp_hip::fmexp_tile<float>(width, depth), 1);
-    }
-    if (dtype_bytes == 8) {
-        if (const auto* c = rpp_hip::fmexp_coal_tile<double>(width, depth)) return report(c, 2);
-        return report(rpp_hip::fmexp_tile<double>(width, depth), 1);
-    }
+    if (dtype_bytes == 4) return report(rpp_hip::fmexp_coal_tile<float>(width, depth), 2);
+    if (dtype_bytes == 8) return report(rpp_hip::fmexp_coal_tile<double>(width, depth), 2);
     return rpp_hip::fail(RPP_HIP_INVALID_ARGUMENT, "dtype_bytes must be 4 or 8");
 }
 

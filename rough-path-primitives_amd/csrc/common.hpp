@@ -210,6 +210,23 @@ __device__ __forceinline__ T cpu_order_level_sum(const DevBasis& B, int F, int R
     return acc;
 }
 
+// One step of the reference's Horner recursion for ft_fmexp and ft_exp with a
+// degree-one exponent (operations/intermediate/ft_fmexp.hpp:66-88,
+// ft_exp.hpp:70-81): the product tile of a word is its single term added to
+// the zero-initialised tile, 0 + a x; the writer scales it by c = 1/d and,
+// on levels inside the multiplier's window, adds the multiplier's word s
+// (ScaledTileAddWriter, ft_multiply.hpp:260-395), else writes c tile alone
+// (ScaledTileWriter, :176-258).  Every operation rounds on its own, as on the
+// reference's CPU, so chained steps agree with it bit for bit (signed zeros
+// included).
+template <typename T>
+__device__ __forceinline__ T horner_ref(T a, T x, T c, T s, bool add) {
+#pragma clang fp contract(off)
+    const T tile = T(0) + a * x;
+    const T v = c * tile;
+    return add ? s + v : v;
+}
+
 // One strided batch of graded vectors (rpp_hip_tensor_batch on the device).
 template <typename T>
 struct View {

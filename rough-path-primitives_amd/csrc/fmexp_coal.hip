@@ -9,9 +9,13 @@
 // r = t + s*NT of every subtree level in registers, so each wavefront load
 // and store covers 64 consecutive coefficients.  The Horner scheme of target
 // level k,
-//     A_M = chain value,  A_i[r] = A_{i-1}[r / W] * x[r % W] / (k - i + 1) + S_i[r],
-// exchanges A_{i-1} through LDS (consecutive lanes read consecutive or equal
-// parents: no bank conflicts), and the last level updates S_k in registers.
+//     A_M = chain value,  A_i[r] = S_i[r] + (1 / (k - i + 1)) (0 + A_{i-1}[r / W] x[r % W]),
+// is the reference's own sequence of roundings (horner_ref), so a call agrees
+// with the reference's ft_fmexp bit for bit; it exchanges A_{i-1} through LDS
+// (consecutive lanes read consecutive or equal parents: no bank conflicts),
+// and the last level updates S_k in registers.  The chain starts from S_0 as
+// the reference's step k + 1 leaves it: S_0 + 0 for k < N (its degree-0 tile
+// is empty), S_0 itself for k = N.
 #include "common.hpp"
 #include "fmexp_coal.hpp"
 
@@ -86,9 +90,10 @@ k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int6
         __syncthreads();
 #pragma unroll
         for (int k = N; k >= M + 1; --k) {
-            T a = chain[0];
+            T a = k == N ? chain[0] : chain[0] + T(0);
 #pragma unroll
-            for (int i = 1; i <= M; ++i) a = (a * recip<T>(k - i + 1)) * xs[(q / cpow(W, M - i)) % W] + chain[i];
+            for (int i = 1; i <= M; ++i)
+                a = horner_ref(a, xs[(q / cpow(W, M - i)) % W], recip<T>(k - i + 1), chain[i], mult.has(i));
 #pragma unroll
             for (int i = M + 1; i <= k; ++i) {
                 const T c = recip<T>(k - i + 1);
@@ -101,7 +106,7 @@ k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int6
                         const int p = r / W;
                         const int letter = r - p * W;
                         const T ap = (i == M + 1) ? a : prev[p];
-                        const T v = (ap * c) * xs[letter] + reg[G::offset(i) + s];
+                        const T v = horner_ref(ap, xs[letter], c, reg[G::offset(i) + s], mult.has(i));
                         if (i == k) {
                             reg[G::offset(i) + s] = v;
                         } else {
@@ -116,11 +121,13 @@ k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int6
         // chain targets M .. 1, descending so each Horner reads old values
 #pragma unroll
         for (int k = M; k >= 1; --k) {
-            T a = chain[0];
+            T a = chain[0] + T(0);
 #pragma unroll
-            for (int i = 1; i <= k; ++i) a = (a * recip<T>(k - i + 1)) * xs[(q / cpow(W, M - i)) % W] + chain[i];
+            for (int i = 1; i <= k; ++i)
+                a = horner_ref(a, xs[(q / cpow(W, M - i)) % W], recip<T>(k - i + 1), chain[i], mult.has(i));
             chain[k] = a;
         }
+        chain[0] = chain[0] + T(0);  // the reference's last step leaves S_0 + 0
         T* dst = out.obj(b);
 #pragma unroll
         for (int i = M + 1; i <= N; ++i) {

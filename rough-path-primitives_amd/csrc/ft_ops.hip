@@ -238,9 +238,11 @@ k_horner(int op, DevBasis B, View<T> out, View<const T> arg, View<const T> mult,
 }
 
 // ft_fmexp when the exponent is a degree-one increment (window (1,1)) and
-// out starts at degree 0:  out_k[w] = sum_j mult_{k-j}[w / d^j] e_j(suffix),
-// e_j(s) = prod_i x[s_i] / j!.  One pass over out instead of N; the result is
-// the same truncated product mult (x) exp(x) the Horner recursion computes.
+// out starts at degree 0, one word at a time: the reference's Horner
+// recursion of the word's target level k along its prefixes,
+//     R_0 = S_0 (+ 0 below the top level),  R_j = mult_j[w_1..w_j] + (0 + R_{j-1} x[w_j]) / (k-j+1),
+// with the reference's roundings (horner_ref).  The prefixes come from
+// repeated division (O(k^2) per word): the strided-layout fallback.
 template <typename T, int W, bool CS1>
 __global__ void __launch_bounds__(kBlock)
 k_fmexp_deg1(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int64_t n) {
@@ -248,6 +250,7 @@ k_fmexp_deg1(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int6
     __shared__ T xs[256];
     __shared__ T inv[kMaxDepth + 2];
     const uint32_t width = L::width(B);
+    const int N = B.depth;
     for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
         T* o = out.obj(b);
         const T* m = mult.obj(b);
@@ -256,20 +259,21 @@ k_fmexp_deg1(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int6
         if (threadIdx.x <= static_cast<unsigned>(B.depth))
             inv[threadIdx.x] = threadIdx.x == 0 ? T(1) : T(1) / T(threadIdx.x);
         __syncthreads();
+        const T m0 = mult.has(0) ? m[0] : T(0);
         for (int k = out.mx; k >= 0; --k) {
             const uint32_t nk = B.pow[k];
             const int64_t ok = B.start[k];
             for (uint32_t w = threadIdx.x; w < nk; w += blockDim.x) {
-                T acc = mult.has(k) ? m[at<CS1>(ok + w, mult.cs)] : T(0);
-                uint32_t p = w;
-                T e = T(1);
+                T r = k == N ? m0 : m0 + T(0);
                 for (int j = 1; j <= k; ++j) {
-                    const uint32_t q = L::div(B, p);
-                    e = e * xs[p - q * width] * inv[j];
-                    p = q;
-                    if (mult.has(k - j)) acc += m[at<CS1>(B.start[k - j] + p, mult.cs)] * e;
+                    uint32_t p = w;  // prefix of j letters: w / width^(k-j)
+                    for (int e = j; e < k; ++e) p = L::div(B, p);
+                    const uint32_t letter = p - L::div(B, p) * width;
+                    const bool in = mult.has(j);
+                    r = horner_ref(r, xs[letter], inv[k - j + 1], in ? m[at<CS1>(B.start[j] + p, mult.cs)] : T(0),
+                                   in);
                 }
-                o[at<CS1>(ok + w, out.cs)] = acc;
+                o[at<CS1>(ok + w, out.cs)] = r;
             }
         }
         __syncthreads();
@@ -278,10 +282,10 @@ k_fmexp_deg1(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int6
 
 // ft_fmexp with a degree-one exponent x and out from degree 0, in Horner
 // form per target level k (out must not alias the multiplier, R7):
-//     A_0 = mult_0,  A_i[p a] = A_{i-1}[p] x[a] / (k-i+1) + mult_i[p a],
-//     out_k = A_k,
-// i.e. sum_i mult_i (x) x^{k-i}/(k-i)!  -- the same truncated series as the
-// reference recursion, at sum_k sum_{i<=k} d^i multiply-adds per object.
+//     A_0 = mult_0 (+ 0 below the top level),
+//     A_i[p a] = mult_i[p a] + (0 + A_{i-1}[p] x[a]) / (k-i+1),   out_k = A_k,
+// the reference's recursion with its roundings (horner_ref), at
+// sum_k sum_{i<=k} d^i steps per object.
 // A work item is (object, first letter q) when split = 1: the words under q
 // evolve independently given the chain value A_1[q].  Thread t owns parents
 // p = t, t + blockDim, ... of a level and computes their W children; the
@@ -314,14 +318,16 @@ k_fmexp_horner(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, in
         for (uint32_t a = threadIdx.x; a < width; a += blockDim.x) xs[a] = x[1 + a];
         __syncthreads();
         const T m0 = mult.has(0) ? m[0] : T(0);
-        // chain value A_1[q] of target k (M = 1): m0 x[q] / k + mult_1[q]
-        const T m1q = (M == 1 && mult.has(1)) ? m[1 + q] : T(0);
+        const bool in1 = mult.has(1);
+        // chain value A_1[q] of target k (M = 1)
+        const T m1q = (M == 1 && in1) ? m[1 + q] : T(0);
         if (threadIdx.x == 0) {
-            if (q == 0 && omx >= 0) o[0] = m0;
-            if (M == 1 && omx >= 1) o[1 + q] = m0 * xs[q] + m1q;
+            if (q == 0 && omx >= 0) o[0] = N >= 1 ? m0 + T(0) : m0;
+            if (M == 1 && omx >= 1) o[1 + q] = horner_ref(N == 1 ? m0 : m0 + T(0), xs[q], T(1), m1q, in1);
         }
         for (int k = omx; k >= M + 1; --k) {
-            const T chain = M == 1 ? m0 * (xs[q] * inv[k]) + m1q : m0;
+            const T a0 = k == N ? m0 : m0 + T(0);
+            const T chain = M == 1 ? horner_ref(a0, xs[q], inv[k], m1q, in1) : a0;
             for (int i = M + 1; i <= k; ++i) {
                 const T c = inv[k - i + 1];
                 const uint32_t np = B.pow[i - 1 - M];  // parents of level i in the item's subtree
@@ -335,12 +341,12 @@ k_fmexp_horner(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, in
                     if constexpr (W > 0) {
 #pragma unroll
                         for (int a = 0; a < W; ++a) {
-                            const T v = ap * (xs[a] * c) + (mi ? m[wb + a] : T(0));
+                            const T v = horner_ref(ap, xs[a], c, mi ? m[wb + a] : T(0), mi);
                             if (i == k) o[wb + a] = v; else cur[p * W + a] = v;
                         }
                     } else {
                         for (uint32_t a = 0; a < width; ++a) {
-                            const T v = ap * (xs[a] * c) + (mi ? m[wb + a] : T(0));
+                            const T v = horner_ref(ap, xs[a], c, mi ? m[wb + a] : T(0), mi);
                             if (i == k) o[wb + a] = v; else cur[p * width + a] = v;
                         }
                     }
@@ -351,30 +357,36 @@ k_fmexp_horner(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, in
     }
 }
 
-// ft_exp of a degree-one argument, out from degree 0: exp(x)_w = prod x / k!.
+// ft_exp of a degree-one argument, out from degree 0, one word at a time:
+// the reference's Horner recursion (operations/intermediate/ft_exp.hpp:70-81)
+// of the word's level k along its prefixes, R_0 = 1,
+// R_j = (0 + R_{j-1} x[w_j]) / (k-j+1), with its roundings (horner_ref);
+// exp(x)_w = prod x / k! up to rounding.  Prefixes by repeated division.
 template <typename T, int W, bool CS1>
 __global__ void __launch_bounds__(kBlock)
 k_exp_deg1(DevBasis B, View<T> out, View<const T> arg, int64_t n) {
     using L = Letters<W>;
     __shared__ T xs[256];
+    __shared__ T inv[kMaxDepth + 2];
     const uint32_t width = L::width(B);
     for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
         T* o = out.obj(b);
         const T* x = arg.obj(b);
         for (uint32_t i = threadIdx.x; i < width; i += blockDim.x) xs[i] = x[at<CS1>(1 + i, arg.cs)];
+        if (threadIdx.x <= static_cast<unsigned>(B.depth))
+            inv[threadIdx.x] = threadIdx.x == 0 ? T(1) : T(1) / T(threadIdx.x);
         __syncthreads();
         for (int k = 0; k <= out.mx; ++k) {
             const uint32_t nk = B.pow[k];
             const int64_t ok = B.start[k];
             for (uint32_t w = threadIdx.x; w < nk; w += blockDim.x) {
-                uint32_t p = w;
-                T e = T(1);
+                T r = T(1);
                 for (int j = 1; j <= k; ++j) {
-                    const uint32_t q = L::div(B, p);
-                    e = e * xs[p - q * width] / T(j);
-                    p = q;
+                    uint32_t p = w;  // prefix of j letters
+                    for (int e = j; e < k; ++e) p = L::div(B, p);
+                    r = horner_ref(r, xs[p - L::div(B, p) * width], inv[k - j + 1], T(0), false);
                 }
-                o[at<CS1>(ok + w, out.cs)] = e;
+                o[at<CS1>(ok + w, out.cs)] = r;
             }
         }
         __syncthreads();
@@ -627,8 +639,6 @@ int launch_horner(int op, const DevBasis& B, int64_t n, const rpp_hip_tensor_bat
     if (op == kOpFmexp && deg1 && contig && out.max_degree == B.depth && mult && B.depth >= 1) {
         if (const FmexpTile<T>* tile = fmexp_coal_tile<T>(B.width, B.depth))
             return tile->launch(B, out, *mult, arg, n, stream);
-        if (const RegTile<T>* tile = fmexp_tile<T>(B.width, B.depth))
-            return tile->fmexp(B, out, *mult, arg, n, stream);
     }
     if (op == kOpFmexp && deg1 && contig && B.width <= 256 && out.max_degree >= 1) {
         // split by first letter when the top-level exchange would not fit

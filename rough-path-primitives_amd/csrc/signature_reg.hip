@@ -333,42 +333,6 @@ k_sig_fwd_reg(DevBasis B, int64_t n_items, const int64_t* order, const T* incr, 
     }
 }
 
-// Plain ft_fmexp (one increment) on the tile: out = mult (x) exp(x) for a
-// degree-one exponent, full output window.  All loads of the multiplier are
-// issued up front into registers, one in-register Chen step, one store: the
-// per-call path reads mult once and writes out once.
-template <typename T, int W, int N, int NT, int M, int RL>
-__global__ void __launch_bounds__(NT)
-k_fmexp_tile(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int64_t n) {
-    using G = Geo<W, N, NT, M, RL>;
-    __shared__ T lds0[G::lds_elems];
-    __shared__ T lds1[G::lds_elems];
-    __shared__ T sstate[G::state_elems];
-    __shared__ T xs[W];
-    for (int64_t item = blockIdx.x; item < n * G::P; item += gridDim.x) {
-        const int64_t b = item / G::P;
-        const int q = static_cast<int>(item - b * G::P);
-        Tile<T, W, N, NT, M, RL> S(sstate);
-        S.load_window(B, mult.obj(b), q, mult.mn, mult.mx);
-        __syncthreads();
-        if (threadIdx.x < W) xs[threadIdx.x] = ex.obj(b)[1 + threadIdx.x];
-        __syncthreads();
-        chen_step_reg<T, W, N, NT, M, RL>(S, xs, lds0, lds1, q);
-        S.template move<true>(B, out.obj(b), 1, q, q == 0);
-    }
-}
-
-template <typename T, int W, int N, int NT, int M, int RL>
-int fmexp_launch(const DevBasis& B, const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& mult,
-                 const rpp_hip_tensor_batch& ex, int64_t n, hipStream_t stream) {
-    using G = Geo<W, N, NT, M, RL>;
-    const int64_t items = n * G::P;
-    const unsigned grid = static_cast<unsigned>(std::min<int64_t>(items, 65535LL * 16));
-    k_fmexp_tile<T, W, N, NT, M, RL><<<grid, NT, 0, stream>>>(B, make_view<T>(out), make_cview<T>(mult),
-                                                          make_cview<T>(ex), n);
-    return check_hip(hipGetLastError(), "k_fmexp_tile launch");
-}
-
 // ------------------------------------------------------------------ backward
 
 // Reverse mode of one Chen step taken at state S (the state before the step),
@@ -725,7 +689,7 @@ int bwd_launch(const DevBasis& B, const rpp_hip_signature_backward_desc& d, cons
 // letters fixed per work-group, forward/backward launchers.
 #define RPP_TILE(T, W, N, NT, M, RL)                                                               \
     {sizeof(T), W, N, NT, M, RL, &fwd_launch<T, W, N, NT, M, RL>,                                  \
-     M <= 1 ? &bwd_launch<T, W, N, NT, (M <= 1 ? M : 1), RL> : nullptr, &fmexp_launch<T, W, N, NT, M, RL>, \
+     M <= 1 ? &bwd_launch<T, W, N, NT, (M <= 1 ? M : 1), RL> : nullptr,                             \
      M <= 1 ? &fmexp_adj_launch<T, W, N, NT, M, RL> : nullptr}
 
 template <typename T>
@@ -754,21 +718,6 @@ const std::vector<RegTile<T>>& reg_tiles() {
     return tiles;
 }
 
-// Plain ft_fmexp prefers the tile that fixes the most letters, then the
-// smallest work-group: more, smaller work-groups per compute unit overlap
-// the loads, the Chen step and the stores of this single-step, HBM-bound call.
-template <typename T>
-const RegTile<T>* fmexp_tile(int width, int depth) {
-    const RegTile<T>* best = nullptr;
-    for (const auto& t : reg_tiles<T>()) {
-        if (t.width != width || t.depth != depth) continue;
-        if (!best || t.split > best->split || (t.split == best->split && t.threads < best->threads) ||
-            (t.split == best->split && t.threads == best->threads && t.reg_levels > best->reg_levels))
-            best = &t;
-    }
-    return best;
-}
-
 // The per-operation adjoint needs a tile fixing at most one letter; prefer
 // one letter, then all levels in registers (one step per call leaves no
 // recurrence for LDS-resident levels to pay off), then the smallest
@@ -786,8 +735,6 @@ const RegTile<T>* fmexp_adj_tile(int width, int depth) {
 
 template const RegTile<float>* fmexp_adj_tile<float>(int, int);
 template const RegTile<double>* fmexp_adj_tile<double>(int, int);
-template const RegTile<float>* fmexp_tile<float>(int, int);
-template const RegTile<double>* fmexp_tile<double>(int, int);
 template const std::vector<RegTile<float>>& reg_tiles<float>();
 template const std::vector<RegTile<double>>& reg_tiles<double>();
 

@@ -18,9 +18,6 @@ struct RegTile {
     int (*forward)(const DevBasis&, const rpp_hip_signature_desc&, const int64_t*, unsigned*, hipStream_t);
     int (*backward)(const DevBasis&, const rpp_hip_signature_backward_desc&, const int64_t*, int64_t, unsigned*,
                     hipStream_t);
-    // one plain ft_fmexp call (degree-one exponent, full output window)
-    int (*fmexp)(const DevBasis&, const rpp_hip_tensor_batch&, const rpp_hip_tensor_batch&,
-                 const rpp_hip_tensor_batch&, int64_t, hipStream_t);
     // per-operation adjoint of that call (split <= 1)
     int (*fmexp_adjoint)(const DevBasis&, const rpp_hip_ft_fmexp_adjoint_desc&, hipStream_t);
 };
@@ -29,10 +26,6 @@ struct RegTile {
 // default when a descriptor names no tile.
 template <typename T>
 const std::vector<RegTile<T>>& reg_tiles();
-
-// The tile used by the plain ft_fmexp call for (width, depth), or nullptr.
-template <typename T>
-const RegTile<T>* fmexp_tile(int width, int depth);
 
 template <typename T>
 const RegTile<T>* fmexp_adj_tile(int width, int depth);

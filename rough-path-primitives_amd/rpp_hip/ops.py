@@ -583,13 +583,13 @@ def lie_basis_load(path) -> tuple:
 
 def fmexp_tile(width: int, depth: int, dtype):
     """(kernel, threads, split) of the tile serving plain ft_fmexp at
-    (width, depth): kernel "k_fmexp_coal" or "k_fmexp_tile"; None when a
-    generic kernel serves it."""
+    (width, depth): kernel "k_fmexp_coal"; None when a generic kernel serves
+    it."""
     nbytes = torch.tensor([], dtype=dtype).element_size()
     th, sp = C.c_int32(0), C.c_int32(0)
     kind = A.lib().rpp_hip_fmexp_tile(nbytes, width, depth, C.byref(th), C.byref(sp))
-    if kind in (1, 2):
-        return ("k_fmexp_coal" if kind == 2 else "k_fmexp_tile"), th.value, sp.value
+    if kind == 2:
+        return "k_fmexp_coal", th.value, sp.value
     return None
 
 

@@ -331,13 +331,33 @@ def test_gpu_against_golden(dev):
     for k in range(L):
         rpp_hip.ft_fmexp(nxt, cur, rpp_hip.Batch(pad_d[:, k * w: k * w + 1 + w], 1, 1), b56)
         cur, nxt = nxt, cur
-    assert_close(cur.cpu().numpy(), s["out"], np.float64, "plain chain")
+    # the plain chain is the reference's own ft_fmexp recursion: bit for bit
+    assert_exact(cur.cpu().numpy(), s["out"], np.float64, "plain chain")
     fused = rpp_hip.signature(t(s["increments"].reshape(-1, w)), t(np.arange(n + 1) * L), b56)
     assert_close(fused.cpu().numpy(), s["out"], np.float64, "fused chain")
     # the log-signature pipeline
     g = fixture("logsig_w3d8")
     n, L, w = g["increments"].shape
     b38 = rpp_hip.TensorBasis(3, 8)
+    # the plain chained ft_fmexp, then ft_log, tensor_to_lie and lie_to_tensor
+    # on its output: the reference's whole log-signature pipeline, bit for bit
+    pad = np.zeros((n, 1 + L * w))
+    pad[:, 1:] = g["increments"].reshape(n, L * w)
+    pad_d = t(pad)
+    cur = t(np.zeros((n, b38.size())))
+    cur[:, 0] = 1
+    nxt = torch.empty_like(cur)
+    for k in range(L):
+        rpp_hip.ft_fmexp(nxt, cur, rpp_hip.Batch(pad_d[:, k * w: k * w + 1 + w], 1, 1), b38)
+        cur, nxt = nxt, cur
+    assert_exact(cur.cpu().numpy(), g["signature"], np.float64, "plain chain (3,8)")
+    lg0 = torch.empty_like(cur)
+    rpp_hip.ft_log(lg0, cur, b38)
+    assert_exact(lg0.cpu().numpy(), g["log"], np.float64, "log of the plain chain (3,8)")
+    lie0 = t(np.zeros(g["lie"].shape))
+    rpp_hip.tensor_to_lie(lie0, lg0, b38)
+    assert_exact(lie0.cpu().numpy()[:, 1:], g["lie"][:, 1:], np.float64, "tensor_to_lie of the plain chain (3,8)")
+    # the fused signature sums in its own order: within the tolerance
     sig = rpp_hip.signature(t(g["increments"].reshape(-1, w)), t(np.arange(n + 1) * L), b38)
     assert_close(sig.cpu().numpy(), g["signature"], np.float64, "signature (3,8)")
     lg = torch.empty_like(sig)

@@ -41,7 +41,7 @@ def test_ft_fmexp(dev, shape, ew, dtype):
     O.orc.ft_fmexp(w, N, expected, mult, ex, ew=ew)
     out = to_dev(np.full((n, dim), np.nan, dtype=dtype), dev)
     rpp_hip.ft_fmexp(out, to_dev(mult, dev), rpp_hip.Batch(to_dev(ex, dev), *ew), basis)
-    assert_close(out.cpu().numpy(), expected, dtype, f"fmexp {shape} {ew}")
+    assert_exact(out.cpu().numpy(), expected, dtype, f"fmexp {shape} {ew}")
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
@@ -69,7 +69,7 @@ def test_ft_fmexp_windows(dev, ranges, out_max, dtype):
     got = out.cpu().numpy()
     m = window_mask(w, N, ow)
     assert np.all(np.isnan(got[:, ~m]))
-    assert_close(got[:, m], expected[:, m], dtype, f"fmexp windows {ranges} {out_max}")
+    assert_exact(got[:, m], expected[:, m], dtype, f"fmexp windows {ranges} {out_max}")
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
@@ -90,12 +90,9 @@ def test_ft_exp_log(dev, shape, op, aw, dtype):
     getattr(O.orc, "ft_" + op)(w, N, expected, arg, aw=aw)
     out = to_dev(np.full((n, dim), np.nan, dtype=dtype), dev)
     getattr(rpp_hip, "ft_" + op)(out, rpp_hip.Batch(to_dev(arg, dev), *aw), basis)
-    if op == "exp" and aw == (1, 1):
-        # the closed form exp(x)_w = prod x_{w_i} / k! of a degree-one argument
-        assert_close(out.cpu().numpy(), expected, dtype, f"{op} {shape} {aw}")
-    else:
-        # the reference's Horner steps with its product order: bit for bit
-        assert_exact(out.cpu().numpy(), expected, dtype, f"{op} {shape} {aw}")
+    # the reference's Horner steps with its product order (for a degree-one
+    # argument, its single-term tiles): bit for bit
+    assert_exact(out.cpu().numpy(), expected, dtype, f"{op} {shape} {aw}")
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
@@ -171,7 +168,7 @@ def test_adjoint_multiplications(dev, side, windows, dtype):
 def test_chained_fmexp_c2_shape(dev, shape, dtype):
     """The plain chain at every shape with a coalesced-ownership tile (width 5,
     depth 6 is the headline shape): paths x increments of chained ft_fmexp
-    against the oracle chain."""
+    against the oracle chain, bit for bit."""
     w, N, n, L = shape
     rng = np.random.default_rng(0xC2 + w * 10 + N)
     basis = rpp_hip.TensorBasis(w, N)
@@ -187,7 +184,7 @@ def test_chained_fmexp_c2_shape(dev, shape, dtype):
     for t in range(L):
         rpp_hip.ft_fmexp(nxt, cur, rpp_hip.Batch(pad_d[:, t * w: t * w + 1 + w], 1, 1), basis)
         cur, nxt = nxt, cur
-    assert_close(cur.cpu().numpy(), expected, dtype, "C2 chain")
+    assert_exact(cur.cpu().numpy(), expected, dtype, "C2 chain")
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
@@ -208,11 +205,11 @@ def test_ft_fmexp_interleaved_and_broadcast(dev, shape, dtype):
     out = torch.zeros((dim, n), dtype=getattr(torch, np.dtype(dtype).name), device=dev)
     rpp_hip.ft_fmexp(rpp_hip.interleaved(out, basis), rpp_hip.interleaved(to_dev(mult.T, dev), basis),
                      rpp_hip.Batch(to_dev(ex, dev), 1, 1), basis)
-    assert_close(out.cpu().numpy().T, expected, dtype, f"interleaved fmexp {shape}")
+    assert_exact(out.cpu().numpy().T, expected, dtype, f"interleaved fmexp {shape}")
     # object-major multiplier, broadcast exponent (Horner kernel)
     out2 = torch.zeros((n, dim), dtype=out.dtype, device=dev)
     rpp_hip.ft_fmexp(out2, to_dev(mult, dev), rpp_hip.Batch(to_dev(ex, dev), 1, 1), basis)
-    assert_close(out2.cpu().numpy(), expected, dtype, f"broadcast fmexp {shape}")
+    assert_exact(out2.cpu().numpy(), expected, dtype, f"broadcast fmexp {shape}")
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
