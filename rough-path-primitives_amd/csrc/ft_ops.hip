@@ -427,10 +427,16 @@ k_pairing(DevBasis B, T* out, int64_t os, View<const T> f, View<const T> x, int6
 
 // out[w] = sum_u op[u] arg[u w]   (left)   or   sum_u arg[w u] op[u]   (right),
 // over op degrees pd with |w| + pd inside arg's window, pd descending as the
-// reference's argument-degree-major loop (cpu/.../ft_adj_lmul.hpp:69-114).
+// reference's argument-degree-major loop (cpu/.../ft_adj_lmul.hpp:69-114),
+// each product and sum rounded on its own: the reference's terms in its
+// order, so both agree with it bit for bit (the right one through the
+// antipode route of ft_adj_rmul.hpp:102-134).
 template <typename T, bool LEFT, bool CS1>
 __global__ void __launch_bounds__(kBlock)
 k_adjoint(DevBasis B, View<T> out, View<const T> op, View<const T> arg, int64_t n, bool accumulate) {
+#pragma clang fp contract(off)
+    using L = Letters<0>;
+    const uint32_t width = L::width(B);
     for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
         T* o = out.obj(b);
         const T* P = op.obj(b);
@@ -450,9 +456,26 @@ k_adjoint(DevBasis B, View<T> out, View<const T> op, View<const T> arg, int64_t 
                             acc += P[at<CS1>(pb + u, op.cs)] *
                                    A[at<CS1>(ab + static_cast<int64_t>(u) * nk + w, arg.cs)];
                     } else {
+                        // the reference's route (antipodes around adj_lmul)
+                        // takes the terms of op word v in the order of
+                        // u = reverse(v) ascending: v follows u's letters
+                        // with reversed weights, carried through u's
+                        // trailing letters width - 1
                         const int64_t base = ab + static_cast<int64_t>(w) * np;
-                        for (uint32_t u = 0; u < np; ++u)
-                            acc += A[at<CS1>(base + u, arg.cs)] * P[at<CS1>(pb + u, op.cs)];
+                        const uint32_t top = pd > 0 ? B.pow[pd - 1] : 0u;
+                        uint32_t v = 0;
+                        for (uint32_t u = 0; u < np; ++u) {
+                            acc += P[at<CS1>(pb + v, op.cs)] * A[at<CS1>(base + v, arg.cs)];
+                            uint32_t wgt = top, uu = u;
+                            while (wgt > 0) {
+                                const uint32_t q = L::div(B, uu);
+                                if (uu - q * width != width - 1) break;
+                                v -= (width - 1) * wgt;
+                                wgt = L::div(B, wgt);
+                                uu = q;
+                            }
+                            v += wgt;
+                        }
                     }
                 }
                 T& dst = o[at<CS1>(B.start[od] + w, out.cs)];

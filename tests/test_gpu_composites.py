@@ -159,7 +159,7 @@ def test_adjoint_multiplications(dev, side, windows, dtype):
     got = out.cpu().numpy()
     m = window_mask(w, N, ow)
     assert np.all(np.isnan(got[:, ~m]))
-    assert_close(got[:, m], expected[:, m], dtype, f"adj_{side} {windows}")
+    assert_exact(got[:, m], expected[:, m], dtype, f"adj_{side} {windows}")
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
