@@ -342,6 +342,23 @@ def fwd_bwd(rpp, torch, O, dev, steps=2, warmup=1):
 
 # ------------------------------------------------------------- ft_mul (C1)
 
+def capi_call_cost(pairs):
+    """The same ft_mul called from C++ through the C-ABI, as the drop-in layer
+    calls it (rpp_hip/capi_call_bench, a child process): host time per call of
+    the enqueueing loop, wall time per call until the stream drains, device
+    time per call."""
+    exe = os.path.join(ROOT, "rough-path-primitives_amd", "rpp_hip", "capi_call_bench")
+    if not os.path.exists(exe):
+        return None
+    import subprocess
+    r = subprocess.run([exe, str(pairs), "2000"], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        return {"error": (r.stderr or r.stdout).strip()[-300:]}
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    res["achieved_gbs"] = 3 * pairs * 341 * 8 / (res["wall_us_per_call"] * 1e-6) / 1e9
+    return res
+
+
 def ft_mul_section(rpp, torch, O, dev):
     w, N = 4, 4
     basis = rpp.TensorBasis(w, N)
@@ -385,6 +402,7 @@ def ft_mul_section(rpp, torch, O, dev):
             gms, _, _ = timed(torch, graph.replay, 20, 3)
             entry["graph_us_per_call"] = gms * 1e3 / calls
             entry["graph_achieved_gbs"] = 3 * n * dim * 8 / (gms / calls * 1e-3) / 1e9
+            entry["cpp_capi"] = capi_call_cost(n)
         if n == 1024:
             # the oracle's restatement of the reference product (the same
             # summation order, bit for bit) on one core: median of 7 runs
@@ -655,7 +673,8 @@ def main():
                     help="comma list of extra sections to run (" + ",".join(SECTIONS) + "), or 'none'")
     ap.add_argument("--only", default=None,
                     help="run one headline variant (plain_fp64, fused_fp64, plain_fp32, fused_fp32), "
-                         "ft_mul_hbm (2^17 width-4 depth-4 products), layout, fused_fwd_bwd (the width-6 "
+                         "ft_mul_hbm (2^17 width-4 depth-4 products), capi_call (1024 width-4 depth-4 "
+                         "products called from C++), layout, fused_fwd_bwd (the width-6 "
                          "depth-6 fused forward + backward), log_stages (the width-3 depth-8 log-signature "
                          "stages) or fp32_drift (the FP32 gradient error sweep behind the checkpoint policy), "
                          "and print its result only")
@@ -691,6 +710,9 @@ def main():
         ms, _, _ = timed(torch, lambda: rpp.ft_mul(out, L, R, basis), args.steps, args.warmup)
         print(json.dumps({"variant": args.only, "us_per_call": ms * 1e3,
                           "achieved_gbs": 3 * n * basis.size() * 8 / (ms * 1e-3) / 1e9}))
+        return
+    if args.only == "capi_call":
+        print(json.dumps({"variant": args.only, **(capi_call_cost(1024) or {})}))
         return
     if args.only == "fused_fwd_bwd":
         # the fused forward + backward of the width 6, depth 6 workload (2048
