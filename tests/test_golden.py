@@ -177,7 +177,7 @@ def test_gpu_shuffle_family_against_golden(dev):
 
     def check(name, o):
         torch.cuda.synchronize()
-        assert_close(o.cpu().numpy(), f[name], np.float64, name)
+        assert_exact(o.cpu().numpy(), f[name], np.float64, name)
 
     o = t(np.full_like(f["x"], 7.0))
     rpp_hip.st_mul(Bt(o, 0, 4), Bt(x, 0, 3), Bt(y, 1, 4), B, beta=-0.5)
