@@ -132,8 +132,28 @@ __device__ __forceinline__ void thread_store(const DevBasis& B, T* o, int64_t cs
         if (q % tpow(W, L - i) == 0) o[(B.start[i] + q / tpow(W, L - i)) * cs] = chain[i];
 }
 
+// Waves per SIMD a tile is compiled for (1: no target).  A SIMD holds 512
+// registers per lane, so a tile a few registers above 512 / w runs w - 1
+// waves.  The FP64 forward tiles at width 6 depth 6 (L = 4) and width 3
+// depth 8 (L = 5) take 170 registers, two waves; held to 168 they run three
+// with one 8-byte spill (2048 x 256 paths at (6,6): 2.93 -> 2.79 ms; 16384 x
+// 64 paths at (3,8): 1.74 -> 1.69 ms).  The FP32 backward at width 6 depth 6
+// (148 registers, three waves) held to 128 runs four (6.42 -> 6.08 ms).
+// Larger cuts spill inside the step loop and run slower: the FP64 backward at
+// width 6 depth 6 held to three waves, 11.6 -> 51 ms; the FP64 forward at
+// width 5 depth 6 (L = 4) held to four, 1.38 -> 1.39 ms.  Results are the same
+// bit for bit: only register allocation changes.
+template <typename T, int W, int N, int L>
+constexpr int thread_fwd_waves() {
+    return sizeof(T) == 8 && ((W == 6 && N == 6 && L == 4) || (W == 3 && N == 8 && L == 5)) ? 3 : 1;
+}
+template <typename T, int W, int N, int L>
+constexpr int thread_bwd_waves() {
+    return sizeof(T) == 4 && W == 6 && N == 6 && L == 4 ? 4 : 1;
+}
+
 template <typename T, int W, int N, int L, int NT>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(thread_fwd_waves<T, W, N, L>())))
 k_sig_fwd_thread(DevBasis B, int n_paths, const int64_t* order, const T* incr, const int64_t* offs, View<T> out,
                  int ck_every, T* ckpts, const int64_t* ck_offs) {
     using G = ThreadGeo<W, N, L>;
@@ -384,7 +404,7 @@ struct WaveSlots {
 };
 
 template <typename T, int W, int N, int L, int NT = BwdGeo<W, N, L>::NT>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(thread_bwd_waves<T, W, N, L>())))
 k_sig_bwd_thread(DevBasis B, int64_t n_items, const int64_t* order, const T* incr, const int64_t* offs,
                  View<const T> sigs, View<const T> cot, T* partial, int ck_every, const T* ckpts,
                  const int64_t* ck_offs) {

@@ -14,7 +14,6 @@ from __future__ import annotations
 import argparse
 import json
 import os
-import time
 
 import torch
 
@@ -46,16 +45,20 @@ def best(width: int, depth: int, dtype, kind: str):
 
 
 def _time(fn, reps):
+    """Seconds per call: events on the current stream around reps calls
+    after one warm-up call, so host work outside the launches is excluded."""
     fn()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
     for _ in range(reps):
         fn()
-    torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / reps
+    end.record()
+    end.synchronize()
+    return start.elapsed_time(end) * 1e-3 / reps
 
 
-def tune(width: int, depth: int, dtype=torch.float64, paths: int = 1024, length: int = 64, reps: int = 3,
+def tune(width: int, depth: int, dtype=torch.float64, paths: int = 1024, length: int = 64, reps: int = 5,
          backward: bool = True, write: bool = True) -> dict:
     from . import ops
     dev = torch.device("cuda:0")
@@ -63,6 +66,8 @@ def tune(width: int, depth: int, dtype=torch.float64, paths: int = 1024, length:
     g = torch.Generator(device="cpu").manual_seed(1234)
     inc = (torch.randn(paths * length, width, generator=g, dtype=torch.float64) / length ** 0.5).to(dev, dtype)
     off = torch.arange(paths + 1, dtype=torch.int64, device=dev) * length
+    # the queue order on the device, so no call synchronises with the host
+    order = ops.path_order(off)
     cands = [(t, s, k, e) for (w, n, t, s, k, e) in ops.signature_tiles(dtype) if (w, n) == (width, depth)]
     cands.append((-1, 0, 0, 0))  # the storage-generic kernel
     thread_bwd = ops.thread_backward_tiles(width, depth, dtype)
@@ -73,11 +78,11 @@ def tune(width: int, depth: int, dtype=torch.float64, paths: int = 1024, length:
         key = f"{t},{s},{k},{e}"
         result["forward"][key] = _time(
             lambda: ops.signature(inc, off, basis, out=sig, tile_threads=t, tile_split=s, tile_kind=k,
-                                  tile_expand=e), reps)
+                                  tile_expand=e, order=order), reps)
         if backward and ((k == 0 and s <= 1) or (k == 2 and (t, s) in thread_bwd)):
             result["backward"][key] = _time(
                 lambda: ops.signature_backward(inc, off, basis, sig, ell, tile_threads=t, tile_split=s,
-                                               tile_expand=e, tile_kind=k), reps)
+                                               tile_expand=e, tile_kind=k, order=order), reps)
     entry = {}
     for kind in ("forward", "backward"):
         if result[kind]:
