@@ -620,9 +620,10 @@ int rpp_hip_signature_tiles(int32_t dtype_bytes, int32_t* out, int32_t capacity)
 
 /* The tile that serves plain ft_fmexp (degree-one exponent, full output
  * window, unit coefficient stride) for (width, depth) and the dtype: returns
- * 2 for a coalesced streaming tile (k_fmexp_coal), writing its threads and
- * leading letters fixed, or 0 when a generic kernel serves it.  No reference
- * counterpart (introspection of this strategy). */
+ * 2 for a coalesced subtree tile (k_fmexp_coal), writing its threads and
+ * leading letters fixed, 3 for a sibling-group wavefront tile (k_fmexp_wave),
+ * writing its threads and 0, or 0 when a generic kernel serves it.  No
+ * reference counterpart (introspection of this strategy). */
 int rpp_hip_fmexp_tile(int32_t dtype_bytes, int32_t width, int32_t depth, int32_t* threads, int32_t* split);
 
 /* ------------------------------------------------------------------ misc */

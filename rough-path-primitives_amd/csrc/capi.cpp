@@ -1083,14 +1083,14 @@ int64_t rpp_hip_signature_backward_scratch_bytes(const rpp_hip_signature_backwar
 
 extern "C" int rpp_hip_fmexp_tile(int32_t dtype_bytes, int32_t width, int32_t depth, int32_t* threads,
                                   int32_t* split) {
-    auto report = [&](const auto* t, int kind) {
+    auto report = [&](const auto* t) {
         if (!t) return 0;
         if (threads) *threads = t->threads;
         if (split) *split = t->split;
-        return kind;
+        return t->kind;
     };
-    if (dtype_bytes == 4) return report(rpp_hip::fmexp_coal_tile<float>(width, depth), 2);
-    if (dtype_bytes == 8) return report(rpp_hip::fmexp_coal_tile<double>(width, depth), 2);
+    if (dtype_bytes == 4) return report(rpp_hip::fmexp_coal_tile<float>(width, depth));
+    if (dtype_bytes == 8) return report(rpp_hip::fmexp_coal_tile<double>(width, depth));
     return rpp_hip::fail(RPP_HIP_INVALID_ARGUMENT, "dtype_bytes must be 4 or 8");
 }
 

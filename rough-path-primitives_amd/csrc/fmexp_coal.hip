@@ -148,6 +148,119 @@ k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int6
     }
 }
 
+// Sibling-group wavefronts (tile kind 3).  A wavefront owns 64 consecutive
+// sibling groups (the W children of one parent word) of one level k of one
+// object.  Lane g forms the Horner value of target k along its group's
+// parent p, A^(k)_{k-1}[p], from the chain words S_0, S_1[p_1], ..,
+// S_{k-1}[p] read straight from memory (neighbouring lanes share most of
+// them, so they come from the caches); the wavefront then writes the 64 W
+// children in coalesced order, lane l taking word l + 64 s and its parent's
+// value by a lane shuffle.  Every load and store of the object's own words
+// covers 64 consecutive coefficients and no work-group barrier is needed:
+// the exponent row is staged in a per-wavefront LDS slot, which replaces
+// W + k - 1 lane-varying global reads per wavefront with LDS reads.  The
+// roundings are horner_ref's in the order k_fmexp_coal uses, so both tiles
+// equal the reference's ft_fmexp bit for bit.  Measured on MI355X against the
+// subtree tile, both built from this file and timed in alternating rounds
+// (microseconds per call, wavefront / subtree): FP64 width 6 depth 6 over 2048
+// objects 322 / 332, width 8 depth 5 over 2048 objects 211 / 221; slower at
+// width 5 depth 6 (252 / 235), width 3 depth 8, width 4 depth 4 and in FP32
+// (width 8 depth 5 110 / 111, width 6 depth 6 226 / 175), where a
+// wavefront's 64 groups hold fewer bytes for its fixed chain and staging work.
+template <int W, int N>
+struct WaveGeo {
+    static constexpr int groups(int k) { return cpow(W, k - 1); }  // sibling groups of level k
+    static constexpr int chunks(int k) { return (groups(k) + 63) / 64; }
+    static constexpr int woff(int k) {
+        int o = 0;
+        for (int j = 1; j < k; ++j) o += chunks(j);
+        return o;
+    }
+    static constexpr int per_object = woff(N + 1);  // wavefronts per object
+    static constexpr int start(int k) {
+        int o = 0;
+        for (int j = 0; j < k; ++j) o += cpow(W, j);
+        return o;
+    }
+};
+
+template <typename T, int W, int N, int K>
+__device__ __forceinline__ void wave_level(T* __restrict__ dst, const T* __restrict__ src, const T* xs, int mn,
+                                           int mx, int c, int lane) {
+    using G = WaveGeo<W, N>;
+    auto has = [&](int i) { return mn <= i && i <= mx; };
+    constexpr int NG = G::groups(K);
+    const int g0 = c * 64;
+    const int nwords = (NG - g0 < 64 ? NG - g0 : 64) * W;
+    const int base = G::start(K) + g0 * W;
+    T own[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+        const int j = lane + 64 * s;
+        own[s] = (j < nwords && has(K)) ? src[base + j] : T(0);
+    }
+    // the lanes past the last group repeat it, so every read stays in range
+    const int p = g0 + lane < NG ? g0 + lane : NG - 1;
+    T pre[K], xl[K], xo[W];
+    pre[0] = has(0) ? src[0] : T(0);
+#pragma unroll
+    for (int i = 1; i < K; ++i) {
+        const int pi = p / cpow(W, K - 1 - i);
+        pre[i] = has(i) ? src[G::start(i) + pi] : T(0);
+        xl[i] = xs[pi % W];
+    }
+    // every exponent letter is read from LDS before the dependent steps
+#pragma unroll
+    for (int s = 0; s < W; ++s) xo[s] = xs[(lane + 64 * s) % W];
+    T a = K == N ? pre[0] : pre[0] + T(0);
+#pragma unroll
+    for (int i = 1; i < K; ++i) a = horner_ref(a, xl[i], recip<T>(K - i + 1), pre[i], has(i));
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+        const int j = lane + 64 * s;
+        const T ap = __shfl(a, j / W, 64);
+        if (j < nwords) dst[base + j] = horner_ref(ap, xo[s], T(1), own[s], has(K));
+    }
+    if (K == 1 && lane == 0) dst[0] = pre[0] + T(0);  // the reference's last step leaves S_0 + 0
+}
+
+// One (object, level, chunk) per wavefront; unit coefficient strides.
+template <typename T, int W, int N, int NT>
+__global__ void __launch_bounds__(NT)
+k_fmexp_wave(T* __restrict__ out, int64_t obs, const T* __restrict__ mult, int64_t mbs, int mn, int mx,
+             const T* __restrict__ ex, int64_t ebs, int64_t n) {
+    using G = WaveGeo<W, N>;
+    static_assert(NT % 64 == 0, "whole wavefronts");
+    __shared__ T xsl[NT / 64][W];
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int64_t gw = static_cast<int64_t>(blockIdx.x) * (NT / 64) + wv;
+    const int64_t b = gw / G::per_object;
+    if (b >= n) return;
+    const int wl = static_cast<int>(gw - b * G::per_object);
+    T* xs = xsl[wv];
+    if (lane < W) xs[lane] = ex[b * ebs + 1 + lane];
+    __builtin_amdgcn_wave_barrier();
+    static_for<N>([&](auto I) {
+        constexpr int K = decltype(I)::value + 1;
+        if (wl >= G::woff(K) && wl < G::woff(K + 1))
+            wave_level<T, W, N, K>(out + b * obs, mult + b * mbs, xs, mn, mx, wl - G::woff(K), lane);
+    });
+}
+
+template <typename T, int W, int N, int NT>
+int wave_launch(const DevBasis&, const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& mult,
+                const rpp_hip_tensor_batch& ex, int64_t n, hipStream_t stream) {
+    using G = WaveGeo<W, N>;
+    const int64_t blocks = (n * G::per_object + NT / 64 - 1) / (NT / 64);
+    if (blocks >= (int64_t{1} << 31))
+        return fail(RPP_HIP_NOT_IMPLEMENTED, "too many objects for one k_fmexp_wave launch");
+    k_fmexp_wave<T, W, N, NT><<<static_cast<unsigned>(blocks), NT, 0, stream>>>(
+        static_cast<T*>(out.data), out.batch_stride, static_cast<const T*>(mult.data), mult.batch_stride,
+        mult.min_degree, mult.max_degree, static_cast<const T*>(ex.data), ex.batch_stride, n);
+    return check_hip(hipGetLastError(), "k_fmexp_wave launch");
+}
+
 template <typename T, int W, int N, int NT, int M, int BPC>
 int coal_launch(const DevBasis& B, const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& mult,
                 const rpp_hip_tensor_batch& ex, int64_t n, hipStream_t stream) {
@@ -163,11 +276,13 @@ int coal_launch(const DevBasis& B, const rpp_hip_tensor_batch& out, const rpp_hi
 }  // namespace
 
 #define RPP_COAL(T, W, N, NT, M, BPC) \
-    FmexpTile<T> { sizeof(T), W, N, NT, M, BPC, &coal_launch<T, W, N, NT, M, BPC> }
+    FmexpTile<T> { 2, sizeof(T), W, N, NT, M, BPC, &coal_launch<T, W, N, NT, M, BPC> }
+#define RPP_WAVE(T, W, N, NT) \
+    FmexpTile<T> { 3, sizeof(T), W, N, NT, 0, 0, &wave_launch<T, W, N, NT> }
 
 // One tile per (width, depth) serves plain ft_fmexp, the fastest of the
-// candidates timed on MI355X over 32 chained calls (threads, split: FP64 / FP32
-// microseconds per call):
+// candidates timed on MI355X.  Subtree tiles over 32 chained calls (threads,
+// split: FP64 / FP32 microseconds per call):
 //   width 5 depth 6, 4096 objects:  (128,2) 249 / 190, (64,2) 250 / 166, (256,1) 275 / 126
 //   width 6 depth 6, 2048 objects:  (128,2) 338 / 179, (256,2) 341 / 181, (256,1) 414 / 283
 //   width 8 depth 5, 2048 objects:  (256,1) 236 / 118, (256,2) 236 / 187, (128,2) 242 / 124
@@ -175,13 +290,15 @@ int coal_launch(const DevBasis& B, const rpp_hip_tensor_batch& out, const rpp_hi
 //   width 4 depth 4, 2^17 objects:  (64,0) 143 / 75, (64,1) 213 / 189
 // An FP32 item moves half the bytes of an FP64 one in the same number of
 // dependent steps, so FP32 prefers fewer, larger items (one prefix letter).
+// Sibling-group wavefront tiles (512 threads) replace them where they measured
+// faster (k_fmexp_wave above): FP64 at width 6 depth 6 and width 8 depth 5.
 template <typename T>
 const std::vector<FmexpTile<T>>& fmexp_coal_tiles() {
     constexpr bool f32 = sizeof(T) == 4;
     static const std::vector<FmexpTile<T>> tiles = {
         f32 ? RPP_COAL(T, 5, 6, 256, 1, 0) : RPP_COAL(T, 5, 6, 128, 2, 0),
-        RPP_COAL(T, 6, 6, 128, 2, 0),
-        RPP_COAL(T, 8, 5, 256, 1, 0),
+        f32 ? RPP_COAL(T, 6, 6, 128, 2, 0) : RPP_WAVE(T, 6, 6, 512),
+        f32 ? RPP_COAL(T, 8, 5, 256, 1, 0) : RPP_WAVE(T, 8, 5, 512),
         RPP_COAL(T, 3, 8, 256, 1, 0),
         RPP_COAL(T, 4, 4, 64, 0, 0),
     };

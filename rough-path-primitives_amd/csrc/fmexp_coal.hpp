@@ -1,4 +1,4 @@
-// Coalesced-ownership tiles of the plain ft_fmexp call (fmexp_coal.hip).
+// Streaming tiles of the plain ft_fmexp call (fmexp_coal.hip).
 #pragma once
 
 #include <vector>
@@ -9,11 +9,12 @@ namespace rpp_hip {
 
 template <typename T>
 struct FmexpTile {
+    int kind;  // 2: coalesced subtree items (k_fmexp_coal), 3: sibling-group wavefronts (k_fmexp_wave)
     int dtype_bytes;
     int width;
     int depth;
     int threads;        // work-group size
-    int split;          // leading letters fixed per work item
+    int split;          // leading letters fixed per work item (kind 2)
     int blocks_per_cu;  // grid cap per compute unit (0: one work-group per item)
     int (*launch)(const DevBasis&, const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& mult,
                   const rpp_hip_tensor_batch& ex, int64_t n, hipStream_t);

@@ -583,13 +583,16 @@ def lie_basis_load(path) -> tuple:
 
 def fmexp_tile(width: int, depth: int, dtype):
     """(kernel, threads, split) of the tile serving plain ft_fmexp at
-    (width, depth): kernel "k_fmexp_coal"; None when a generic kernel serves
-    it."""
+    (width, depth): kernel "k_fmexp_coal" (subtree items, split leading
+    letters) or "k_fmexp_wave" (sibling-group wavefronts, split 0); None when
+    a generic kernel serves it."""
     nbytes = torch.tensor([], dtype=dtype).element_size()
     th, sp = C.c_int32(0), C.c_int32(0)
     kind = A.lib().rpp_hip_fmexp_tile(nbytes, width, depth, C.byref(th), C.byref(sp))
     if kind == 2:
         return "k_fmexp_coal", th.value, sp.value
+    if kind == 3:
+        return "k_fmexp_wave", th.value, sp.value
     return None
 
 

@@ -166,7 +166,7 @@ def test_adjoint_multiplications(dev, side, windows, dtype):
 @pytest.mark.parametrize("shape", [(5, 6, 64, 16), (6, 6, 24, 6), (8, 5, 24, 6), (3, 8, 64, 8), (4, 4, 300, 8)],
                          ids=["w5d6", "w6d6", "w8d5", "w3d8", "w4d4"])
 def test_chained_fmexp_c2_shape(dev, shape, dtype):
-    """The plain chain at every shape with a coalesced-ownership tile (width 5,
+    """The plain chain at every shape with a streaming tile (width 5,
     depth 6 is the headline shape): paths x increments of chained ft_fmexp
     against the oracle chain, bit for bit."""
     w, N, n, L = shape
@@ -185,6 +185,33 @@ def test_chained_fmexp_c2_shape(dev, shape, dtype):
         rpp_hip.ft_fmexp(nxt, cur, rpp_hip.Batch(pad_d[:, t * w: t * w + 1 + w], 1, 1), basis)
         cur, nxt = nxt, cur
     assert_exact(cur.cpu().numpy(), expected, dtype, "C2 chain")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("shape", [(5, 6), (6, 6), (8, 5), (3, 8), (4, 4)], ids=["w5d6", "w6d6", "w8d5", "w3d8", "w4d4"])
+@pytest.mark.parametrize("mw", [(0, 0), (1, 0), (0, -1), (2, 0), (2, -1)],
+                         ids=["full", "from1", "low", "from2", "mid"])
+@pytest.mark.parametrize("n", [1, 7])
+def test_ft_fmexp_streaming_tiles_multiplier_windows(dev, shape, mw, n, dtype):
+    """The streaming tiles (subtree items or sibling-group wavefronts, by
+    shape and dtype) with a degree-one exponent and a full output window, for
+    multiplier windows that drop low, high and middle degrees (zero-extended,
+    never read outside: NaN there), against the oracle bit for bit."""
+    w, N = shape
+    assert rpp_hip.fmexp_tile(w, N, torch.float64 if dtype == np.float64 else torch.float32) is not None
+    lo, hi = mw[0], N + mw[1]
+    rng = np.random.default_rng(700 + 10 * w + N + n)
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    mult = damped_batch(rng, n, w, N, dtype)
+    mult[:, ~window_mask(w, N, (lo, hi))] = np.nan
+    ex = deg1(rng, n, w, N, dtype)
+    ex[:, 0] = np.nan
+    expected = np.full((n, dim), np.nan, dtype=dtype)
+    O.orc.ft_fmexp(w, N, expected, mult, ex, mw=(lo, hi), ew=(1, 1))
+    out = to_dev(np.full((n, dim), np.nan, dtype=dtype), dev)
+    rpp_hip.ft_fmexp(out, rpp_hip.Batch(to_dev(mult, dev), lo, hi), rpp_hip.Batch(to_dev(ex, dev), 1, 1), basis)
+    assert_exact(out.cpu().numpy(), expected, dtype, f"fmexp tile {shape} mult {(lo, hi)}")
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
