@@ -7,22 +7,26 @@
 // there the top degrees of both factors enter element-wise: out_N[w] =
 // sum_{j<N} P_{N-j}[w / d^j] Y_j[w mod d^j] + P_0 Y_N[w], and nothing reads
 // out_N afterwards.  So one work-group keeps only degrees 0..N-1 of Y and of
-// P in LDS (2 start_of_degree(N) values: about 2/width of the object), reads
-// Y_N from HBM at the last step and writes every degree of that step to HBM
-// directly.  Each earlier step computes its new coefficients into registers
-// from the old P and Y, then writes them back: two barriers per step.  HBM
+// P in LDS (about 1/width of the object each), reads Y_N from HBM at the last
+// step and writes every degree of that step to HBM directly.  P is kept twice:
+// each earlier step reads one copy and writes its new coefficients into the
+// other, so a step ends with one barrier and holds nothing in registers.  HBM
 // sees one read of arg and one write of out per object.  A work-group loads
 // the lower degrees of its next object into registers as soon as the current
 // one is staged, so that read overlaps the Horner steps, and the last step
-// reads each top-degree argument word before the LDS reads of its sum (width
-// 3, depth 8, 16384 objects: FP64 1.14 -> 0.96 ms, FP32 0.91 -> 0.81 ms;
-// results unchanged bit for bit, as the arithmetic is the same).  The shape is fixed
-// at compile time, so every degree offset and division by width^j is a
-// constant.  Each sum adds its terms in the order of the reference's CPU
-// product (cpu_term_no_rhs0: ascending rhs degree below the first right-tiled
-// degree, the right-tiled order at and above it), every product and sum
-// rounded on its own (contraction off), so the logarithm agrees with the
-// reference's cpu::single_thread ft_log bit for bit.
+// reads each top-degree argument word before the LDS reads of its sum.  The
+// shape is fixed at compile time, so every degree offset and division by
+// width^j is a constant.  Each sum adds its terms in the order of the
+// reference's CPU product (cpu_term_no_rhs0: ascending rhs degree below the
+// first right-tiled degree, the right-tiled order at and above it), every
+// product and sum rounded on its own (contraction off), so the logarithm
+// agrees with the reference's cpu::single_thread ft_log bit for bit.
+//
+// Width 3, depth 8 over 8256 objects on MI355X, microseconds per call: FP64
+// 558 with P updated in place (two barriers per step, new values staged in
+// registers, one work-group per compute unit) -> 339 double-buffered (75
+// registers, held to 64 so that two work-groups share a compute unit); FP32
+// 630 -> 498.  Every one of the 16 compiled shapes gains, 1-50 %.
 #include "common.hpp"
 #include "log_lds.hpp"
 
@@ -51,40 +55,37 @@ struct LogGeo {
     static constexpr int dim = start(N + 1);
     static constexpr int low = start(N);  // degrees 0..N-1, the LDS-resident part
     static constexpr int slots(int k, int nt) { return (words(k) + nt - 1) / nt; }
-    static constexpr int reg_offset(int k, int nt) {
-        int o = 0;
-        for (int i = 1; i < k; ++i) o += slots(i, nt);
-        return o;
-    }
 };
 
-// LDS of one work-group: Y and P on degrees 0..N-1.
+// LDS of one work-group: Y and two copies of P on degrees 0..N-1.
 template <typename T, int W, int N>
 constexpr size_t log_lds_bytes() {
-    return 2 * static_cast<size_t>(LogGeo<W, N>::low) * sizeof(T);
+    return 3 * static_cast<size_t>(LogGeo<W, N>::low) * sizeof(T);
 }
 
-// At most 80 KiB, so that LDS leaves room for two work-groups per compute
-// unit, whose barrier-separated steps then overlap; registers decide whether
-// two are resident (below).
-constexpr size_t kLogLdsCap = 80 * 1024;
+constexpr size_t kLogLdsMax = 160 * 1024;
 
-// 1024 threads for objects of more than 4096 coefficients, 512 below.  At
-// width 3, depth 8 in FP64 (16384 objects) 1024 threads take about 100
-// registers, so one work-group per compute unit (1.13 ms); held to 64
-// registers for two, they spill and run slower (1.59 ms).  512 threads needed
-// 141 registers (1.77 ms).
+// Work-groups of a compute unit by LDS, and the waves per SIMD that many
+// work-groups of nt threads make (at most 8).
+template <typename T, int W, int N>
+constexpr int log_groups_per_cu() {
+    return static_cast<int>(kLogLdsMax / log_lds_bytes<T, W, N>());
+}
+template <typename T, int W, int N>
+constexpr int log_waves(int nt) {
+    const int w = log_groups_per_cu<T, W, N>() * nt / 256;
+    return w < 8 ? w : 8;
+}
+
+// 512 threads where LDS still allows 8 waves per SIMD with them, else 1024;
+// the kernel is compiled for the waves that LDS allows, so its registers do
+// not cap the occupancy.  Measured over the 16 compiled shapes in both types
+// (three candidates each: 512 or 1024 threads double-buffered, and the
+// in-place form): this rule takes the fastest in 31 of 32 cases, and is 3 %
+// behind at width 6 depth 5 FP64 (261 against 253 us with 1024 threads).
 template <typename T, int W, int N>
 constexpr int log_threads() {
-    return LogGeo<W, N>::dim > 4096 ? 1024 : 512;
-}
-
-// FP32 work-groups are held to 64 registers, so two share a compute unit
-// (width 3 depth 8 over 16384 objects: 0.98 -> 0.89 ms); in FP64 that limit
-// spills and one work-group per compute unit is faster.
-template <typename T>
-constexpr int log_waves() {
-    return sizeof(T) == 4 ? 8 : 1;
+    return log_waves<T, W, N>(512) >= 8 ? 512 : 1024;
 }
 
 // sum_u P_{K-j}[w / width^j] * Y_j[w mod width^j] over the terms u = 0 ..
@@ -111,17 +112,17 @@ __device__ __forceinline__ T log_level_sum(const T* ps, const T* ys, int w, int 
     return acc;
 }
 
-template <typename T, int W, int N, int NT>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(log_waves<T>())))
+template <typename T, int W, int N, int NT, int WPE>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_log_lds(View<T> out, View<const T> arg, int64_t n) {
     using G = LogGeo<W, N>;
     extern __shared__ __align__(16) unsigned char log_raw[];
     T* ys = reinterpret_cast<T*>(log_raw);
-    T* ps = ys + G::low;
+    T* pa = ys + G::low;
+    T* pb = pa + G::low;
     const int t = threadIdx.x;
     const int ylo = max(arg.mn, 1), yhi = min(arg.mx, N);
     const bool top = N >= ylo && N <= yhi;
-    T reg[G::reg_offset(N, NT) > 0 ? G::reg_offset(N, NT) : 1];
     // degrees 0..N-1 of the object to stage next: loaded while the previous
     // object computes, so its HBM latency is hidden behind the Horner steps
     constexpr int SL = (G::low + NT - 1) / NT;
@@ -146,15 +147,17 @@ k_log_lds(View<T> out, View<const T> arg, int64_t n) {
 #pragma unroll
         for (int s = 0; s < SL; ++s) {
             const int i = t + s * NT;
-            if (i < G::low) {
-                ys[i] = yv[s];
-                ps[i] = T(0);
-            }
+            if (i < G::low) ys[i] = yv[s];
         }
-        if (t == 0) ps[0] = (N % 2 == 0 ? T(-1) : T(1)) / T(N);
+        // the first step reads only P_0 = c_N
+        if (t == 0) pa[0] = (N % 2 == 0 ? T(-1) : T(1)) / T(N);
         __syncthreads();
         if (b + gridDim.x < n) load_low(b + gridDim.x);
-        // steps d = N..2 reach degree K = N - d + 1 <= N - 1
+        // steps d = N..2 reach degree K = N - d + 1 <= N - 1; each reads pc
+        // on degrees 0..K-1 and writes degrees 1..K of pn, and the next step
+        // reads pn on degrees 0..K only
+        T* pc = pa;
+        T* pn = pb;
 #pragma unroll 1
         for (int d = N; d >= 2; --d) {
             const int K = N - d + 1;
@@ -167,8 +170,7 @@ k_log_lds(View<T> out, View<const T> arg, int64_t n) {
                         for (int s = 0; s < G::slots(k, NT); ++s) {
                             const int w = t + s * NT;
                             if (G::words(k) % NT == 0 || w < G::words(k))
-                                reg[G::reg_offset(k, NT) + s] =
-                                    log_level_sum<T, W, N, k, false>(ps, ys, w, ylo, yhi, T(0), T(0), false);
+                                pn[G::start(k) + w] = log_level_sum<T, W, N, k, false>(pc, ys, w, ylo, yhi, T(0), T(0), false);
                             // keep the scheduler from hoisting every slot's LDS reads at
                             // once: the in-flight values would not fit in registers
                             __builtin_amdgcn_sched_barrier(0);
@@ -176,19 +178,12 @@ k_log_lds(View<T> out, View<const T> arg, int64_t n) {
                     }
                 }
             });
-            __syncthreads();
-#pragma unroll
-            for (int k = 1; k < N; ++k) {
-                if (k > K) continue;
-#pragma unroll
-                for (int s = 0; s < G::slots(k, NT); ++s) {
-                    const int w = t + s * NT;
-                    if (G::words(k) % NT == 0 || w < G::words(k)) ps[G::start(k) + w] = reg[G::reg_offset(k, NT) + s];
-                }
-            }
             // the next step's identity term (degree 0 of the product is 0)
-            if (t == 0) ps[0] = ((d - 1) % 2 == 0 ? T(-1) : T(1)) / T(d - 1);
+            if (t == 0) pn[0] = ((d - 1) % 2 == 0 ? T(-1) : T(1)) / T(d - 1);
             __syncthreads();
+            T* tmp = pc;
+            pc = pn;
+            pn = tmp;
         }
         // the last step (d = 1, P_0 = c_1 = 1) straight to HBM; degree 0 of
         // the product is 0
@@ -198,39 +193,61 @@ k_log_lds(View<T> out, View<const T> arg, int64_t n) {
             constexpr int k = decltype(kc)::value;
             if constexpr (k >= 1) {
                 for (int w = t; w < G::words(k); w += NT)
-                    o[G::start(k) + w] = log_level_sum<T, W, N, k, false>(ps, ys, w, ylo, yhi, T(0), T(0), false);
+                    o[G::start(k) + w] = log_level_sum<T, W, N, k, false>(pc, ys, w, ylo, yhi, T(0), T(0), false);
             }
         });
-        const T p0 = ps[0];
+        const T p0 = pc[0];
         for (int w = t; w < G::words(N); w += NT) {
             // issued before the LDS reads of the sum, whose latency it overlaps
             const T y = top ? a[G::start(N) + w] : T(0);
-            o[G::start(N) + w] = log_level_sum<T, W, N, N, true>(ps, ys, w, ylo, yhi, p0, y, top);
+            o[G::start(N) + w] = log_level_sum<T, W, N, N, true>(pc, ys, w, ylo, yhi, p0, y, top);
         }
     }
 }
 
-template <typename T, int W, int N>
-int log_launch(int64_t n, const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& arg, hipStream_t stream) {
+template <typename T, int W, int N, int NT, int WPE>
+int log_launch_with(int64_t n, const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& arg,
+                    hipStream_t stream) {
     constexpr size_t lds = log_lds_bytes<T, W, N>();
-    constexpr int NT = log_threads<T, W, N>();
-    static const bool raised = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_log_lds<T, W, N, NT>),
+    static_assert(lds <= kLogLdsMax, "the LDS of one compute unit");
+    const auto kernel = &k_log_lds<T, W, N, NT, WPE>;
+    static const bool raised = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                                    static_cast<int>(lds)) == hipSuccess;
     if (!raised && lds > 64 * 1024) return fail(RPP_HIP_OUT_OF_RESOURCES, "cannot raise the LDS limit");
-    const int64_t per_cu = std::max<int64_t>(1, static_cast<int64_t>((160 * 1024) / lds));
+    // objects are dealt to work-groups in turn, so the grid is what the
+    // compute units hold at once (threads, registers and this LDS size)
+    int resident = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kernel, NT, lds) != hipSuccess) resident = 1;
+    const int64_t per_cu = std::max(1, resident);
     const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(n, device_cu_count() * per_cu)));
-    k_log_lds<T, W, N, NT><<<grid, NT, lds, stream>>>(make_view<T>(out), make_cview<T>(arg), n);
+    kernel<<<grid, NT, lds, stream>>>(make_view<T>(out), make_cview<T>(arg), n);
     return check_hip(hipGetLastError(), "k_log_lds launch");
+}
+
+template <typename T, int W, int N>
+int log_launch(int64_t n, const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& arg, hipStream_t stream) {
+    constexpr int NT = log_threads<T, W, N>();
+    return log_launch_with<T, W, N, NT, log_waves<T, W, N>(NT)>(n, out, arg, stream);
 }
 
 template <typename T>
 using LogFn = int (*)(int64_t, const rpp_hip_tensor_batch&, const rpp_hip_tensor_batch&, hipStream_t);
 
+// The launcher of a shape whose LDS fits one compute unit, else nullptr.
+template <typename T, int W, int N>
+constexpr LogFn<T> log_fn() {
+    if constexpr (log_lds_bytes<T, W, N>() <= kLogLdsMax) {
+        return &log_launch<T, W, N>;
+    } else {
+        return nullptr;
+    }
+}
+
 template <typename T>
 LogFn<T> log_kernel(int width, int depth) {
 #define RPP_LOG(W, N) \
-    if (width == W && depth == N && log_lds_bytes<T, W, N>() <= kLogLdsCap) return &log_launch<T, W, N>;
+    if (width == W && depth == N) return log_fn<T, W, N>();
     RPP_LOG(2, 8) RPP_LOG(2, 10) RPP_LOG(2, 12) RPP_LOG(3, 6) RPP_LOG(3, 7) RPP_LOG(3, 8) RPP_LOG(4, 5)
     RPP_LOG(4, 6) RPP_LOG(4, 7) RPP_LOG(5, 4) RPP_LOG(5, 5) RPP_LOG(5, 6) RPP_LOG(6, 4) RPP_LOG(6, 5)
     RPP_LOG(8, 4) RPP_LOG(8, 5)

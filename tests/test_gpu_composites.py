@@ -364,3 +364,23 @@ def test_ft_log_lds_shapes(dev, shape, aw, dtype):
     got = out.cpu().numpy()
     assert_close(got, expected, dtype, f"log {shape} {aw}")
     assert np.array_equal(got, expected), f"log {shape} {aw}: not bit-identical, {rel_err(got, expected):.3e}"
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("shape,n", [((3, 8), 2600), ((2, 8), 9000), ((5, 6), 1100)], ids=["w3d8", "w2d8", "w5d6"])
+def test_ft_log_lds_many_objects_per_work_group(dev, shape, n, dtype):
+    """More objects than the compute units hold work-groups (at most 8 per
+    unit at these shapes), so each work-group logs several objects in turn:
+    the next object's staging loads, the alternating copies of P and the
+    reuse of LDS across objects, against the oracle bit for bit."""
+    w, N = shape
+    rng = np.random.default_rng(1300 + 10 * w + N)
+    basis = rpp_hip.TensorBasis(w, N)
+    dim = basis.size()
+    arg = damped_batch(rng, n, w, N, dtype)
+    expected = np.full((n, dim), np.nan, dtype=dtype)
+    O.orc.ft_log(w, N, expected, arg)
+    out = to_dev(np.full((n, dim), np.nan, dtype=dtype), dev)
+    rpp_hip.ft_log(out, to_dev(arg, dev), basis)
+    got = out.cpu().numpy()
+    assert np.array_equal(got, expected), f"log {shape} x {n}: not bit-identical, {rel_err(got, expected):.3e}"
