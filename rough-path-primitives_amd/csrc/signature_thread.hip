@@ -152,11 +152,23 @@ constexpr int thread_bwd_waves() {
     return sizeof(T) == 4 && W == 6 && N == 6 && L == 4 ? 4 : 1;
 }
 
-template <typename T, int W, int N, int L, int NT>
+// CK: the forward also stores checkpoints.  Without them the step loop
+// carries no checkpoint state or stores, which frees registers: FP64 width 5
+// depth 6 (L = 4) 140 -> 108 (three waves per SIMD -> four), and the loop
+// takes the form the compiler schedules best per type, measured over the
+// benchmark shapes against the checkpointing loop (ms per launch): FP64 a
+// ring of register sets indexed by step parity ((5,6) 4096 x 128 paths
+// 1.23 -> 0.93, (6,6) 2048 x 256 2.55 -> 2.11, (3,8) 16384 x 64 1.47 -> 1.33,
+// (8,5) 8192 uneven 9.47 -> 8.14; the plain loop without checkpoints 1.00,
+// 2.30, 1.50, 8.78), FP32 the plain loop ((5,6) 0.61 -> 0.59, (6,6) 1.48 ->
+// 1.33, (8,5) 5.39 -> 5.02, (3,8) 0.84 -> 0.80; the ring 0.66, 1.34, 5.61,
+// 0.77).  The arithmetic is the same in every form, bit for bit.
+template <typename T, int W, int N, int L, int NT, bool CK>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(thread_fwd_waves<T, W, N, L>())))
-k_sig_fwd_thread(DevBasis B, int n_paths, const int64_t* order, const T* incr, const int64_t* offs, View<T> out,
+k_sig_fwd_thread(DevBasis B, int n_paths, const int64_t* order, const T* __restrict__ incr, const int64_t* offs, View<T> out,
                  int ck_every, T* ckpts, const int64_t* ck_offs) {
     using G = ThreadGeo<W, N, L>;
+    constexpr int LQ = L > 0 ? L : 1;
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * NT + threadIdx.x;
     // a wavefront lies inside one path's padded range (NT and PP are
     // multiples of 64), so its path index is uniform
@@ -167,7 +179,7 @@ k_sig_fwd_thread(DevBasis B, int n_paths, const int64_t* order, const T* incr, c
     const int q = live ? lane_q : 0;
     const int64_t path = order ? order[slot] : slot;
     const int64_t t0 = offs[path], t1 = offs[path + 1];
-    int ql[L > 0 ? L : 1];  // letters of q, first (most significant) first
+    int ql[LQ];  // letters of q, first (most significant) first
 #pragma unroll
     for (int i = 0; i < L; ++i) ql[i] = (q / tpow(W, L - 1 - i)) % W;
     T chain[L + 1];
@@ -177,38 +189,53 @@ k_sig_fwd_thread(DevBasis B, int n_paths, const int64_t* order, const T* incr, c
     T sub[G::total];
 #pragma unroll
     for (int r = 0; r < G::total; ++r) sub[r] = T(0);
-    const int64_t dim = B.start[N + 1];
     // Increments one step ahead, in two register sets used in turn (the loop
     // is unrolled by two, so no copies): the row, uniform across the
     // wavefront, and the prefix letters' entries, loaded from the row pointer
     // with 32-bit lane offsets.
-    T xa[W], xqa[L > 0 ? L : 1], xb[W], xqb[L > 0 ? L : 1];
-    auto load = [&](T (&x)[W], T (&xq)[L > 0 ? L : 1], int64_t t) {
+    auto load = [&](T (&x)[W], T (&xq)[LQ], int64_t t) {
         const T* row = incr + t * W;
 #pragma unroll
         for (int b = 0; b < W; ++b) x[b] = row[b];
 #pragma unroll
         for (int i = 0; i < L; ++i) xq[i] = row[ql[i]];
     };
-    // checkpoints: a countdown instead of a division per increment
-    int ck_left = ck_every;
-    int64_t ck_index = ck_every > 0 ? ck_offs[path] : 0;
-    auto checkpoint = [&]() {
-        if (ck_every > 0 && --ck_left == 0) {
-            ck_left = ck_every;
-            if (live) thread_store<T, W, N, L>(B, ckpts + ck_index * dim, 1, q, chain, sub);
-            ++ck_index;
+    if constexpr (!CK && sizeof(T) == 8) {
+        T xr[2][W], xqr[2][LQ];
+        if (t0 < t1) load(xr[0], xqr[0], t0);
+        for (int64_t t = t0; t < t1; t += 2) {
+            static_for<2>([&](auto I) {
+                constexpr int s = decltype(I)::value;
+                const int64_t tt = t + s;
+                if (tt < t1) {
+                    if (tt + 1 < t1) load(xr[s ^ 1], xqr[s ^ 1], tt + 1);
+                    thread_step<T, W, N, L>(chain, sub, xr[s], xqr[s]);
+                }
+            });
         }
-    };
-    if (t0 < t1) load(xa, xqa, t0);
-    for (int64_t t = t0; t < t1; t += 2) {
-        if (t + 1 < t1) load(xb, xqb, t + 1);
-        thread_step<T, W, N, L>(chain, sub, xa, xqa);
-        checkpoint();
-        if (t + 1 >= t1) break;
-        if (t + 2 < t1) load(xa, xqa, t + 2);
-        thread_step<T, W, N, L>(chain, sub, xb, xqb);
-        checkpoint();
+    } else {
+        T xa[W], xqa[LQ], xb[W], xqb[LQ];
+        // checkpoints: a countdown instead of a division per increment
+        int ck_left = ck_every;
+        int64_t ck_index = CK && ck_every > 0 ? ck_offs[path] : 0;
+        const int64_t dim = B.start[N + 1];
+        auto checkpoint = [&]() {
+            if (CK && ck_every > 0 && --ck_left == 0) {
+                ck_left = ck_every;
+                if (live) thread_store<T, W, N, L>(B, ckpts + ck_index * dim, 1, q, chain, sub);
+                ++ck_index;
+            }
+        };
+        if (t0 < t1) load(xa, xqa, t0);
+        for (int64_t t = t0; t < t1; t += 2) {
+            if (t + 1 < t1) load(xb, xqb, t + 1);
+            thread_step<T, W, N, L>(chain, sub, xa, xqa);
+            checkpoint();
+            if (t + 1 >= t1) break;
+            if (t + 2 < t1) load(xa, xqa, t + 2);
+            thread_step<T, W, N, L>(chain, sub, xb, xqb);
+            checkpoint();
+        }
     }
     if (live) thread_store<T, W, N, L>(B, out.obj(path), out.cs, q, chain, sub);
 }
@@ -221,9 +248,11 @@ int thread_fwd_launch(const DevBasis& B, const rpp_hip_signature_desc& d, const 
         return fail(RPP_HIP_NOT_IMPLEMENTED, "too many paths for one thread-subtree launch");
     const int64_t threads = d.num_paths * G::PP;
     const unsigned grid = static_cast<unsigned>((threads + NT - 1) / NT);
-    k_sig_fwd_thread<T, W, N, L, NT><<<grid, NT, 0, stream>>>(
-        B, static_cast<int>(d.num_paths), order, static_cast<const T*>(d.increments), d.offsets, make_view<T>(d.out),
-        d.checkpoint_every, static_cast<T*>(d.checkpoints), d.ckpt_offsets);
+    const auto kernel = d.checkpoint_every > 0 ? &k_sig_fwd_thread<T, W, N, L, NT, true>
+                                               : &k_sig_fwd_thread<T, W, N, L, NT, false>;
+    kernel<<<grid, NT, 0, stream>>>(B, static_cast<int>(d.num_paths), order, static_cast<const T*>(d.increments),
+                                    d.offsets, make_view<T>(d.out), d.checkpoint_every,
+                                    static_cast<T*>(d.checkpoints), d.ckpt_offsets);
     return check_hip(hipGetLastError(), "k_sig_fwd_thread launch");
 }
 
