@@ -53,10 +53,18 @@ struct CoalGeo {
     static constexpr int P = cpow(W, M);
 };
 
-template <typename T, int W, int N, int NT, int M>
+// FULL: the multiplier window holds every degree, so its tests fold away.
+template <typename T, int W, int N, int NT, int M, bool FULL>
 __global__ void __launch_bounds__(NT)
-k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int64_t n) {
+k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult_, View<const T> ex, int64_t n) {
     using G = CoalGeo<W, N, NT, M>;
+    struct Mult {
+        View<const T> v;
+        __device__ const T* obj(int64_t b) const { return v.obj(b); }
+        __device__ bool has(int k) const { return FULL || v.has(k); }
+    };
+    const Mult mult{mult_};
+    const int64_t mcs = mult_.cs;
     __shared__ T lds0[G::lds_elems];
     __shared__ T lds1[G::lds_elems];
     __shared__ T xs[W];
@@ -73,16 +81,16 @@ k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult, View<const T> ex, int6
 #pragma unroll
         for (int i = 0; i <= M; ++i) {
             const int pre = q / cpow(W, M - i);
-            chain[i] = mult.has(i) ? src[(B.start[i] + pre) * mult.cs] : T(0);
+            chain[i] = mult.has(i) ? src[(B.start[i] + pre) * mcs] : T(0);
         }
 #pragma unroll
         for (int i = M + 1; i <= N; ++i) {
             const bool in = mult.has(i);
-            const T* lvl = src + (B.start[i] + static_cast<int64_t>(q) * G::words(i)) * mult.cs;
+            const T* lvl = src + (B.start[i] + static_cast<int64_t>(q) * G::words(i)) * mcs;
 #pragma unroll
             for (int s = 0; s < G::slots(i); ++s) {
                 const int r = t + s * NT;
-                if (G::words(i) % NT == 0 || r < G::words(i)) reg[G::offset(i) + s] = in ? lvl[r * mult.cs] : T(0);
+                if (G::words(i) % NT == 0 || r < G::words(i)) reg[G::offset(i) + s] = in ? lvl[r * mcs] : T(0);
             }
         }
         __syncthreads();  // the previous item's readers of xs and LDS are done
@@ -261,22 +269,28 @@ int wave_launch(const DevBasis&, const rpp_hip_tensor_batch& out, const rpp_hip_
     return check_hip(hipGetLastError(), "k_fmexp_wave launch");
 }
 
-template <typename T, int W, int N, int NT, int M, int BPC>
+// FW: the tile also has a full-window instance, taken when the multiplier
+// holds every degree (faster only at some shapes; see the tile table).
+template <typename T, int W, int N, int NT, int M, int BPC, bool FW>
 int coal_launch(const DevBasis& B, const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& mult,
                 const rpp_hip_tensor_batch& ex, int64_t n, hipStream_t stream) {
     using G = CoalGeo<W, N, NT, M>;
     const int64_t items = n * G::P;
     const int64_t cap = BPC > 0 ? static_cast<int64_t>(device_cu_count()) * BPC : (int64_t{1} << 30);
     const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min(items, cap)));
-    k_fmexp_coal<T, W, N, NT, M><<<grid, NT, 0, stream>>>(B, make_view<T>(out), make_cview<T>(mult),
-                                                          make_cview<T>(ex), n);
+    if (FW && mult.min_degree <= 0 && mult.max_degree >= N)
+        k_fmexp_coal<T, W, N, NT, M, true><<<grid, NT, 0, stream>>>(B, make_view<T>(out), make_cview<T>(mult),
+                                                                    make_cview<T>(ex), n);
+    else
+        k_fmexp_coal<T, W, N, NT, M, false><<<grid, NT, 0, stream>>>(B, make_view<T>(out), make_cview<T>(mult),
+                                                                     make_cview<T>(ex), n);
     return check_hip(hipGetLastError(), "k_fmexp_coal launch");
 }
 
 }  // namespace
 
-#define RPP_COAL(T, W, N, NT, M, BPC) \
-    FmexpTile<T> { 2, sizeof(T), W, N, NT, M, BPC, &coal_launch<T, W, N, NT, M, BPC> }
+#define RPP_COAL(T, W, N, NT, M, BPC, FW) \
+    FmexpTile<T> { 2, sizeof(T), W, N, NT, M, BPC, &coal_launch<T, W, N, NT, M, BPC, FW> }
 #define RPP_WAVE(T, W, N, NT) \
     FmexpTile<T> { 3, sizeof(T), W, N, NT, 0, 0, &wave_launch<T, W, N, NT> }
 
@@ -292,15 +306,21 @@ int coal_launch(const DevBasis& B, const rpp_hip_tensor_batch& out, const rpp_hi
 // dependent steps, so FP32 prefers fewer, larger items (one prefix letter).
 // Sibling-group wavefront tiles (512 threads) replace them where they measured
 // faster (k_fmexp_wave above): FP64 at width 6 depth 6 and width 8 depth 5.
+// The full-window instance (last field), against the windowed one on a
+// full-window multiplier, microseconds per call:
+//   width 3 depth 8, 8192 objects:  FP64 252 -> 237, FP32 154 -> 134
+//   width 6 depth 6, 2048 objects:  FP32 176 -> 168
+//   width 5 depth 6, 4096 objects:  FP32 119 -> 114, FP64 237 -> 269 (not taken)
+//   width 8 depth 5, 2048 objects:  FP32 111 -> 116 (not taken)
 template <typename T>
 const std::vector<FmexpTile<T>>& fmexp_coal_tiles() {
     constexpr bool f32 = sizeof(T) == 4;
     static const std::vector<FmexpTile<T>> tiles = {
-        f32 ? RPP_COAL(T, 5, 6, 256, 1, 0) : RPP_COAL(T, 5, 6, 128, 2, 0),
-        f32 ? RPP_COAL(T, 6, 6, 128, 2, 0) : RPP_WAVE(T, 6, 6, 512),
-        f32 ? RPP_COAL(T, 8, 5, 256, 1, 0) : RPP_WAVE(T, 8, 5, 512),
-        RPP_COAL(T, 3, 8, 256, 1, 0),
-        RPP_COAL(T, 4, 4, 64, 0, 0),
+        f32 ? RPP_COAL(T, 5, 6, 256, 1, 0, true) : RPP_COAL(T, 5, 6, 128, 2, 0, false),
+        f32 ? RPP_COAL(T, 6, 6, 128, 2, 0, true) : RPP_WAVE(T, 6, 6, 512),
+        f32 ? RPP_COAL(T, 8, 5, 256, 1, 0, false) : RPP_WAVE(T, 8, 5, 512),
+        RPP_COAL(T, 3, 8, 256, 1, 0, true),
+        RPP_COAL(T, 4, 4, 64, 0, 0, false),
     };
     return tiles;
 }
