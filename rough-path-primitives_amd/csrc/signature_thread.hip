@@ -343,13 +343,18 @@ __device__ __forceinline__ T lanes_move(T v, F&& f) {
     }
 }
 
-// x + the value of the lane that the DPP control CTRL names (within a row of
-// 16 lanes).
+// The value of the lane that the DPP control CTRL names (within a row of 16
+// lanes).  Every control used here names a lane for every lane, so the move
+// needs no prior value in its destination.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_move(uint32_t u) {
+    return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(u), CTRL, 0xf, 0xf, true));
+}
+
+// x + the value of the lane that the DPP control CTRL names.
 template <int CTRL, typename T>
 __device__ __forceinline__ T add_dpp(T x) {
-    return x + lanes_move(x, [](uint32_t u) {
-               return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(u), CTRL, 0xf, 0xf, false));
-           });
+    return x + lanes_move(x, [](uint32_t u) { return dpp_move<CTRL>(u); });
 }
 
 // x summed over the lane pairs that differ in lane bit HALF (32 or 16) in
@@ -416,10 +421,7 @@ struct WaveSlots {
             const bool up = (lane & 8) != 0;
             const T send = up ? s[0] : s[1];
             const T keep = up ? s[1] : s[0];
-            s[0] = keep + lanes_move(send, [](uint32_t u) {
-                       return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(u), 0x128, 0xf, 0xf,
-                                                                                false));  // row_ror:8
-                   });
+            s[0] = keep + lanes_move(send, [](uint32_t u) { return dpp_move<0x128>(u); });  // row_ror:8
         } else {
             s[0] = add_dpp<0x128>(s[0]);  // row_ror:8
         }
@@ -485,6 +487,13 @@ k_sig_bwd_thread(DevBasis B, int64_t n_items, const int64_t* order, const T* inc
         for (int rr = 0; rr < GG::words(i); ++rr)
             subG[GG::offset(i) + rr] = g0[(B.start[i] + static_cast<int64_t>(q) * GG::words(i) + rr) * cot.cs];
     T gL = g0[(B.start[L] + q) * cot.cs];
+    // Padding threads hold zero cotangents, so every value they add to the
+    // xbar sums is zero and no step needs to mask them out.
+    if (!live) {
+        gL = T(0);
+#pragma unroll
+        for (int rr = 0; rr < GG::total; ++rr) subG[rr] = T(0);
+    }
     // the walk this thread executes, if any: level ex, chain word ej under a,
     // and that word's cotangent
     int ex = 0, ej = 0;
@@ -658,10 +667,6 @@ k_sig_bwd_thread(DevBasis B, int64_t n_items, const int64_t* order, const T* inc
             if (live && L > 1) {
 #pragma unroll
                 for (int k = 0; k <= N - L; ++k) vbuf[vslot][BG::voff(L - 1) + t * BG::vin(L - 1) + k] = vout[k];
-            }
-            if (!live) {
-#pragma unroll
-                for (int b = 0; b < W; ++b) xbar[b] = T(0);
             }
             const T xs = WS::sum(xbar, lane);
             if (WS::leader(lane) && WS::slot(lane) < W) red[xslot][t >> 6][WS::slot(lane)] = xs;
