@@ -269,15 +269,18 @@ def test_lie_conversions_on_windows(dev, shape, to_lie, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_lie_conversions_large_batch(dev, dtype):
-    """(3,8) over 4099 objects: more object groups than resident work-groups
-    (each work-group takes several in turn) and a last group that is only
-    partly filled; both conversions bit-identical to the oracle."""
-    w, N, n = 3, 8, 4099
+@pytest.mark.parametrize("shape", [(3, 8, 4099), (5, 6, 1031)], ids=["w3d8", "w5d6"])
+def test_lie_conversions_large_batch(dev, shape, dtype):
+    """(3,8) over 4099 objects and (5,6) over 1031: more object groups than
+    resident work-groups (each work-group takes several in turn) and a last
+    group that is only partly filled, with one thread per row serving every
+    staged object (2, 4 or 8 of them by shape and type); both conversions
+    bit-identical to the oracle."""
+    w, N, n = shape
     basis = rpp_hip.TensorBasis(w, N)
     lie = rpp_hip.lie_basis(w, N)
     ts, dim = lie["true_size"], lie["tensor_dim"]
-    rng = np.random.default_rng(0x38)
+    rng = np.random.default_rng(0x38 + w)
     x = rng.uniform(-1, 1, size=(n, dim)).astype(dtype)
     exp_lie = _lie_expected(lie, x, True, ts, dim, dtype)
     lie_out = to_dev(np.full((n, ts), 7.0, dtype=dtype), dev)
