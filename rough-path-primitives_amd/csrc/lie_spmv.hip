@@ -72,11 +72,16 @@ __device__ __forceinline__ void lie_term(T (&acc)[J], uint32_t w, const T* tab, 
 // FP32 0.55 -> 0.26 ms against one object per thread).  With fewer rows, J =
 // 1 and short-row matrices take R = 3 or 4 rows per thread, so that a row's
 // two dependent loads (descriptor, entries) overlap those of the others.
-template <typename T, int OB, int R, int D, int J>
-__global__ void __launch_bounds__(kLieThreads) __attribute__((amdgpu_waves_per_eu(8)))
-k_lie_rows(T* out, int64_t obs, int64_t ocs, int64_t obeg, const T* arg, int64_t abs_, int64_t acs, int64_t abeg,
+// WPE: the waves per SIMD the register budget is set for (8 where two
+// work-groups share a compute unit, 4 where one takes the whole LDS).
+// UNIT: both coefficient strides are 1 (object-major batches), so no
+// address needs a stride product.
+template <typename T, int OB, int R, int D, int J, int WPE, bool UNIT>
+__global__ void __launch_bounds__(kLieThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_lie_rows(T* out, int64_t obs, int64_t ocs_, int64_t obeg, const T* arg, int64_t abs_, int64_t acs_, int64_t abeg,
            uint32_t cols, uint32_t col0, const uint4* packed4, const int4* rowdesc, const T* table, int table_size,
            int64_t row0, int rows, int64_t n) {
+    const int64_t ocs = UNIT ? 1 : ocs_, acs = UNIT ? 1 : acs_;
     extern __shared__ __align__(16) unsigned char lie_raw[];
     T* tab = reinterpret_cast<T*>(lie_raw);
     T* xs = reinterpret_cast<T*>(lie_raw + lie_table_bytes<T>(table_size));
@@ -187,10 +192,23 @@ int lie_objects_per_group(const DeviceCsr& m, int64_t cols) {
     return 0;
 }
 
-template <typename T, int OB, int R, int D, int J = 1>
-int lie_launch(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg, const DeviceCsr& m,
-               int64_t row0, int64_t rows, int64_t col0, hipStream_t stream) {
-    static const bool raised = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_lie_rows<T, OB, R, D, J>),
+// Long-row matrices with one thread per row for every object: the largest
+// group (up to 8 objects) the whole LDS holds.  Larger groups share each
+// entry's decoding among more objects, which measured faster than two
+// work-groups of half the size per compute unit (tensor_to_lie at width 3
+// depth 8 over 16384 objects, FP32: 4 objects 0.64 ms, two groups of 2 0.74).
+template <typename T>
+int lie_long_objects(const DeviceCsr& m, int64_t cols) {
+    for (int ob : {8, 4, 2, 1})
+        if (lie_lds_bytes<T>(m, cols, ob) <= kLieLdsMax) return ob;
+    return 0;
+}
+
+template <typename T, int OB, int R, int D, int J, int WPE, bool UNIT>
+int lie_launch_with(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg, const DeviceCsr& m,
+                    int64_t row0, int64_t rows, int64_t col0, hipStream_t stream) {
+    const auto kernel = &k_lie_rows<T, OB, R, D, J, WPE, UNIT>;
+    static const bool raised = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                                    static_cast<int>(kLieLdsMax)) == hipSuccess;
     const size_t lds = lie_lds_bytes<T>(m, arg.size, OB);
@@ -198,16 +216,23 @@ int lie_launch(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_
     // object groups are dealt to work-groups in turn, so the grid is what the
     // compute units hold at once (threads, registers and this LDS size)
     int resident = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_lie_rows<T, OB, R, D, J>, kLieThreads, lds) != hipSuccess)
-        resident = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kernel, kLieThreads, lds) != hipSuccess) resident = 1;
     const int64_t per_cu = std::max(1, resident);
     const int64_t groups = (n + OB - 1) / OB;
     const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min(groups, device_cu_count() * per_cu)));
-    k_lie_rows<T, OB, R, D, J><<<grid, kLieThreads, lds, stream>>>(
+    kernel<<<grid, kLieThreads, lds, stream>>>(
         static_cast<T*>(out.data), out.batch_stride, out.coeff_stride, out.begin, static_cast<const T*>(arg.data),
         arg.batch_stride, arg.coeff_stride, arg.begin, static_cast<uint32_t>(arg.size), static_cast<uint32_t>(col0),
         m.packed4, m.rowdesc, static_cast<const T*>(m.table), m.table_size, row0, static_cast<int>(rows), n);
     return check_hip(hipGetLastError(), "k_lie_rows launch");
+}
+
+template <typename T, int OB, int R, int D, int J = 1, int WPE = 8>
+int lie_launch(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg, const DeviceCsr& m,
+               int64_t row0, int64_t rows, int64_t col0, hipStream_t stream) {
+    if (out.coeff_stride == 1 && arg.coeff_stride == 1)
+        return lie_launch_with<T, OB, R, D, J, WPE, true>(n, out, arg, m, row0, rows, col0, stream);
+    return lie_launch_with<T, OB, R, D, J, WPE, false>(n, out, arg, m, row0, rows, col0, stream);
 }
 
 }  // namespace
@@ -225,6 +250,25 @@ int launch_lie_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_ve
     // otherwise, rows per thread of short-row matrices: as many as fit in 64
     // registers
     constexpr int RS = sizeof(T) == 8 ? 3 : 4;
+    if (long_rows && joint) {
+        // one work-group per compute unit where the group takes more than
+        // half the LDS: registers for 4 waves per SIMD, six entry groups in
+        // flight (width 3 depth 8, 16384 objects: FP64 1.09 -> 1.07 ms)
+        const int ob = lie_long_objects<T>(m, arg.size);
+        const bool whole = lie_lds_bytes<T>(m, arg.size, ob) > kLieLdsShared;
+        switch (ob * (whole ? -1 : 1)) {
+            // FP64 with 8 objects: two groups in flight fit in 64 registers
+            case 8: return lie_launch<T, 8, 1, sizeof(T) == 8 ? 2 : 4, 8>(n, out, arg, m, row0, rows, col0, stream);
+            case 4: return lie_launch<T, 4, 1, 4, 4>(n, out, arg, m, row0, rows, col0, stream);
+            case 2: return lie_launch<T, 2, 1, 4, 2>(n, out, arg, m, row0, rows, col0, stream);
+            case 1: return lie_launch<T, 1, 1, 4, 1>(n, out, arg, m, row0, rows, col0, stream);
+            case -8: return lie_launch<T, 8, 1, 6, 8, 4>(n, out, arg, m, row0, rows, col0, stream);
+            case -4: return lie_launch<T, 4, 1, 6, 4, 4>(n, out, arg, m, row0, rows, col0, stream);
+            case -2: return lie_launch<T, 2, 1, 6, 2, 4>(n, out, arg, m, row0, rows, col0, stream);
+            case -1: return lie_launch<T, 1, 1, 6, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
+            default: return fail(RPP_HIP_NOT_IMPLEMENTED, "Lie argument does not fit in LDS");
+        }
+    }
     switch (lie_objects_per_group<T>(m, arg.size) * (long_rows ? -1 : 1)) {
         case 8:
             return joint ? lie_launch<T, 8, 1, 2, 8>(n, out, arg, m, row0, rows, col0, stream)
@@ -236,15 +280,9 @@ int launch_lie_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_ve
             return joint ? lie_launch<T, 2, 1, 2, 2>(n, out, arg, m, row0, rows, col0, stream)
                          : lie_launch<T, 2, RS, 1>(n, out, arg, m, row0, rows, col0, stream);
         case 1: return lie_launch<T, 1, RS, 1>(n, out, arg, m, row0, rows, col0, stream);
-        case -8:
-            return joint ? lie_launch<T, 8, 1, 4, 8>(n, out, arg, m, row0, rows, col0, stream)
-                         : lie_launch<T, 8, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
-        case -4:
-            return joint ? lie_launch<T, 4, 1, 4, 4>(n, out, arg, m, row0, rows, col0, stream)
-                         : lie_launch<T, 4, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
-        case -2:
-            return joint ? lie_launch<T, 2, 1, 4, 2>(n, out, arg, m, row0, rows, col0, stream)
-                         : lie_launch<T, 2, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
+        case -8: return lie_launch<T, 8, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
+        case -4: return lie_launch<T, 4, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
+        case -2: return lie_launch<T, 2, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
         case -1: return lie_launch<T, 1, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
         default: return fail(RPP_HIP_NOT_IMPLEMENTED, "Lie argument does not fit in LDS");
     }
