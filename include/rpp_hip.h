@@ -506,7 +506,10 @@ int rpp_hip_lie_basis_load(const char* path, int32_t* width, int32_t* depth);
  *   checkpoint_every / checkpoints: optional (0 / NULL).  When set, the
  *               state after every checkpoint_every-th increment of path p is
  *               stored at checkpoints + (ckpt_offsets[p] + j) * dim, used by
- *               the backward pass to re-anchor its reconstruction.
+ *               the backward pass to re-anchor its reconstruction.  Degrees
+ *               0 .. depth-1 are stored; the top degree, which no backward
+ *               step reads, is left unwritten by the thread-owned subtree
+ *               tiles (tile kind 2).
  *   order / num_increments: optional.  NULL order: the library reads the
  *               offsets back (one host synchronisation) and queues paths by
  *               decreasing length.  Otherwise order is a device permutation

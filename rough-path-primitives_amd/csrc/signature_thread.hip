@@ -116,6 +116,14 @@ __device__ __forceinline__ void thread_step(T (&chain)[L + 1], T (&sub)[TOT], co
     for (int k = 1; k <= L; ++k) chain[k] = nch[k];
 }
 
+// v, opaque to the optimiser (an empty asm statement that claims to change
+// it): addresses derived from the result are computed where they are used,
+// not hoisted out of the step loop to occupy registers across it.
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 // Store a thread's subtree and the chain words it owns.
 template <typename T, int W, int N, int L, int TOT = ThreadGeo<W, N, L>::total>
 __device__ __forceinline__ void thread_store(const DevBasis& B, T* o, int64_t cs, int q, const T (&chain)[L + 1],
@@ -219,10 +227,13 @@ k_sig_fwd_thread(DevBasis B, int n_paths, const int64_t* order, const T* __restr
         int ck_left = ck_every;
         int64_t ck_index = CK && ck_every > 0 ? ck_offs[path] : 0;
         const int64_t dim = B.start[N + 1];
+        // a checkpoint holds the state below the top level, the part a
+        // backward step reads (the top level is 83 % of the words at width 6
+        // depth 6, so this writes a sixth of the bytes)
         auto checkpoint = [&]() {
             if (CK && ck_every > 0 && --ck_left == 0) {
                 ck_left = ck_every;
-                if (live) thread_store<T, W, N, L>(B, ckpts + ck_index * dim, 1, q, chain, sub);
+                if (live) thread_store<T, W, N - 1, L, G::total>(B, ckpts + ck_index * dim, 1, opaque(q), chain, sub);
                 ++ck_index;
             }
         };
@@ -557,13 +568,14 @@ k_sig_bwd_thread(DevBasis B, int64_t n_items, const int64_t* order, const T* inc
                 for (int rr = 0; rr < GS::total; ++rr) subS[rr] = T(0);
             } else if (CK && ck_every > 0 && (tl % ck_every) == 0) {
                 const T* c0 = ckpts + (ck_offs[path] + tl / ck_every - 1) * dim;
+                const int qo = opaque(q);
 #pragma unroll
-                for (int i = 0; i <= L; ++i) chain[i] = c0[B.start[i] + q / tpow(W, L - i)];
+                for (int i = 0; i <= L; ++i) chain[i] = c0[B.start[i] + qo / tpow(W, L - i)];
 #pragma unroll
                 for (int i = L + 1; i < N; ++i)
 #pragma unroll
                     for (int rr = 0; rr < GS::words(i); ++rr)
-                        subS[GS::offset(i) + rr] = c0[B.start[i] + static_cast<int64_t>(q) * GS::words(i) + rr];
+                        subS[GS::offset(i) + rr] = c0[B.start[i] + static_cast<int64_t>(qo) * GS::words(i) + rr];
             } else {
                 thread_step<T, W, N - 1, L>(chain, subS, xneg, xqneg);
             }
