@@ -102,6 +102,14 @@ constexpr int cpu_term_no_rhs0(int w, int k, int t) {
 // f, decltype(i)::value is a constant expression, so index arithmetic built
 // from it (degree offsets, word counts, a term order) is evaluated by the
 // compiler's front end rather than left to loop unrolling.
+// v, opaque to the optimiser (an empty asm statement that claims to change
+// it): values derived from the result are computed where they are used, not
+// hoisted out of a loop to occupy registers across it.
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 template <typename F, int... Is>
 __device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Is...>) {
     (f(std::integral_constant<int, Is>{}), ...);

@@ -116,14 +116,6 @@ __device__ __forceinline__ void thread_step(T (&chain)[L + 1], T (&sub)[TOT], co
     for (int k = 1; k <= L; ++k) chain[k] = nch[k];
 }
 
-// v, opaque to the optimiser (an empty asm statement that claims to change
-// it): addresses derived from the result are computed where they are used,
-// not hoisted out of the step loop to occupy registers across it.
-__device__ __forceinline__ int opaque(int v) {
-    asm volatile("" : "+v"(v));
-    return v;
-}
-
 // Store a thread's subtree and the chain words it owns.
 template <typename T, int W, int N, int L, int TOT = ThreadGeo<W, N, L>::total>
 __device__ __forceinline__ void thread_store(const DevBasis& B, T* o, int64_t cs, int q, const T (&chain)[L + 1],
