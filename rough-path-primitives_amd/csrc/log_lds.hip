@@ -26,7 +26,9 @@
 // 558 with P updated in place (two barriers per step, new values staged in
 // registers, one work-group per compute unit) -> 339 double-buffered (75
 // registers, held to 64 so that two work-groups share a compute unit); FP32
-// 630 -> 498.  Every one of the 16 compiled shapes gains, 1-50 %.
+// 630 -> 498.  Every one of the 16 compiled shapes gains, 1-50 %.  In FP32 a
+// thread sums blocks of width consecutive words at most shapes, sharing each
+// P read among them (log_block_instance).
 #include "common.hpp"
 #include "log_lds.hpp"
 
@@ -112,8 +114,40 @@ __device__ __forceinline__ T log_level_sum(const T* ps, const T* ys, int w, int 
     return acc;
 }
 
+// The sums of log_level_sum for the W consecutive words W m .. W m + W - 1 of
+// degree K, each in the same term order: for j >= 1 the W words share
+// P_{K-j}[m / width^(j-1)] and read Y_j[width (m mod width^(j-1)) + b], so one
+// P read serves W words.  TOP: the term j = K = N adds p0 * y[b].
+template <typename T, int W, int N, int K, bool TOP>
+__device__ __forceinline__ void log_block_sum(T (&acc)[W], const T* ps, const T* ys, int m, int ylo, int yhi, T p0,
+                                              const T (&y)[W], bool top) {
+    using G = LogGeo<W, N>;
+    constexpr int kF = cpu_first_tiled_degree(W), kR = cpu_right_letters(W);
+#pragma unroll
+    for (int b = 0; b < W; ++b) acc[b] = T(0);
+    static_for<K>([&](auto uc) {
+        constexpr int j = cpu_term_no_rhs0_fr(kF, kR, W == 1, K, decltype(uc)::value);
+        if constexpr (TOP && j == K) {
+            if (top) {
+#pragma unroll
+                for (int b = 0; b < W; ++b) acc[b] += p0 * y[b];
+            }
+        } else {
+            constexpr int wj1 = G::words(j - 1), sk = G::start(K - j), sj = G::start(j);
+            if (j >= ylo && j <= yhi) {
+                const int p = m / wj1;
+                const T pv = ps[sk + p];
+                const T* yy = ys + sj + W * (m - p * wj1);
+#pragma unroll
+                for (int b = 0; b < W; ++b) acc[b] += pv * yy[b];
+            }
+        }
+    });
+}
+
 // FULL: the argument window holds degrees 1..N, so its tests fold away.
-template <typename T, int W, int N, int NT, int WPE, bool FULL>
+// BLK: each thread sums blocks of W consecutive words (log_block_sum).
+template <typename T, int W, int N, int NT, int WPE, bool FULL, bool BLK = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_log_lds(View<T> out, View<const T> arg, int64_t n) {
     using G = LogGeo<W, N>;
@@ -165,7 +199,23 @@ k_log_lds(View<T> out, View<const T> arg, int64_t n) {
             // new P_k = sum_{j=1..k} P_{k-j} (x) Y_j for 1 <= k <= K
             static_for<N>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
-                if constexpr (k >= 1) {
+                if constexpr (k >= 1 && BLK) {
+                    if (k <= K) {
+                        constexpr int nb = G::words(k - 1);
+#pragma unroll
+                        for (int s = 0; s < (nb + NT - 1) / NT; ++s) {
+                            const int m = t + s * NT;
+                            if (nb % NT == 0 || m < nb) {
+                                T acc[W];
+                                const T yz[W] = {};
+                                log_block_sum<T, W, N, k, false>(acc, pc, ys, m, ylo, yhi, T(0), yz, false);
+#pragma unroll
+                                for (int b = 0; b < W; ++b) pn[G::start(k) + W * m + b] = acc[b];
+                            }
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
+                    }
+                } else if constexpr (k >= 1) {
                     if (k <= K) {
 #pragma unroll
                         for (int s = 0; s < G::slots(k, NT); ++s) {
@@ -192,26 +242,47 @@ k_log_lds(View<T> out, View<const T> arg, int64_t n) {
         if (t == 0) o[0] = T(0);
         static_for<N>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            if constexpr (k >= 1) {
+            if constexpr (k >= 1 && BLK) {
+                for (int m = t; m < G::words(k - 1); m += NT) {
+                    T acc[W];
+                    const T yz[W] = {};
+                    log_block_sum<T, W, N, k, false>(acc, pc, ys, m, ylo, yhi, T(0), yz, false);
+#pragma unroll
+                    for (int b = 0; b < W; ++b) o[G::start(k) + W * m + b] = acc[b];
+                }
+            } else if constexpr (k >= 1) {
                 for (int w = t; w < G::words(k); w += NT)
                     o[G::start(k) + w] = log_level_sum<T, W, N, k, false>(pc, ys, w, ylo, yhi, T(0), T(0), false);
             }
         });
         const T p0 = pc[0];
-        for (int w = t; w < G::words(N); w += NT) {
-            // issued before the LDS reads of the sum, whose latency it overlaps
-            const T y = top ? a[G::start(N) + w] : T(0);
-            o[G::start(N) + w] = log_level_sum<T, W, N, N, true>(pc, ys, w, ylo, yhi, p0, y, top);
+        if constexpr (BLK) {
+            for (int m = t; m < G::words(N - 1); m += NT) {
+                // issued before the LDS reads of the sums, whose latency they overlap
+                T y[W];
+#pragma unroll
+                for (int b = 0; b < W; ++b) y[b] = top ? a[G::start(N) + W * m + b] : T(0);
+                T acc[W];
+                log_block_sum<T, W, N, N, true>(acc, pc, ys, m, ylo, yhi, p0, y, top);
+#pragma unroll
+                for (int b = 0; b < W; ++b) o[G::start(N) + W * m + b] = acc[b];
+            }
+        } else {
+            for (int w = t; w < G::words(N); w += NT) {
+                // issued before the LDS reads of the sum, whose latency it overlaps
+                const T y = top ? a[G::start(N) + w] : T(0);
+                o[G::start(N) + w] = log_level_sum<T, W, N, N, true>(pc, ys, w, ylo, yhi, p0, y, top);
+            }
         }
     }
 }
 
-template <typename T, int W, int N, int NT, int WPE, bool FULL>
+template <typename T, int W, int N, int NT, int WPE, bool FULL, bool BLK = false>
 int log_launch_with(int64_t n, const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& arg,
                     hipStream_t stream) {
     constexpr size_t lds = log_lds_bytes<T, W, N>();
     static_assert(lds <= kLogLdsMax, "the LDS of one compute unit");
-    const auto kernel = &k_log_lds<T, W, N, NT, WPE, FULL>;
+    const auto kernel = &k_log_lds<T, W, N, NT, WPE, FULL, BLK>;
     static const bool raised = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                                    static_cast<int>(lds)) == hipSuccess;
@@ -238,12 +309,27 @@ constexpr bool log_full_instance() {
     return !(sizeof(T) == 8 && ((W == 2 && N == 10) || (W == 4 && N == 6)));
 }
 
+// Blocks of W words per thread (log_block_sum): one P read serves W words.
+// Measured against one word per thread over the 16 compiled shapes (full
+// window, microseconds per call): FP32 faster at 11 of them, e.g. width 3
+// depth 8 427 -> 268, width 5 depth 6 316 -> 262, width 4 depth 6 271 -> 232,
+// width 6 depth 4 366 -> 317; slower at widths 2, width 3 depth 6 and width 4
+// depth 5 (e.g. width 2 depth 8 617 -> 731), which keep one word per thread;
+// FP64 slower at every shape (width 3 depth 8 269 -> 287: its W sums and
+// top-degree words in flight exceed the 64-register budget).
+template <typename T, int W, int N>
+constexpr bool log_block_instance() {
+    return sizeof(T) == 4 && W >= 3 && !(W == 3 && N == 6) && !(W == 4 && N == 5);
+}
+
 template <typename T, int W, int N>
 int log_launch(int64_t n, const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& arg, hipStream_t stream) {
     constexpr int NT = log_threads<T, W, N>();
+    constexpr int WP = log_waves<T, W, N>(NT);
+    constexpr bool BLK = log_block_instance<T, W, N>();
     if (log_full_instance<T, W, N>() && arg.min_degree <= 1 && arg.max_degree >= N)
-        return log_launch_with<T, W, N, NT, log_waves<T, W, N>(NT), true>(n, out, arg, stream);
-    return log_launch_with<T, W, N, NT, log_waves<T, W, N>(NT), false>(n, out, arg, stream);
+        return log_launch_with<T, W, N, NT, WP, true, BLK>(n, out, arg, stream);
+    return log_launch_with<T, W, N, NT, WP, false, BLK>(n, out, arg, stream);
 }
 
 template <typename T>
