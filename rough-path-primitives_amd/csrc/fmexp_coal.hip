@@ -54,7 +54,10 @@ struct CoalGeo {
 };
 
 // FULL: the multiplier window holds every degree, so its tests fold away.
-template <typename T, int W, int N, int NT, int M, bool FULL>
+// XCD: items are taken in XCD ranges (below); else item v is work-group v's.
+// NTS: the subtree stores are non-temporal (the output is not read again
+// before the caches have turned over).
+template <typename T, int W, int N, int NT, int M, bool FULL, bool XCD, bool NTS>
 __global__ void __launch_bounds__(NT)
 k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult_, View<const T> ex, int64_t n) {
     using G = CoalGeo<W, N, NT, M>;
@@ -69,7 +72,16 @@ k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult_, View<const T> ex, int
     __shared__ T lds1[G::lds_elems];
     __shared__ T xs[W];
     const int t = threadIdx.x;
-    for (int64_t item = blockIdx.x; item < n * G::P; item += gridDim.x) {
+    // Work-groups are dispatched to the eight XCDs round robin, so with a
+    // grid that is a multiple of 8, v mod 8 names the XCD of virtual item v.
+    // XCD x takes the contiguous item range [x per, (x + 1) per): the items
+    // of one object share partial cache lines of its lower subtree levels
+    // and its chain words, and these then meet in one XCD's L2.
+    const int64_t items = n * G::P;
+    const int64_t per = XCD ? (items + 7) >> 3 : items;
+    for (int64_t v = blockIdx.x; v < (XCD ? per * 8 : items); v += gridDim.x) {
+        const int64_t item = XCD ? (v & 7) * per + (v >> 3) : v;
+        if (XCD && item >= items) continue;
         const int64_t b = item / G::P;
         const int q = static_cast<int>(item - b * G::P);
         const T* src = mult.obj(b);
@@ -143,7 +155,13 @@ k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult_, View<const T> ex, int
 #pragma unroll
             for (int s = 0; s < G::slots(i); ++s) {
                 const int r = t + s * NT;
-                if (G::words(i) % NT == 0 || r < G::words(i)) lvl[r * out.cs] = reg[G::offset(i) + s];
+                if (G::words(i) % NT == 0 || r < G::words(i)) {
+                    if constexpr (NTS) {
+                        __builtin_nontemporal_store(reg[G::offset(i) + s], &lvl[r * out.cs]);
+                    } else {
+                        lvl[r * out.cs] = reg[G::offset(i) + s];
+                    }
+                }
             }
         }
         // chain word of level i is written by the item whose remaining
@@ -227,12 +245,16 @@ __device__ __forceinline__ void wave_level(T* __restrict__ dst, const T* __restr
     for (int s = 0; s < W; ++s) {
         const int j = lane + 64 * s;
         const T ap = __shfl(a, j / W, 64);
-        if (j < nwords) dst[base + j] = horner_ref(ap, xo[s], T(1), own[s], has(K));
+        if (j < nwords) __builtin_nontemporal_store(horner_ref(ap, xo[s], T(1), own[s], has(K)), &dst[base + j]);
     }
     if (K == 1 && lane == 0) dst[0] = pre[0] + T(0);  // the reference's last step leaves S_0 + 0
 }
 
-// One (object, level, chunk) per wavefront; unit coefficient strides.
+// One (object, level, chunk) per wavefront; unit coefficient strides; the
+// output stores are non-temporal.  Work-groups take objects in launch order: taking them in one contiguous
+// range per XCD, as k_fmexp_coal's FP64 width 5 depth 6 tile does, measured
+// slower here (FP64 width 6 depth 6 over 2048 objects 323 -> 335 us per
+// call, width 8 depth 5 212 -> 217).
 template <typename T, int W, int N, int NT>
 __global__ void __launch_bounds__(NT)
 k_fmexp_wave(T* __restrict__ out, int64_t obs, const T* __restrict__ mult, int64_t mbs, int mn, int mx,
@@ -271,26 +293,28 @@ int wave_launch(const DevBasis&, const rpp_hip_tensor_batch& out, const rpp_hip_
 
 // FW: the tile also has a full-window instance, taken when the multiplier
 // holds every degree (faster only at some shapes; see the tile table).
-template <typename T, int W, int N, int NT, int M, int BPC, bool FW>
+template <typename T, int W, int N, int NT, int M, int BPC, bool FW, bool XCD, bool NTS>
 int coal_launch(const DevBasis& B, const rpp_hip_tensor_batch& out, const rpp_hip_tensor_batch& mult,
                 const rpp_hip_tensor_batch& ex, int64_t n, hipStream_t stream) {
     using G = CoalGeo<W, N, NT, M>;
-    const int64_t items = n * G::P;
+    // XCD: virtual items, the items rounded up to a multiple of 8 (one range
+    // per XCD); a persistent grid is a multiple of 8 too (compute units x BPC)
+    const int64_t virt = XCD ? (n * G::P + 7) / 8 * 8 : n * G::P;
     const int64_t cap = BPC > 0 ? static_cast<int64_t>(device_cu_count()) * BPC : (int64_t{1} << 30);
-    const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min(items, cap)));
+    const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min(virt, cap)));
     if (FW && mult.min_degree <= 0 && mult.max_degree >= N)
-        k_fmexp_coal<T, W, N, NT, M, true><<<grid, NT, 0, stream>>>(B, make_view<T>(out), make_cview<T>(mult),
+        k_fmexp_coal<T, W, N, NT, M, true, XCD, NTS><<<grid, NT, 0, stream>>>(B, make_view<T>(out), make_cview<T>(mult),
                                                                     make_cview<T>(ex), n);
     else
-        k_fmexp_coal<T, W, N, NT, M, false><<<grid, NT, 0, stream>>>(B, make_view<T>(out), make_cview<T>(mult),
+        k_fmexp_coal<T, W, N, NT, M, false, XCD, NTS><<<grid, NT, 0, stream>>>(B, make_view<T>(out), make_cview<T>(mult),
                                                                      make_cview<T>(ex), n);
     return check_hip(hipGetLastError(), "k_fmexp_coal launch");
 }
 
 }  // namespace
 
-#define RPP_COAL(T, W, N, NT, M, BPC, FW) \
-    FmexpTile<T> { 2, sizeof(T), W, N, NT, M, BPC, &coal_launch<T, W, N, NT, M, BPC, FW> }
+#define RPP_COAL(T, W, N, NT, M, BPC, FW, XCD, NTS) \
+    FmexpTile<T> { 2, sizeof(T), W, N, NT, M, BPC, &coal_launch<T, W, N, NT, M, BPC, FW, XCD, NTS> }
 #define RPP_WAVE(T, W, N, NT) \
     FmexpTile<T> { 3, sizeof(T), W, N, NT, 0, 0, &wave_launch<T, W, N, NT> }
 
@@ -312,15 +336,25 @@ int coal_launch(const DevBasis& B, const rpp_hip_tensor_batch& out, const rpp_hi
 //   width 6 depth 6, 2048 objects:  FP32 176 -> 168
 //   width 5 depth 6, 4096 objects:  FP32 119 -> 114, FP64 237 -> 269 (not taken)
 //   width 8 depth 5, 2048 objects:  FP32 111 -> 116 (not taken)
+// Items in XCD ranges (XCD field), microseconds per call, alternating runs:
+//   width 5 depth 6, 4096 objects:  FP64 235 -> 227 (taken); FP32 116 -> 118,
+//   and no gain at the other shapes (not taken)
+// Non-temporal subtree stores (last field; the wavefront tiles always), one
+// box, alternating runs, microseconds per call:
+//   FP64: width 4 depth 4 159 -> 134, width 3 depth 8 243 -> 238, width 6
+//   depth 6 324 -> 319, width 8 depth 5 212 -> 207, width 5 depth 6
+//   232 -> 238 (not taken)
+//   FP32: width 6 depth 6 199 -> 170, width 4 depth 4 82 -> 76, width 5
+//   depth 6 126 -> 120, width 8 depth 5 117 -> 111, width 3 depth 8 147 -> 140
 template <typename T>
 const std::vector<FmexpTile<T>>& fmexp_coal_tiles() {
     constexpr bool f32 = sizeof(T) == 4;
     static const std::vector<FmexpTile<T>> tiles = {
-        f32 ? RPP_COAL(T, 5, 6, 256, 1, 0, true) : RPP_COAL(T, 5, 6, 128, 2, 0, false),
-        f32 ? RPP_COAL(T, 6, 6, 128, 2, 0, true) : RPP_WAVE(T, 6, 6, 512),
-        f32 ? RPP_COAL(T, 8, 5, 256, 1, 0, false) : RPP_WAVE(T, 8, 5, 512),
-        RPP_COAL(T, 3, 8, 256, 1, 0, true),
-        RPP_COAL(T, 4, 4, 64, 0, 0, false),
+        f32 ? RPP_COAL(T, 5, 6, 256, 1, 0, true, false, true) : RPP_COAL(T, 5, 6, 128, 2, 0, false, true, false),
+        f32 ? RPP_COAL(T, 6, 6, 128, 2, 0, true, false, true) : RPP_WAVE(T, 6, 6, 512),
+        f32 ? RPP_COAL(T, 8, 5, 256, 1, 0, false, false, true) : RPP_WAVE(T, 8, 5, 512),
+        RPP_COAL(T, 3, 8, 256, 1, 0, true, false, true),
+        RPP_COAL(T, 4, 4, 64, 0, 0, false, false, true),
     };
     return tiles;
 }
