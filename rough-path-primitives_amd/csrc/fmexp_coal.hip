@@ -157,7 +157,7 @@ k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult_, View<const T> ex, int
                 const int r = t + s * NT;
                 if (G::words(i) % NT == 0 || r < G::words(i)) {
                     if constexpr (NTS) {
-                        __builtin_nontemporal_store(reg[G::offset(i) + s], &lvl[r * out.cs]);
+                        stream_store(reg[G::offset(i) + s], &lvl[r * out.cs]);
                     } else {
                         lvl[r * out.cs] = reg[G::offset(i) + s];
                     }
@@ -245,7 +245,7 @@ __device__ __forceinline__ void wave_level(T* __restrict__ dst, const T* __restr
     for (int s = 0; s < W; ++s) {
         const int j = lane + 64 * s;
         const T ap = __shfl(a, j / W, 64);
-        if (j < nwords) __builtin_nontemporal_store(horner_ref(ap, xo[s], T(1), own[s], has(K)), &dst[base + j]);
+        if (j < nwords) stream_store(horner_ref(ap, xo[s], T(1), own[s], has(K)), &dst[base + j]);
     }
     if (K == 1 && lane == 0) dst[0] = pre[0] + T(0);  // the reference's last step leaves S_0 + 0
 }

@@ -162,6 +162,8 @@ k_lie_rows(T* out, int64_t obs, int64_t ocs_, int64_t obeg, const T* arg, int64_
 #pragma unroll
                     for (int j = 0; j < D; ++j) u[r][j] = nx[r][j];
             }
+            // ordinary stores (non-temporal ones measured no faster for
+            // tensor_to_lie and 4 % slower for lie_to_tensor at width 3 depth 8)
 #pragma unroll
             for (int r = 0; r < R; ++r)
 #pragma unroll
