@@ -11,7 +11,8 @@
 // degree offset, word count, division by width^j and the term order are
 // constants, so the inner loops unroll into LDS reads, multiplies and adds.
 // A work-group takes one object (a grid-stride loop covers batches beyond the
-// grid limit): it stages the object's lhs and rhs windows in LDS with
+// grid limit; objects are taken in one contiguous range per XCD, 2^17 pairs
+// at width 4 depth 4 FP64 194 -> 189.5 us): it stages the object's lhs and rhs windows in LDS with
 // coalesced loads, then writes every output word once with coalesced stores.  All operand reads of an object precede its first write,
 // so out may alias lhs or rhs; the addend is read at the index being written.
 #include "common.hpp"
@@ -66,7 +67,14 @@ k_product_small(View<T> out, View<const T> lhs, View<const T> rhs, View<const T>
     const bool lw = lhs.mn <= lhs.mx, rw = rhs.mn <= rhs.mx;
     const int lb = lw ? degree_start<W, N>(lhs.mn) : 0, le = lw ? degree_start<W, N>(lhs.mx + 1) : 0;
     const int rb = rw ? degree_start<W, N>(rhs.mn) : 0, re = rw ? degree_start<W, N>(rhs.mx + 1) : 0;
-    for (int64_t b = blockIdx.x; b < n; b += gridDim.x) {
+    // Work-groups are dispatched to the eight XCDs round robin and the grid
+    // is a multiple of 8, so v mod 8 names the XCD of virtual object v: XCD
+    // x takes the contiguous objects [x per, (x + 1) per), and the cache
+    // lines that neighbouring objects share meet in one L2.
+    const int64_t per = (n + 7) >> 3;
+    for (int64_t v = blockIdx.x; v < per * 8; v += gridDim.x) {
+        const int64_t b = (v & 7) * per + (v >> 3);
+        if (b >= n) continue;
         const T* L = lhs.obj(b);
         const T* R = rhs.obj(b);
         // compile-time trip count: every load of the object is issued before
@@ -128,7 +136,7 @@ int small_launch(int64_t n, const rpp_hip_tensor_batch& out, const rpp_hip_tenso
     constexpr int NT = dim <= 128 ? 64 : (dim <= 1024 ? 128 : 256);
     // one object per work-group: the hardware scheduler keeps every compute
     // unit full and no work-group waits on a second object's loads
-    const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(n, 0x7fffffff)));
+    const unsigned grid = static_cast<unsigned>(std::max<int64_t>(8, std::min<int64_t>((n + 7) / 8 * 8, 0x7ffffff8)));
     const View<const T> a = add ? make_cview<T>(*add) : View<const T>{nullptr, 0, 1, 1, 0};
     auto full = [](const rpp_hip_tensor_batch& t) { return t.min_degree == 0 && t.max_degree == N; };
     if (!add && full(out) && full(lhs) && full(rhs))
