@@ -549,8 +549,10 @@ int upload(const Csr& h, const std::vector<int64_t>& row_blocks, DeviceCsr& d) {
             const int64_t len = h.offsets[static_cast<size_t>(r) + 1] - h.offsets[static_cast<size_t>(r)];
             group[static_cast<size_t>(r) + 1] = group[static_cast<size_t>(r)] + (len + 3) / 4;
         }
-        if (group.back() >= (int64_t{1} << 31)) return 0;  // no packed form
-        std::vector<uint32_t> words(static_cast<size_t>(group.back()) * 4, pad);
+        if (group.back() + kLiePackSlack >= (int64_t{1} << 31)) return 0;  // no packed form
+        // kLiePackSlack padding groups after the last row: the packed kernel
+        // loads up to that many groups past a row's end without a test
+        std::vector<uint32_t> words(static_cast<size_t>(group.back() + kLiePackSlack) * 4, pad);
         for (int64_t r = 0; r < h.rows; ++r) {
             size_t at = static_cast<size_t>(group[static_cast<size_t>(r)]) * 4;
             for (int64_t e = h.offsets[static_cast<size_t>(r)]; e < h.offsets[static_cast<size_t>(r) + 1]; ++e)

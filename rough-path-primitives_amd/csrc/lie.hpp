@@ -51,6 +51,11 @@ std::shared_ptr<const LieMetadata> load_lie_metadata(const std::string& path, st
 // may already be built from it).
 int install_lie_metadata(std::shared_ptr<const LieMetadata> md, std::string* err);
 
+// Padding groups after the last row of a packed matrix (DeviceCsr::packed4):
+// the packed kernel's prefetch reads groups past a row's end, at indices it
+// holds below the row's end group + kLiePackSlack, so never past the array.
+constexpr int64_t kLiePackSlack = 16;
+
 struct DeviceCsr {
     void* values = nullptr;
     int64_t* indices = nullptr;
@@ -63,6 +68,7 @@ struct DeviceCsr {
     // entries with the word (table_size - 1) << 24 | 0xffffff, whose table
     // value is -0.0 (a term that leaves any sum unchanged), and read as
     // uint4 groups.
+    // kLiePackSlack padding groups follow the last row.
     uint4* packed4 = nullptr;
     void* table = nullptr;
     int32_t table_size = 0;  // 0: no packed form

@@ -75,8 +75,9 @@ __device__ __forceinline__ void lie_term(T (&acc)[J], uint32_t w, const T* tab, 
 // WPE: the waves per SIMD the register budget is set for (8 where two
 // work-groups share a compute unit, 4 where one takes the whole LDS).
 // UNIT: both coefficient strides are 1 (object-major batches), so no
-// address needs a stride product.
-template <typename T, int OB, int R, int D, int J, int WPE, bool UNIT>
+// address needs a stride product.  PF: long-row matrices load their entry
+// groups with no test (below).
+template <typename T, int OB, int R, int D, int J, int WPE, bool UNIT, bool PF>
 __global__ void __launch_bounds__(kLieThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_lie_rows(T* out, int64_t obs, int64_t ocs_, int64_t obeg, const T* arg, int64_t abs_, int64_t acs_, int64_t abeg,
            uint32_t cols, uint32_t col0, const uint4* packed4, const int4* rowdesc, const T* table, int table_size,
@@ -86,6 +87,7 @@ k_lie_rows(T* out, int64_t obs, int64_t ocs_, int64_t obeg, const T* arg, int64_
     T* tab = reinterpret_cast<T*>(lie_raw);
     T* xs = reinterpret_cast<T*>(lie_raw + lie_table_bytes<T>(table_size));
     static_assert(OB % J == 0, "a row slot serves whole objects");
+    constexpr int slack_last = static_cast<int>(kLiePackSlack) - 1;
     constexpr int kStage = kLieThreads / OB;        // staging steps per pass
     constexpr int kSlots = kLieThreads / (OB / J);  // row slots
     const uint32_t t = threadIdx.x;
@@ -133,18 +135,33 @@ k_lie_rows(T* out, int64_t obs, int64_t ocs_, int64_t obeg, const T* arg, int64_
 #pragma unroll
                 for (int j = 0; j < J; ++j) acc[r][j] = T(0);
             }
+            // PF: every group load is unconditional: groups past a row's end
+            // (the next rows', or the slack groups after the last row) are
+            // loaded and never summed; an index is held below the row's end
+            // group plus the slack, so it stays in the array.  With the loads
+            // under a branch the compiler waits for the whole prefetch before
+            // the current groups' sums (tensor_to_lie at width 3 depth 8 over
+            // 16384 objects FP64: 1.12 -> 1.02 ms without the branch).  Short
+            // rows keep the tests: most of their prefetches would pass the
+            // row's end (lie_to_tensor 0.467 -> 0.475 ms without them).
+            auto group = [&](int r, int i) {
+                if constexpr (PF) {
+                    return packed4[min(d[r].y + i, d[r].z + slack_last)];
+                } else {
+                    return i < len[r] ? packed4[d[r].y + i] : make_uint4(0, 0, 0, 0);
+                }
+            };
             uint4 u[R][D];
 #pragma unroll
             for (int r = 0; r < R; ++r)
 #pragma unroll
-                for (int j = 0; j < D; ++j) u[r][j] = j < len[r] ? packed4[d[r].y + j] : make_uint4(0, 0, 0, 0);
+                for (int j = 0; j < D; ++j) u[r][j] = group(r, j);
             for (int k = 0; k < most; k += D) {
                 uint4 nx[R][D];
 #pragma unroll
                 for (int r = 0; r < R; ++r)
 #pragma unroll
-                    for (int j = 0; j < D; ++j)
-                        nx[r][j] = k + D + j < len[r] ? packed4[d[r].y + k + D + j] : make_uint4(0, 0, 0, 0);
+                    for (int j = 0; j < D; ++j) nx[r][j] = group(r, k + D + j);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
 #pragma unroll
@@ -206,10 +223,10 @@ int lie_long_objects(const DeviceCsr& m, int64_t cols) {
     return 0;
 }
 
-template <typename T, int OB, int R, int D, int J, int WPE, bool UNIT>
+template <typename T, int OB, int R, int D, int J, int WPE, bool UNIT, bool PF>
 int lie_launch_with(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg, const DeviceCsr& m,
                     int64_t row0, int64_t rows, int64_t col0, hipStream_t stream) {
-    const auto kernel = &k_lie_rows<T, OB, R, D, J, WPE, UNIT>;
+    const auto kernel = &k_lie_rows<T, OB, R, D, J, WPE, UNIT, PF>;
     static const bool raised = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                                    static_cast<int>(kLieLdsMax)) == hipSuccess;
@@ -229,12 +246,12 @@ int lie_launch_with(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_ve
     return check_hip(hipGetLastError(), "k_lie_rows launch");
 }
 
-template <typename T, int OB, int R, int D, int J = 1, int WPE = 8>
+template <typename T, int OB, int R, int D, int J = 1, int WPE = 8, bool PF = false>
 int lie_launch(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_vector_batch& arg, const DeviceCsr& m,
                int64_t row0, int64_t rows, int64_t col0, hipStream_t stream) {
     if (out.coeff_stride == 1 && arg.coeff_stride == 1)
-        return lie_launch_with<T, OB, R, D, J, WPE, true>(n, out, arg, m, row0, rows, col0, stream);
-    return lie_launch_with<T, OB, R, D, J, WPE, false>(n, out, arg, m, row0, rows, col0, stream);
+        return lie_launch_with<T, OB, R, D, J, WPE, true, PF>(n, out, arg, m, row0, rows, col0, stream);
+    return lie_launch_with<T, OB, R, D, J, WPE, false, PF>(n, out, arg, m, row0, rows, col0, stream);
 }
 
 }  // namespace
@@ -254,20 +271,22 @@ int launch_lie_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_ve
     constexpr int RS = sizeof(T) == 8 ? 3 : 4;
     if (long_rows && joint) {
         // one work-group per compute unit where the group takes more than
-        // half the LDS: registers for 4 waves per SIMD, six entry groups in
-        // flight (width 3 depth 8, 16384 objects: FP64 1.09 -> 1.07 ms)
+        // half the LDS: registers for 4 waves per SIMD, four entry groups in
+        // flight while four are summed (width 3 depth 8, 16384 objects,
+        // against six and eight: FP64 1.00 / 1.03 / 1.10 ms, FP32 0.657 /
+        // 0.663 / 0.680 ms)
         const int ob = lie_long_objects<T>(m, arg.size);
         const bool whole = lie_lds_bytes<T>(m, arg.size, ob) > kLieLdsShared;
         switch (ob * (whole ? -1 : 1)) {
             // FP64 with 8 objects: two groups in flight fit in 64 registers
-            case 8: return lie_launch<T, 8, 1, sizeof(T) == 8 ? 2 : 4, 8>(n, out, arg, m, row0, rows, col0, stream);
-            case 4: return lie_launch<T, 4, 1, 4, 4>(n, out, arg, m, row0, rows, col0, stream);
-            case 2: return lie_launch<T, 2, 1, 4, 2>(n, out, arg, m, row0, rows, col0, stream);
-            case 1: return lie_launch<T, 1, 1, 4, 1>(n, out, arg, m, row0, rows, col0, stream);
-            case -8: return lie_launch<T, 8, 1, 6, 8, 4>(n, out, arg, m, row0, rows, col0, stream);
-            case -4: return lie_launch<T, 4, 1, 6, 4, 4>(n, out, arg, m, row0, rows, col0, stream);
-            case -2: return lie_launch<T, 2, 1, 6, 2, 4>(n, out, arg, m, row0, rows, col0, stream);
-            case -1: return lie_launch<T, 1, 1, 6, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
+            case 8: return lie_launch<T, 8, 1, sizeof(T) == 8 ? 2 : 4, 8, 8, true>(n, out, arg, m, row0, rows, col0, stream);
+            case 4: return lie_launch<T, 4, 1, 4, 4, 8, true>(n, out, arg, m, row0, rows, col0, stream);
+            case 2: return lie_launch<T, 2, 1, 4, 2, 8, true>(n, out, arg, m, row0, rows, col0, stream);
+            case 1: return lie_launch<T, 1, 1, 4, 1, 8, true>(n, out, arg, m, row0, rows, col0, stream);
+            case -8: return lie_launch<T, 8, 1, 4, 8, 4, true>(n, out, arg, m, row0, rows, col0, stream);
+            case -4: return lie_launch<T, 4, 1, 4, 4, 4, true>(n, out, arg, m, row0, rows, col0, stream);
+            case -2: return lie_launch<T, 2, 1, 4, 2, 4, true>(n, out, arg, m, row0, rows, col0, stream);
+            case -1: return lie_launch<T, 1, 1, 4, 1, 4, true>(n, out, arg, m, row0, rows, col0, stream);
             default: return fail(RPP_HIP_NOT_IMPLEMENTED, "Lie argument does not fit in LDS");
         }
     }
@@ -282,10 +301,10 @@ int launch_lie_spmv(int64_t n, const rpp_hip_vector_batch& out, const rpp_hip_ve
             return joint ? lie_launch<T, 2, 1, 2, 2>(n, out, arg, m, row0, rows, col0, stream)
                          : lie_launch<T, 2, RS, 1>(n, out, arg, m, row0, rows, col0, stream);
         case 1: return lie_launch<T, 1, RS, 1>(n, out, arg, m, row0, rows, col0, stream);
-        case -8: return lie_launch<T, 8, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
-        case -4: return lie_launch<T, 4, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
-        case -2: return lie_launch<T, 2, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
-        case -1: return lie_launch<T, 1, 1, 4>(n, out, arg, m, row0, rows, col0, stream);
+        case -8: return lie_launch<T, 8, 1, 4, 1, 8, true>(n, out, arg, m, row0, rows, col0, stream);
+        case -4: return lie_launch<T, 4, 1, 4, 1, 8, true>(n, out, arg, m, row0, rows, col0, stream);
+        case -2: return lie_launch<T, 2, 1, 4, 1, 8, true>(n, out, arg, m, row0, rows, col0, stream);
+        case -1: return lie_launch<T, 1, 1, 4, 1, 8, true>(n, out, arg, m, row0, rows, col0, stream);
         default: return fail(RPP_HIP_NOT_IMPLEMENTED, "Lie argument does not fit in LDS");
     }
 }
