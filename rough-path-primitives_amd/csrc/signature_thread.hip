@@ -167,12 +167,31 @@ constexpr int thread_bwd_waves() {
 // 2.30, 1.50, 8.78), FP32 the plain loop ((5,6) 0.61 -> 0.59, (6,6) 1.48 ->
 // 1.33, (8,5) 5.39 -> 5.02, (3,8) 0.84 -> 0.80; the ring 0.66, 1.34, 5.61,
 // 0.77).  The arithmetic is the same in every form, bit for bit.
+// UL: the next increment's loads are issued with no test (the last step
+// loads its own row again), so the compiler can wait for the current row
+// alone instead of every load in flight.  Measured against the tested loads
+// (ms per launch): FP64 width 6 depth 6 over 2048 x 256 paths 2.47 -> 2.36,
+// width 5 depth 6 over 4096 x 128 1.086 -> 1.048, width 3 depth 8 over
+// 16384 x 64 1.39 -> 1.35, FP32 width 8 depth 5 over the uneven 8192 paths
+// 5.33 -> 5.22 (taken); FP64 width 8 depth 5 8.47 -> 8.70, FP32 width 5
+// depth 6 0.644 -> 0.72 (not taken).  The backward's loads measured no
+// faster that way.
+template <typename T, int W, int N, int L>
+constexpr bool thread_fwd_uncond() {
+    if constexpr (sizeof(T) == 8) {
+        return (W == 5 && N == 6) || (W == 6 && N == 6) || (W == 3 && N == 8);
+    } else {
+        return W == 8 && N == 5;
+    }
+}
+
 template <typename T, int W, int N, int L, int NT, bool CK>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(thread_fwd_waves<T, W, N, L>())))
 k_sig_fwd_thread(DevBasis B, int n_paths, const int64_t* order, const T* __restrict__ incr, const int64_t* offs, View<T> out,
                  int ck_every, T* ckpts, const int64_t* ck_offs) {
     using G = ThreadGeo<W, N, L>;
     constexpr int LQ = L > 0 ? L : 1;
+    constexpr bool UL = thread_fwd_uncond<T, W, N, L>();
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * NT + threadIdx.x;
     // a wavefront lies inside one path's padded range (NT and PP are
     // multiples of 64), so its path index is uniform
@@ -212,7 +231,7 @@ k_sig_fwd_thread(DevBasis B, int n_paths, const int64_t* order, const T* __restr
                 constexpr int s = decltype(I)::value;
                 const int64_t tt = t + s;
                 if (tt < t1) {
-                    if (tt + 1 < t1) load(xr[s ^ 1], xqr[s ^ 1], tt + 1);
+                    if (UL || tt + 1 < t1) load(xr[s ^ 1], xqr[s ^ 1], tt + 1 < t1 ? tt + 1 : tt);
                     thread_step<T, W, N, L>(chain, sub, xr[s], xqr[s]);
                 }
             });
@@ -235,11 +254,11 @@ k_sig_fwd_thread(DevBasis B, int n_paths, const int64_t* order, const T* __restr
         };
         if (t0 < t1) load(xa, xqa, t0);
         for (int64_t t = t0; t < t1; t += 2) {
-            if (t + 1 < t1) load(xb, xqb, t + 1);
+            if (UL || t + 1 < t1) load(xb, xqb, t + 1 < t1 ? t + 1 : t);
             thread_step<T, W, N, L>(chain, sub, xa, xqa);
             checkpoint();
             if (t + 1 >= t1) break;
-            if (t + 2 < t1) load(xa, xqa, t + 2);
+            if (UL || t + 2 < t1) load(xa, xqa, t + 2 < t1 ? t + 2 : t + 1);
             thread_step<T, W, N, L>(chain, sub, xb, xqb);
             checkpoint();
         }
