@@ -236,9 +236,11 @@ k_log_lds(View<T> out, View<const T> arg, int64_t n) {
             pc = pn;
             pn = tmp;
         }
-        // the last step (d = 1, P_0 = c_1 = 1) straight to HBM with
-        // non-temporal stores (width 3 depth 8 over 16384 objects FP64:
-        // 0.573 -> 0.566 ms); degree 0 of the product is 0
+        // the last step (d = 1, P_0 = c_1 = 1) straight to HBM; degree 0 of
+        // the product is 0.  Ordinary stores: non-temporal ones measured 1 %
+        // faster at width 3 depth 8 over 16384 objects FP64 (0.573 -> 0.566
+        // ms) but wrote 1.05 x the output bytes (partial lines at object
+        // edges go to HBM unmerged)
         T* o = out.obj(b);
         if (t == 0) o[0] = T(0);
         static_for<N>([&](auto kc) {
@@ -249,11 +251,11 @@ k_log_lds(View<T> out, View<const T> arg, int64_t n) {
                     const T yz[W] = {};
                     log_block_sum<T, W, N, k, false>(acc, pc, ys, m, ylo, yhi, T(0), yz, false);
 #pragma unroll
-                    for (int b = 0; b < W; ++b) stream_store(acc[b], &o[G::start(k) + W * m + b]);
+                    for (int b = 0; b < W; ++b) o[G::start(k) + W * m + b] = acc[b];
                 }
             } else if constexpr (k >= 1) {
                 for (int w = t; w < G::words(k); w += NT)
-                    stream_store(log_level_sum<T, W, N, k, false>(pc, ys, w, ylo, yhi, T(0), T(0), false), &o[G::start(k) + w]);
+                    o[G::start(k) + w] = log_level_sum<T, W, N, k, false>(pc, ys, w, ylo, yhi, T(0), T(0), false);
             }
         });
         const T p0 = pc[0];
@@ -266,13 +268,13 @@ k_log_lds(View<T> out, View<const T> arg, int64_t n) {
                 T acc[W];
                 log_block_sum<T, W, N, N, true>(acc, pc, ys, m, ylo, yhi, p0, y, top);
 #pragma unroll
-                for (int b = 0; b < W; ++b) stream_store(acc[b], &o[G::start(N) + W * m + b]);
+                for (int b = 0; b < W; ++b) o[G::start(N) + W * m + b] = acc[b];
             }
         } else {
             for (int w = t; w < G::words(N); w += NT) {
                 // issued before the LDS reads of the sum, whose latency it overlaps
                 const T y = top ? a[G::start(N) + w] : T(0);
-                stream_store(log_level_sum<T, W, N, N, true>(pc, ys, w, ylo, yhi, p0, y, top), &o[G::start(N) + w]);
+                o[G::start(N) + w] = log_level_sum<T, W, N, N, true>(pc, ys, w, ylo, yhi, p0, y, top);
             }
         }
     }
