@@ -176,8 +176,15 @@ def _sfx(*batches) -> str:
     raise RppHipError(8, f"unsupported dtype {dt}")
 
 
+# The current stream's raw handle without building a torch.cuda.Stream object
+# (2.7 us per call on the host, against 0.1 us): every operation asks for it.
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def _stream(stream) -> int:
     if stream is None:
+        if _raw_stream is not None:
+            return _raw_stream(torch.cuda.current_device())
         return torch.cuda.current_stream().cuda_stream
     if isinstance(stream, torch.cuda.Stream):
         return stream.cuda_stream
