@@ -343,7 +343,8 @@ int coal_launch(const DevBasis& B, const rpp_hip_tensor_batch& out, const rpp_hi
 // box, alternating runs, microseconds per call:
 //   FP64: width 4 depth 4 159 -> 134, width 3 depth 8 243 -> 238, width 6
 //   depth 6 324 -> 319, width 8 depth 5 212 -> 207, width 5 depth 6
-//   232 -> 238 (not taken)
+//   232 -> 238 (not taken; non-temporal for its top level only, 28.9 ->
+//   30.7 ms per 128-call chain)
 //   FP32: width 6 depth 6 199 -> 170, width 4 depth 4 82 -> 76, width 5
 //   depth 6 126 -> 120, width 8 depth 5 117 -> 111, width 3 depth 8 147 -> 140
 template <typename T>
