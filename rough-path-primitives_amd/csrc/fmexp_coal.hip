@@ -76,7 +76,8 @@ k_fmexp_coal(DevBasis B, View<T> out, View<const T> mult_, View<const T> ex, int
     // grid that is a multiple of 8, v mod 8 names the XCD of virtual item v.
     // XCD x takes the contiguous item range [x per, (x + 1) per): the items
     // of one object share partial cache lines of its lower subtree levels
-    // and its chain words, and these then meet in one XCD's L2.
+    // and its chain words, and these then meet in one XCD's L2.  The map is a
+    // bijection of items, so results never depend on the dispatch order.
     const int64_t items = n * G::P;
     const int64_t per = XCD ? (items + 7) >> 3 : items;
     for (int64_t v = blockIdx.x; v < (XCD ? per * 8 : items); v += gridDim.x) {

@@ -70,7 +70,8 @@ k_product_small(View<T> out, View<const T> lhs, View<const T> rhs, View<const T>
     // Work-groups are dispatched to the eight XCDs round robin and the grid
     // is a multiple of 8, so v mod 8 names the XCD of virtual object v: XCD
     // x takes the contiguous objects [x per, (x + 1) per), and the cache
-    // lines that neighbouring objects share meet in one L2.
+    // lines that neighbouring objects share meet in one L2.  The map is a
+    // bijection of objects, so results never depend on the dispatch order.
     const int64_t per = (n + 7) >> 3;
     for (int64_t v = blockIdx.x; v < per * 8; v += gridDim.x) {
         const int64_t b = (v & 7) * per + (v >> 3);
