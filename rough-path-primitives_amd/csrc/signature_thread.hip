@@ -170,12 +170,12 @@ constexpr int thread_bwd_waves() {
 // UL: the next increment's loads are issued with no test (the last step
 // loads its own row again), so the compiler can wait for the current row
 // alone instead of every load in flight.  Measured against the tested loads
-// (ms per launch): FP64 width 6 depth 6 over 2048 x 256 paths 2.47 -> 2.36,
-// width 5 depth 6 over 4096 x 128 1.086 -> 1.048, width 3 depth 8 over
-// 16384 x 64 1.39 -> 1.35, FP32 width 8 depth 5 over the uneven 8192 paths
-// 5.33 -> 5.22 (taken); FP64 width 8 depth 5 8.47 -> 8.70, FP32 width 5
-// depth 6 0.644 -> 0.72 (not taken).  The backward's loads measured no
-// faster that way.
+// (ms per launch, alternating runs): FP64 width 6 depth 6 over 2048 x 256
+// paths 2.51 -> 2.40, width 5 depth 6 over 4096 x 128 1.10 -> 1.07, width 3
+// depth 8 over 16384 x 64 1.41 -> 1.385, FP32 width 8 depth 5 over the
+// uneven 8192 paths 5.39 -> 5.29 (taken); FP64 width 8 depth 5 8.47 -> 8.70,
+// FP32 width 5 depth 6 0.644 -> 0.72 (not taken).  The backward's loads
+// measured no faster that way.
 template <typename T, int W, int N, int L>
 constexpr bool thread_fwd_uncond() {
     if constexpr (sizeof(T) == 8) {
